@@ -1,0 +1,218 @@
+/*
+ * slemm_gpu.h -- C ABI of the MI355X (gfx950) engine for SLEMM's stochastic-Lanczos REML path.
+ *
+ * The reference (jiang18/slemm v0.90.1) has no plugin or FFI layer: its operator seam is the
+ * set of C++ functions in src/reml.cpp that the Krylov engine and the REML drivers call
+ * (lnr_op_select, rmatvec, weighted_rmatvec, matvec, L_Seed, L_Solve, SLQ_LDet) plus the
+ * bed decoding and counting in src/input.cpp / src/weight.cpp and the GEBV loop in
+ * src/pred.cpp.  Every entry point below names the reference function it replaces.
+ *
+ * Conventions
+ *  - Every function returns an int status (SLEMM_OK == 0).  On failure a message is kept per
+ *    host thread and returned by slemm_gpu_last_error().  No C++ exception crosses this ABI.
+ *  - Dense matrices are column-major float32 with an explicit leading dimension, exactly like
+ *    the reference's Eigen MatrixXf arguments.  Pointers documented as "device" live in HBM
+ *    (hipMalloc'd or torch tensors); pointers documented as "host" are ordinary memory.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the default stream).  Calls that
+ *    return host results synchronise that stream before returning.
+ *  - A handle is used by one host thread at a time.
+ */
+#ifndef SLEMM_GPU_H_
+#define SLEMM_GPU_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SLEMM_OK = 0,
+  SLEMM_ERR_ARG = 1,     /* invalid argument (sizes, null pointers, unknown kind) */
+  SLEMM_ERR_HIP = 2,     /* a HIP runtime call failed */
+  SLEMM_ERR_ALLOC = 3,   /* device allocation failed */
+  SLEMM_ERR_STATE = 4,   /* handle not ready (e.g. model not set) */
+  SLEMM_ERR_RANK = 5,    /* fixed-effects design not of full rank (src/reml.cpp:586,769) */
+  SLEMM_ERR_DATA = 6     /* malformed input data (e.g. truncated bed, src/input.cpp:366) */
+};
+
+/* Linear operator kinds of lnr_op_select (src/reml.cpp:96-118). */
+enum {
+  SLEMM_OP_LDET = 0,     /* H0_ldet_mv     (src/reml.cpp:52-94):  R^-1/2 Zc W Zc' R^-1/2 v + tau0 v */
+  SLEMM_OP_QUADFORM = 1  /* H0_quadform_mv (src/reml.cpp:8-50):   (I-QQ')R^-1/2 Zc W Zc' R^-1/2 (I-QQ') v + tau0 v */
+};
+
+/* Arithmetic used for the genotype block products. */
+enum {
+  SLEMM_PREC_FP32 = 0,   /* FP32 operands on FP32 MFMA, FP32 accumulation (plain path) */
+  SLEMM_PREC_BF16X2 = 1  /* vector block split into BF16 hi + lo, BF16 MFMA, FP32 accumulation */
+};
+
+typedef struct slemm_geno slemm_geno;   /* device-resident packed genotypes + model terms */
+typedef struct slemm_lseed slemm_lseed; /* device-resident Lanczos seed system (struct lseed) */
+
+const char* slemm_gpu_last_error(void);
+int slemm_gpu_version(void);
+int slemm_gpu_device_count(int* count);
+
+/* ---------------------------------------------------------------- genotypes ---------------- */
+
+/* Replaces the bed part of read_plink_bin_geno_file_and_generate_pxvb_data
+ * (src/input.cpp:343-386: individual compaction) and the marker reorder (:446-454).
+ * bed_rows: host, m_raw rows of row_bytes = ceil(n_raw/4) bytes (the .bed body after the 3
+ * magic bytes).  keep_idx: host, n_keep strictly increasing fam indices of retained
+ * individuals (NULL = all).  snp_idx: host, the m bed rows to retain, in analysis order
+ * (NULL = all rows in file order).  Codes are kept verbatim (00,01,10,11); dosage = low+high. */
+int slemm_gpu_geno_create(const uint8_t* bed_rows, int64_t n_raw, int64_t m_raw, int64_t row_bytes,
+                          const int64_t* keep_idx, int64_t n_keep,
+                          const int64_t* snp_idx, int64_t m, slemm_geno** out);
+
+/* New rows-subset handle built on the device from an existing one (MAF/HWE filtering and
+ * group ordering of src/input.cpp:389-454 without re-reading the bed). */
+int slemm_gpu_geno_subset(const slemm_geno* h, const int64_t* snp_idx, int64_t m, slemm_geno** out);
+
+/* Synthetic bed generated in HBM: per-SNP p ~ U(maf_lo, maf_hi), genotype ~ Binomial(2,p)
+ * (A2 count), `missing` fraction written as code 01.  Counter-based hashing, so the content
+ * depends only on (seed, n, m); no reference counterpart (test and benchmark input). */
+int slemm_gpu_geno_synthesize(int64_t n, int64_t m, uint64_t seed, float maf_lo, float maf_hi,
+                              float missing, slemm_geno** out);
+
+int slemm_gpu_geno_destroy(slemm_geno* h);
+int slemm_gpu_geno_dims(const slemm_geno* h, int64_t* n, int64_t* m);
+
+/* Copy packed SNP-major rows [j0, j0+cnt) to host (ceil(n/4) bytes each, PLINK code layout,
+ * padding codes 00).  Exposes the decoded kmat of src/input.cpp:374-377 for bit-exact checks. */
+int slemm_gpu_geno_download(const slemm_geno* h, int64_t j0, int64_t cnt, uint8_t* rows);
+
+/* Per-SNP bit counts.  popcnt = number of set bits (the nz_bits of src/reml.cpp:616-619 and
+ * src/weight.cpp:8-11); n2 = both bits, n1 = exactly one bit, n0 = none (src/input.cpp:14-19).
+ * gm (optional) = float(popcnt)/float(n), the genotype mean of src/reml.cpp:620.
+ * All outputs are host arrays of length m; any may be NULL. */
+int slemm_gpu_geno_counts(slemm_geno* h, int32_t* popcnt, int32_t* n0, int32_t* n1, int32_t* n2,
+                          float* gm);
+
+/* ---------------------------------------------------------------- model terms -------------- */
+
+/* Sets the terms the operators need, all host arrays:
+ *   persnp (m, NULL = all ones): per-SNP heritability weights; gw = (gm(2-gm)/2)^-1 / m *
+ *     persnp/mean(persnp) as src/reml.cpp:622;
+ *   rvec (n, NULL = all ones): error-variance weights R; r_rsqrt = rvec^-1/2 (src/reml.cpp:574);
+ *   Q (n x c column-major, ldq): orthonormal basis of R^-1/2 X (c may be 0).
+ * gm is recomputed on the device from the bit counts. */
+int slemm_gpu_set_model(slemm_geno* h, const float* persnp, const float* rvec,
+                        const float* Q, int64_t ldq, int c);
+
+/* Host copies of the model terms (each may be NULL). */
+int slemm_gpu_get_model(const slemm_geno* h, float* gm, float* gw, float* r_rsqrt);
+
+/* ---------------------------------------------------------------- operators ---------------- */
+
+/* lnr_op_select (src/reml.cpp:96-118) over all t columns in one batch.
+ * V, out: device, n x t column-major (ldv, ldo >= n).  kind applies to every column unless
+ * col_kind (host, t entries) is given, in which case each column uses its own kind, so
+ * probes (LDET) and the phenotype (QUADFORM) share one pass over the genotypes. */
+int slemm_gpu_op(slemm_geno* h, int kind, const int* col_kind, float tau0,
+                 const float* V, int64_t ldv, int t, float* out, int64_t ldo,
+                 int precision, void* stream);
+
+/* rmatvec (src/reml.cpp:159-195): out (m x t, ldo) = Zc' V.
+ * weighted_rmatvec (src/reml.cpp:120-157) when weighted != 0: out = W Zc' V.
+ * V device n x t (ldv).  If scale_rr != 0 the rows of V are first multiplied by r_rsqrt. */
+int slemm_gpu_zt(slemm_geno* h, const float* V, int64_t ldv, int t, float* out, int64_t ldo,
+                 int weighted, int scale_rr, int precision, void* stream);
+
+/* matvec (src/reml.cpp:197-233): out (n x t, ldo) = Zc U with U device m x t (ldu). */
+int slemm_gpu_z(slemm_geno* h, const float* U, int64_t ldu, int t, float* out, int64_t ldo,
+                int precision, void* stream);
+
+/* computeTotalGeneticValuePLINK inner loop (src/pred.cpp:112-130): gebv (device, n doubles)
+ * = sum_j alpha_j * z_ij over raw dosages z (not centred), with z -> 2-z where flip_j != 0.
+ * alpha device (m doubles), flip host or NULL (m bytes).  FP64 accumulation. */
+int slemm_gpu_gebv(slemm_geno* h, const double* alpha, const uint8_t* flip, double* gebv,
+                   void* stream);
+
+/* GRAMMAR-gamma score terms of bin/slemm_gamma.py:306-319 for every SNP j (device outputs,
+ * m floats each): xy_j = x_c' Py, xx_j = || (I-QQ') R^-1/2 x_c ||^2, with x_c the mean-
+ * centred dosage (mean = popcnt/n).  Py device (n floats).  skip_mac: SNPs with
+ * popcnt <= skip_mac or 2n-popcnt <= skip_mac get NaN (the reference's `--mac` filter). */
+int slemm_gpu_gwas_score(slemm_geno* h, const float* Py, float skip_mac, float* xy, float* xx,
+                         void* stream);
+
+/* ---------------------------------------------------------------- Krylov engine ------------ */
+
+/* L_Seed (src/reml.cpp:246-339): t lock-step Lanczos/CG recurrences on the operator of
+ * `kind` (or col_kind per column) at shift tau0, starting from B (device n x t, ldb).
+ * Dot products, norms and all recurrence scalars are FP64.
+ * col_group (host, t entries, or NULL = one group): several independent seed systems in one
+ * batch -- each group stops on its own, exactly as a separate L_Seed call would, when
+ * max over its columns of |rho|/||b|| <= tol, or after maxit-1 steps; the batch ends when
+ * every group has stopped.  keep_cols: the leading keep_cols columns keep every basis
+ * vector U_j in HBM (needed by slemm_gpu_lsolve); the other columns keep only their
+ * tridiagonal coefficients (SLQ probes; the reference frees their U at :634-636). */
+int slemm_gpu_lseed(slemm_geno* h, int kind, const int* col_kind, const int* col_group, float tau0,
+                    const float* B, int64_t ldb, int t, float tol, int maxit, int keep_cols,
+                    int precision, int verbose, void* stream, slemm_lseed** out);
+
+/* t, the largest step count, and whether every group converged. */
+int slemm_gpu_lseed_info(const slemm_lseed* s, int* t, int* steps, int* converged);
+/* Per-column step count j_k (host, t ints). */
+int slemm_gpu_lseed_col_steps(const slemm_lseed* s, int* steps);
+/* beta (host, t x (steps+1), column-major) and delta (t x steps); entries past a column's own
+ * step count are zero.  Same layout as struct lseed's beta/delta (src/reml.cpp:333-336). */
+int slemm_gpu_lseed_coeffs(const slemm_lseed* s, double* beta, double* delta);
+int slemm_gpu_lseed_destroy(slemm_lseed* s);
+
+/* L_Solve (src/reml.cpp:369-431): X (device n x t2, ldx) solving (A + sigma I) X = B by
+ * replaying the shifted CG recurrence on the stored basis of columns [col0, col0+t2), which
+ * must be kept columns.  B device n x t2 (ldb) -- the same vectors the seed started from. */
+int slemm_gpu_lsolve(const slemm_lseed* s, int col0, int t2, const float* B, int64_t ldb,
+                     float sigma, float tol, float* X, int64_t ldx, void* stream);
+
+/* Tridiagonal spectral data for SLQ (src/reml.cpp:637-646) in FP64 on the host for column k:
+ * D, W (host, steps_k each): eigenvalues and first eigenvector components. */
+int slemm_gpu_lseed_spectrum(const slemm_lseed* s, int k, double* D, double* W);
+
+/* SLQ_LDet (src/reml.cpp:440-451) in FP64 over the probe columns [col0, col0+nv):
+ * n/nv * sum_k sum_i W_ki^2 log(D_ki + sigma). */
+int slemm_gpu_slq_ldet(const slemm_lseed* s, int col0, int nv, double sigma, double* ldet);
+
+/* ---------------------------------------------------------------- REML drivers ------------- */
+
+typedef struct {
+  float vg, ve, h2, lrt;        /* variance components, h2, LRT (0 for LMC) */
+  int lanczos_steps[3];         /* Lanczos steps of the seed systems (y, Q, probes) */
+  int criterion_evals;          /* REML_criterion / REML_first_order evaluations */
+  double seconds_seed;          /* wall time of the seed systems */
+  double seconds_total;         /* wall time from resident bed to final estimates */
+} slemm_reml_stats;
+
+/* SLDF_REML (src/reml.cpp:549-690).  Host inputs: X (n x c, ldx) covariates, y (n).  Probes:
+ * Rademacher from std::mt19937(rng_seed+54321) as src/reml.cpp:597-605.  Host outputs:
+ * Py (n), snp_blup (m), blue (c), var_blue (c x c).  persnp/rvec given by set_model. */
+int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y,
+                        int rng_seed, int n_V, float tol_L, float s2max, float s2min, float tol_VC,
+                        int precision, int verbose,
+                        float* Py, float* snp_blup, float* blue, float* var_blue,
+                        slemm_reml_stats* stats);
+
+/* LMC_REML without --lmm (src/reml.cpp:730-882).  MC samples from std::mt19937(rng_seed+54321)
+ * with std::normal_distribution<float> (src/reml.cpp:795-809). */
+int slemm_gpu_lmc_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y,
+                       int rng_seed, int n_MC, float tol_L, float s2max, float s2min,
+                       int precision, int verbose,
+                       float* Py, float* snp_blup, float* blue, slemm_reml_stats* stats);
+
+/* ---------------------------------------------------------------- instrumentation ---------- */
+
+/* Kernel timing of the genotype block products (no reference counterpart; the reference only
+ * prints wall-clock stamps, src/option.cpp:21-34).  When enabled, every block-product launch
+ * is bracketed by HIP events on its own stream.  pass: 0 = Z'V pass, 1 = Z U pass. */
+int slemm_gpu_timing(int enable);
+int slemm_gpu_timing_read(int pass, int precision, double* total_ms, int* launches);
+int slemm_gpu_timing_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLEMM_GPU_H_ */

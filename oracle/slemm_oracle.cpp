@@ -1,0 +1,484 @@
+// ORACLE -- test infrastructure only.  CPU restatement of the reference's algorithms for the
+// stochastic-Lanczos REML path, used by tests/ as the checker and by bench.py as the CPU
+// baseline.  Nothing in the product (slemm_amd/) links, imports or runs this code.
+//
+// Every function cites the reference file:line it restates.  Arithmetic follows the
+// reference: FP32 everywhere (MatrixXf/VectorXf) with the genotype bits unpacked per SNP
+// subset into a float panel and two matrix products per subset.  The reference's GEMM engine
+// (Eigen -> MKL sgemm) is not available, so products are written as plain OpenMP loops; the
+// summation order therefore differs from MKL's (parity at FP32 rounding level only).
+//
+// Genotype bits use the reference's kmat representation: one byte (0/1) per bit, row j holds
+// 2n bits, individual i at bits 2i (low) and 2i+1 (high), exactly as vector<bool> kmat[j].
+#include <omp.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+extern "C" {
+
+// --------------------------------------------------------------------------- input -------
+// read_plink_bin_geno_file_and_generate_pxvb_data, bed part (src/input.cpp:343-386) plus the
+// marker reorder (:446-454).  bed: whole file (3 magic bytes are skipped unchecked, :352).
+// bindi[n_raw], bmarker[m_raw]: retained flags.  order[m_keep]: for each final row, the rank
+// of that marker among the retained markers in bim order (marker2index).  kmat: m_keep x 2*n_keep
+// bytes, preset to 1 as the reference presets kmat to true (src/option.cpp:347).
+// Returns 0, or -1 when the file is truncated ("problem with the BED file", :366).
+int orc_decode_bed(const uint8_t* bed, int64_t bed_bytes, int64_t n_raw, int64_t m_raw, const uint8_t* bindi,
+                   const uint8_t* bmarker, const int64_t* order, int64_t m_keep, uint8_t* kmat) {
+  int64_t n_keep = 0, m_ret = 0;
+  for (int64_t i = 0; i < n_raw; ++i) n_keep += bindi[i] ? 1 : 0;
+  for (int64_t j = 0; j < m_raw; ++j) m_ret += bmarker[j] ? 1 : 0;
+  const int64_t w = 2 * n_keep;
+  std::vector<uint8_t> km((size_t)m_ret * w, 1);
+  int64_t pos = 3;
+  auto read = [&](uint8_t& ch) -> bool {
+    if (pos >= bed_bytes) return false;
+    ch = bed[pos++];
+    return true;
+  };
+  int64_t snp_indx = 0;
+  for (int64_t j = 0; j < m_raw; ++j) {
+    uint8_t ch;
+    if (!bmarker[j]) {
+      for (int64_t i = 0; i < n_raw; i += 4) read(ch);
+      continue;
+    }
+    for (int64_t i = 0, indi = 0; i < n_raw;) {
+      if (!read(ch)) return -1;
+      int k = 0;
+      while (k < 7 && i < n_raw) {
+        if (!bindi[i]) {
+          k += 2;
+        } else {
+          // only zero bits are written into the preset-true row (:374-377)
+          if (!((ch >> k) & 1)) km[(size_t)snp_indx * w + indi] = 0;
+          ++indi; ++k;
+          if (!((ch >> k) & 1)) km[(size_t)snp_indx * w + indi] = 0;
+          ++indi; ++k;
+        }
+        ++i;
+      }
+    }
+    ++snp_indx;
+    if (snp_indx == m_ret) break;
+  }
+  for (int64_t r = 0; r < m_keep; ++r)
+    std::memcpy(kmat + (size_t)r * w, km.data() + (size_t)order[r] * w, (size_t)w);
+  return 0;
+}
+
+// Bit counts: nz_bits (src/reml.cpp:613-621 -> gm = nz_bits/float(n)), and the HWE genotype
+// counts of cal_hwep_from_genobits (src/input.cpp:12-21): n2 = both bits, n1 = one bit, n0 = none.
+void orc_counts(const uint8_t* kmat, int64_t m, int64_t n, int32_t* popcnt, int32_t* n0, int32_t* n1,
+                int32_t* n2, float* gm) {
+#pragma omp parallel for
+  for (int64_t j = 0; j < m; ++j) {
+    const uint8_t* b = kmat + (size_t)j * 2 * n;
+    int nz = 0, c0 = 0, c1 = 0, c2 = 0;
+    for (int64_t k = 0; k < 2 * n; ++k)
+      if (b[k]) ++nz;
+    for (int64_t k = 0; k < 2 * n; k += 2) {
+      if (b[k] && b[k + 1]) ++c2;
+      else if (b[k] || b[k + 1]) ++c1;
+      else ++c0;
+    }
+    popcnt[j] = nz;
+    n0[j] = c0;
+    n1[j] = c1;
+    n2[j] = c2;
+    gm[j] = nz / float(n);
+  }
+}
+
+// cal_maf_from_genobits (src/weight.cpp:3-15): f = nz_bits / float(2n); maf = |[f>0.5] - f|.
+void orc_maf(const uint8_t* kmat, int64_t m, int64_t n, float* maf) {
+  const int nb = (int)(2 * n);
+  for (int64_t j = 0; j < m; ++j) {
+    const uint8_t* b = kmat + (size_t)j * 2 * n;
+    int nz = 0;
+    for (int k = 0; k < nb; ++k)
+      if (b[k]) ++nz;
+    const float f = nz / float(nb);
+    maf[j] = std::fabs((f > 0.5f ? 1.0f : 0.0f) - f);
+  }
+}
+
+// gw of src/reml.cpp:622: (gm(2-gm)/2)^-1 / m * persnp/mean(persnp), FP32.
+void orc_gw(const float* gm, const float* persnp, int64_t m, float* gw) {
+  float mean = 1.0f;
+  if (persnp) {
+    double s = 0;
+    for (int64_t j = 0; j < m; ++j) s += persnp[j];
+    mean = (float)(s / (double)m);
+  }
+  for (int64_t j = 0; j < m; ++j) {
+    const float v = (gm[j] * (2.0f - gm[j])) / 2.0f;
+    gw[j] = ((1.0f / v) / (float)m) * (persnp ? persnp[j] / mean : 1.0f);
+  }
+}
+
+// ------------------------------------------------------------------------ operators -------
+namespace {
+
+struct Panel {
+  // row-major n x s panel of centred dosages (Z - m) for one SNP subset (src/reml.cpp:29-35)
+  std::vector<float> a;
+  int64_t n = 0;
+  int s = 0;
+};
+
+void unpack(const uint8_t* kmat, int64_t n, int64_t j0, int s, const float* gm, Panel& P) {
+  P.n = n;
+  P.s = s;
+  P.a.assign((size_t)n * s, 0.f);
+#pragma omp parallel for
+  for (int j = 0; j < s; ++j) {
+    const uint8_t* b = kmat + (size_t)(j0 + j) * 2 * n;
+    const float mg = gm[j0 + j];
+    for (int64_t kk = 0; kk < n; ++kk) P.a[(size_t)kk * s + j] = (float)(b[2 * kk] + b[2 * kk + 1]) - mg;
+  }
+}
+
+// T (s x t, row-major) = P' * V   with V given row-major n x t
+void pt_v(const Panel& P, const float* Vr, int t, float* T) {
+  const int s = P.s;
+  const int64_t n = P.n;
+#pragma omp parallel for schedule(static)
+  for (int j0 = 0; j0 < s; j0 += 16) {
+    const int jb = std::min(16, s - j0);
+    std::vector<float> acc((size_t)jb * t, 0.f);
+    for (int64_t i = 0; i < n; ++i) {
+      const float* pr = &P.a[(size_t)i * s + j0];
+      const float* vr = Vr + (size_t)i * t;
+      for (int jj = 0; jj < jb; ++jj) {
+        const float a = pr[jj];
+        float* ar = &acc[(size_t)jj * t];
+        for (int k = 0; k < t; ++k) ar[k] += a * vr[k];
+      }
+    }
+    std::memcpy(T + (size_t)j0 * t, acc.data(), sizeof(float) * jb * t);
+  }
+}
+
+// Out (n x t, row-major) += P * T   with T row-major s x t
+void p_t(const Panel& P, const float* T, int t, float* Outr) {
+  const int s = P.s;
+  const int64_t n = P.n;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const float* pr = &P.a[(size_t)i * s];
+    float* o = Outr + (size_t)i * t;
+    for (int j = 0; j < s; ++j) {
+      const float a = pr[j];
+      const float* tr = T + (size_t)j * t;
+      for (int k = 0; k < t; ++k) o[k] += a * tr[k];
+    }
+  }
+}
+
+void to_rowmajor(const float* V, int64_t ld, int64_t n, int t, std::vector<float>& Vr) {
+  Vr.resize((size_t)n * t);
+  for (int k = 0; k < t; ++k)
+    for (int64_t i = 0; i < n; ++i) Vr[(size_t)i * t + k] = V[i + (size_t)k * ld];
+}
+
+void from_rowmajor(const std::vector<float>& Vr, int64_t n, int t, float* V, int64_t ld) {
+  for (int k = 0; k < t; ++k)
+    for (int64_t i = 0; i < n; ++i) V[i + (size_t)k * ld] = Vr[(size_t)i * t + k];
+}
+
+// sum over SNP subsets of Zc_s * (w_s o (Zc_s' * Vr))  (the common core of :28-47 and :71-90)
+void zwzt(const uint8_t* kmat, int64_t n, int64_t m, const float* gm, const float* w, const std::vector<float>& Vr,
+          int t, int subset_size, std::vector<float>& Outr) {
+  Outr.assign((size_t)n * t, 0.f);
+  Panel P;
+  std::vector<float> T;
+  for (int64_t j0 = 0; j0 < m; j0 += subset_size) {
+    const int s = (int)std::min<int64_t>(subset_size, m - j0);
+    unpack(kmat, n, j0, s, gm, P);
+    T.assign((size_t)s * t, 0.f);
+    pt_v(P, Vr.data(), t, T.data());
+    for (int j = 0; j < s; ++j)
+      for (int k = 0; k < t; ++k) T[(size_t)j * t + k] *= w[j0 + j];
+    p_t(P, T.data(), t, Outr.data());
+  }
+}
+
+// Q' X (c x t) and X - Q (Q' X), FP32 like the reference
+void project(const float* Q, int64_t n, int c, std::vector<float>& Xr, int t) {
+  if (c == 0) return;
+  std::vector<float> QtX((size_t)c * t, 0.f);
+  for (int q = 0; q < c; ++q)
+    for (int64_t i = 0; i < n; ++i)
+      for (int k = 0; k < t; ++k) QtX[(size_t)q * t + k] += Q[i + (size_t)q * n] * Xr[(size_t)i * t + k];
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < t; ++k) {
+      float s = 0.f;
+      for (int q = 0; q < c; ++q) s += Q[i + (size_t)q * n] * QtX[(size_t)q * t + k];
+      Xr[(size_t)i * t + k] -= s;
+    }
+}
+
+}  // namespace
+
+// H0_quadform_mv (src/reml.cpp:8-50, kind 1) and H0_ldet_mv (:52-94, kind 0).
+// V, out: n x t column-major (ld n).  Q: n x c column-major.  rr: n.
+void orc_op(int kind, const uint8_t* kmat, int64_t n, int64_t m, const float* gm, const float* gw,
+            const float* rr, float tau0, const float* Q, int c, const float* V, int t, int subset_size,
+            float* out) {
+  std::vector<float> Vr, Xr, Or;
+  to_rowmajor(V, n, n, t, Vr);
+  Xr = Vr;
+  if (kind == 1) project(Q, n, c, Xr, t);                         // v - Q(Q'v)       (:26)
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < t; ++k) Xr[(size_t)i * t + k] *= rr[i];   // rr o (...)
+  zwzt(kmat, n, m, gm, gw, Xr, t, subset_size, Or);
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < t; ++k) Or[(size_t)i * t + k] *= rr[i];   // (:48 / :91)
+  if (kind == 1) project(Q, n, c, Or, t);                         // - Q(Q'ZPv)       (:49)
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < t; ++k) Or[(size_t)i * t + k] += tau0 * Vr[(size_t)i * t + k];
+  from_rowmajor(Or, n, t, out, n);
+}
+
+// rmatvec (src/reml.cpp:159-195): out (m x t) = (Z-m)'V; weighted_rmatvec (:120-157) when gw.
+void orc_rmatvec(const uint8_t* kmat, int64_t n, int64_t m, const float* gm, const float* gw, const float* V,
+                 int t, int subset_size, float* out) {
+  std::vector<float> Vr;
+  to_rowmajor(V, n, n, t, Vr);
+  Panel P;
+  std::vector<float> T;
+  for (int64_t j0 = 0; j0 < m; j0 += subset_size) {
+    const int s = (int)std::min<int64_t>(subset_size, m - j0);
+    unpack(kmat, n, j0, s, gm, P);
+    T.assign((size_t)s * t, 0.f);
+    pt_v(P, Vr.data(), t, T.data());
+    for (int j = 0; j < s; ++j)
+      for (int k = 0; k < t; ++k)
+        out[(j0 + j) + (size_t)k * m] = gw ? gw[j0 + j] * T[(size_t)j * t + k] : T[(size_t)j * t + k];
+  }
+}
+
+// matvec (src/reml.cpp:197-233): out (n x t) = (Z-m) U, U m x t.
+void orc_matvec(const uint8_t* kmat, int64_t n, int64_t m, const float* gm, const float* U, int t,
+                int subset_size, float* out) {
+  std::vector<float> Ur, Or((size_t)n * t, 0.f);
+  to_rowmajor(U, m, m, t, Ur);
+  Panel P;
+  for (int64_t j0 = 0; j0 < m; j0 += subset_size) {
+    const int s = (int)std::min<int64_t>(subset_size, m - j0);
+    unpack(kmat, n, j0, s, gm, P);
+    p_t(P, Ur.data() + (size_t)j0 * t, t, Or.data());
+  }
+  from_rowmajor(Or, n, t, out, n);
+}
+
+// ------------------------------------------------------------------------ Krylov ----------
+struct OrcSeed {
+  int64_t n = 0;
+  int t = 0;
+  int steps = 0;
+  bool converged = false;
+  std::vector<std::vector<float>> U;  // column-major n x t each
+  std::vector<float> beta, delta;     // t x (steps+1), t x steps (column-major)
+};
+
+// L_Seed (src/reml.cpp:246-339), FP32.  Returns a handle.
+void* orc_lseed(int kind, const uint8_t* kmat, int64_t n, int64_t m, const float* gm, const float* gw,
+                const float* rr, float tau0, const float* Q, int c, const float* B, int t, float tol, int maxit,
+                int subset_size) {
+  auto* s = new OrcSeed();
+  s->n = n;
+  s->t = t;
+  std::vector<float> Bn(t), rho((size_t)t * maxit, 0.f), beta((size_t)t * maxit, 0.f),
+      omega((size_t)t * maxit, 0.f), gamma((size_t)t * maxit, 1.f), delta((size_t)t * maxit, 0.f);
+  for (int k = 0; k < t; ++k) {
+    float ss = 0.f;
+    for (int64_t i = 0; i < n; ++i) ss += B[i + (size_t)k * n] * B[i + (size_t)k * n];
+    Bn[k] = std::sqrt(ss);
+    rho[k] = Bn[k];
+    beta[k] = Bn[k];
+  }
+  s->U.emplace_back((size_t)n * t);
+  for (int k = 0; k < t; ++k)
+    for (int64_t i = 0; i < n; ++i) s->U[0][i + (size_t)k * n] = B[i + (size_t)k * n] / beta[k];
+  std::vector<float> tmp((size_t)n * t);
+  int j = 0;
+  bool cnvg = false;
+  while (j < maxit - 1) {
+    s->U.emplace_back((size_t)n * t, 0.f);
+    orc_op(kind, kmat, n, m, gm, gw, rr, tau0, Q, c, s->U[j].data(), t, subset_size, tmp.data());
+    for (int k = 0; k < t; ++k) {
+      float* tk = &tmp[(size_t)k * n];
+      const float* uj = &s->U[j][(size_t)k * n];
+      if (j > 0) {
+        const float* up = &s->U[j - 1][(size_t)k * n];
+        const float bj = beta[(size_t)j * t + k];
+        for (int64_t i = 0; i < n; ++i) tk[i] -= up[i] * bj;
+      }
+      float d = 0.f;
+      for (int64_t i = 0; i < n; ++i) d += uj[i] * tk[i];
+      delta[(size_t)j * t + k] = d;
+      float nn = 0.f;
+      for (int64_t i = 0; i < n; ++i) {
+        tk[i] -= uj[i] * d;
+        nn += tk[i] * tk[i];
+      }
+      const float b1 = std::sqrt(nn);
+      beta[(size_t)(j + 1) * t + k] = b1;
+      float* un = &s->U[j + 1][(size_t)k * n];
+      for (int64_t i = 0; i < n; ++i) un[i] = tk[i] / b1;
+      // CG coefficients (:304-307)
+      const size_t o = (size_t)j * t + k;
+      if (j == 0) gamma[o] = 1.f / delta[o];
+      else gamma[o] = 1.f / (delta[o] - omega[o - t] / gamma[o - t]);
+      omega[o] = (b1 * gamma[o]) * (b1 * gamma[o]);
+      rho[o + t] = -b1 * gamma[o] * rho[o];
+    }
+    ++j;
+    float res = 0.f;
+    for (int k = 0; k < t; ++k) {
+      const float* un = &s->U[j][(size_t)k * n];
+      float nr = 0.f;
+      const float r = rho[(size_t)j * t + k];
+      for (int64_t i = 0; i < n; ++i) nr += (un[i] * r) * (un[i] * r);
+      res = std::max(res, std::sqrt(nr) / Bn[k]);
+    }
+    if (res <= tol) {
+      cnvg = true;
+      break;
+    }
+  }
+  s->steps = j;
+  s->converged = cnvg;
+  s->beta.assign(beta.begin(), beta.begin() + (size_t)t * (j + 1));
+  s->delta.assign(delta.begin(), delta.begin() + (size_t)t * j);
+  return s;
+}
+
+int orc_lseed_steps(void* h) { return static_cast<OrcSeed*>(h)->steps; }
+int orc_lseed_converged(void* h) { return static_cast<OrcSeed*>(h)->converged ? 1 : 0; }
+
+// beta: t x (steps+1), delta: t x steps, column-major (row k = RHS column k)
+void orc_lseed_coeffs(void* h, float* beta, float* delta) {
+  auto* s = static_cast<OrcSeed*>(h);
+  std::memcpy(beta, s->beta.data(), sizeof(float) * s->beta.size());
+  std::memcpy(delta, s->delta.data(), sizeof(float) * s->delta.size());
+}
+
+void orc_lseed_free(void* h) { delete static_cast<OrcSeed*>(h); }
+
+// L_Solve (src/reml.cpp:369-431), FP32.  B, X: n x t column-major.
+void orc_lsolve(void* h, const float* B, float sigma, float tol, float* X) {
+  auto* s = static_cast<OrcSeed*>(h);
+  const int64_t n = s->n;
+  const int t = s->t;
+  const int maxit = (int)s->U.size();
+  std::vector<float> Bn(t), P(B, B + (size_t)n * t), R((size_t)n * t);
+  std::vector<float> omega((size_t)t * maxit, 0.f), gamma((size_t)t * maxit, 1.f), rho((size_t)t * maxit, 0.f);
+  for (int k = 0; k < t; ++k) {
+    float ss = 0.f;
+    for (int64_t i = 0; i < n; ++i) ss += B[i + (size_t)k * n] * B[i + (size_t)k * n];
+    Bn[k] = std::sqrt(ss);
+    rho[k] = Bn[k];
+  }
+  std::fill(X, X + (size_t)n * t, 0.f);
+  int j = 0;
+  while (j < maxit - 1) {
+    float res = 0.f;
+    for (int k = 0; k < t; ++k) {
+      const size_t o = (size_t)j * t + k;
+      const float dj = s->delta[o] + sigma;
+      if (j == 0) gamma[o] = 1.f / dj;
+      else gamma[o] = 1.f / (dj - omega[o - t] / gamma[o - t]);
+      const float b1 = s->beta[o + t];
+      omega[o] = (b1 * gamma[o]) * (b1 * gamma[o]);
+      rho[o + t] = -b1 * gamma[o] * rho[o];
+      const float* un = &s->U[j + 1][(size_t)k * n];
+      float nr = 0.f;
+      for (int64_t i = 0; i < n; ++i) {
+        const size_t e = i + (size_t)k * n;
+        X[e] += P[e] * gamma[o];
+        R[e] = un[i] * rho[o + t];
+        P[e] = R[e] + P[e] * omega[o];
+        nr += R[e] * R[e];
+      }
+      res = std::max(res, std::sqrt(nr) / Bn[k]);
+    }
+    ++j;
+    if (res <= tol) break;
+  }
+}
+
+// ------------------------------------------------------------------------ RNG -------------
+// Rademacher probes of SLDF_REML (src/reml.cpp:597-605): mt19937(seed+54321), runif<float>(0,1),
+// V(i,j) = -1 if runif < 0.5 else 1, then each column divided by its norm.  V: n x nv col-major.
+void orc_rademacher(int seed, int64_t n, int nv, float* V) {
+  std::mt19937 rng(seed + 54321);
+  std::uniform_real_distribution<float> runif(0.0, 1.0);
+  for (int j = 0; j < nv; ++j)
+    for (int64_t i = 0; i < n; ++i) V[i + (size_t)j * n] = runif(rng) < 0.5 ? -1.0f : 1.0f;
+  for (int j = 0; j < nv; ++j) {
+    float ss = 0.f;
+    for (int64_t i = 0; i < n; ++i) ss += V[i + (size_t)j * n] * V[i + (size_t)j * n];
+    const float nr = std::sqrt(ss);
+    for (int64_t i = 0; i < n; ++i) V[i + (size_t)j * n] /= nr;
+  }
+}
+
+// MC samples of LMC_REML (src/reml.cpp:794-809): mt19937(seed+54321), normal<float>(0,1);
+// u (m x nmc col-major) drawn first, then e (n x nmc) from the same stream.
+void orc_lmc_samples(int seed, int64_t n, int64_t m, int nmc, float* u, float* e) {
+  std::mt19937 rng(seed + 54321);
+  std::normal_distribution<float> rnorm(0.0, 1.0);
+  for (int j = 0; j < nmc; ++j)
+    for (int64_t i = 0; i < m; ++i) u[i + (size_t)j * m] = rnorm(rng);
+  for (int j = 0; j < nmc; ++j)
+    for (int64_t i = 0; i < n; ++i) e[i + (size_t)j * n] = rnorm(rng);
+}
+
+// ------------------------------------------------------------------------ GEBV ------------
+// computeTotalGeneticValuePLINK bed loop (src/pred.cpp:104-130): tgv (FP64) over all fam
+// individuals, SNPs in bim order, z = low+high bit, z -> 2-z where bflip, tgv += z*alpha[j].
+int orc_gebv(const uint8_t* bed, int64_t bed_bytes, int64_t indi_num, int64_t snp_num, const uint8_t* bmarker,
+             const uint8_t* bflip, const double* alpha, double* tgv) {
+  int64_t last_true = -1;
+  for (int64_t j = snp_num - 1; j >= 0; --j)
+    if (bmarker[j]) {
+      last_true = j;
+      break;
+    }
+  if (last_true < 0) return -2;
+  std::vector<int> z(indi_num);
+  for (int64_t i = 0; i < indi_num; ++i) tgv[i] = 0.0;
+  int64_t pos = 3;
+  for (int64_t j = 0; j < snp_num; ++j) {
+    if (!bmarker[j]) {
+      pos += (indi_num + 3) / 4;
+      continue;
+    }
+    for (int64_t i = 0; i < indi_num;) {
+      if (pos >= bed_bytes) return -1;
+      const uint8_t ch = bed[pos++];
+      int k = 0;
+      while (k < 7 && i < indi_num) {
+        z[i] = ((ch >> k) & 1) + ((ch >> (k + 1)) & 1);  // b[k++] + b[k++] (:123), sequenced
+        k += 2;
+        ++i;
+      }
+    }
+    for (int64_t i = 0; i < indi_num; ++i) {
+      const int zi = bflip[j] ? 2 - z[i] : z[i];
+      tgv[i] += (double)zi * alpha[j];
+    }
+    if (j == last_true) break;
+  }
+  return 0;
+}
+
+}  // extern "C"

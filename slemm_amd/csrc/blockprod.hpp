@@ -1,0 +1,74 @@
+// Launch interface of the genotype block-product kernels (blockprod.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace slemm {
+
+enum { PREC_FP32 = SLEMM_PREC_FP32, PREC_BF16X2 = SLEMM_PREC_BF16X2 };
+enum { EPI_PASS1 = 0, EPI_PASS2 = 1 };
+
+struct BlockProdArgs {
+  const uint8_t* G = nullptr;  // packed rows
+  int64_t ldg = 0;             // bytes per row (= C/4)
+  int64_t nsuper = 0;          // C / 512
+  const void* X = nullptr;     // packed operand
+  int64_t lo_offset16 = 0;     // BF16X2: offset of the lo half in 16-byte units
+  int KP = 0;                  // padded column count (row stride of X)
+  int t = 0;                   // live columns
+  int64_t rows = 0;            // live rows
+  const double* colsum = nullptr;  // pass 1: s_k = sum_i X_ik ; pass 2: u_k = sum_j gm_j X_jk
+  // pass 1
+  const double* xsum = nullptr;    // column sums of the un-centred operand (null: not centred)
+  double xsum_n = 0.0;             // number of live reduction rows (n)
+  const int32_t* popcnt = nullptr; // per-row bit counts (rows padded with zeros)
+  const float* gm = nullptr;
+  const float* gw = nullptr;   // null: unweighted
+  void* out2 = nullptr;        // pass-2 operand (may be null)
+  int out2_mode = 0;
+  int64_t out2_lo_offset = 0;  // in bf16 elements
+  double* upart = nullptr;     // per-workgroup sum_j gm_j T_jk (may be null)
+  // both
+  float* out = nullptr;        // column-major result (pass 1: optional, pass 2: required)
+  int64_t ldo = 0;
+  // pass 2
+  const float* rr = nullptr;   // null: no scaling
+  const float* v = nullptr;    // null: no tau0 v term
+  int64_t ldv = 0;
+  float tau0 = 0.f;
+  const int* kinds = nullptr;  // per-column kinds (device); LDET columns get tau0 v here
+};
+
+struct PackArgs {
+  const float* V = nullptr;
+  int64_t ldv = 0;
+  int t = 0;
+  int KP = 0;
+  int64_t n = 0;
+  int64_t rows_pad = 0;        // multiple of 256
+  const float* rr = nullptr;
+  const float* Q = nullptr;
+  int64_t ldq = 0;
+  int c = 0;
+  const double* qtv = nullptr;  // c x KP (row q stride KP); null: no projection
+  const int* kinds = nullptr;
+  int mode = 0;
+  void* X = nullptr;            // null: only the column sums of x are produced
+  int64_t lo_offset16 = 0;
+  const double* xsum = nullptr; // if set, x_ik -= float(xsum_k / n) before packing
+  double* part = nullptr;       // (rows_pad/256) x KP partial column sums
+  const float* wsum = nullptr;  // optional row weights of the column sums (sum_i wsum_i x_ik)
+};
+
+int blockprod_nt(int t);
+int64_t blockprod_rows_per_wg(int NT);
+void launch_blockprod(const BlockProdArgs& a, int precision, int epi, int NT, int64_t rows_pad, hipStream_t st);
+void launch_pack(const PackArgs& a, hipStream_t st);
+void launch_sum_parts(const double* part, int64_t nparts, int KP, double* out, hipStream_t st);
+void tn_product(const float* A, int64_t lda, int c, const float* B, int64_t ldb, int t, int64_t n,
+                double* out, int ldo, DevBuf& scratch, hipStream_t st);
+void launch_quad_finish(float* out, int64_t ldo, const float* V, int64_t ldv, const float* Q, int64_t ldq,
+                        int c, const double* b, int KP, const int* kinds, int t, int64_t n, float tau0,
+                        hipStream_t st);
+
+}  // namespace slemm

@@ -1,0 +1,111 @@
+// Shared definitions of the gfx950 engine: status handling, handle layout, small helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/slemm_gpu.h"
+
+namespace slemm {
+
+// Reduction dimension (individuals for Z'V, SNPs for Z U) and row counts of both packed copies
+// are padded to this many elements: one "super-chunk" of the block-product kernels, 128 bytes
+// of 2-bit codes per row, and a whole number of 512-row workgroup tiles.
+constexpr int64_t kPad = 512;
+
+inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// Status plumbing: every C entry point runs its body through guard(), which turns C++
+// exceptions into a status code and a per-thread message.
+struct Error {
+  int code;
+  std::string msg;
+};
+void set_error(const std::string& msg);
+[[noreturn]] void fail(int code, const std::string& msg);
+void hip_check(hipError_t e, const char* what, const char* file, int line);
+#define HIPCHECK(x) ::slemm::hip_check((x), #x, __FILE__, __LINE__)
+#define REQUIRE(cond, msg) \
+  do { if (!(cond)) ::slemm::fail(SLEMM_ERR_ARG, (msg)); } while (0)
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return SLEMM_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return SLEMM_ERR_ALLOC;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return SLEMM_ERR_ARG;
+  }
+}
+
+// Device buffer owned by a handle; grows on demand, never shrinks.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t nbytes);
+  void release();
+  template <class T> T* as() const { return static_cast<T*>(p); }
+  ~DevBuf() { release(); }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+}  // namespace slemm
+
+// Device-resident genotype set.  Two packed copies of the same 2-bit codes:
+//  gs: SNP-major,        rows = SNPs (m_rows = round_up(m, kPad)),  n_pad/4 bytes per row;
+//  gi: individual-major, rows = individuals (n_rows = n_pad),        m_pad/4 bytes per row.
+// Individual i of a row sits in bits 2(i%4), 2(i%4)+1 of byte i/4 (PLINK order); padding
+// codes are 00 (dosage 0), so padded rows/columns contribute nothing to any product.
+struct slemm_geno {
+  int device = 0;
+  int64_t n = 0, m = 0;
+  int64_t n_pad = 0, m_pad = 0;
+  uint8_t* gs = nullptr;
+  uint8_t* gi = nullptr;
+  int64_t ld_gs = 0, ld_gi = 0;  // bytes per row
+
+  // model terms (device): gm, gw padded to m_pad with zeros; rr padded to n_pad with zeros.
+  int32_t* popcnt = nullptr;  // m
+  float* gm = nullptr;
+  float* gw = nullptr;
+  float* rr = nullptr;
+  float* Q = nullptr;  // n_pad x c, column-major, padding rows zero
+  int c = 0;
+  bool counts_ready = false;
+  bool model_ready = false;
+
+  // workspaces of the operator pipeline
+  slemm::DevBuf x1, x2;        // packed block operands of pass 1 / pass 2
+  slemm::DevBuf part1, part2;  // per-workgroup column partial sums
+  slemm::DevBuf colsum;        // reduced column sums (double)
+  slemm::DevBuf qtv;           // Q' V  (c x t, double)
+  slemm::DevBuf tmp;           // n_pad x t float
+  slemm::DevBuf kinds;         // per-column operator kinds
+  slemm::DevBuf redpart;       // partials of column reductions (double)
+  std::vector<int> kinds_host;
+};
+
+namespace slemm {
+
+// Operator pipeline pieces shared by op.hip and lanczos.hip.
+void ensure_counts(slemm_geno* h, hipStream_t st);
+void require_model(const slemm_geno* h);
+void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const float* V, int64_t ldv,
+            int t, float* out, int64_t ldo, int precision, hipStream_t st);
+
+// Column dot products in FP64: res[k] = sum_i A[i,k]*B[i,k] for k < t over n rows.
+void col_dots(slemm_geno* h, const float* A, int64_t lda, const float* B, int64_t ldb, int64_t n,
+              int t, double* res_dev, hipStream_t st);
+
+}  // namespace slemm

@@ -1,0 +1,498 @@
+// Packed genotype storage on the device: upload/compaction of PLINK bed rows, the
+// individual-major transposed copy, bit counts, synthetic beds and the model terms.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace slemm {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+void hip_check(hipError_t e, const char* what, const char* file, int line) {
+  if (e == hipSuccess) return;
+  std::string msg = std::string(what) + " failed: " + hipGetErrorString(e) + " (" + file + ":" +
+                    std::to_string(line) + ")";
+  fail(e == hipErrorOutOfMemory ? SLEMM_ERR_ALLOC : SLEMM_ERR_HIP, msg);
+}
+
+void DevBuf::reserve(size_t nbytes) {
+  if (nbytes <= bytes) return;
+  release();
+  hipError_t e = hipMalloc(&p, nbytes);
+  if (e != hipSuccess) {
+    p = nullptr;
+    bytes = 0;
+    hip_check(e, "hipMalloc(workspace)", __FILE__, __LINE__);
+  }
+  bytes = nbytes;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+// ------------------------------------------------------------------------------ kernels ----
+
+// One thread per output byte: gathers the 4 kept individuals of that byte from a raw bed row.
+// keep == nullptr means "all individuals"; bits beyond n are forced to code 00.
+__global__ void k_compact(const uint8_t* __restrict__ raw, int64_t raw_ld, int rows,
+                          const int32_t* __restrict__ keep, int64_t n, uint8_t* __restrict__ gs,
+                          int64_t ld_gs) {
+  const int64_t ob = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (r >= rows || ob >= ld_gs) return;
+  const uint8_t* src = raw + (int64_t)r * raw_ld;
+  uint32_t out = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t kk = ob * 4 + q;
+    if (kk < n) {
+      const int64_t i = keep ? (int64_t)keep[kk] : kk;
+      const uint32_t code = (src[i >> 2] >> (2 * (i & 3))) & 3u;
+      out |= code << (2 * q);
+    }
+  }
+  gs[(int64_t)r * ld_gs + ob] = (uint8_t)out;
+}
+
+// gs (SNP-major) -> gi (individual-major).  Tile: 128 SNPs x 256 individuals; 256 threads,
+// thread t writes the 32 bytes (128 SNPs) of individual i0+t.
+__global__ __launch_bounds__(256) void k_transpose(const uint8_t* __restrict__ gs, int64_t ld_gs,
+                                                   uint8_t* __restrict__ gi, int64_t ld_gi) {
+  __shared__ uint32_t tile[128][64 / 4 + 1];  // 128 SNP rows x 64 bytes (+pad against conflicts)
+  const int64_t i0 = (int64_t)blockIdx.x * 256;
+  const int64_t j0 = (int64_t)blockIdx.y * 128;
+  const int t = threadIdx.x;
+  // cooperative load: 128 rows x 16 dwords = 2048 dwords, 8 per thread
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int idx = q * 256 + t;
+    const int row = idx >> 4, col = idx & 15;
+    tile[row][col] = *reinterpret_cast<const uint32_t*>(gs + (j0 + row) * ld_gs + i0 / 4 + col * 4);
+  }
+  __syncthreads();
+  const int dw = t >> 4;          // dword holding individual t within a row
+  const int sh = 2 * (t & 15);    // bit offset of its code in that dword
+  uint32_t outw[8];
+#pragma unroll
+  for (int w = 0; w < 8; ++w) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const uint32_t code = (tile[w * 16 + q][dw] >> sh) & 3u;
+      acc |= code << (2 * q);
+    }
+    outw[w] = acc;
+  }
+  uint4* dst = reinterpret_cast<uint4*>(gi + (i0 + t) * ld_gi + j0 / 4);
+  dst[0] = make_uint4(outw[0], outw[1], outw[2], outw[3]);
+  dst[1] = make_uint4(outw[4], outw[5], outw[6], outw[7]);
+}
+
+// One wave per SNP row: popcount, count of 11 codes, count of nonzero codes; gm = pc/n.
+__global__ __launch_bounds__(256) void k_counts(const uint8_t* __restrict__ gs, int64_t ld_gs,
+                                                int64_t m, int64_t n, int32_t* __restrict__ pc,
+                                                int32_t* __restrict__ c2, int32_t* __restrict__ cany,
+                                                float* __restrict__ gm) {
+  const int lane = threadIdx.x & 63;
+  const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= m) return;
+  const uint4* row = reinterpret_cast<const uint4*>(gs + j * ld_gs);
+  const int64_t nv = ld_gs / 16;
+  int s_pc = 0, s_2 = 0, s_any = 0;
+  for (int64_t v = lane; v < nv; v += 64) {
+    const uint4 w4 = row[v];
+    const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t lo = w[q] & 0x55555555u, hi = (w[q] >> 1) & 0x55555555u;
+      s_pc += __popc(w[q]);
+      s_2 += __popc(lo & hi);
+      s_any += __popc(lo | hi);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s_pc += __shfl_xor(s_pc, off);
+    s_2 += __shfl_xor(s_2, off);
+    s_any += __shfl_xor(s_any, off);
+  }
+  if (lane == 0) {
+    pc[j] = s_pc;
+    c2[j] = s_2;
+    cany[j] = s_any;
+    gm[j] = __fdiv_rn((float)s_pc, (float)n);  // src/reml.cpp:620, IEEE round-to-nearest
+  }
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Synthetic bed: thread per output byte of gs.  p_j ~ U(lo,hi); code 01 with prob `missing`,
+// else A2 count ~ Bin(2,p_j) coded 00/10/11.
+__global__ void k_synth(uint8_t* __restrict__ gs, int64_t ld_gs, int64_t n, int64_t m,
+                        uint64_t seed, float lo, float hi, float missing) {
+  const int64_t ob = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (j >= m || ob >= ld_gs) return;
+  const uint64_t hj = mix64(seed ^ mix64((uint64_t)j + 0x1234567ull));
+  const float p = lo + (hi - lo) * (float)((hj >> 40) * (1.0 / 16777216.0));
+  const uint32_t pthr = (uint32_t)(p * 2097152.0f);            // 21-bit threshold
+  const uint32_t mthr = (uint32_t)(missing * 2097152.0f);
+  uint32_t out = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t i = ob * 4 + q;
+    if (i < n) {
+      const uint64_t r = mix64(hj ^ ((uint64_t)i * 0xD1B54A32D192ED03ull));
+      const uint32_t u1 = (uint32_t)(r & 0x1FFFFF), u2 = (uint32_t)((r >> 21) & 0x1FFFFF),
+                     u3 = (uint32_t)((r >> 42) & 0x1FFFFF);
+      uint32_t code;
+      if (u3 < mthr) {
+        code = 1u;  // missing
+      } else {
+        const uint32_t d = (u1 < pthr) + (u2 < pthr);
+        code = d == 0 ? 0u : (d == 1 ? 2u : 3u);
+      }
+      out |= code << (2 * q);
+    }
+  }
+  gs[j * ld_gs + ob] = (uint8_t)out;
+}
+
+// Gathers whole SNP-major rows (subset / reorder on the device).
+__global__ void k_gather_rows(const uint8_t* __restrict__ src, int64_t ld, const int64_t* __restrict__ idx,
+                              int64_t rows, uint8_t* __restrict__ dst) {
+  const int64_t r = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (r >= rows) return;
+  const uint4* s = reinterpret_cast<const uint4*>(src + idx[r] * ld);
+  uint4* d = reinterpret_cast<uint4*>(dst + r * ld);
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < ld / 16;
+       v += (int64_t)gridDim.x * blockDim.x)
+    d[v] = s[v];
+}
+
+// ------------------------------------------------------------------------------ helpers ----
+
+static slemm_geno* alloc_geno(int64_t n, int64_t m) {
+  auto* h = new slemm_geno();
+  HIPCHECK(hipGetDevice(&h->device));
+  h->n = n;
+  h->m = m;
+  h->n_pad = round_up(n, kPad);
+  h->m_pad = round_up(m, kPad);
+  h->ld_gs = h->n_pad / 4;
+  h->ld_gi = h->m_pad / 4;
+  try {
+    HIPCHECK(hipMalloc(&h->gs, (size_t)h->m_pad * h->ld_gs));
+    HIPCHECK(hipMalloc(&h->gi, (size_t)h->n_pad * h->ld_gi));
+    HIPCHECK(hipMemset(h->gs, 0, (size_t)h->m_pad * h->ld_gs));
+    HIPCHECK(hipMalloc(&h->popcnt, sizeof(int32_t) * h->m_pad));
+    HIPCHECK(hipMalloc(&h->gm, sizeof(float) * h->m_pad));
+    HIPCHECK(hipMalloc(&h->gw, sizeof(float) * h->m_pad));
+    HIPCHECK(hipMalloc(&h->rr, sizeof(float) * h->n_pad));
+    HIPCHECK(hipMemset(h->popcnt, 0, sizeof(int32_t) * h->m_pad));
+    HIPCHECK(hipMemset(h->gm, 0, sizeof(float) * h->m_pad));
+    HIPCHECK(hipMemset(h->gw, 0, sizeof(float) * h->m_pad));
+    HIPCHECK(hipMemset(h->rr, 0, sizeof(float) * h->n_pad));
+  } catch (...) {
+    slemm_gpu_geno_destroy(h);
+    throw;
+  }
+  return h;
+}
+
+static void build_transpose(slemm_geno* h, hipStream_t st) {
+  dim3 grid((unsigned)(h->n_pad / 256), (unsigned)(h->m_pad / 128));
+  k_transpose<<<grid, 256, 0, st>>>(h->gs, h->ld_gs, h->gi, h->ld_gi);
+  HIPCHECK(hipGetLastError());
+}
+
+void ensure_counts(slemm_geno* h, hipStream_t st) {
+  if (h->counts_ready) return;
+  DevBuf c2, cany;
+  c2.reserve(sizeof(int32_t) * h->m);
+  cany.reserve(sizeof(int32_t) * h->m);
+  k_counts<<<(unsigned)((h->m + 3) / 4), 256, 0, st>>>(h->gs, h->ld_gs, h->m, h->n, h->popcnt,
+                                                      c2.as<int32_t>(), cany.as<int32_t>(), h->gm);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(st));
+  h->counts_ready = true;
+}
+
+void require_model(const slemm_geno* h) {
+  if (!h->model_ready) fail(SLEMM_ERR_STATE, "model terms not set: call slemm_gpu_set_model first");
+}
+
+}  // namespace slemm
+
+using namespace slemm;
+
+// ------------------------------------------------------------------------------ C ABI ------
+
+extern "C" const char* slemm_gpu_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int slemm_gpu_version(void) { return 1; }
+
+extern "C" int slemm_gpu_device_count(int* count) {
+  return guard([&] {
+    REQUIRE(count, "count is null");
+    HIPCHECK(hipGetDeviceCount(count));
+  });
+}
+
+extern "C" int slemm_gpu_geno_create(const uint8_t* bed_rows, int64_t n_raw, int64_t m_raw,
+                                     int64_t row_bytes, const int64_t* keep_idx, int64_t n_keep,
+                                     const int64_t* snp_idx, int64_t m, slemm_geno** out) {
+  return guard([&] {
+    REQUIRE(out, "out is null");
+    *out = nullptr;
+    REQUIRE(bed_rows, "bed_rows is null");
+    REQUIRE(n_raw > 0 && m_raw > 0, "empty genotype matrix");
+    REQUIRE(row_bytes >= (n_raw + 3) / 4, "row_bytes < ceil(n_raw/4)");
+    REQUIRE(n_raw < (int64_t)1 << 31, "n_raw too large");
+    int64_t n = keep_idx ? n_keep : n_raw;
+    REQUIRE(n > 0, "individual set retained for analysis is empty");
+    if (!snp_idx) m = m_raw;
+    REQUIRE(m > 0, "marker set retained for analysis is empty");
+    std::vector<int32_t> keep32;
+    if (keep_idx) {
+      keep32.resize(n);
+      for (int64_t k = 0; k < n; ++k) {
+        REQUIRE(keep_idx[k] >= 0 && keep_idx[k] < n_raw, "keep_idx out of range");
+        REQUIRE(k == 0 || keep_idx[k] > keep_idx[k - 1], "keep_idx must be strictly increasing");
+        keep32[k] = (int32_t)keep_idx[k];
+      }
+    }
+    if (snp_idx)
+      for (int64_t j = 0; j < m; ++j)
+        REQUIRE(snp_idx[j] >= 0 && snp_idx[j] < m_raw, "snp_idx out of range");
+
+    slemm_geno* h = alloc_geno(n, m);
+    try {
+      DevBuf dkeep, stage;
+      if (keep_idx) {
+        dkeep.reserve(sizeof(int32_t) * n);
+        HIPCHECK(hipMemcpy(dkeep.p, keep32.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+      }
+      // stream rows through a staging buffer of at most ~256 MB
+      const int64_t chunk =
+          std::max<int64_t>(1, std::min<int64_t>({m, (256ll << 20) / row_bytes, (int64_t)65535}));
+      stage.reserve((size_t)chunk * row_bytes);
+      std::vector<uint8_t> hbuf;
+      if (snp_idx) hbuf.resize((size_t)chunk * row_bytes);
+      for (int64_t j0 = 0; j0 < m; j0 += chunk) {
+        const int64_t rows = std::min(chunk, m - j0);
+        const uint8_t* src;
+        if (snp_idx) {
+          for (int64_t r = 0; r < rows; ++r)
+            std::memcpy(hbuf.data() + r * row_bytes, bed_rows + snp_idx[j0 + r] * row_bytes, row_bytes);
+          src = hbuf.data();
+        } else {
+          src = bed_rows + j0 * row_bytes;
+        }
+        HIPCHECK(hipMemcpy(stage.p, src, (size_t)rows * row_bytes, hipMemcpyHostToDevice));
+        dim3 grid((unsigned)((h->ld_gs + 255) / 256), (unsigned)rows);
+        k_compact<<<grid, 256>>>(stage.as<uint8_t>(), row_bytes, (int)rows, dkeep.as<int32_t>(), n,
+                                 h->gs + j0 * h->ld_gs, h->ld_gs);
+        HIPCHECK(hipGetLastError());
+      }
+      build_transpose(h, nullptr);
+      HIPCHECK(hipDeviceSynchronize());
+    } catch (...) {
+      slemm_gpu_geno_destroy(h);
+      throw;
+    }
+    *out = h;
+  });
+}
+
+extern "C" int slemm_gpu_geno_subset(const slemm_geno* hs, const int64_t* snp_idx, int64_t m,
+                                     slemm_geno** out) {
+  return guard([&] {
+    REQUIRE(hs && snp_idx && out, "null argument");
+    *out = nullptr;
+    REQUIRE(m > 0, "marker set retained for analysis is empty");
+    for (int64_t j = 0; j < m; ++j) REQUIRE(snp_idx[j] >= 0 && snp_idx[j] < hs->m, "snp_idx out of range");
+    HIPCHECK(hipSetDevice(hs->device));
+    slemm_geno* h = alloc_geno(hs->n, m);
+    try {
+      DevBuf didx;
+      didx.reserve(sizeof(int64_t) * m);
+      HIPCHECK(hipMemcpy(didx.p, snp_idx, sizeof(int64_t) * m, hipMemcpyHostToDevice));
+      dim3 grid(8, (unsigned)std::min<int64_t>(m, 65535), (unsigned)((m + 65534) / 65535));
+      k_gather_rows<<<grid, 256>>>(hs->gs, hs->ld_gs, didx.as<int64_t>(), m, h->gs);
+      HIPCHECK(hipGetLastError());
+      build_transpose(h, nullptr);
+      HIPCHECK(hipDeviceSynchronize());
+    } catch (...) {
+      slemm_gpu_geno_destroy(h);
+      throw;
+    }
+    *out = h;
+  });
+}
+
+extern "C" int slemm_gpu_geno_synthesize(int64_t n, int64_t m, uint64_t seed, float maf_lo,
+                                         float maf_hi, float missing, slemm_geno** out) {
+  return guard([&] {
+    REQUIRE(out, "out is null");
+    *out = nullptr;
+    REQUIRE(n > 0 && m > 0, "empty genotype matrix");
+    REQUIRE(n < (int64_t)1 << 31, "n too large");
+    REQUIRE(maf_lo >= 0 && maf_hi <= 1 && maf_lo <= maf_hi, "bad allele-frequency range");
+    REQUIRE(missing >= 0 && missing < 1, "bad missing rate");
+    slemm_geno* h = alloc_geno(n, m);
+    try {
+      dim3 grid((unsigned)((h->ld_gs + 255) / 256), (unsigned)std::min<int64_t>(m, 65535),
+                (unsigned)((m + 65534) / 65535));
+      k_synth<<<grid, 256>>>(h->gs, h->ld_gs, n, m, seed, maf_lo, maf_hi, missing);
+      HIPCHECK(hipGetLastError());
+      build_transpose(h, nullptr);
+      HIPCHECK(hipDeviceSynchronize());
+    } catch (...) {
+      slemm_gpu_geno_destroy(h);
+      throw;
+    }
+    *out = h;
+  });
+}
+
+extern "C" int slemm_gpu_geno_destroy(slemm_geno* h) {
+  return guard([&] {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    (void)hipDeviceSynchronize();
+    if (h->gs) (void)hipFree(h->gs);
+    if (h->gi) (void)hipFree(h->gi);
+    if (h->popcnt) (void)hipFree(h->popcnt);
+    if (h->gm) (void)hipFree(h->gm);
+    if (h->gw) (void)hipFree(h->gw);
+    if (h->rr) (void)hipFree(h->rr);
+    if (h->Q) (void)hipFree(h->Q);
+    delete h;
+  });
+}
+
+extern "C" int slemm_gpu_geno_dims(const slemm_geno* h, int64_t* n, int64_t* m) {
+  return guard([&] {
+    REQUIRE(h, "null handle");
+    if (n) *n = h->n;
+    if (m) *m = h->m;
+  });
+}
+
+extern "C" int slemm_gpu_geno_download(const slemm_geno* h, int64_t j0, int64_t cnt, uint8_t* rows) {
+  return guard([&] {
+    REQUIRE(h && rows, "null argument");
+    REQUIRE(j0 >= 0 && cnt >= 0 && j0 + cnt <= h->m, "row range out of bounds");
+    HIPCHECK(hipSetDevice(h->device));
+    const int64_t rb = (h->n + 3) / 4;
+    HIPCHECK(hipMemcpy2D(rows, rb, h->gs + j0 * h->ld_gs, h->ld_gs, rb, cnt, hipMemcpyDeviceToHost));
+  });
+}
+
+extern "C" int slemm_gpu_geno_counts(slemm_geno* h, int32_t* popcnt, int32_t* n0, int32_t* n1,
+                                     int32_t* n2, float* gm) {
+  return guard([&] {
+    REQUIRE(h, "null handle");
+    HIPCHECK(hipSetDevice(h->device));
+    DevBuf c2, cany;
+    c2.reserve(sizeof(int32_t) * h->m);
+    cany.reserve(sizeof(int32_t) * h->m);
+    k_counts<<<(unsigned)((h->m + 3) / 4), 256>>>(h->gs, h->ld_gs, h->m, h->n, h->popcnt,
+                                                  c2.as<int32_t>(), cany.as<int32_t>(), h->gm);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipDeviceSynchronize());
+    h->counts_ready = true;
+    std::vector<int32_t> v2(h->m), vany(h->m);
+    HIPCHECK(hipMemcpy(v2.data(), c2.p, sizeof(int32_t) * h->m, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(vany.data(), cany.p, sizeof(int32_t) * h->m, hipMemcpyDeviceToHost));
+    if (popcnt) HIPCHECK(hipMemcpy(popcnt, h->popcnt, sizeof(int32_t) * h->m, hipMemcpyDeviceToHost));
+    if (gm) HIPCHECK(hipMemcpy(gm, h->gm, sizeof(float) * h->m, hipMemcpyDeviceToHost));
+    for (int64_t j = 0; j < h->m; ++j) {
+      if (n2) n2[j] = v2[j];
+      if (n1) n1[j] = vany[j] - v2[j];
+      if (n0) n0[j] = (int32_t)(h->n - vany[j]);
+    }
+  });
+}
+
+extern "C" int slemm_gpu_set_model(slemm_geno* h, const float* persnp, const float* rvec,
+                                   const float* Q, int64_t ldq, int c) {
+  return guard([&] {
+    REQUIRE(h, "null handle");
+    REQUIRE(c >= 0 && c <= 64, "number of covariates out of [0, 64]");
+    REQUIRE(c == 0 || (Q && ldq >= h->n), "Q is null or ldq < n");
+    HIPCHECK(hipSetDevice(h->device));
+    ensure_counts(h, nullptr);
+    const int64_t n = h->n, m = h->m;
+    std::vector<float> gm(m), gw(m), rr(n);
+    HIPCHECK(hipMemcpy(gm.data(), h->gm, sizeof(float) * m, hipMemcpyDeviceToHost));
+    // mean of the per-SNP weights (src/reml.cpp:622 persnp.mean()); exact for unit weights
+    float mean = 1.0f;
+    if (persnp) {
+      double s = 0;
+      for (int64_t j = 0; j < m; ++j) s += persnp[j];
+      mean = (float)(s / (double)m);
+      REQUIRE(mean > 0, "per-SNP weights must have a positive mean");
+    }
+    for (int64_t j = 0; j < m; ++j) {
+      const float g = gm[j];
+      const float v = (g * (2.0f - g)) / 2.0f;           // src/reml.cpp:622, float arithmetic
+      const float wj = persnp ? persnp[j] / mean : 1.0f;
+      gw[j] = ((1.0f / v) / (float)m) * wj;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+      if (rvec) {
+        REQUIRE(rvec[i] > 0, "error-variance weights must be positive");
+        rr[i] = 1.0f / std::sqrt(rvec[i]);                // src/reml.cpp:574
+      } else {
+        rr[i] = 1.0f;
+      }
+    }
+    for (int64_t j = 0; j < m; ++j)
+      REQUIRE(std::isfinite(gw[j]), "monomorphic SNP (allele frequency 0 or 1) gives an infinite weight");
+    HIPCHECK(hipMemcpy(h->gw, gw.data(), sizeof(float) * m, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(h->rr, rr.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    if (h->Q) {
+      (void)hipFree(h->Q);
+      h->Q = nullptr;
+    }
+    h->c = c;
+    if (c > 0) {
+      HIPCHECK(hipMalloc(&h->Q, sizeof(float) * h->n_pad * c));
+      HIPCHECK(hipMemset(h->Q, 0, sizeof(float) * h->n_pad * c));
+      HIPCHECK(hipMemcpy2D(h->Q, sizeof(float) * h->n_pad, Q, sizeof(float) * ldq, sizeof(float) * n, c,
+                           hipMemcpyHostToDevice));
+    }
+    h->model_ready = true;
+  });
+}
+
+extern "C" int slemm_gpu_get_model(const slemm_geno* h, float* gm, float* gw, float* r_rsqrt) {
+  return guard([&] {
+    REQUIRE(h, "null handle");
+    require_model(h);
+    HIPCHECK(hipSetDevice(h->device));
+    if (gm) HIPCHECK(hipMemcpy(gm, h->gm, sizeof(float) * h->m, hipMemcpyDeviceToHost));
+    if (gw) HIPCHECK(hipMemcpy(gw, h->gw, sizeof(float) * h->m, hipMemcpyDeviceToHost));
+    if (r_rsqrt) HIPCHECK(hipMemcpy(r_rsqrt, h->rr, sizeof(float) * h->n, hipMemcpyDeviceToHost));
+  });
+}

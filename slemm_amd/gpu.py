@@ -1,0 +1,280 @@
+"""Python mirror of the reference's operator and Krylov interfaces over the gfx950 engine.
+
+Names, argument meaning and error behaviour follow src/reml.cpp: ``lnr_op_select`` (:96-118),
+``rmatvec`` (:159-195), ``weighted_rmatvec`` (:120-157), ``matvec`` (:197-233),
+``L_Seed`` (:246-339), ``L_Solve`` (:369-431), ``SLQ_LDet`` (:440-451), and the GEBV loop of
+src/pred.cpp:112-130.  Matrices are torch float32 tensors of shape (rows, cols) on the GPU;
+internally they are handed to the C ABI column-major (a contiguous (cols, rows) buffer), the
+layout of the reference's Eigen ``MatrixXf``.  PyTorch is only the device-memory carrier.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+PRECISIONS = {"fp32": L.PREC_FP32, "bf16x2": L.PREC_BF16X2}
+OPS = {"ldet": L.OP_LDET, "quadform": L.OP_QUADFORM}
+
+
+def _prec(p) -> int:
+    if isinstance(p, str):
+        if p not in PRECISIONS:
+            raise ValueError(f"unknown precision {p!r}")
+        return PRECISIONS[p]
+    return int(p)
+
+
+def _colmajor(x: torch.Tensor, rows: int) -> torch.Tensor:
+    """(rows, t) or (rows,) float32 CUDA tensor -> contiguous (t, rows) buffer."""
+    if x.dim() == 1:
+        x = x[:, None]
+    if x.shape[0] != rows:
+        raise ValueError(f"expected {rows} rows, got {tuple(x.shape)}")
+    if not x.is_cuda or x.dtype != torch.float32:
+        raise TypeError("expected a float32 CUDA tensor")
+    return x.t().contiguous()
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _np_ptr(a: Optional[np.ndarray], ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _stream() -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Geno:
+    """Device-resident packed genotypes (both layouts) plus the operator's model terms."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+        lib = L.load()
+        n, m = C.c_int64(), C.c_int64()
+        L.check(lib.slemm_gpu_geno_dims(self._h, C.byref(n), C.byref(m)))
+        self.n, self.m = n.value, m.value
+
+    # ------------------------------------------------------------------ construction
+    @classmethod
+    def from_bed(cls, bed_rows: np.ndarray, n_raw: int, keep_idx: Optional[Sequence[int]] = None,
+                 snp_idx: Optional[Sequence[int]] = None) -> "Geno":
+        """bed_rows: uint8 array (m_raw, ceil(n_raw/4)) -- the .bed body after the magic bytes."""
+        lib = L.load()
+        rows = np.ascontiguousarray(bed_rows, dtype=np.uint8)
+        if rows.ndim != 2:
+            raise ValueError("bed_rows must be 2-D (SNPs x bytes)")
+        keep = None if keep_idx is None else np.ascontiguousarray(keep_idx, dtype=np.int64)
+        snps = None if snp_idx is None else np.ascontiguousarray(snp_idx, dtype=np.int64)
+        h = C.c_void_p()
+        L.check(lib.slemm_gpu_geno_create(
+            _np_ptr(rows, C.c_uint8), n_raw, rows.shape[0], rows.shape[1],
+            _np_ptr(keep, C.c_int64), 0 if keep is None else len(keep),
+            _np_ptr(snps, C.c_int64), 0 if snps is None else len(snps), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def synthesize(cls, n: int, m: int, seed: int = 1, maf_lo: float = 0.05, maf_hi: float = 0.5,
+                   missing: float = 0.0) -> "Geno":
+        lib = L.load()
+        h = C.c_void_p()
+        L.check(lib.slemm_gpu_geno_synthesize(n, m, seed, maf_lo, maf_hi, missing, C.byref(h)))
+        return cls(h)
+
+    def subset(self, snp_idx: Sequence[int]) -> "Geno":
+        lib = L.load()
+        idx = np.ascontiguousarray(snp_idx, dtype=np.int64)
+        h = C.c_void_p()
+        L.check(lib.slemm_gpu_geno_subset(self._h, _np_ptr(idx, C.c_int64), len(idx), C.byref(h)))
+        return Geno(h)
+
+    def close(self) -> None:
+        if self._h:
+            L.check(L.load().slemm_gpu_geno_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    # ------------------------------------------------------------------ inspection
+    def download(self, j0: int = 0, cnt: Optional[int] = None) -> np.ndarray:
+        cnt = self.m - j0 if cnt is None else cnt
+        out = np.zeros((cnt, (self.n + 3) // 4), dtype=np.uint8)
+        L.check(L.load().slemm_gpu_geno_download(self._h, j0, cnt, _np_ptr(out, C.c_uint8)))
+        return out
+
+    def counts(self):
+        """(popcnt, n0, n1, n2, gm) host arrays -- src/input.cpp:12-21, src/reml.cpp:612-621."""
+        m = self.m
+        pc, n0, n1, n2 = (np.zeros(m, np.int32) for _ in range(4))
+        gm = np.zeros(m, np.float32)
+        L.check(L.load().slemm_gpu_geno_counts(self._h, _np_ptr(pc, C.c_int32), _np_ptr(n0, C.c_int32),
+                                               _np_ptr(n1, C.c_int32), _np_ptr(n2, C.c_int32),
+                                               _np_ptr(gm, C.c_float)))
+        return pc, n0, n1, n2, gm
+
+    def set_model(self, persnp: Optional[np.ndarray] = None, rvec: Optional[np.ndarray] = None,
+                  Q: Optional[np.ndarray] = None) -> None:
+        ps = None if persnp is None else np.ascontiguousarray(persnp, dtype=np.float32)
+        rv = None if rvec is None else np.ascontiguousarray(rvec, dtype=np.float32)
+        c = 0
+        q = None
+        if Q is not None:
+            Qa = np.asarray(Q, dtype=np.float32)
+            if Qa.ndim == 1:
+                Qa = Qa[:, None]
+            c = Qa.shape[1]
+            q = np.asfortranarray(Qa)
+        L.check(L.load().slemm_gpu_set_model(self._h, _np_ptr(ps, C.c_float), _np_ptr(rv, C.c_float),
+                                             _np_ptr(q, C.c_float), self.n, c))
+
+    def model(self):
+        gm = np.zeros(self.m, np.float32)
+        gw = np.zeros(self.m, np.float32)
+        rr = np.zeros(self.n, np.float32)
+        L.check(L.load().slemm_gpu_get_model(self._h, _np_ptr(gm, C.c_float), _np_ptr(gw, C.c_float),
+                                             _np_ptr(rr, C.c_float)))
+        return gm, gw, rr
+
+    # ------------------------------------------------------------------ operators
+    def lnr_op_select(self, lnr, tau0: float, v: torch.Tensor, precision="bf16x2",
+                      col_kind: Optional[Sequence] = None) -> torch.Tensor:
+        """src/reml.cpp:96-118 for all columns of v (n, t) in one batch; returns (n, t)."""
+        if isinstance(lnr, str):
+            if lnr not in OPS:
+                raise ValueError("Unknown linear operator in lnr_op_select()")
+            kind = OPS[lnr]
+        else:
+            kind = int(lnr)
+        vc = _colmajor(v, self.n)
+        t = vc.shape[0]
+        out = torch.empty_like(vc)
+        ck = None
+        if col_kind is not None:
+            ck_arr = (C.c_int * t)(*[OPS[x] if isinstance(x, str) else int(x) for x in col_kind])
+            ck = C.cast(ck_arr, C.POINTER(C.c_int))
+        L.check(L.load().slemm_gpu_op(self._h, kind, ck, tau0, _ptr(vc), self.n, t, _ptr(out), self.n,
+                                      _prec(precision), _stream()))
+        return out.t()
+
+    def rmatvec(self, v: torch.Tensor, precision="bf16x2", weighted: bool = False,
+                scale_rr: bool = False) -> torch.Tensor:
+        """(Z-m)'v (src/reml.cpp:159-195); W(Z-m)'v when weighted (:120-157).  (n,t) -> (m,t)."""
+        vc = _colmajor(v, self.n)
+        t = vc.shape[0]
+        out = torch.empty((t, self.m), dtype=torch.float32, device=vc.device)
+        L.check(L.load().slemm_gpu_zt(self._h, _ptr(vc), self.n, t, _ptr(out), self.m, int(weighted),
+                                      int(scale_rr), _prec(precision), _stream()))
+        return out.t()
+
+    def weighted_rmatvec(self, v: torch.Tensor, precision="bf16x2") -> torch.Tensor:
+        return self.rmatvec(v, precision, weighted=True)
+
+    def matvec(self, u: torch.Tensor, precision="bf16x2") -> torch.Tensor:
+        """(Z-m)u (src/reml.cpp:197-233).  (m,t) -> (n,t)."""
+        uc = _colmajor(u, self.m)
+        t = uc.shape[0]
+        out = torch.empty((t, self.n), dtype=torch.float32, device=uc.device)
+        L.check(L.load().slemm_gpu_z(self._h, _ptr(uc), self.m, t, _ptr(out), self.n, _prec(precision),
+                                     _stream()))
+        return out.t()
+
+    def gebv(self, alpha: torch.Tensor, flip: Optional[np.ndarray] = None) -> torch.Tensor:
+        """sum_j alpha_j z_ij over raw dosages, z -> 2-z where flip (src/pred.cpp:112-130)."""
+        if alpha.dtype != torch.float64 or not alpha.is_cuda or alpha.numel() != self.m:
+            raise TypeError("alpha must be a float64 CUDA tensor of length m")
+        a = alpha.contiguous()
+        out = torch.empty(self.n, dtype=torch.float64, device=a.device)
+        fl = None if flip is None else np.ascontiguousarray(flip, dtype=np.uint8)
+        L.check(L.load().slemm_gpu_gebv(self._h, _ptr(a), _np_ptr(fl, C.c_uint8), _ptr(out), _stream()))
+        return out
+
+    # ------------------------------------------------------------------ Krylov engine
+    def L_Seed(self, lnr, tau0: float, B: torch.Tensor, tol: float = 5e-4, maxit: int = 600,
+               keep_cols: Optional[int] = None, precision="bf16x2", col_kind=None, col_group=None,
+               verbose: bool = False) -> "LSeed":
+        kind = OPS[lnr] if isinstance(lnr, str) else int(lnr)
+        Bc = _colmajor(B, self.n)
+        t = Bc.shape[0]
+        keep = t if keep_cols is None else keep_cols
+        ck = cg = None
+        if col_kind is not None:
+            ck = C.cast((C.c_int * t)(*[OPS[x] if isinstance(x, str) else int(x) for x in col_kind]),
+                        C.POINTER(C.c_int))
+        if col_group is not None:
+            cg = C.cast((C.c_int * t)(*[int(x) for x in col_group]), C.POINTER(C.c_int))
+        h = C.c_void_p()
+        L.check(L.load().slemm_gpu_lseed(self._h, kind, ck, cg, tau0, _ptr(Bc), self.n, t, tol, maxit, keep,
+                                         _prec(precision), int(verbose), _stream(), C.byref(h)))
+        return LSeed(h, self.n, t)
+
+
+class LSeed:
+    """struct lseed (src/reml.cpp:240-244) kept on the device."""
+
+    def __init__(self, handle, n: int, t: int):
+        self._h = handle
+        self.n, self.t = n, t
+        lib = L.load()
+        tt, steps, conv = C.c_int(), C.c_int(), C.c_int()
+        L.check(lib.slemm_gpu_lseed_info(self._h, C.byref(tt), C.byref(steps), C.byref(conv)))
+        self.steps, self.converged = steps.value, bool(conv.value)
+        cs = np.zeros(t, dtype=np.int32)
+        L.check(lib.slemm_gpu_lseed_col_steps(self._h, cs.ctypes.data_as(C.POINTER(C.c_int))))
+        self.col_steps = cs
+
+    def coeffs(self):
+        """beta (t, steps+1), delta (t, steps) as in struct lseed."""
+        J = self.steps
+        beta = np.zeros((J + 1, self.t), dtype=np.float64)
+        delta = np.zeros((max(J, 1), self.t), dtype=np.float64)
+        L.check(L.load().slemm_gpu_lseed_coeffs(self._h, _np_ptr(beta, C.c_double), _np_ptr(delta, C.c_double)))
+        return beta.T.copy(), delta[:J].T.copy()
+
+    def L_Solve(self, B: torch.Tensor, sigma: float, tol: float = 1e-6, col0: int = 0) -> torch.Tensor:
+        Bc = _colmajor(B, self.n)
+        t2 = Bc.shape[0]
+        X = torch.empty_like(Bc)
+        L.check(L.load().slemm_gpu_lsolve(self._h, col0, t2, _ptr(Bc), self.n, sigma, tol, _ptr(X), self.n,
+                                          _stream()))
+        return X.t()
+
+    def spectrum(self, k: int):
+        J = int(self.col_steps[k])
+        D = np.zeros(J)
+        W = np.zeros(J)
+        L.check(L.load().slemm_gpu_lseed_spectrum(self._h, k, _np_ptr(D, C.c_double), _np_ptr(W, C.c_double)))
+        return D, W
+
+    def SLQ_LDet(self, sigma: float, col0: int = 0, nv: Optional[int] = None) -> float:
+        nv = self.t - col0 if nv is None else nv
+        out = C.c_double()
+        L.check(L.load().slemm_gpu_slq_ldet(self._h, col0, nv, sigma, C.byref(out)))
+        return out.value
+
+    def close(self):
+        if self._h:
+            L.check(L.load().slemm_gpu_lseed_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

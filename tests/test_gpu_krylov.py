@@ -1,0 +1,75 @@
+"""GPU Krylov engine (L_Seed / L_Solve / SLQ) against exact dense FP64 results and the FP32
+CPU restatement of src/reml.cpp:246-451."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from tests.problems import make_problem, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def prob(gpu):
+    p = make_problem(n=1200, m=1500, seed=23, weights=True, c=2)
+    g = gpu.Geno.from_bed(p.rows, p.n)
+    g.set_model(None, p.rvec, p.Q)
+    return p, g
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
+def test_lseed_lsolve_against_dense_solve(prob, prec):
+    p, g = prob
+    D = p.dense()
+    tau0 = (1 - 0.7) / 0.7
+    rng = np.random.default_rng(1)
+    y = rng.normal(size=p.n).astype(np.float32)
+    yp = (y - p.Q @ (p.Q.T @ y)).astype(np.float32)
+    B = np.concatenate([yp[:, None], p.Q], axis=1)
+    kinds = ["quadform"] + ["ldet"] * p.Q.shape[1]
+    groups = [0] + [1] * p.Q.shape[1]
+    s = g.L_Seed("ldet", tau0, torch.from_numpy(B).cuda(), tol=1e-7, maxit=400, keep_cols=B.shape[1],
+                 precision=prec, col_kind=kinds, col_group=groups)
+    for h2 in (0.1, 0.3, 0.6):
+        tau = (1 - h2) / h2
+        X = s.L_Solve(torch.from_numpy(B[:, :1]).cuda(), tau - tau0, tol=1e-6, col0=0).cpu().numpy()
+        ref = D.solve("quadform", tau, B[:, :1])
+        assert rel_err(X, ref) < (1e-4 if prec == "fp32" else 5e-4)
+        XQ = s.L_Solve(torch.from_numpy(p.Q).cuda(), tau - tau0, tol=1e-6, col0=1).cpu().numpy()
+        refQ = D.solve("ldet", tau, p.Q)
+        assert rel_err(XQ, refQ) < (1e-4 if prec == "fp32" else 5e-4)
+
+
+def test_lseed_matches_cpu_restatement(prob):
+    """Same inputs, same stopping rule: step counts agree and the tridiagonal coefficients match
+    the FP32 CPU restatement to FP32 rounding (the first steps, before rounding compounds)."""
+    p, g = prob
+    tau0 = (1 - 0.7) / 0.7
+    V = oracle.rademacher(0, p.n, 8)
+    mdl = p.model()
+    ref = mdl.L_Seed("ldet", tau0, V, tol=5e-4)
+    s = g.L_Seed("ldet", tau0, torch.from_numpy(np.ascontiguousarray(V)).cuda(), tol=5e-4, keep_cols=0,
+                 precision="fp32")
+    assert abs(s.steps - ref.steps) <= 1
+    beta, delta = s.coeffs()
+    k = min(5, ref.steps)
+    assert rel_err(delta[:, :k], ref.delta[:, :k]) < 1e-4
+    assert rel_err(beta[:, :k + 1], ref.beta[:, :k + 1]) < 1e-4
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
+def test_slq_against_exact_quadrature(prob, prec):
+    """For fixed probes, SLQ with converged Lanczos equals sum_k v_k' log(A + sigma I) v_k."""
+    p, g = prob
+    D = p.dense()
+    tau0 = (1 - 0.7) / 0.7
+    nv = 6
+    V = oracle.rademacher(3, p.n, nv)
+    s = g.L_Seed("ldet", tau0, torch.from_numpy(np.ascontiguousarray(V)).cuda(), tol=1e-6, keep_cols=0,
+                 precision=prec, maxit=500)
+    for h2 in (0.2, 0.5):
+        tau = (1 - h2) / h2
+        est = s.SLQ_LDet(tau - tau0)
+        exact = D.quad_logm("ldet", tau, V) * p.n / nv
+        assert abs(est - exact) / abs(exact) < (1e-6 if prec == "fp32" else 2e-5)

@@ -1,0 +1,142 @@
+"""GPU parity of the genotype engine against the oracle (bit-exact integer work) and the FP64
+dense oracle (floating point).  Every call goes through the C ABI of libslemm_gpu.so."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from oracle import bedio
+from tests.problems import make_problem, rel_err
+
+pytestmark = pytest.mark.gpu
+
+# relative-error bounds (max |err| / max |ref|) per precision, stated against FP64 dense
+TOL = {"fp32": 2e-5, "bf16x2": 3e-4}
+
+
+def _geno(gpu, p, keep=None, snps=None, n_raw=None):
+    return gpu.Geno.from_bed(p.rows, p.n if n_raw is None else n_raw, keep, snps)
+
+
+def test_decode_bit_exact_with_keep_and_order(gpu):
+    p = make_problem(n=1037, m=611, seed=3, missing=0.02)
+    rng = np.random.default_rng(7)
+    keep = np.sort(rng.choice(p.n, size=801, replace=False))
+    order = rng.permutation(p.m)[:517]
+    g = _geno(gpu, p, keep, order)
+    got = bedio.decode_codes(g.download(), len(keep))
+    bindi = np.zeros(p.n, np.uint8)
+    bindi[keep] = 1
+    kmat = oracle.decode_bed(bedio.bed_bytes(p.rows), p.n, p.m, bindi, None, order)
+    want = (kmat[:, 0::2] | (kmat[:, 1::2] << 1)).astype(np.uint8)
+    assert np.array_equal(got, want)
+    # padding codes beyond n are zero
+    full = g.download()
+    assert np.all((full[:, -1] >> (2 * (len(keep) % 4))) == 0) if len(keep) % 4 else True
+
+
+def test_counts_bit_exact(gpu):
+    p = make_problem(n=2051, m=733, seed=5, missing=0.03)
+    g = _geno(gpu, p)
+    pc, n0, n1, n2, gm = g.counts()
+    opc, on0, on1, on2, ogm = oracle.counts(p.kmat)
+    assert np.array_equal(pc, opc) and np.array_equal(n0, on0)
+    assert np.array_equal(n1, on1) and np.array_equal(n2, on2)
+    assert np.array_equal(gm.view(np.uint32), ogm.view(np.uint32))  # IEEE f32 divide, bit-exact
+
+
+def test_model_terms_bit_exact(gpu):
+    p = make_problem(n=600, m=500, seed=9, weights=True)
+    g = _geno(gpu, p)
+    g.set_model(None, p.rvec, p.Q)
+    gm, gw, rr = g.model()
+    assert np.array_equal(gw.view(np.uint32), p.gw.view(np.uint32))
+    assert np.array_equal(rr, (1.0 / np.sqrt(p.rvec.astype(np.float64))).astype(np.float32)) or rel_err(rr, p.rr) < 1e-7
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
+@pytest.mark.parametrize("t", [1, 7, 33, 66])
+def test_op_ldet_and_quadform(gpu, prec, t):
+    p = make_problem(n=913, m=1201, seed=11, weights=True, c=2)
+    g = _geno(gpu, p)
+    g.set_model(None, p.rvec, p.Q)
+    D = p.dense()
+    rng = np.random.default_rng(t)
+    V = rng.normal(size=(p.n, t)).astype(np.float32)
+    Vd = torch.from_numpy(V).cuda()
+    tau0 = 0.4286
+    for kind in ("ldet", "quadform"):
+        out = g.lnr_op_select(kind, tau0, Vd, precision=prec).cpu().numpy()
+        ref = D.op(kind, tau0, V)
+        assert rel_err(out, ref) < TOL[prec], (kind, prec, t, rel_err(out, ref))
+    # mixed batch: one pass serves both kinds
+    kinds = ["quadform" if k % 3 == 0 else "ldet" for k in range(t)]
+    out = g.lnr_op_select("ldet", tau0, Vd, precision=prec, col_kind=kinds).cpu().numpy()
+    for k, kd in enumerate(kinds):
+        ref = D.op(kd, tau0, V[:, k:k + 1])[:, 0]
+        assert rel_err(out[:, k], ref) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
+def test_rmatvec_matvec(gpu, prec):
+    p = make_problem(n=1500, m=2100, seed=13)
+    g = _geno(gpu, p)
+    g.set_model(None, None, p.Q)
+    D = p.dense()
+    rng = np.random.default_rng(2)
+    V = rng.normal(size=(p.n, 5)).astype(np.float32)
+    U = rng.normal(size=(p.m, 3)).astype(np.float32)
+    zt = g.rmatvec(torch.from_numpy(V).cuda(), precision=prec).cpu().numpy()
+    assert rel_err(zt, D.rmatvec(V)) < TOL[prec]
+    wzt = g.weighted_rmatvec(torch.from_numpy(V).cuda(), precision=prec).cpu().numpy()
+    assert rel_err(wzt, D.rmatvec(V, weighted=True)) < TOL[prec]
+    z = g.matvec(torch.from_numpy(U).cuda(), precision=prec).cpu().numpy()
+    assert rel_err(z, D.matvec(U)) < TOL[prec]
+
+
+def test_centring_of_constant_vector(gpu):
+    """Zc' 1 = popcnt - n gm is zero up to the rounding of gm: the rank-1 centring epilogue must
+    not lose it to cancellation (the intercept column of Q hits exactly this case)."""
+    p = make_problem(n=4099, m=300, seed=17)
+    g = _geno(gpu, p)
+    g.set_model(None, None, p.Q)
+    one = torch.full((p.n, 1), 1.0 / np.sqrt(p.n), dtype=torch.float32, device="cuda")
+    D = p.dense()
+    ref = D.rmatvec(one.cpu().numpy())
+    for prec in ("fp32", "bf16x2"):
+        got = g.rmatvec(one, precision=prec).cpu().numpy()
+        # the column is removed as its mean and re-enters only through the exact term
+        # xbar (popcnt - n gm): the result is as accurate as that FP64 product
+        assert np.max(np.abs(got - ref)) < 1e-6 + 1e-4 * np.max(np.abs(ref)), prec
+
+
+def test_synthesized_bed_consistent(gpu):
+    g = gpu.Geno.synthesize(3000, 700, seed=5, missing=0.01)
+    rows = g.download()
+    codes = bedio.decode_codes(rows, 3000)
+    kmat = bedio.kmat_of_codes(codes)
+    pc, n0, n1, n2, gm = g.counts()
+    opc, on0, on1, on2, ogm = oracle.counts(kmat)
+    assert np.array_equal(pc, opc) and np.array_equal(n2, on2) and np.array_equal(n1, on1)
+    f = pc / (2 * 3000.0)
+    assert 0.04 < f.mean() < 0.3
+    miss = np.mean(codes == 1)
+    assert 0.005 < miss < 0.02
+    # the transposed copy agrees with the SNP-major one: matvec vs dense
+    g.set_model()
+    from oracle.dense import Dense, dosage_of_kmat
+    D = Dense(dosage_of_kmat(kmat), ogm, oracle.gw(ogm), np.ones(3000))
+    U = np.random.default_rng(0).normal(size=(700, 2)).astype(np.float32)
+    z = g.matvec(torch.from_numpy(U).cuda(), precision="fp32").cpu().numpy()
+    assert rel_err(z, D.matvec(U)) < TOL["fp32"]
+
+
+def test_gebv_bit_exact(gpu):
+    p = make_problem(n=3001, m=800, seed=19, missing=0.01)
+    rng = np.random.default_rng(4)
+    alpha = rng.normal(size=p.m)
+    flip = (rng.random(p.m) < 0.3).astype(np.uint8)
+    g = _geno(gpu, p)
+    got = g.gebv(torch.from_numpy(alpha).cuda(), flip).cpu().numpy()
+    want = oracle.gebv(bedio.bed_bytes(p.rows), p.n, p.m, np.ones(p.m, np.uint8), flip, alpha)
+    assert np.array_equal(got, want)
