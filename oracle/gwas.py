@@ -1,0 +1,75 @@
+"""ORACLE -- test infrastructure only.  GWAS statistics of the REML path restated in numpy:
+the gstat quadratic-form statistic of src/output.cpp:98-117 and the GRAMMAR-gamma score test
+of bin/slemm_gamma.py:90-105, :296-325."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def one_df_chisq_pval(x: float) -> float:
+    """getOneDfChisqPval (src/chisq.h:14-18)."""
+    return 1.0 if x <= 0 else math.erfc(math.sqrt(x) * math.sqrt(0.5))
+
+
+def gstat_chisq(xPy: float, xPx: float, vg: float) -> float:
+    """ChiSq column of .gstat.txt (src/output.cpp:105,115), float arithmetic."""
+    xPy, xPx, vg = np.float32(xPy), np.float32(xPx), np.float32(vg)
+    return float((xPy * xPy) / (vg * xPx))
+
+
+def mac_of_maf(maf: float, pop_size: int) -> int:
+    """MAC column of .gstat.txt (src/output.cpp:112)."""
+    return int(np.float32(maf) * 2 * pop_size + 0.5)
+
+
+def mean_wo_outliers(x) -> float:
+    """Correction factor: mean of x inside the 1.5-IQR fences (bin/slemm_gamma.py:90-105)."""
+    x = list(x)
+    q1 = np.percentile(x, 25)
+    q3 = np.percentile(x, 75)
+    lower = q1 - (q3 - q1) * 1.5
+    upper = q3 + (q3 - q1) * 1.5
+    rsum, rcnt = 0.0, 0.0
+    for v in x:
+        if lower < v < upper:
+            rsum += v
+            rcnt += 1
+    return rsum / rcnt
+
+
+def gamma_scores(Z: np.ndarray, Py: np.ndarray, rr: np.ndarray, Q: np.ndarray, mac: float = 0.0):
+    """Per-SNP xy = x_c'Py and xx = ||(I-QQ')R^-1/2 x_c||^2 (bin/slemm_gamma.py:306-319) for the
+    dosage matrix Z (n x m); SNPs at or below the MAC filter get NaN."""
+    Z = np.asarray(Z, np.float64)
+    n, m = Z.shape
+    xy = np.empty(m)
+    xx = np.empty(m)
+    Py = np.asarray(Py, np.float64)
+    rr = np.asarray(rr, np.float64)
+    Q = np.asarray(Q, np.float64).reshape(n, -1)
+    for j in range(m):
+        x = Z[:, j].copy()
+        s = x.sum()
+        if s <= mac or 2 * n - s <= mac:
+            xy[j] = xx[j] = np.nan
+            continue
+        x -= s / n
+        xy[j] = x @ Py
+        x *= rr
+        x -= Q @ (Q.T @ x)
+        xx[j] = x @ x
+    return xy, xx
+
+
+def gamma_stats(xy, xx, cfactor: float, snp_var: float):
+    """BETA, SE, CHISQ, P of bin/slemm_gamma.py:321-325."""
+    from scipy.stats import chi2
+
+    xy = np.asarray(xy, np.float64)
+    xx = np.asarray(xx, np.float64)
+    beta = xy / xx * cfactor
+    serr = np.sqrt((snp_var * cfactor) / xx)
+    chisq = np.square(beta / serr)
+    return beta, serr, chisq, chi2.sf(chisq, 1)

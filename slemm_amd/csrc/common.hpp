@@ -94,12 +94,16 @@ struct slemm_geno {
   slemm::DevBuf kinds;         // per-column operator kinds
   slemm::DevBuf redpart;       // partials of column reductions (double)
   std::vector<int> kinds_host;
+  std::vector<float> rr_host;    // r_rsqrt (n)
+  std::vector<float> rvec_host;  // R (n)
 };
 
 namespace slemm {
 
 // Operator pipeline pieces shared by op.hip and lanczos.hip.
 void ensure_counts(slemm_geno* h, hipStream_t st);
+// Replaces the covariate basis Q (host n x c column-major) of the model terms.
+void set_Q(slemm_geno* h, const float* Q, int c);
 void require_model(const slemm_geno* h);
 void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const float* V, int64_t ldv,
             int t, float* out, int64_t ldo, int precision, hipStream_t st);

@@ -236,6 +236,20 @@ void ensure_counts(slemm_geno* h, hipStream_t st) {
   h->counts_ready = true;
 }
 
+void set_Q(slemm_geno* h, const float* Q, int c) {
+  if (h->Q) {
+    (void)hipFree(h->Q);
+    h->Q = nullptr;
+  }
+  h->c = c;
+  if (c > 0) {
+    HIPCHECK(hipMalloc(&h->Q, sizeof(float) * h->n_pad * c));
+    HIPCHECK(hipMemset(h->Q, 0, sizeof(float) * h->n_pad * c));
+    HIPCHECK(hipMemcpy2D(h->Q, sizeof(float) * h->n_pad, Q, sizeof(float) * h->n, sizeof(float) * h->n, c,
+                         hipMemcpyHostToDevice));
+  }
+}
+
 void require_model(const slemm_geno* h) {
   if (!h->model_ready) fail(SLEMM_ERR_STATE, "model terms not set: call slemm_gpu_set_model first");
 }
@@ -471,17 +485,12 @@ extern "C" int slemm_gpu_set_model(slemm_geno* h, const float* persnp, const flo
       REQUIRE(std::isfinite(gw[j]), "monomorphic SNP (allele frequency 0 or 1) gives an infinite weight");
     HIPCHECK(hipMemcpy(h->gw, gw.data(), sizeof(float) * m, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(h->rr, rr.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-    if (h->Q) {
-      (void)hipFree(h->Q);
-      h->Q = nullptr;
-    }
-    h->c = c;
-    if (c > 0) {
-      HIPCHECK(hipMalloc(&h->Q, sizeof(float) * h->n_pad * c));
-      HIPCHECK(hipMemset(h->Q, 0, sizeof(float) * h->n_pad * c));
-      HIPCHECK(hipMemcpy2D(h->Q, sizeof(float) * h->n_pad, Q, sizeof(float) * ldq, sizeof(float) * n, c,
-                           hipMemcpyHostToDevice));
-    }
+    h->rr_host = rr;
+    h->rvec_host.assign(n, 1.0f);
+    if (rvec) std::copy(rvec, rvec + n, h->rvec_host.begin());
+    std::vector<float> qc((size_t)n * c);
+    for (int q = 0; q < c; ++q) std::copy(Q + (size_t)q * ldq, Q + (size_t)q * ldq + n, qc.begin() + (size_t)q * n);
+    set_Q(h, qc.data(), c);
     h->model_ready = true;
   });
 }

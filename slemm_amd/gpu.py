@@ -205,6 +205,48 @@ class Geno:
         L.check(L.load().slemm_gpu_gebv(self._h, _ptr(a), _np_ptr(fl, C.c_uint8), _ptr(out), _stream()))
         return out
 
+    # ------------------------------------------------------------------ REML drivers
+    def SLDF_REML(self, X: np.ndarray, y: np.ndarray, rng_seed: int = 0, n_V: int = 30, tol_L: float = 5e-4,
+                  s2max: float = 0.7, s2min: float = 1e-4, tol_VC: float = 1e-4, precision="bf16x2",
+                  verbose: bool = False) -> dict:
+        """SLDF_REML (src/reml.cpp:549-690); persnp and rvec come from set_model."""
+        Xf = np.asfortranarray(np.asarray(X, np.float32).reshape(self.n, -1))
+        c = Xf.shape[1]
+        yv = np.ascontiguousarray(y, np.float32)
+        Py = np.zeros(self.n, np.float32)
+        blup = np.zeros(self.m, np.float32)
+        blue = np.zeros(c, np.float32)
+        vb = np.zeros((c, c), np.float32, order="F")
+        st = L.RemlStats()
+        L.check(L.load().slemm_gpu_sldf_reml(
+            self._h, _np_ptr(Xf, C.c_float), self.n, c, _np_ptr(yv, C.c_float), rng_seed, n_V, tol_L, s2max,
+            s2min, tol_VC, _prec(precision), int(verbose), _np_ptr(Py, C.c_float), _np_ptr(blup, C.c_float),
+            _np_ptr(blue, C.c_float), _np_ptr(vb, C.c_float), C.byref(st)))
+        return dict(vg=st.vg, ve=st.ve, h2=st.h2, lrt=st.lrt, Py=Py, snp_blup=blup, blue=blue, var_blue=vb,
+                    steps=tuple(st.lanczos_steps), evals=st.criterion_evals, seconds_seed=st.seconds_seed,
+                    seconds_total=st.seconds_total)
+
+    def LMC_REML(self, X: np.ndarray, y: np.ndarray, rng_seed: int = 0, n_MC: Optional[int] = None,
+                 tol_L: float = 5e-4, s2max: float = 0.7, s2min: float = 1e-4, precision="bf16x2",
+                 verbose: bool = False) -> dict:
+        """LMC_REML (src/reml.cpp:730-882, without --lmm); n_MC default as src/option.cpp:414."""
+        if n_MC is None:
+            n_MC = max(min(int(4e9 / self.n / self.n), 30), 3)
+        Xf = np.asfortranarray(np.asarray(X, np.float32).reshape(self.n, -1))
+        c = Xf.shape[1]
+        yv = np.ascontiguousarray(y, np.float32)
+        Py = np.zeros(self.n, np.float32)
+        blup = np.zeros(self.m, np.float32)
+        blue = np.zeros(c, np.float32)
+        st = L.RemlStats()
+        L.check(L.load().slemm_gpu_lmc_reml(
+            self._h, _np_ptr(Xf, C.c_float), self.n, c, _np_ptr(yv, C.c_float), rng_seed, n_MC, tol_L, s2max,
+            s2min, _prec(precision), int(verbose), _np_ptr(Py, C.c_float), _np_ptr(blup, C.c_float),
+            _np_ptr(blue, C.c_float), C.byref(st)))
+        return dict(vg=st.vg, ve=st.ve, h2=st.h2, Py=Py, snp_blup=blup, blue=blue, steps=st.lanczos_steps[0],
+                    evals=st.criterion_evals, n_MC=n_MC, seconds_seed=st.seconds_seed,
+                    seconds_total=st.seconds_total)
+
     # ------------------------------------------------------------------ Krylov engine
     def L_Seed(self, lnr, tau0: float, B: torch.Tensor, tol: float = 5e-4, maxit: int = 600,
                keep_cols: Optional[int] = None, precision="bf16x2", col_kind=None, col_group=None,

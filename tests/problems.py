@@ -58,3 +58,17 @@ def rel_err(a, b) -> float:
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def make_phenotype(p: Problem, h2=0.5, n_causal=200, seed=0) -> np.ndarray:
+    """y = Z beta + e with beta ~ N(0, 1) on n_causal SNPs and var(e) set for the target h2."""
+    rng = np.random.default_rng(seed + 77)
+    idx = rng.choice(p.m, size=min(n_causal, p.m), replace=False)
+    beta = rng.normal(size=len(idx))
+    g = p.Z[:, idx] @ beta
+    g -= g.mean()
+    ve = g.var() * (1 - h2) / h2
+    e = rng.normal(scale=np.sqrt(ve), size=p.n)
+    if p.rvec is not None:
+        e *= np.sqrt(p.rvec)
+    return (g + e + 3.0).astype(np.float32)

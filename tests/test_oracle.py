@@ -1,0 +1,277 @@
+"""Pins of the oracle (CPU, no GPU): known answers, brute force, closed forms, the FP64 dense
+oracle, the libstdc++ RNG definitions and the reference's example outputs."""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import bedio, gwas
+from oracle import reml as oreml
+from oracle.dense import Dense, dosage_of_kmat, reml_loglik
+from tests.problems import make_phenotype, make_problem, rel_err
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _build(oracle_lib):
+    return oracle_lib
+
+
+# ------------------------------------------------------------------------------ bed decode ----
+def test_decode_known_answer_byte():
+    # 0b11_10_01_00: individuals 0..3 carry codes 00, 01 (missing), 10 (het), 11 -> dosages 0,1,1,2
+    bed = bedio.MAGIC + bytes([0b11100100])
+    kmat = oracle.decode_bed(bed, 4, 1)
+    assert kmat.tolist() == [[0, 0, 1, 0, 0, 1, 1, 1]]
+    assert dosage_of_kmat(kmat)[:, 0].tolist() == [0, 1, 1, 2]
+
+
+def test_decode_matches_numpy_codec_with_masks_and_order():
+    rng = np.random.default_rng(0)
+    n, m = 37, 23
+    codes = rng.integers(0, 4, size=(m, n)).astype(np.uint8)
+    rows = bedio.encode_codes(codes)
+    bindi = (rng.random(n) < 0.7).astype(np.uint8)
+    bmarker = (rng.random(m) < 0.8).astype(np.uint8)
+    kept = np.flatnonzero(bmarker)
+    order = rng.permutation(len(kept))
+    kmat = oracle.decode_bed(bedio.bed_bytes(rows), n, m, bindi, bmarker, order)
+    want = bedio.kmat_of_codes(codes[kept][order][:, bindi.astype(bool)])
+    assert np.array_equal(kmat, want)
+
+
+def test_decode_truncated_bed_raises():
+    rows = bedio.encode_codes(np.zeros((5, 9), np.uint8))
+    with pytest.raises(ValueError, match="problem with the BED file"):
+        oracle.decode_bed(bedio.bed_bytes(rows)[:-3], 9, 5)
+
+
+# ------------------------------------------------------------------------------ counts ----------
+def test_counts_brute_force_and_frequencies():
+    rng = np.random.default_rng(1)
+    codes = rng.integers(0, 4, size=(50, 333)).astype(np.uint8)
+    kmat = bedio.kmat_of_codes(codes)
+    pc, n0, n1, n2, gm = oracle.counts(kmat)
+    assert np.array_equal(pc, kmat.sum(axis=1))
+    assert np.array_equal(n2, (codes == 3).sum(axis=1))
+    assert np.array_equal(n1, ((codes == 1) | (codes == 2)).sum(axis=1))
+    assert np.array_equal(n0, (codes == 0).sum(axis=1))
+    assert np.array_equal(gm, (pc.astype(np.float32) / np.float32(333)).astype(np.float32))
+    f = pc.astype(np.float32) / np.float32(666)
+    assert np.array_equal(oracle.maf(kmat), np.abs((f > 0.5).astype(np.float32) - f))
+
+
+def test_gw_closed_form():
+    gm = np.array([0.1, 0.5, 1.0, 1.9], np.float32)
+    ps = np.array([1.0, 2.0, 3.0, 2.0], np.float32)
+    v = (gm * (np.float32(2) - gm)) / np.float32(2)
+    want = (np.float32(1) / v) / np.float32(4) * (ps / np.float32(2.0))
+    assert np.allclose(oracle.gw(gm, ps), want, rtol=1e-7, atol=0)
+    want1 = (np.float32(1) / v) / np.float32(4)
+    assert np.array_equal(oracle.gw(gm), want1.astype(np.float32))
+
+
+# ------------------------------------------------------------------------------ operators -------
+@pytest.fixture(scope="module")
+def small():
+    p = make_problem(n=403, m=517, seed=4, weights=True, c=2)
+    return p, p.model(subset_size=100), p.dense()
+
+
+def test_operators_vs_dense(small):
+    p, mdl, D = small
+    V = np.random.default_rng(3).normal(size=(p.n, 5)).astype(np.float32)
+    for kind in ("ldet", "quadform"):
+        assert rel_err(mdl.op(kind, 0.43, V), D.op(kind, 0.43, V)) < 2e-5
+    assert rel_err(mdl.rmatvec(V), D.rmatvec(V)) < 2e-5
+    assert rel_err(mdl.rmatvec(V, weighted=True), D.rmatvec(V, weighted=True)) < 2e-5
+    U = np.random.default_rng(4).normal(size=(p.m, 3)).astype(np.float32)
+    assert rel_err(mdl.matvec(U), D.matvec(U)) < 2e-5
+
+
+def test_centring_identity(small):
+    """Zc'v = Z'v - gm (1'v) and Zc u = Z u - 1 (gm'u): the identities the kernels' epilogues use."""
+    p, mdl, D = small
+    v = np.random.default_rng(5).normal(size=p.n)
+    u = np.random.default_rng(6).normal(size=p.m)
+    assert np.allclose(D.Zc.T @ v, p.Z.T @ v - p.gm.astype(np.float64) * v.sum(), rtol=1e-12, atol=1e-9)
+    assert np.allclose(D.Zc @ u, p.Z @ u - np.dot(p.gm.astype(np.float64), u), rtol=1e-12, atol=1e-9)
+
+
+def test_quadform_is_projected_ldet(small):
+    p, mdl, D = small
+    P = np.eye(p.n) - p.Q.astype(np.float64) @ p.Q.T.astype(np.float64)
+    assert np.allclose(D.A("quadform", 0.0), P @ D.A("ldet", 0.0) @ P, atol=1e-10)
+
+
+def test_lseed_lsolve_vs_dense_solve(small):
+    p, mdl, D = small
+    B = np.random.default_rng(7).normal(size=(p.n, 2)).astype(np.float32)
+    tau0 = 0.4286
+    s = mdl.L_Seed("ldet", tau0, B, tol=1e-6)
+    assert s.converged
+    for tau in (0.5, 2.0, 9.0):
+        X = s.L_Solve(B, tau - tau0, tol=1e-6)
+        assert rel_err(X, D.solve("ldet", tau, B)) < 1e-4
+    # Jacobi matrix of a converged Lanczos run: its extreme Ritz values bracket A's spectrum ends
+    lam = np.linalg.eigvalsh(D.A("ldet", tau0))
+    T = np.diag(s.delta[0]) + np.diag(s.beta[0, 1:s.steps], 1) + np.diag(s.beta[0, 1:s.steps], -1)
+    ritz = np.linalg.eigvalsh(T.astype(np.float64))
+    assert lam[0] - 1e-4 <= ritz[0] and ritz[-1] <= lam[-1] + 1e-3 * lam[-1]
+    assert abs(ritz[-1] - lam[-1]) / lam[-1] < 1e-3
+
+
+def test_slq_vs_exact_log(small):
+    p, mdl, D = small
+    V = oracle.rademacher(1, p.n, 4)
+    tau0 = 0.4286
+    s = mdl.L_Seed("ldet", tau0, V, tol=1e-6)
+    for tau in (0.6, 3.0):
+        est = sum(float(np.sum(W ** 2 * np.log(Dv + tau - tau0)))
+                  for Dv, W in (oreml.slq_spectrum(s, k) for k in range(4)))
+        exact = D.quad_logm("ldet", tau, V)
+        assert abs(est - exact) / abs(exact) < 1e-4
+
+
+# ------------------------------------------------------------------------------ RNG -------------
+def _mt_raw(seed, count):
+    bg = np.random.MT19937()
+    bg._legacy_seeding(seed)
+    return bg.random_raw(count).astype(np.uint64)
+
+
+def test_rademacher_is_mt19937_uniform_float():
+    """std::mt19937(seed+54321) + uniform_real_distribution<float>(0,1): u = float(x) / 2^32 from
+    one 32-bit draw (generate_canonical<float,24>); v = -1 if u < 0.5 (src/reml.cpp:597-605)."""
+    n, nv, seed = 50, 3, 11
+    x = _mt_raw(seed + 54321, n * nv)
+    u = x.astype(np.float32) / np.float32(4294967296.0)
+    u = np.where(u >= 1, np.nextafter(np.float32(1), np.float32(0)), u)
+    signs = np.where(u < 0.5, -1.0, 1.0).reshape(nv, n).T
+    V = oracle.rademacher(seed, n, nv)
+    assert np.array_equal(np.sign(V), signs)
+    assert np.allclose(np.abs(V), np.float32(1) / np.sqrt(np.float32(n)), rtol=0, atol=1e-7)
+
+
+def test_lmc_samples_are_mt19937_marsaglia_polar():
+    """normal_distribution<float> in libstdc++: Marsaglia polar method on pairs of
+    2u-1 draws, caching the second deviate (src/reml.cpp:795-809)."""
+    n, m, nmc, seed = 7, 9, 2, 5
+    u_o, e_o = oracle.lmc_samples(seed, n, m, nmc)
+    raw = _mt_raw(seed + 54321, 4 * (m + n) * nmc + 64)
+    pos = [0]
+
+    def unif():
+        v = np.float32(raw[pos[0]]) / np.float32(4294967296.0)
+        pos[0] += 1
+        return v
+
+    out = []
+    saved = None
+    while len(out) < (m + n) * nmc:
+        if saved is not None:
+            out.append(saved)
+            saved = None
+            continue
+        while True:
+            x = np.float32(2.0) * unif() - np.float32(1.0)
+            y = np.float32(2.0) * unif() - np.float32(1.0)
+            r2 = x * x + y * y
+            if r2 <= 1.0 and r2 != 0.0:
+                break
+        mult = np.sqrt(np.float32(-2.0) * np.log(r2) / r2)
+        saved = x * mult
+        out.append(y * mult)
+    want = np.array(out, np.float32)
+    got = np.concatenate([u_o.T.ravel(), e_o.T.ravel()])
+    assert np.allclose(got, want, rtol=2e-6, atol=1e-7)
+
+
+# ------------------------------------------------------------------------------ GEBV ------------
+def test_gebv_exact():
+    rng = np.random.default_rng(9)
+    n, m = 61, 40
+    codes = rng.integers(0, 4, size=(m, n)).astype(np.uint8)
+    rows = bedio.encode_codes(codes)
+    bm = (rng.random(m) < 0.8).astype(np.uint8)
+    bf = (rng.random(m) < 0.4).astype(np.uint8)
+    alpha = rng.normal(size=m)
+    got = oracle.gebv(bedio.bed_bytes(rows), n, m, bm, bf, alpha)
+    z = bedio.dosage_of_codes(codes).astype(np.int64)
+    z = np.where(bf[:, None] == 1, 2 - z, z)
+    want = np.zeros(n)
+    for j in range(m):  # same order as the reference: SNP by SNP, FP64
+        if bm[j]:
+            want += z[j] * alpha[j]
+    assert np.array_equal(got, want)
+
+
+# ------------------------------------------------------------------------------ REML drivers ----
+def test_sldf_restatement_near_exact_optimum():
+    p = make_problem(n=900, m=1200, seed=12)
+    y = make_phenotype(p, 0.4, seed=1)
+    r = oreml.SLDF_REML(p.kmat, p.gm, p.gw, None, p.X, y, n_V=50)
+    hs = np.linspace(0.1, 0.8, 71)
+    ll = [reml_loglik(p.Z, p.gm, p.gw, p.X, y, h) for h in hs]
+    assert abs(r["h2"] - hs[int(np.argmax(ll))]) < 0.03
+    assert r["evals"] == 21  # 2 + ceil(ln(1e-4/0.6999)/ln 0.618) golden-section evaluations
+    # with a single intercept the BLUE is the GLS mean, and Vg = y'Py/(n-c) holds at the optimum
+    assert abs(r["vg"] - oreml.vg_from_py(y, r["Py"], 1)) / r["vg"] < 1e-3
+
+
+def test_lmc_restatement_near_exact_optimum():
+    p = make_problem(n=900, m=1200, seed=13)
+    y = make_phenotype(p, 0.5, seed=2)
+    r = oreml.LMC_REML(p.kmat, p.gm, p.gw, None, p.X, y)
+    hs = np.linspace(0.1, 0.9, 81)
+    ll = [reml_loglik(p.Z, p.gm, p.gw, p.X, y, h) for h in hs]
+    assert r["n_MC"] == 30
+    assert abs(r["h2"] - hs[int(np.argmax(ll))]) < 0.06
+
+
+# ------------------------------------------------------------------------------ example outputs -
+@pytest.fixture(scope="module")
+def ex10k():
+    d = np.load(GOLDEN / "ex10k_reml.npz")
+    meta = json.loads((GOLDEN / "ex10k_meta.json").read_text())
+    return d, meta
+
+
+def test_example_vg_from_py(ex10k):
+    d, meta = ex10k
+    vg = oreml.vg_from_py(d["y"], d["Py"], meta["c"])
+    assert abs(vg - meta["Vg"]) / meta["Vg"] < 5e-6
+    assert abs(meta["Vg"] / (meta["Vg"] + meta["Ve"]) - meta["h2"]) < 1e-6
+    assert abs(float(np.sum(d["Py"].astype(np.float64)))) < 1e-3 * float(np.abs(d["Py"]).sum()) / 1e3
+    assert meta["pop_size"] == len(d["y"])
+
+
+def test_example_gstat_statistics(ex10k):
+    _, meta = ex10k
+    for row in meta["gstat"]:
+        chisq = gwas.gstat_chisq(float(row["xPy"]), float(row["xPx"]), meta["Vg"])
+        assert abs(chisq - float(row["ChiSq"])) / float(row["ChiSq"]) < 2e-5
+        assert abs(gwas.one_df_chisq_pval(float(row["ChiSq"])) - float(row["Pval"])) < 1e-5
+        assert gwas.mac_of_maf(float(row["MAF"]), meta["pop_size"]) == int(row["MAC"])
+
+
+def test_example_gamma_statistics(ex10k):
+    _, meta = ex10k
+    hdr = meta["gamma_header"]
+    ib, isd, ic, ip = (hdr.index(k) for k in ("BETA", "SE", "CHISQ", "P"))
+    for r in meta["gamma_rows"]:
+        beta, se, chisq, pv = float(r[ib]), float(r[isd]), float(r[ic]), float(r[ip])
+        assert abs((beta / se) ** 2 - chisq) <= 1e-5 * max(chisq, 1e-3)
+        assert abs(math.erfc(math.sqrt(chisq / 2)) - pv) < 1e-6
+
+
+def test_example_maf_are_allele_fractions(ex10k):
+    _, meta = ex10k
+    two_n = 2 * meta["pop_size"]
+    for f in meta["maf"]:
+        k = round(f * two_n)
+        assert abs(f - k / two_n) < 1e-6
