@@ -123,6 +123,8 @@ def main():
     def step(prec):
         return g.lnr_op_select("ldet", tau0, V, precision=prec, col_kind=kinds)
 
+    tune = g.autotune(K, reps=2)
+
     for _ in range(args.warmup):
         step("bf16x2")
     torch.cuda.synchronize()
@@ -180,6 +182,7 @@ def main():
         "fp32_path": {"ms_per_step": round(dt32 * 1e3, 3), "GB/s": round(2 * pb / dt32 / 1e9, 2),
                       "speedup_bf16x2_over_fp32": round(dt32 / dt, 2)},
         "bed_generate_s": round(t_load, 3),
+        "autotune": tune,
         "roofline": {
             "bound": "mfma",
             "kernel": f"k_blockprod bf16x2 {dom}",

@@ -202,6 +202,16 @@ int slemm_gpu_lmc_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const 
                        int precision, int verbose,
                        float* Py, float* snp_blup, float* blue, slemm_reml_stats* stats);
 
+/* ---------------------------------------------------------------- tuning ------------------- */
+
+/* Autotuner of the split-BF16 block products (no reference counterpart; the reference's only
+ * blocking knob is --subset_size, src/option.cpp:75,297).  Times every kernel variant (B operand
+ * from global memory with 4 or 8 row tiles per wave, or LDS-staged in 4- or 2-step groups) at
+ * three grid sizes for both passes of a t-column operator on this handle's shape, and keeps the
+ * fastest per pass for later calls.  Host outputs (any may be NULL): variant[2], target[2]
+ * (workgroups the grid aims for), ms[26] = best per pass, then pass*12 + variant*3 + size index. */
+int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, int* target, double* ms);
+
 /* ---------------------------------------------------------------- instrumentation ---------- */
 
 /* Kernel timing of the genotype block products (no reference counterpart; the reference only

@@ -84,3 +84,51 @@ def reml_loglik(Z, gm, gw, X, y, h2: float, rr=None) -> float:
     P = Hi - Hi @ X @ np.linalg.solve(XtHiX, X.T @ Hi)
     yPy = float(y @ P @ y)
     return -0.5 * ((n - c) * np.log(yPy) + np.linalg.slogdet(H)[1] + np.linalg.slogdet(XtHiX)[1])
+
+
+def sldf_criterion(D: "Dense", y_proj, Q, V, h2: float, tau0: float):
+    """REML_criterion (src/reml.cpp:457-489) evaluated exactly in FP64: yPy and Q'H^-1Q by dense
+    solves, and the SLQ log-determinant replaced by its limit n/nV sum_k v_k' log(A + sigma I) v_k
+    for the same probes V.  Returns (slogLL, vg, ve, lrt)."""
+    n = D.n
+    c = Q.shape[1]
+    tau = (1 - h2) / h2
+    yp = np.asarray(y_proj, np.float64)
+    x = D.solve("quadform", tau, yp)
+    yPy = float(yp @ x)
+    ySy = float(yp @ yp)
+    vg = yPy / (n - c)
+    ve = vg * tau
+    ldetH = D.quad_logm("ldet", tau, V) * n / V.shape[1]
+    XQ = D.solve("ldet", tau, np.asarray(Q, np.float64))
+    logdet = np.linalg.slogdet(np.asarray(Q, np.float64).T @ XQ)[1]
+    lrt = n * np.log(ySy / yPy) - ldetH
+    return (n - c) * np.log(ySy / yPy) - logdet - ldetH, vg, ve, lrt
+
+
+def sldf_gss(D: "Dense", y_proj, Q, V, s2min=1e-4, s2max=0.7, tol=1e-4):
+    """REML_GSS (src/reml.cpp:493-547) in FP64 over sldf_criterion; returns the last evaluated
+    point (vg, ve, lrt, h2) exactly as the reference does."""
+    invphi = (np.sqrt(5) - 1) / 2
+    invphi2 = (3 - np.sqrt(5)) / 2
+    a, b = s2min, s2max
+    h = b - a
+    tau0 = (1 - s2max) / s2max
+    nitr = 1 if h < tol else int(np.ceil(np.log(tol / h) / np.log(invphi)))
+    c, d = a + invphi2 * h, a + invphi * h
+    fc, *_ = sldf_criterion(D, y_proj, Q, V, c, tau0)
+    fd, vg, ve, lrt = sldf_criterion(D, y_proj, Q, V, d, tau0)
+    last = (vg, ve, lrt)
+    for _ in range(nitr):
+        if fc > fd:
+            b, d, fd = d, c, fc
+            h *= invphi
+            c = a + invphi2 * h
+            fc, *last = sldf_criterion(D, y_proj, Q, V, c, tau0)
+        else:
+            a, c, fc = c, d, fd
+            h *= invphi
+            d = a + invphi * h
+            fd, *last = sldf_criterion(D, y_proj, Q, V, d, tau0)
+    vg, ve, lrt = last
+    return vg, ve, lrt, vg / (vg + ve)

@@ -256,27 +256,42 @@ int blockprod_nt(int t) {
 int64_t blockprod_rows_per_wg(int NT) { return NT <= 2 ? 512 : 256; }
 
 // Optional per-launch timing of the block-product kernels with HIP events recorded on the
-// launch stream (slemm_gpu_timing_*).  Events are created lazily and read back on demand.
+// launch stream (slemm_gpu_timing_*).  Events are created per launch and read back on demand.
 namespace {
 struct TimingRec {
   hipEvent_t beg, end;
-  int epi, precision;
-  int64_t rows, red, t;
+  int pass, precision;
 };
 bool g_timing = false;
 std::vector<TimingRec> g_timing_recs;
 }  // namespace
 
+TimingToken timing_begin(hipStream_t st) {
+  TimingToken tok;
+  if (!g_timing) return tok;
+  HIPCHECK(hipEventCreate(&tok.beg));
+  HIPCHECK(hipEventRecord(tok.beg, st));
+  tok.on = true;
+  return tok;
+}
+
+void timing_end(TimingToken& tok, hipStream_t st, int pass, int precision) {
+  if (!tok.on) return;
+  TimingRec rec{};
+  rec.beg = tok.beg;
+  HIPCHECK(hipEventCreate(&rec.end));
+  HIPCHECK(hipEventRecord(rec.end, st));
+  rec.pass = pass;
+  rec.precision = precision;
+  g_timing_recs.push_back(rec);
+  tok.on = false;
+}
+
 void launch_blockprod(const BlockProdArgs& a, int precision, int epi, int NT, int64_t rows_pad,
                       hipStream_t st) {
   if (rows_pad % blockprod_rows_per_wg(NT) != 0) fail(SLEMM_ERR_ARG, "row padding mismatch");
   if (a.KP % (16 * NT) != 0) fail(SLEMM_ERR_ARG, "column padding mismatch");
-  TimingRec rec{};
-  if (g_timing) {
-    HIPCHECK(hipEventCreate(&rec.beg));
-    HIPCHECK(hipEventCreate(&rec.end));
-    HIPCHECK(hipEventRecord(rec.beg, st));
-  }
+  TimingToken tok = timing_begin(st);
   if (precision == PREC_BF16X2) {
     if (epi == EPI_PASS1) launch_nt<PREC_BF16X2, EPI_PASS1>(a, NT, rows_pad, st);
     else launch_nt<PREC_BF16X2, EPI_PASS2>(a, NT, rows_pad, st);
@@ -284,15 +299,7 @@ void launch_blockprod(const BlockProdArgs& a, int precision, int epi, int NT, in
     if (epi == EPI_PASS1) launch_nt<PREC_FP32, EPI_PASS1>(a, NT, rows_pad, st);
     else launch_nt<PREC_FP32, EPI_PASS2>(a, NT, rows_pad, st);
   }
-  if (g_timing) {
-    HIPCHECK(hipEventRecord(rec.end, st));
-    rec.epi = epi;
-    rec.precision = precision;
-    rec.rows = a.rows;
-    rec.red = a.nsuper * kPad;
-    rec.t = a.t;
-    g_timing_recs.push_back(rec);
-  }
+  timing_end(tok, st, epi, precision);
 }
 
 }  // namespace slemm
@@ -309,7 +316,7 @@ extern "C" int slemm_gpu_timing_read(int pass, int precision, double* total_ms, 
     double tot = 0.0;
     int cnt = 0;
     for (auto& r : g_timing_recs) {
-      if (r.epi != pass || r.precision != precision) continue;
+      if (r.pass != pass || r.precision != precision) continue;
       HIPCHECK(hipEventSynchronize(r.end));
       float ms = 0.f;
       HIPCHECK(hipEventElapsedTime(&ms, r.beg, r.end));

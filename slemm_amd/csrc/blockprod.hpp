@@ -60,6 +60,78 @@ struct PackArgs {
   const float* wsum = nullptr;  // optional row weights of the column sums (sum_i wsum_i x_ik)
 };
 
+// ---- split-BF16 block product with LDS-staged vector block (blockprod_lds.hip)
+constexpr int kLdsRT = 4;  // 16-row tiles per wave; a workgroup covers 256 rows
+
+struct LdsArgs {
+  const uint8_t* G = nullptr;
+  int64_t ldg = 0;
+  int64_t nsuper = 0;         // reduction length / 512
+  int64_t ns_per_split = 0;   // super-chunks per gridDim.z slice
+  const void* X = nullptr;    // BF16 hi block, lo block at lo_offset16 (16-byte units)
+  int64_t lo_offset16 = 0;
+  int KP = 0;
+  int64_t rows_pad = 0;
+  float* part = nullptr;      // nsplit x KP x rows_pad partial products (column-major per split)
+  bool squared = false;       // unpack squared dosages {0,1,1,4} instead of {0,1,1,2}
+};
+
+struct Fin1Args {
+  const float* part = nullptr;
+  int nsplit = 1;
+  int64_t rows_pad = 0, rows = 0;
+  int KP = 0, t = 0;
+  const double* colsum = nullptr;  // s_k
+  const double* xsum = nullptr;    // un-centred sums (null: none)
+  double xsum_n = 0.0;
+  const int32_t* popcnt = nullptr;
+  const float* gm = nullptr;
+  const float* gw = nullptr;       // null: unweighted
+  void* X2 = nullptr;              // BF16 pass-2 operand (null: none)
+  int64_t lo_offset16 = 0;
+  double* upart = nullptr;         // (rows_pad/256) x KP partial sums of gm * represented T
+  float* out = nullptr;            // optional column-major rows x t
+  int64_t ldo = 0;
+  bool raw = false;                // out = summed partial products, no centring or weighting
+};
+
+struct Fin2Args {
+  const float* part = nullptr;
+  int nsplit = 1;
+  int64_t rows_pad = 0, rows = 0;
+  int KP = 0, t = 0;
+  const double* colsum = nullptr;  // u_k
+  const float* rr = nullptr;
+  const float* v = nullptr;
+  int64_t ldv = 0;
+  float tau0 = 0.f;
+  const int* kinds = nullptr;
+  float* out = nullptr;
+  int64_t ldo = 0;
+};
+
+// Block-product variants (selected per shape by the autotuner, slemm_gpu_autotune):
+//  BP_REG : B fragments read from global memory by every wave, no LDS (2 waves/SIMD)
+//  BP_LDS4: B staged in LDS per 4-step group, double-buffered, codes prefetched a chunk ahead
+//  BP_LDS2: B staged in LDS per 2-step group, double-buffered, no code prefetch
+//  BP_REG8: as BP_REG with 8 row tiles per wave (half the B traffic per MFMA, 1 wave/SIMD)
+enum { BP_REG = 0, BP_LDS4 = 1, BP_LDS2 = 2, BP_REG8 = 3, BP_NVARIANTS = 4 };
+
+int lds_nt(int t);
+// number of reduction splits so that the grid has about target_wgs workgroups
+int lds_splits(int64_t rows_pad, int64_t nsuper, int64_t target_wgs, int64_t& ns_per);
+void launch_bp(const LdsArgs& a, int NT, int nsplit, int variant, hipStream_t st);
+void launch_fin1(const Fin1Args& a, hipStream_t st);
+void launch_fin2(const Fin2Args& a, hipStream_t st);
+
+// Kernel timing hooks (HIP events on the launch stream), shared by all block-product launchers.
+struct TimingToken {
+  hipEvent_t beg = nullptr;
+  bool on = false;
+};
+TimingToken timing_begin(hipStream_t st);
+void timing_end(TimingToken& tok, hipStream_t st, int pass, int precision);
+
 int blockprod_nt(int t);
 int64_t blockprod_rows_per_wg(int NT);
 void launch_blockprod(const BlockProdArgs& a, int precision, int epi, int NT, int64_t rows_pad, hipStream_t st);

@@ -1,0 +1,361 @@
+// Split-BF16 genotype block product with the vector block staged in LDS.
+//
+// Same product as k_blockprod (P = G X with G the 2-bit codes unpacked to BF16 dosages and X the
+// hi/lo BF16 split of the vector block), restructured for gfx950's data paths:
+//  * the four waves of a workgroup share every B-operand fragment through LDS: a step group of
+//    TS MFMA steps (TS*32 reduction elements x KB columns x hi/lo) is copied once per workgroup
+//    with 16-byte loads and read back with conflict-free ds_read_b128, double-buffered so the
+//    copy of group q+1 overlaps the MFMAs of group q;
+//  * the codes of a wave's rows stay in registers for a whole 512-element super-chunk and the
+//    next super-chunk's codes are fetched one chunk ahead;
+//  * the reduction can be split over gridDim.z so that passes with few rows (Z U over the
+//    individuals, small SNP sets) still fill all 256 CUs; each split writes raw FP32 partial
+//    products and the epilogue kernels below sum the splits in a fixed order (deterministic).
+#include <hip/hip_runtime.h>
+
+#include "blockprod.hpp"
+
+namespace slemm {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+namespace {
+
+__device__ __forceinline__ uint32_t bf16_bits_rn(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ float bf16_to_float(uint32_t b) { return __uint_as_float(b << 16); }
+
+// 8 two-bit codes -> 8 BF16 dosages with one byte-permute per pair of codes.  Table indexed by
+// the raw code c: low byte {00,80,80,00}[c], high byte {00,3F,3F,40}[c] (BF16 of 0,1,1,2).
+// SQ selects squared dosages {0,1,1,4}: low byte {00,80,80,80}[c] (BF16(4) = 0x4080).
+template <bool SQ = false>
+__device__ __forceinline__ bf16x8_t unpack8(uint32_t p16) {
+  constexpr uint32_t lo_tab = SQ ? 0x80808000u : 0x00808000u;
+  uint32_t w[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t t = (p16 >> (4 * q)) & 0xFu;
+    const uint32_t x = (t | (t << 14)) & 0x00030003u;  // c0 -> byte 0, c1 -> byte 2
+    w[q] = __builtin_amdgcn_perm(0x403F3F00u, lo_tab, x * 0x0101u + 0x04000400u);
+  }
+  return __builtin_bit_cast(bf16x8_t, make_uint4(w[0], w[1], w[2], w[3]));
+}
+
+}  // namespace
+
+template <int RT, int NT, int TS, bool PREF, bool SQ>
+__global__ __launch_bounds__(256) void k_bp_lds(LdsArgs a) {
+  constexpr int KB = 16 * NT;
+  constexpr int GPS = 16 / TS;                 // step groups per super-chunk
+  constexpr int NSTAGE = TS * 8 * KB / 256;    // 16-byte loads per thread per step group
+  __shared__ uint4 sB[2][TS * 2 * 4 * KB];      // [buf][tt][piece][g][col]
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t row0 = (int64_t)blockIdx.x * (64 * RT) + wave * (16 * RT);
+  const int col0 = blockIdx.y * KB;
+  const int64_t s0 = (int64_t)blockIdx.z * a.ns_per_split;
+  const int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
+  const int64_t ngrp = (s1 - s0) * GPS;
+  const uint4* X = reinterpret_cast<const uint4*>(a.X);
+
+  f32x4_t acc[RT][NT];
+#pragma unroll
+  for (int i = 0; i < RT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+
+  uint4 stage[NSTAGE];
+  auto load_group = [&](int64_t q) {
+#pragma unroll
+    for (int e = 0; e < NSTAGE; ++e) {
+      const int L = tid + 256 * e;
+      const int col = L % KB;
+      int rest = L / KB;
+      const int gg = rest & 3;
+      rest >>= 2;
+      const int piece = rest & 1;
+      const int tt = rest >> 1;
+      const int64_t qs = q * TS + tt;
+      const int64_t s = s0 + qs / 16;
+      const int64_t blk = s * 64 + gg * 16 + (qs % 16);
+      stage[e] = X[piece * a.lo_offset16 + blk * a.KP + col0 + col];
+    }
+  };
+  auto store_group = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < NSTAGE; ++e) sB[buf][tid + 256 * e] = stage[e];
+  };
+
+  const uint8_t* gbase = a.G + (row0 + r) * a.ldg + g * 32;
+  uint32_t pk[RT][8], pn[PREF ? RT : 1][8];
+  auto load_codes = [&](int64_t s, auto& dst) {
+#pragma unroll
+    for (int i = 0; i < RT; ++i) {
+      const u32x4_t* src = reinterpret_cast<const u32x4_t*>(gbase + (int64_t)i * 16 * a.ldg + s * 128);
+      const u32x4_t q0 = __builtin_nontemporal_load(src);
+      const u32x4_t q1 = __builtin_nontemporal_load(src + 1);
+      dst[i][0] = q0.x; dst[i][1] = q0.y; dst[i][2] = q0.z; dst[i][3] = q0.w;
+      dst[i][4] = q1.x; dst[i][5] = q1.y; dst[i][6] = q1.z; dst[i][7] = q1.w;
+    }
+  };
+
+  if (ngrp > 0) {
+    load_group(0);
+    store_group(0);
+    load_codes(s0, pk);
+  }
+  for (int64_t q = 0; q < ngrp; ++q) {
+    __syncthreads();
+    const int buf = (int)(q & 1);
+    if (q + 1 < ngrp) load_group(q + 1);
+    const int gq = (int)(q % GPS);
+    if constexpr (PREF) {
+      if (gq == 0 && q + GPS < ngrp) load_codes(s0 + q / GPS + 1, pn);
+    } else {
+      if (gq == 0 && q > 0) load_codes(s0 + q / GPS, pk);
+    }
+    const uint4* sb = sB[buf];
+#pragma unroll
+    for (int tt = 0; tt < TS; ++tt) {
+      const int t = gq * TS + tt;
+      bf16x8_t av[RT];
+#pragma unroll
+      for (int i = 0; i < RT; ++i) av[i] = unpack8<SQ>((pk[i][t >> 1] >> (16 * (t & 1))) & 0xFFFFu);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, sb[((tt * 2 + 0) * 4 + g) * KB + j * 16 + r]);
+        const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, sb[((tt * 2 + 1) * 4 + g) * KB + j * 16 + r]);
+#pragma unroll
+        for (int i = 0; i < RT; ++i) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bh, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl, acc[i][j], 0, 0, 0);
+        }
+      }
+    }
+    if constexpr (PREF) {
+      if (gq == GPS - 1 && q + 1 < ngrp) {
+#pragma unroll
+        for (int i = 0; i < RT; ++i)
+#pragma unroll
+          for (int w = 0; w < 8; ++w) pk[i][w] = pn[i][w];
+      }
+    }
+    if (q + 1 < ngrp) store_group(buf ^ 1);
+  }
+
+  // raw partial products, column-major per split: part[(z*KP + col) * rows_pad + row]
+#pragma unroll
+  for (int i = 0; i < RT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int64_t col = col0 + 16 * j + r;
+      const int64_t row = row0 + 16 * i + 4 * g;
+      float4* dst = reinterpret_cast<float4*>(a.part + ((int64_t)blockIdx.z * a.KP + col) * a.rows_pad + row);
+      *dst = make_float4(acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]);
+    }
+}
+
+// Register-only variant: every wave reads its B fragments straight from global memory (L2/MALL
+// hits after the first wave of a workgroup wave-front); no LDS, no barriers, lower register
+// use, so two waves fit per SIMD.  Same split-K partial output as k_bp_lds.
+template <int RT, int NT, bool SQ>
+__global__ __launch_bounds__(256) void k_bp_reg(LdsArgs a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t row0 = (int64_t)blockIdx.x * (64 * RT) + wave * (16 * RT);
+  const int col0 = blockIdx.y * (16 * NT);
+  const int64_t s0 = (int64_t)blockIdx.z * a.ns_per_split;
+  const int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
+  const uint4* xh = reinterpret_cast<const uint4*>(a.X);
+  const uint4* xl = xh + a.lo_offset16;
+  f32x4_t acc[RT][NT];
+#pragma unroll
+  for (int i = 0; i < RT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  const uint8_t* gbase = a.G + (row0 + r) * a.ldg + g * 32;
+  for (int64_t s = s0; s < s1; ++s) {
+    uint32_t pk[RT][8];
+#pragma unroll
+    for (int i = 0; i < RT; ++i) {
+      const u32x4_t* src = reinterpret_cast<const u32x4_t*>(gbase + (int64_t)i * 16 * a.ldg + s * 128);
+      const u32x4_t q0 = __builtin_nontemporal_load(src);
+      const u32x4_t q1 = __builtin_nontemporal_load(src + 1);
+      pk[i][0] = q0.x; pk[i][1] = q0.y; pk[i][2] = q0.z; pk[i][3] = q0.w;
+      pk[i][4] = q1.x; pk[i][5] = q1.y; pk[i][6] = q1.z; pk[i][7] = q1.w;
+    }
+    const int64_t blk0 = s * 64 + g * 16;
+#pragma unroll 4
+    for (int t = 0; t < 16; ++t) {
+      bf16x8_t av[RT];
+#pragma unroll
+      for (int i = 0; i < RT; ++i) av[i] = unpack8<SQ>((pk[i][t >> 1] >> (16 * (t & 1))) & 0xFFFFu);
+      const int64_t boff = (blk0 + t) * a.KP + col0 + r;
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, xh[boff + j * 16]);
+        const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, xl[boff + j * 16]);
+#pragma unroll
+        for (int i = 0; i < RT; ++i) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bh, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl, acc[i][j], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < RT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int64_t col = col0 + 16 * j + r;
+      const int64_t row = row0 + 16 * i + 4 * g;
+      float4* dst = reinterpret_cast<float4*>(a.part + ((int64_t)blockIdx.z * a.KP + col) * a.rows_pad + row);
+      *dst = make_float4(acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]);
+    }
+}
+
+// Pass-1 epilogue from the split partials: T_jk = w_j (P_jk - gm_j s_k + xbar_k (pc_j - n gm_j));
+// writes the BF16 hi/lo pass-2 operand, optionally the column-major result, and per-block
+// partial sums of gm_j * represented(T_jk).  Thread (tx = column, ty): 8 blocks of 8 rows.
+__global__ __launch_bounds__(256) void k_fin1(Fin1Args a) {
+  __shared__ double red[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int k = blockIdx.x * 64 + tx;
+  const int64_t rbase = (int64_t)blockIdx.y * 256 + ty * 64;
+  double usum = 0.0;
+  if (k < a.KP) {
+    const double sk = a.colsum[k];
+    const double xb = (a.xsum && k < a.t) ? (double)(float)(a.xsum[k] / a.xsum_n) : 0.0;
+    for (int b8 = 0; b8 < 8; ++b8) {
+      const int64_t i0 = rbase + b8 * 8;
+      float P[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int z = 0; z < a.nsplit; ++z) {
+        const float4* src = reinterpret_cast<const float4*>(a.part + ((int64_t)z * a.KP + k) * a.rows_pad + i0);
+        const float4 p0 = src[0], p1 = src[1];
+        P[0] += p0.x; P[1] += p0.y; P[2] += p0.z; P[3] += p0.w;
+        P[4] += p1.x; P[5] += p1.y; P[6] += p1.z; P[7] += p1.w;
+      }
+      uint32_t hb[8], lb[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int64_t row = i0 + e;
+        const float gmv = a.gm[row];
+        const float wv = a.gw ? a.gw[row] : 1.0f;
+        const double rem = xb * ((double)a.popcnt[row] - a.xsum_n * (double)gmv);
+        const float tv = a.raw ? P[e] : wv * (float)((double)P[e] - (double)gmv * sk + rem);
+        hb[e] = bf16_bits_rn(tv);
+        lb[e] = bf16_bits_rn(tv - bf16_to_float(hb[e]));
+        usum += (double)gmv * ((double)bf16_to_float(hb[e]) + (double)bf16_to_float(lb[e]));
+        if (a.out && k < a.t && row < a.rows) a.out[row + (int64_t)k * a.ldo] = tv;
+      }
+      if (a.X2) {
+        uint4* x2 = reinterpret_cast<uint4*>(a.X2);
+        const int64_t idx = (i0 >> 3) * a.KP + k;
+        x2[idx] = make_uint4(hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16), hb[6] | (hb[7] << 16));
+        x2[idx + a.lo_offset16] =
+            make_uint4(lb[0] | (lb[1] << 16), lb[2] | (lb[3] << 16), lb[4] | (lb[5] << 16), lb[6] | (lb[7] << 16));
+      }
+    }
+  }
+  red[ty][tx] = usum;
+  __syncthreads();
+  if (ty == 0 && k < a.KP && a.upart)
+    a.upart[(int64_t)blockIdx.y * a.KP + k] = (red[0][tx] + red[1][tx]) + (red[2][tx] + red[3][tx]);
+}
+
+// Pass-2 epilogue: out_ik = rr_i (sum_z P_ik - u_k) (+ tau0 v_ik for LDET columns).
+__global__ __launch_bounds__(256) void k_fin2(Fin2Args a) {
+  const int k = blockIdx.y;
+  const int64_t i = 4 * ((int64_t)blockIdx.x * 256 + threadIdx.x);
+  if (i >= a.rows || k >= a.t) return;
+  float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int z = 0; z < a.nsplit; ++z) {
+    const float4 p = *reinterpret_cast<const float4*>(a.part + ((int64_t)z * a.KP + k) * a.rows_pad + i);
+    P.x += p.x; P.y += p.y; P.z += p.z; P.w += p.w;
+  }
+  const float Pv[4] = {P.x, P.y, P.z, P.w};
+  const double uk = a.colsum[k];
+  const bool add_tau = a.v && (a.kinds == nullptr || a.kinds[k] == 0);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int64_t row = i + e;
+    if (row >= a.rows) break;
+    float zv = (float)((double)Pv[e] - uk);
+    if (a.rr) zv *= a.rr[row];
+    if (add_tau) zv += a.tau0 * a.v[row + (int64_t)k * a.ldv];
+    a.out[row + (int64_t)k * a.ldo] = zv;
+  }
+}
+
+// ---------------------------------------------------------------------------- launchers -----
+int lds_nt(int t) {
+  const int nt = (t + 15) / 16;
+  return nt < 8 ? nt : 8;
+}
+
+int lds_splits(int64_t rows_pad, int64_t nsuper, int64_t target_wgs, int64_t& ns_per) {
+  const int64_t wgs = rows_pad / (64 * kLdsRT);  // 256-row workgroups
+  int64_t S = (target_wgs + wgs - 1) / wgs;
+  if (S < 1) S = 1;
+  if (S > nsuper) S = nsuper;
+  ns_per = (nsuper + S - 1) / S;
+  return (int)((nsuper + ns_per - 1) / ns_per);
+}
+
+template <int NT, bool SQ>
+static void launch_variant(const LdsArgs& a, int variant, dim3 grid, hipStream_t st) {
+  switch (variant) {
+    case BP_REG: k_bp_reg<kLdsRT, NT, SQ><<<grid, 256, 0, st>>>(a); break;
+    case BP_REG8:
+      grid.x /= 2;  // 8 row tiles per wave: 512 rows per workgroup
+      if constexpr (NT <= 5) k_bp_reg<8, NT, SQ><<<grid, 256, 0, st>>>(a);
+      else k_bp_reg<kLdsRT, NT, SQ><<<dim3(grid.x * 2, grid.y, grid.z), 256, 0, st>>>(a);
+      break;
+    case BP_LDS4: k_bp_lds<kLdsRT, NT, 4, true, SQ><<<grid, 256, 0, st>>>(a); break;
+    case BP_LDS2: k_bp_lds<kLdsRT, NT, 2, false, SQ><<<grid, 256, 0, st>>>(a); break;
+    default: fail(SLEMM_ERR_ARG, "unknown block-product variant");
+  }
+}
+
+void launch_bp(const LdsArgs& a, int NT, int nsplit, int variant, hipStream_t st) {
+  if (a.rows_pad % (64 * kLdsRT) != 0) fail(SLEMM_ERR_ARG, "row padding mismatch");
+  if (a.KP % (16 * NT) != 0) fail(SLEMM_ERR_ARG, "column padding mismatch");
+  dim3 grid((unsigned)(a.rows_pad / (64 * kLdsRT)), (unsigned)(a.KP / (16 * NT)), (unsigned)nsplit);
+  if (a.squared) {
+    if (NT != 1) fail(SLEMM_ERR_ARG, "squared-dosage products support one column tile");
+    launch_variant<1, true>(a, variant, grid, st);
+  } else {
+    switch (NT) {
+      case 1: launch_variant<1, false>(a, variant, grid, st); break;
+      case 2: launch_variant<2, false>(a, variant, grid, st); break;
+      case 3: launch_variant<3, false>(a, variant, grid, st); break;
+      case 4: launch_variant<4, false>(a, variant, grid, st); break;
+      case 5: launch_variant<5, false>(a, variant, grid, st); break;
+      case 6: launch_variant<6, false>(a, variant, grid, st); break;
+      case 7: launch_variant<7, false>(a, variant, grid, st); break;
+      case 8: launch_variant<8, false>(a, variant, grid, st); break;
+      default: fail(SLEMM_ERR_ARG, "unsupported column tile count");
+    }
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_fin1(const Fin1Args& a, hipStream_t st) {
+  dim3 grid((unsigned)((a.KP + 63) / 64), (unsigned)(a.rows_pad / 256));
+  k_fin1<<<grid, 256, 0, st>>>(a);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_fin2(const Fin2Args& a, hipStream_t st) {
+  dim3 grid((unsigned)((a.rows + 1023) / 1024), (unsigned)a.t);
+  k_fin2<<<grid, 256, 0, st>>>(a);
+  HIPCHECK(hipGetLastError());
+}
+
+}  // namespace slemm

@@ -93,7 +93,12 @@ struct slemm_geno {
   slemm::DevBuf tmp;           // n_pad x t float
   slemm::DevBuf kinds;         // per-column operator kinds
   slemm::DevBuf redpart;       // partials of column reductions (double)
+  slemm::DevBuf partbuf;       // split-reduction partial products of the block products
   std::vector<int> kinds_host;
+  // block-product variant and grid target per (pass, column tiles), set by the autotuner
+  int bp_variant[2][9] = {{0, 0, 0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  int bp_target[2][9] = {{1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024},
+                         {1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024}};
   std::vector<float> rr_host;    // r_rsqrt (n)
   std::vector<float> rvec_host;  // R (n)
 };

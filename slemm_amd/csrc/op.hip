@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -56,7 +57,7 @@ const int* upload_kinds(slemm_geno* h, int kind, const int* col_kind, int t, boo
 // column means removed.  colsum layout: [0,KP) sums of the represented centred operand,
 // [KP,2KP) pass-2 centring sums, [2KP,3KP) sums of the un-centred operand.
 void pack_pass1(slemm_geno* h, const Layout& L, const float* V, int64_t ldv, int t, const float* rr,
-                const double* qtv, const int* kinds, int precision, hipStream_t st) {
+                const double* qtv, const int* kinds, int precision, hipStream_t st, bool center = true) {
   h->x1.reserve(L.x1_bytes);
   const int64_t nparts = h->n_pad / 256;
   h->part1.reserve(sizeof(double) * nparts * L.KP);
@@ -66,10 +67,12 @@ void pack_pass1(slemm_geno* h, const Layout& L, const float* V, int64_t ldv, int
   p.V = V; p.ldv = ldv; p.t = t; p.KP = L.KP; p.n = h->n; p.rows_pad = h->n_pad;
   p.rr = rr; p.Q = h->Q; p.ldq = h->n_pad; p.c = h->c; p.qtv = qtv; p.kinds = kinds;
   p.mode = precision; p.X = nullptr; p.lo_offset16 = L.lo1_16; p.part = h->part1.as<double>();
-  launch_pack(p, st);  // sums of the un-centred operand
-  launch_sum_parts(h->part1.as<double>(), nparts, L.KP, xsum, st);
+  if (center) {
+    launch_pack(p, st);  // sums of the un-centred operand
+    launch_sum_parts(h->part1.as<double>(), nparts, L.KP, xsum, st);
+    p.xsum = xsum;
+  }
   p.X = h->x1.p;
-  p.xsum = xsum;
   launch_pack(p, st);  // centred operand and its represented column sums
   launch_sum_parts(h->part1.as<double>(), nparts, L.KP, h->colsum.as<double>(), st);
 }
@@ -78,6 +81,75 @@ void centring_args(slemm_geno* h, const Layout& L, BlockProdArgs& a) {
   a.xsum = h->colsum.as<double>() + 2 * L.KP;
   a.xsum_n = (double)h->n;
   a.popcnt = h->popcnt;
+}
+
+// Split-BF16 products run on the split-reduction block-product kernels (variant and grid size
+// per shape from the autotuner); SLEMM_BLOCKPROD_V1=1 selects the fused register kernel of
+// blockprod.hip instead.  SLEMM_BP_VARIANT / SLEMM_BP_TARGET force a variant / grid target.
+bool use_lds(int precision) {
+  static const bool v1 = std::getenv("SLEMM_BLOCKPROD_V1") != nullptr;
+  return precision == PREC_BF16X2 && !v1;
+}
+
+struct BpChoice {
+  int variant;
+  int64_t target;
+};
+
+BpChoice bp_choice(const slemm_geno* h, int pass, int NT) {
+  static const char* ev = std::getenv("SLEMM_BP_VARIANT");
+  static const char* et = std::getenv("SLEMM_BP_TARGET");
+  BpChoice c{h->bp_variant[pass][NT], h->bp_target[pass][NT]};
+  if (ev) c.variant = std::atoi(ev);
+  if (et) c.target = std::atoll(et);
+  return c;
+}
+
+int64_t pass_splits(const slemm_geno* h, int pass, int NT, int64_t& nsp) {
+  const BpChoice c = bp_choice(h, pass, NT);
+  return pass == 0 ? lds_splits(h->m_pad, h->n_pad / kPad, c.target, nsp)
+                   : lds_splits(h->n_pad, h->m_pad / kPad, c.target, nsp);
+}
+
+// Z'X1 pass, epilogue T = W (Zc'x) -> pass-2 operand x2 / out / upart.
+void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, double* upart, float* out,
+               int64_t ldo, hipStream_t st, bool raw = false, bool squared = false) {
+  int64_t nsp = 0;
+  const int64_t nsuper = h->n_pad / kPad;
+  const int S = (int)pass_splits(h, 0, L.NT, nsp);
+  h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * h->m_pad);
+  LdsArgs la;
+  la.G = h->gs; la.ldg = h->ld_gs; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x1.p;
+  la.lo_offset16 = L.lo1_16; la.KP = L.KP; la.rows_pad = h->m_pad; la.part = h->partbuf.as<float>();
+  la.squared = squared;
+  TimingToken tok = timing_begin(st);
+  launch_bp(la, L.NT, S, bp_choice(h, 0, L.NT).variant, st);
+  timing_end(tok, st, EPI_PASS1, PREC_BF16X2);
+  Fin1Args f;
+  f.part = h->partbuf.as<float>(); f.nsplit = S; f.rows_pad = h->m_pad; f.rows = h->m; f.KP = L.KP; f.t = t;
+  f.colsum = h->colsum.as<double>(); f.xsum = h->colsum.as<double>() + 2 * L.KP; f.xsum_n = (double)h->n;
+  f.popcnt = h->popcnt; f.gm = h->gm; f.gw = weighted ? h->gw : nullptr; f.X2 = x2; f.lo_offset16 = L.lo2_16;
+  f.upart = upart; f.out = out; f.ldo = ldo; f.raw = raw;
+  launch_fin1(f, st);
+}
+
+// Z X2 pass, epilogue out = rr (P - u) (+ tau0 v for LDET columns).
+void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const float* rr, const float* V,
+               int64_t ldv, float tau0, const int* kinds, float* out, int64_t ldo, hipStream_t st) {
+  int64_t nsp = 0;
+  const int64_t nsuper = h->m_pad / kPad;
+  const int S = (int)pass_splits(h, 1, L.NT, nsp);
+  h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * h->n_pad);
+  LdsArgs la;
+  la.G = h->gi; la.ldg = h->ld_gi; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x2.p;
+  la.lo_offset16 = L.lo2_16; la.KP = L.KP; la.rows_pad = h->n_pad; la.part = h->partbuf.as<float>();
+  TimingToken tok = timing_begin(st);
+  launch_bp(la, L.NT, S, bp_choice(h, 1, L.NT).variant, st);
+  timing_end(tok, st, EPI_PASS2, PREC_BF16X2);
+  Fin2Args f;
+  f.part = h->partbuf.as<float>(); f.nsplit = S; f.rows_pad = h->n_pad; f.rows = h->n; f.KP = L.KP; f.t = t;
+  f.colsum = u; f.rr = rr; f.v = V; f.ldv = ldv; f.tau0 = tau0; f.kinds = kinds; f.out = out; f.ldo = ldo;
+  launch_fin2(f, st);
 }
 
 void check_dense(const float* p, int64_t ld, int64_t rows, int t, const char* what) {
@@ -110,6 +182,21 @@ void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const floa
 
   // pass 1: T = W (Z' X1 - gm s'), written as the pass-2 operand
   h->x2.reserve(L.x2_bytes);
+  if (use_lds(precision)) {
+    const int64_t nparts = h->m_pad / 256;
+    h->part2.reserve(sizeof(double) * nparts * L.KP);
+    {  // size the split-partial buffer for both passes before anything is in flight
+      int64_t nsp = 0;
+      const int64_t S1 = pass_splits(h, 0, L.NT, nsp);
+      const int64_t S2 = pass_splits(h, 1, L.NT, nsp);
+      h->partbuf.reserve(sizeof(float) * L.KP *
+                         std::max<size_t>((size_t)S1 * h->m_pad, (size_t)S2 * h->n_pad));
+    }
+    pass1_lds(h, L, t, true, h->x2.p, h->part2.as<double>(), nullptr, 0, st);
+    double* u = h->colsum.as<double>() + L.KP;
+    launch_sum_parts(h->part2.as<double>(), nparts, L.KP, u, st);
+    pass2_lds(h, L, t, u, h->rr, V, ldv, tau0, kinds, out, ldo, st);
+  } else {
   const int64_t nwg1 = h->m_pad / blockprod_rows_per_wg(L.NT);
   h->part2.reserve(sizeof(double) * nwg1 * L.KP);
   BlockProdArgs a;
@@ -128,6 +215,7 @@ void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const floa
   b.KP = L.KP; b.t = t; b.rows = h->n; b.colsum = u; b.out = out; b.ldo = ldo;
   b.rr = h->rr; b.v = V; b.ldv = ldv; b.tau0 = tau0; b.kinds = kinds;
   launch_blockprod(b, precision, EPI_PASS2, L.NT, h->n_pad, st);
+  }
 
   if (any_quad) {
     double* bq = nullptr;
@@ -170,9 +258,136 @@ __global__ __launch_bounds__(64) void k_gebv(const uint8_t* __restrict__ gs, int
     if (i0 + q < n) out[i0 + q] = acc[q];
 }
 
+// GRAMMAR-gamma score columns: [Py, rr o Q_1..c, rr^2] (n x (2+c), column-major, ld n).
+__global__ void k_gwas_cols(const float* __restrict__ Py, const float* __restrict__ rr, const float* __restrict__ Q,
+                            int64_t ldq, int c, int64_t n, float* __restrict__ V) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float r = rr[i];
+  V[i] = Py[i];
+  for (int q = 0; q < c; ++q) V[i + (int64_t)(1 + q) * n] = r * Q[i + (int64_t)q * ldq];
+  V[i + (int64_t)(1 + c) * n] = r * r;
+}
+
+// xy_j = x_c'Py and xx_j = ||(I-QQ')R^-1/2 x_c||^2 = sum rr^2 (d-mu)^2 - ||Q'(rr o x_c)||^2 with
+// sum rr^2 (d-mu)^2 = S2 - 2 mu C1 - mu^2 R2  (S2 = sum rr^2 d^2, C1 = Zc'(rr^2), R2 = sum rr^2).
+__global__ void k_gwas_combine(const float* __restrict__ A, int64_t lda, int c, const float* __restrict__ S2,
+                               const float* __restrict__ gm, const int32_t* __restrict__ pc, int64_t m, int64_t n,
+                               double R2, float mac, float* __restrict__ xy, float* __restrict__ xx) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const float s = (float)pc[j];
+  if (s <= mac || 2.0f * (float)n - s <= mac) {
+    xy[j] = __builtin_nanf("");
+    xx[j] = __builtin_nanf("");
+    return;
+  }
+  const double mu = gm[j];
+  double proj = 0.0;
+  for (int q = 0; q < c; ++q) {
+    const double a = A[j + (int64_t)(1 + q) * lda];
+    proj += a * a;
+  }
+  const double C1 = A[j + (int64_t)(1 + c) * lda];
+  xy[j] = A[j];
+  xx[j] = (float)((double)S2[j] - 2.0 * mu * C1 - mu * mu * R2 - proj);
+}
+
 }  // namespace slemm
 
 using namespace slemm;
+
+extern "C" int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, int* target, double* best_ms) {
+  return guard([&] {
+    REQUIRE(h, "null handle");
+    REQUIRE(t > 0 && reps > 0, "t and reps must be positive");
+    HIPCHECK(hipSetDevice(h->device));
+    const Layout L = layout_for(h, t, PREC_BF16X2);
+    h->x1.reserve(L.x1_bytes);
+    h->x2.reserve(L.x2_bytes);
+    HIPCHECK(hipMemset(h->x1.p, 0, L.x1_bytes));
+    HIPCHECK(hipMemset(h->x2.p, 0, L.x2_bytes));
+    const int64_t targets[] = {512, 1024, 2048};
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    for (int pass = 0; pass < 2; ++pass) {
+      const int64_t rows_pad = pass == 0 ? h->m_pad : h->n_pad;
+      const int64_t nsuper = (pass == 0 ? h->n_pad : h->m_pad) / kPad;
+      double best = 1e300;
+      int bv = 0;
+      int64_t bt = 1024;
+      for (int v = 0; v < BP_NVARIANTS; ++v)
+        for (int ti = 0; ti < 3; ++ti) {
+          const int64_t tg = targets[ti];
+          int64_t nsp = 0;
+          const int S = lds_splits(rows_pad, nsuper, tg, nsp);
+          h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * rows_pad);
+          LdsArgs la;
+          la.G = pass == 0 ? h->gs : h->gi;
+          la.ldg = pass == 0 ? h->ld_gs : h->ld_gi;
+          la.nsuper = nsuper; la.ns_per_split = nsp; la.X = pass == 0 ? h->x1.p : h->x2.p;
+          la.lo_offset16 = pass == 0 ? L.lo1_16 : L.lo2_16; la.KP = L.KP; la.rows_pad = rows_pad;
+          la.part = h->partbuf.as<float>();
+          launch_bp(la, L.NT, S, v, nullptr);  // warm-up
+          HIPCHECK(hipEventRecord(e0, nullptr));
+          for (int r = 0; r < reps; ++r) launch_bp(la, L.NT, S, v, nullptr);
+          HIPCHECK(hipEventRecord(e1, nullptr));
+          HIPCHECK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+          const double avg = ms / reps;
+          if (best_ms) best_ms[2 + pass * (BP_NVARIANTS * 3) + v * 3 + ti] = avg;
+          if (avg < best) {
+            best = avg;
+            bv = v;
+            bt = tg;
+          }
+        }
+      h->bp_variant[pass][L.NT] = bv;
+      h->bp_target[pass][L.NT] = (int)bt;
+      if (variant) variant[pass] = bv;
+      if (target) target[pass] = (int)bt;
+      if (best_ms) best_ms[pass] = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}
+
+extern "C" int slemm_gpu_gwas_score(slemm_geno* h, const float* Py, float skip_mac, float* xy, float* xx,
+                                    void* stream) {
+  return guard([&] {
+    REQUIRE(h && Py && xy && xx, "null argument");
+    HIPCHECK(hipSetDevice(h->device));
+    require_model(h);
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = h->n, m = h->m;
+    const int c = h->c;
+    const int t = 2 + c;
+    DevBuf cols, A, B;
+    cols.reserve(sizeof(float) * n * t);
+    A.reserve(sizeof(float) * m * t);
+    B.reserve(sizeof(float) * m);
+    k_gwas_cols<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(Py, h->rr, h->Q, h->n_pad, c, n, cols.as<float>());
+    HIPCHECK(hipGetLastError());
+    // centred products Zc'[Py, rr o Q, rr^2]
+    const Layout LA = layout_for(h, t, PREC_BF16X2);
+    pack_pass1(h, LA, cols.as<float>(), n, t, nullptr, nullptr, nullptr, PREC_BF16X2, st);
+    pass1_lds(h, LA, t, false, nullptr, nullptr, A.as<float>(), m, st);
+    // raw squared-dosage product Z2'(rr^2)
+    const Layout LB = layout_for(h, 1, PREC_BF16X2);
+    pack_pass1(h, LB, cols.as<float>() + (size_t)(1 + c) * n, n, 1, nullptr, nullptr, nullptr, PREC_BF16X2, st,
+               false);
+    pass1_lds(h, LB, 1, false, nullptr, nullptr, B.as<float>(), m, st, true, true);
+    double R2 = 0.0;
+    for (int64_t i = 0; i < n; ++i) R2 += (double)h->rr_host[i] * h->rr_host[i];
+    k_gwas_combine<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(A.as<float>(), m, c, B.as<float>(), h->gm,
+                                                                h->popcnt, m, n, R2, skip_mac, xy, xx);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+  });
+}
 
 extern "C" int slemm_gpu_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const float* V,
                             int64_t ldv, int t, float* out, int64_t ldo, int precision, void* stream) {
@@ -196,6 +411,10 @@ extern "C" int slemm_gpu_zt(slemm_geno* h, const float* V, int64_t ldv, int t, f
     if (weighted || scale_rr) require_model(h);
     const Layout L = layout_for(h, t, precision);
     pack_pass1(h, L, V, ldv, t, scale_rr ? h->rr : nullptr, nullptr, nullptr, precision, st);
+    if (use_lds(precision)) {
+      pass1_lds(h, L, t, weighted != 0, nullptr, nullptr, out, ldo, st);
+      return;
+    }
     BlockProdArgs a;
     a.G = h->gs; a.ldg = h->ld_gs; a.nsuper = h->n_pad / kPad; a.X = h->x1.p; a.lo_offset16 = L.lo1_16;
     a.KP = L.KP; a.t = t; a.rows = h->m; a.colsum = h->colsum.as<double>();
@@ -228,6 +447,10 @@ extern "C" int slemm_gpu_z(slemm_geno* h, const float* U, int64_t ldu, int t, fl
     launch_pack(p, st);
     double* u = h->colsum.as<double>() + L.KP;
     launch_sum_parts(h->part2.as<double>(), nparts, L.KP, u, st);
+    if (use_lds(precision)) {
+      pass2_lds(h, L, t, u, nullptr, nullptr, 0, 0.f, nullptr, out, ldo, st);
+      return;
+    }
     BlockProdArgs b;
     b.G = h->gi; b.ldg = h->ld_gi; b.nsuper = h->m_pad / kPad; b.X = h->x2.p; b.lo_offset16 = L.lo2_16;
     b.KP = L.KP; b.t = t; b.rows = h->n; b.colsum = u; b.out = out; b.ldo = ldo;

@@ -450,7 +450,3 @@ extern "C" int slemm_gpu_lmc_reml(slemm_geno* h, const float* X, int64_t ldx, in
     }
   });
 }
-
-extern "C" int slemm_gpu_gwas_score(slemm_geno*, const float*, float, float*, float*, void*) {
-  return guard([&] { fail(SLEMM_ERR_STATE, "slemm_gpu_gwas_score: not available in this build"); });
-}

@@ -205,6 +205,18 @@ class Geno:
         L.check(L.load().slemm_gpu_gebv(self._h, _ptr(a), _np_ptr(fl, C.c_uint8), _ptr(out), _stream()))
         return out
 
+    def autotune(self, t: int, reps: int = 2) -> dict:
+        """Times the block-product variants for a t-column operator and keeps the fastest."""
+        v = (C.c_int * 2)()
+        tg = (C.c_int * 2)()
+        ms = (C.c_double * 26)(*([float("nan")] * 26))
+        L.check(L.load().slemm_gpu_autotune(self._h, t, reps, v, tg, ms))
+        names = ["reg", "lds4", "lds2", "reg8"]
+        table = {f"pass{p + 1}/{names[vv]}/{[512, 1024, 2048][ti]}": round(ms[2 + p * 12 + vv * 3 + ti], 3)
+                 for p in range(2) for vv in range(4) for ti in range(3)}
+        return {"pass1": (names[v[0]], tg[0], round(ms[0], 3)), "pass2": (names[v[1]], tg[1], round(ms[1], 3)),
+                "table_ms": table}
+
     # ------------------------------------------------------------------ REML drivers
     def SLDF_REML(self, X: np.ndarray, y: np.ndarray, rng_seed: int = 0, n_V: int = 30, tol_L: float = 5e-4,
                   s2max: float = 0.7, s2min: float = 1e-4, tol_VC: float = 1e-4, precision="bf16x2",

