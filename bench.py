@@ -75,6 +75,39 @@ def cpu_baseline(g, n, m, K, m_sample, reps):
     }
 
 
+def reml_time_to_converge(g, n, m, nv, n_causal, h2):
+    """SLDF-REML (src/reml.cpp:549-690) on the resident bed for a phenotype simulated from it:
+    y = Zc beta + e, beta ~ N(0,1) on n_causal random SNPs, var(e) set for the target h2.
+    Time from the resident bed to the final variance components, split-BF16 and FP32."""
+    import numpy as np
+    import torch
+
+    rng = np.random.default_rng(11)
+    beta = np.zeros(m, np.float32)
+    beta[rng.choice(m, size=n_causal, replace=False)] = rng.normal(size=n_causal).astype(np.float32)
+    gv = g.matvec(torch.from_numpy(beta).cuda(), precision="fp32")[:, 0].double().cpu().numpy()
+    ve = gv.var() * (1 - h2) / h2
+    y = (gv + rng.normal(scale=np.sqrt(ve), size=n) + 10.0).astype(np.float32)
+    X = np.ones((n, 1), np.float32)
+    out = {"workload": f"SLDF-REML, {nv} Rademacher probes, simulated h2={h2} with {n_causal} causal SNPs",
+           "realised_h2": round(float(gv.var() / (gv.var() + ve)), 5)}
+    res = {}
+    for prec in ("bf16x2", "fp32"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = g.SLDF_REML(X, y, rng_seed=0, n_V=nv, precision=prec)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        res[prec] = r
+        out[prec] = {"seconds": round(wall, 3), "seconds_seed": round(r["seconds_seed"], 3),
+                     "lanczos_steps": list(r["steps"]), "h2": round(float(r["h2"]), 6),
+                     "vg": float(r["vg"]), "ve": float(r["ve"]), "lrt": float(r["lrt"])}
+    out["speedup_bf16x2_over_fp32"] = round(out["fp32"]["seconds"] / out["bf16x2"]["seconds"], 2)
+    out["vc_rel_diff_bf16x2_vs_fp32"] = {
+        k: float(abs(res["bf16x2"][k] - res["fp32"][k]) / abs(res["fp32"][k])) for k in ("vg", "ve", "h2")}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +119,9 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=2)
     ap.add_argument("--cpu-snps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-reml", action="store_true")
+    ap.add_argument("--causal", type=int, default=5000)
+    ap.add_argument("--h2", type=float, default=0.3)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -155,6 +191,8 @@ def main():
     f2, _ = kernel_ms(lib, 1, L.PREC_FP32)
     lib.slemm_gpu_timing_reset()
 
+    reml = None if args.no_reml else reml_time_to_converge(g, n, m, nv, args.causal, args.h2)
+
     pb = pass_bytes(n, m, K)
     value = 2 * pb / dt / 1e9
     flops = 2.0 * n * m * K
@@ -183,6 +221,7 @@ def main():
                       "speedup_bf16x2_over_fp32": round(dt32 / dt, 2)},
         "bed_generate_s": round(t_load, 3),
         "autotune": tune,
+        "reml_time_to_converge": reml,
         "roofline": {
             "bound": "mfma",
             "kernel": f"k_blockprod bf16x2 {dom}",

@@ -127,16 +127,17 @@ __global__ __launch_bounds__(256) void k_bp_lds(LdsArgs a) {
       bf16x8_t av[RT];
 #pragma unroll
       for (int i = 0; i < RT; ++i) av[i] = unpack8<SQ>((pk[i][t >> 1] >> (16 * (t & 1))) & 0xFFFFu);
+      // all hi products of the step, then all lo products: RT*NT independent MFMAs separate
+      // the two updates of one accumulator
 #pragma unroll
-      for (int j = 0; j < NT; ++j) {
-        const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, sb[((tt * 2 + 0) * 4 + g) * KB + j * 16 + r]);
-        const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, sb[((tt * 2 + 1) * 4 + g) * KB + j * 16 + r]);
+      for (int piece = 0; piece < 2; ++piece)
 #pragma unroll
-        for (int i = 0; i < RT; ++i) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bh, acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl, acc[i][j], 0, 0, 0);
+        for (int j = 0; j < NT; ++j) {
+          const bf16x8_t b = __builtin_bit_cast(bf16x8_t, sb[((tt * 2 + piece) * 4 + g) * KB + j * 16 + r]);
+#pragma unroll
+          for (int i = 0; i < RT; ++i)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], b, acc[i][j], 0, 0, 0);
         }
-      }
     }
     if constexpr (PREF) {
       if (gq == GPS - 1 && q + 1 < ngrp) {
@@ -197,16 +198,23 @@ __global__ __launch_bounds__(256) void k_bp_reg(LdsArgs a) {
 #pragma unroll
       for (int i = 0; i < RT; ++i) av[i] = unpack8<SQ>((pk[i][t >> 1] >> (16 * (t & 1))) & 0xFFFFu);
       const int64_t boff = (blk0 + t) * a.KP + col0 + r;
+      bf16x8_t bh[NT], bl[NT];
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
-        const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, xh[boff + j * 16]);
-        const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, xl[boff + j * 16]);
-#pragma unroll
-        for (int i = 0; i < RT; ++i) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bh, acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl, acc[i][j], 0, 0, 0);
-        }
+        bh[j] = __builtin_bit_cast(bf16x8_t, xh[boff + j * 16]);
+        bl[j] = __builtin_bit_cast(bf16x8_t, xl[boff + j * 16]);
       }
+      // all hi products of the step, then all lo products (independent MFMAs in between)
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+#pragma unroll
+        for (int i = 0; i < RT; ++i)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+#pragma unroll
+        for (int i = 0; i < RT; ++i)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl[j], acc[i][j], 0, 0, 0);
     }
   }
 #pragma unroll
