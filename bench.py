@@ -75,6 +75,48 @@ def cpu_baseline(g, n, m, K, m_sample, reps):
     }
 
 
+def small_k_passes(g, lib, L, n, m, ks=(1, 3, 7, 16), reps=3):
+    """HBM-bound regime (K*P <~ 39): Z'V and Z U passes at small K (GEBV K=1, GWAS K=1+c+1,
+    LMC K=1+2*3, ...), kernel time from the library's HIP events; plus the FP64 GEBV kernel."""
+    import torch
+
+    out = {}
+    for K in ks:
+        V = torch.randn(n, K, device="cuda")
+        U = torch.randn(m, K, device="cuda")
+        g.autotune(K, reps=2)
+        g.rmatvec(V)
+        g.matvec(U)
+        torch.cuda.synchronize()
+        lib.slemm_gpu_timing_reset()
+        lib.slemm_gpu_timing(1)
+        for _ in range(reps):
+            g.rmatvec(V)
+            g.matvec(U)
+        torch.cuda.synchronize()
+        lib.slemm_gpu_timing(0)
+        p1, _ = kernel_ms(lib, 0, L.PREC_BF16X2)
+        p2, _ = kernel_ms(lib, 1, L.PREC_BF16X2)
+        lib.slemm_gpu_timing_reset()
+        b = pass_bytes(n, m, K)
+        out[f"K={K}"] = {"pass1_ms": round(p1, 3), "pass2_ms": round(p2, 3),
+                         "pass1_GB/s": round(b / p1 / 1e6, 1), "pass2_GB/s": round(b / p2 / 1e6, 1),
+                         "pass1_hbm_frac": round(b / p1 / 1e6 / HBM_PEAK_GBS, 3),
+                         "pass2_hbm_frac": round(b / p2 / 1e6 / HBM_PEAK_GBS, 3)}
+    alpha = torch.randn(m, dtype=torch.float64, device="cuda")
+    g.gebv(alpha)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.gebv(alpha)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    gb = m * ((n + 3) // 4) + m * 8 + n * 8
+    out["gebv_fp64"] = {"ms": round(dt * 1e3, 3), "GB/s": round(gb / dt / 1e9, 1),
+                        "hbm_frac": round(gb / dt / 1e9 / HBM_PEAK_GBS, 3)}
+    return out
+
+
 def reml_time_to_converge(g, n, m, nv, n_causal, h2):
     """SLDF-REML (src/reml.cpp:549-690) on the resident bed for a phenotype simulated from it:
     y = Zc beta + e, beta ~ N(0,1) on n_causal random SNPs, var(e) set for the target h2.
@@ -120,6 +162,7 @@ def main():
     ap.add_argument("--cpu-snps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-reml", action="store_true")
+    ap.add_argument("--no-smallk", action="store_true")
     ap.add_argument("--causal", type=int, default=5000)
     ap.add_argument("--h2", type=float, default=0.3)
     args = ap.parse_args()
@@ -192,6 +235,7 @@ def main():
     lib.slemm_gpu_timing_reset()
 
     reml = None if args.no_reml else reml_time_to_converge(g, n, m, nv, args.causal, args.h2)
+    smallk = None if args.no_smallk else small_k_passes(g, lib, L, n, m)
 
     pb = pass_bytes(n, m, K)
     value = 2 * pb / dt / 1e9
@@ -222,6 +266,7 @@ def main():
         "bed_generate_s": round(t_load, 3),
         "autotune": tune,
         "reml_time_to_converge": reml,
+        "small_k_passes": smallk,
         "roofline": {
             "bound": "mfma",
             "kernel": f"k_blockprod bf16x2 {dom}",
