@@ -29,6 +29,12 @@ BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA
 FP32_PEAK_TFLOPS = 157.3
 
 
+# HBM bytes per launch of the split-BF16 block product measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE on this workload (fetch counter scaled x2.0, write x1.0 against a 1 GiB device copy);
+# recorded with the commands and calibration in profiles/blockprod_c2_k65_hbm_traffic.md.
+MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 14.441e9 + 0.160e9, "pass2": 15.010e9 + 0.096e9}}
+
+
 def pass_bytes(n, m, K):
     return m * ((n + 3) // 4) + n * K * 4 + m * K * 4
 
@@ -163,6 +169,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-reml", action="store_true")
     ap.add_argument("--no-smallk", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device copy first (reference kernel for HBM byte counters)")
     ap.add_argument("--causal", type=int, default=5000)
     ap.add_argument("--h2", type=float, default=0.3)
     args = ap.parse_args()
@@ -202,7 +211,12 @@ def main():
     def step(prec):
         return g.lnr_op_select("ldet", tau0, V, precision=prec, col_kind=kinds)
 
-    tune = g.autotune(K, reps=2)
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = src.clone()
+        torch.cuda.synchronize()
+        del src, dst
+    tune = None if args.no_autotune else g.autotune(K, reps=2)
 
     for _ in range(args.warmup):
         step("bf16x2")
@@ -274,7 +288,7 @@ def main():
             "peak": BF16_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(ach_tf / BF16_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": (MEASURED_TRAFFIC.get((n, m, K), {}).get("pass2" if k2 >= k1 else "pass1")),
             "algorithmic_per_launch": {"flop": flops, "bytes": pb},
             "hbm_GB/s_of_kernel": round(pb / (dom_ms * 1e-3) / 1e9, 1),
             "mfma_issue_factor": round(2 * (16 * -(-K // 16)) / K, 3),

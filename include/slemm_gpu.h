@@ -206,10 +206,11 @@ int slemm_gpu_lmc_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const 
 
 /* Autotuner of the split-BF16 block products (no reference counterpart; the reference's only
  * blocking knob is --subset_size, src/option.cpp:75,297).  Times every kernel variant (B operand
- * from global memory with 4 or 8 row tiles per wave, or LDS-staged in 4- or 2-step groups) at
- * three grid sizes for both passes of a t-column operator on this handle's shape, and keeps the
- * fastest per pass for later calls.  Host outputs (any may be NULL): variant[2], target[2]
- * (workgroups the grid aims for), ms[26] = best per pass, then pass*12 + variant*3 + size index. */
+ * from global memory with 4 or 8 row tiles per wave or with table-driven code unpacking, or
+ * LDS-staged in 4- or 2-step groups) at three grid sizes for both passes of a t-column operator
+ * on this handle's shape, and keeps the fastest per pass for later calls.  Host outputs (any may
+ * be NULL): variant[2], target[2] (workgroups the grid aims for), ms[32] = best per pass, then
+ * pass*15 + variant*3 + grid-size index. */
 int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, int* target, double* ms);
 
 /* ---------------------------------------------------------------- instrumentation ---------- */

@@ -115,7 +115,9 @@ struct Fin2Args {
 //  BP_LDS4: B staged in LDS per 4-step group, double-buffered, codes prefetched a chunk ahead
 //  BP_LDS2: B staged in LDS per 2-step group, double-buffered, no code prefetch
 //  BP_REG8: as BP_REG with 8 row tiles per wave (half the B traffic per MFMA, 1 wave/SIMD)
-enum { BP_REG = 0, BP_LDS4 = 1, BP_LDS2 = 2, BP_REG8 = 3, BP_NVARIANTS = 4 };
+//  BP_REGL: as BP_REG with the codes unpacked through a 16-entry LDS table (fewer VALU ops;
+//           pays off when few MFMAs share each unpacked operand, i.e. small K)
+enum { BP_REG = 0, BP_LDS4 = 1, BP_LDS2 = 2, BP_REG8 = 3, BP_REGL = 4, BP_NVARIANTS = 5 };
 
 int lds_nt(int t);
 // number of reduction splits so that the grid has about target_wgs workgroups

@@ -165,8 +165,18 @@ __global__ __launch_bounds__(256) void k_bp_lds(LdsArgs a) {
 // Register-only variant: every wave reads its B fragments straight from global memory (L2/MALL
 // hits after the first wave of a workgroup wave-front); no LDS, no barriers, lower register
 // use, so two waves fit per SIMD.  Same split-K partial output as k_bp_lds.
-template <int RT, int NT, bool SQ>
+template <int RT, int NT, bool SQ, bool LUT = false>
 __global__ __launch_bounds__(256) void k_bp_reg(LdsArgs a) {
+  // LUT: codes are unpacked through a 16-entry LDS table (nibble = two codes -> two BF16
+  // dosages), one broadcast-friendly ds_read_b32 per pair instead of the permute sequence
+  __shared__ uint32_t lut[16];
+  if constexpr (LUT) {
+    if (threadIdx.x < 16) {
+      const uint32_t bf[4] = {0x0000u, 0x3F80u, 0x3F80u, SQ ? 0x4080u : 0x4000u};  // BF16 of d or d^2
+      lut[threadIdx.x] = bf[threadIdx.x & 3] | (bf[threadIdx.x >> 2] << 16);
+    }
+    __syncthreads();
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 15, g = lane >> 4;
   const int64_t row0 = (int64_t)blockIdx.x * (64 * RT) + wave * (16 * RT);
@@ -196,7 +206,15 @@ __global__ __launch_bounds__(256) void k_bp_reg(LdsArgs a) {
     for (int t = 0; t < 16; ++t) {
       bf16x8_t av[RT];
 #pragma unroll
-      for (int i = 0; i < RT; ++i) av[i] = unpack8<SQ>((pk[i][t >> 1] >> (16 * (t & 1))) & 0xFFFFu);
+      for (int i = 0; i < RT; ++i) {
+        const uint32_t p16 = (pk[i][t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
+        if constexpr (LUT) {
+          av[i] = __builtin_bit_cast(bf16x8_t, make_uint4(lut[p16 & 0xF], lut[(p16 >> 4) & 0xF],
+                                                          lut[(p16 >> 8) & 0xF], lut[p16 >> 12]));
+        } else {
+          av[i] = unpack8<SQ>(p16);
+        }
+      }
       const int64_t boff = (blk0 + t) * a.KP + col0 + r;
       bf16x8_t bh[NT], bl[NT];
 #pragma unroll
@@ -320,6 +338,7 @@ template <int NT, bool SQ>
 static void launch_variant(const LdsArgs& a, int variant, dim3 grid, hipStream_t st) {
   switch (variant) {
     case BP_REG: k_bp_reg<kLdsRT, NT, SQ><<<grid, 256, 0, st>>>(a); break;
+    case BP_REGL: k_bp_reg<kLdsRT, NT, SQ, true><<<grid, 256, 0, st>>>(a); break;
     case BP_REG8:
       grid.x /= 2;  // 8 row tiles per wave: 512 rows per workgroup
       if constexpr (NT <= 5) k_bp_reg<8, NT, SQ><<<grid, 256, 0, st>>>(a);

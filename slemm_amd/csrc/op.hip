@@ -229,33 +229,40 @@ void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const floa
 }
 
 // --------------------------------------------------------------------------- GEBV ----------
-// One wave per 1024 individuals; thread holds 16 FP64 accumulators (4 bytes of codes) and walks
-// the SNPs in order, so every individual's sum is accumulated in the same order as the
-// reference loop over SNPs.  z*alpha is exact for z in {0,1,2}, so fma == mul+add.
-__global__ __launch_bounds__(64) void k_gebv(const uint8_t* __restrict__ gs, int64_t ld_gs, int64_t m,
-                                             int64_t n, const double* __restrict__ alpha,
-                                             const uint8_t* __restrict__ flip, double* __restrict__ out) {
-  const int64_t i0 = (int64_t)blockIdx.x * 1024 + threadIdx.x * 16;
-  if (i0 >= n) return;  // n_pad is a multiple of 16, so i0 < n implies i0 + 16 <= n_pad
-  double acc[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) acc[q] = 0.0;
-  const uint8_t* base = gs + i0 / 4;
-  for (int64_t j = 0; j < m; ++j) {
-    const uint32_t w = *reinterpret_cast<const uint32_t*>(base + j * ld_gs);
-    const double aj = alpha[j];
-    const bool fj = flip && flip[j];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const uint32_t code = (w >> (2 * q)) & 3u;
+// One thread per individual over the individual-major copy: each thread walks its own row of
+// codes SNP by SNP, so every individual's FP64 sum is accumulated in exactly the reference's
+// order (tgv += z*alpha_j, src/pred.cpp:128); z*alpha is exact for z in {0,1,2}, so the fma
+// rounds like mul+add.  Chunks of 256 SNPs: 64 bytes of codes per thread (full 64-byte sectors
+// of its row), alpha and the flip flags of the chunk staged in LDS and read as broadcasts.
+__global__ __launch_bounds__(256) void k_gebv(const uint8_t* __restrict__ gi, int64_t ld_gi, int64_t m,
+                                              int64_t n, const double* __restrict__ alpha,
+                                              const uint8_t* __restrict__ flip, double* __restrict__ out) {
+  __shared__ double sa[256];
+  __shared__ uint32_t sf[256];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;  // < n_pad: rows exist in gi
+  const uint4* row = reinterpret_cast<const uint4*>(gi + i * ld_gi);
+  double acc = 0.0;
+  for (int64_t j0 = 0; j0 < m; j0 += 256) {
+    __syncthreads();
+    {
+      const int64_t j = j0 + threadIdx.x;
+      sa[threadIdx.x] = j < m ? alpha[j] : 0.0;
+      sf[threadIdx.x] = (j < m && flip && flip[j]) ? 1u : 0u;
+    }
+    const uint4* src = row + j0 / 64;
+    const uint4 c0 = src[0], c1 = src[1], c2 = src[2], c3 = src[3];
+    __syncthreads();
+    const uint32_t w[16] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w,
+                            c2.x, c2.y, c2.z, c2.w, c3.x, c3.y, c3.z, c3.w};
+    const int jn = (int)min<int64_t>(256, m - j0);
+    for (int jj = 0; jj < jn; ++jj) {
+      const uint32_t code = (w[jj >> 4] >> (2 * (jj & 15))) & 3u;
       uint32_t d = (code & 1u) + (code >> 1);
-      if (fj) d = 2u - d;
-      acc[q] = fma((double)d, aj, acc[q]);
+      if (sf[jj]) d = 2u - d;
+      acc = fma((double)d, sa[jj], acc);
     }
   }
-#pragma unroll
-  for (int q = 0; q < 16; ++q)
-    if (i0 + q < n) out[i0 + q] = acc[q];
+  if (i < n) out[i] = acc;
 }
 
 // GRAMMAR-gamma score columns: [Py, rr o Q_1..c, rr^2] (n x (2+c), column-major, ld n).
@@ -469,7 +476,7 @@ extern "C" int slemm_gpu_gebv(slemm_geno* h, const double* alpha, const uint8_t*
       dflip.reserve(h->m);
       HIPCHECK(hipMemcpyAsync(dflip.p, flip, h->m, hipMemcpyHostToDevice, st));
     }
-    k_gebv<<<(unsigned)((h->n_pad + 1023) / 1024), 64, 0, st>>>(h->gs, h->ld_gs, h->m, h->n, alpha,
+    k_gebv<<<(unsigned)(h->n_pad / 256), 256, 0, st>>>(h->gi, h->ld_gi, h->m, h->n, alpha,
                                                       dflip.as<uint8_t>(), gebv);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(st));
