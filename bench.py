@@ -156,6 +156,73 @@ def reml_time_to_converge(g, n, m, nv, n_causal, h2):
     return out
 
 
+def scale_run(n, m, nv, n_causal, h2, seed=5):
+    """Whole-pipeline run at biobank shape: synthetic bed with allele frequencies drawn from
+    Beta(0.8, 0.8) and kept where MAF >= 0.01, phenotype y = Zc beta + e with beta ~ N(0,1) on
+    n_causal SNPs and var(e) set for h2; then SLDF-REML with nv probes (src/reml.cpp:549-690),
+    GEBV of every individual from the SNP BLUPs (src/pred.cpp:112-130) and the GRAMMAR-gamma
+    score terms of every SNP (bin/slemm_gamma.py:306-319).  Wall-clock seconds per stage."""
+    import numpy as np
+    import torch
+    from slemm_amd import gpu
+
+    rng = np.random.default_rng(seed)
+    p = rng.beta(0.8, 0.8, size=4 * m)
+    p = p[np.minimum(p, 1 - p) >= 0.01][:m]
+    assert p.size == m
+    out = {"workload": (f"n={n} m={m}, MAF~Beta(0.8,0.8)>=0.01, h2={h2}, {n_causal} causal, "
+                        f"SLDF-REML with {nv} probes + GEBV + GWAS score")}
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    g = gpu.Geno.synthesize_p(n, p, seed=seed)
+    torch.cuda.synchronize()
+    out["generate_s"] = round(time.perf_counter() - t_all, 3)
+    g.set_model(None, None, None)
+    beta = np.zeros(m, np.float32)
+    beta[rng.choice(m, size=n_causal, replace=False)] = rng.normal(size=n_causal).astype(np.float32)
+    gv = g.matvec(torch.from_numpy(beta).cuda(), precision="fp32")[:, 0].double().cpu().numpy()
+    ve = gv.var() * (1 - h2) / h2
+    y = (gv + rng.normal(scale=np.sqrt(ve), size=n) + 10.0).astype(np.float32)
+    X = np.ones((n, 1), np.float32)
+    out["realised_h2"] = round(float(gv.var() / (gv.var() + ve)), 5)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = g.SLDF_REML(X, y, rng_seed=0, n_V=nv, precision="bf16x2")
+    torch.cuda.synchronize()
+    out["reml"] = {"seconds": round(time.perf_counter() - t0, 3), "seconds_seed": round(r["seconds_seed"], 3),
+                   "lanczos_steps": list(r["steps"]), "criterion_evals": r["evals"],
+                   "h2": round(float(r["h2"]), 5), "vg": float(r["vg"]), "ve": float(r["ve"])}
+
+    alpha = torch.from_numpy(r["snp_blup"].astype(np.float64)).cuda()
+    g.gebv(alpha)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gebv = g.gebv(alpha)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out["gebv"] = {"ms": round(dt * 1e3, 3),
+                   "GB/s": round((m * ((n + 3) // 4) + 8 * (m + n)) / dt / 1e9, 1),
+                   "corr_with_true_genetic_value": round(float(np.corrcoef(gebv.cpu().numpy(), gv)[0, 1]), 4)}
+
+    Py = torch.from_numpy(r["Py"]).cuda()
+    g.gwas_score(Py)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xy, xx = g.gwas_score(Py)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    s2 = (xy.double() ** 2 / xx.double()).cpu().numpy()   # chi-square up to the gamma factor
+    causal = beta != 0
+    out["gwas_score"] = {"ms": round(dt * 1e3, 3), "snps": m,
+                         "score_enrichment_causal_over_null":
+                             round(float(np.nanmean(s2[causal]) / np.nanmean(s2[~causal])), 3)}
+    torch.cuda.synchronize()
+    out["total_s"] = round(time.perf_counter() - t_all, 3)
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +241,10 @@ def main():
                     help="run a 1 GiB device copy first (reference kernel for HBM byte counters)")
     ap.add_argument("--causal", type=int, default=5000)
     ap.add_argument("--h2", type=float, default=0.3)
+    ap.add_argument("--no-scale", action="store_true",
+                    help="skip the n=1M m=60k REML + GEBV + GWAS pipeline run")
+    ap.add_argument("--scale-n", type=int, default=1_000_000)
+    ap.add_argument("--scale-m", type=int, default=60_000)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -301,6 +372,12 @@ def main():
             result["cpu_baseline"] = {"value": None, "error": repr(e)}
     else:
         result["cpu_baseline"] = None
+    if not args.no_scale:
+        g.close()
+        del V
+        torch.cuda.empty_cache()
+        print(f"# scale pipeline n={args.scale_n} m={args.scale_m}", file=sys.stderr, flush=True)
+        result["scale_pipeline"] = scale_run(args.scale_n, args.scale_m, 100, 2000, 0.4)
     print(json.dumps(result), flush=True)
 
 

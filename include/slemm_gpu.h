@@ -76,6 +76,10 @@ int slemm_gpu_geno_subset(const slemm_geno* h, const int64_t* snp_idx, int64_t m
  * depends only on (seed, n, m); no reference counterpart (test and benchmark input). */
 int slemm_gpu_geno_synthesize(int64_t n, int64_t m, uint64_t seed, float maf_lo, float maf_hi,
                               float missing, slemm_geno** out);
+/* Same, with the allele frequency of every SNP given (p: m floats in [0,1], host memory), e.g.
+ * drawn from a Beta distribution on the host.  No reference counterpart. */
+int slemm_gpu_geno_synthesize_p(int64_t n, int64_t m, uint64_t seed, const float* p, float missing,
+                                slemm_geno** out);
 
 int slemm_gpu_geno_destroy(slemm_geno* h);
 int slemm_gpu_geno_dims(const slemm_geno* h, int64_t* n, int64_t* m);

@@ -147,12 +147,12 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 // Synthetic bed: thread per output byte of gs.  p_j ~ U(lo,hi); code 01 with prob `missing`,
 // else A2 count ~ Bin(2,p_j) coded 00/10/11.
 __global__ void k_synth(uint8_t* __restrict__ gs, int64_t ld_gs, int64_t n, int64_t m,
-                        uint64_t seed, float lo, float hi, float missing) {
+                        uint64_t seed, float lo, float hi, float missing, const float* __restrict__ pj) {
   const int64_t ob = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t j = blockIdx.y + (int64_t)blockIdx.z * 65535;
   if (j >= m || ob >= ld_gs) return;
   const uint64_t hj = mix64(seed ^ mix64((uint64_t)j + 0x1234567ull));
-  const float p = lo + (hi - lo) * (float)((hj >> 40) * (1.0 / 16777216.0));
+  const float p = pj ? pj[j] : lo + (hi - lo) * (float)((hj >> 40) * (1.0 / 16777216.0));
   const uint32_t pthr = (uint32_t)(p * 2097152.0f);            // 21-bit threshold
   const uint32_t mthr = (uint32_t)(missing * 2097152.0f);
   uint32_t out = 0;
@@ -363,29 +363,47 @@ extern "C" int slemm_gpu_geno_subset(const slemm_geno* hs, const int64_t* snp_id
   });
 }
 
+static void synthesize(int64_t n, int64_t m, uint64_t seed, float maf_lo, float maf_hi, float missing,
+                       const float* p_host, slemm_geno** out) {
+  REQUIRE(out, "out is null");
+  *out = nullptr;
+  REQUIRE(n > 0 && m > 0, "empty genotype matrix");
+  REQUIRE(n < (int64_t)1 << 31, "n too large");
+  REQUIRE(maf_lo >= 0 && maf_hi <= 1 && maf_lo <= maf_hi, "bad allele-frequency range");
+  REQUIRE(missing >= 0 && missing < 1, "bad missing rate");
+  if (p_host)
+    for (int64_t j = 0; j < m; ++j) REQUIRE(p_host[j] >= 0 && p_host[j] <= 1, "allele frequencies must be in [0,1]");
+  slemm_geno* h = alloc_geno(n, m);
+  try {
+    DevBuf dp;
+    if (p_host) {
+      dp.reserve(sizeof(float) * m);
+      HIPCHECK(hipMemcpy(dp.p, p_host, sizeof(float) * m, hipMemcpyHostToDevice));
+    }
+    dim3 grid((unsigned)((h->ld_gs + 255) / 256), (unsigned)std::min<int64_t>(m, 65535),
+              (unsigned)((m + 65534) / 65535));
+    k_synth<<<grid, 256>>>(h->gs, h->ld_gs, n, m, seed, maf_lo, maf_hi, missing, dp.as<float>());
+    HIPCHECK(hipGetLastError());
+    build_transpose(h, nullptr);
+    HIPCHECK(hipDeviceSynchronize());
+  } catch (...) {
+    slemm_gpu_geno_destroy(h);
+    throw;
+  }
+  *out = h;
+}
+
+extern "C" int slemm_gpu_geno_synthesize_p(int64_t n, int64_t m, uint64_t seed, const float* p, float missing,
+                                           slemm_geno** out) {
+  return guard([&] {
+    REQUIRE(p, "p is null");
+    synthesize(n, m, seed, 0.f, 1.f, missing, p, out);
+  });
+}
+
 extern "C" int slemm_gpu_geno_synthesize(int64_t n, int64_t m, uint64_t seed, float maf_lo,
                                          float maf_hi, float missing, slemm_geno** out) {
-  return guard([&] {
-    REQUIRE(out, "out is null");
-    *out = nullptr;
-    REQUIRE(n > 0 && m > 0, "empty genotype matrix");
-    REQUIRE(n < (int64_t)1 << 31, "n too large");
-    REQUIRE(maf_lo >= 0 && maf_hi <= 1 && maf_lo <= maf_hi, "bad allele-frequency range");
-    REQUIRE(missing >= 0 && missing < 1, "bad missing rate");
-    slemm_geno* h = alloc_geno(n, m);
-    try {
-      dim3 grid((unsigned)((h->ld_gs + 255) / 256), (unsigned)std::min<int64_t>(m, 65535),
-                (unsigned)((m + 65534) / 65535));
-      k_synth<<<grid, 256>>>(h->gs, h->ld_gs, n, m, seed, maf_lo, maf_hi, missing);
-      HIPCHECK(hipGetLastError());
-      build_transpose(h, nullptr);
-      HIPCHECK(hipDeviceSynchronize());
-    } catch (...) {
-      slemm_gpu_geno_destroy(h);
-      throw;
-    }
-    *out = h;
-  });
+  return guard([&] { synthesize(n, m, seed, maf_lo, maf_hi, missing, nullptr, out); });
 }
 
 extern "C" int slemm_gpu_geno_destroy(slemm_geno* h) {

@@ -90,6 +90,16 @@ class Geno:
         L.check(lib.slemm_gpu_geno_synthesize(n, m, seed, maf_lo, maf_hi, missing, C.byref(h)))
         return cls(h)
 
+    @classmethod
+    def synthesize_p(cls, n: int, p: np.ndarray, seed: int = 1, missing: float = 0.0) -> "Geno":
+        """Synthetic genotypes with the given per-SNP A2 allele frequencies (len(p) SNPs)."""
+        lib = L.load()
+        pf = np.ascontiguousarray(p, dtype=np.float32)
+        h = C.c_void_p()
+        L.check(lib.slemm_gpu_geno_synthesize_p(n, len(pf), seed, _np_ptr(pf, C.c_float), missing,
+                                                C.byref(h)))
+        return cls(h)
+
     def subset(self, snp_idx: Sequence[int]) -> "Geno":
         lib = L.load()
         idx = np.ascontiguousarray(snp_idx, dtype=np.int64)
@@ -204,6 +214,18 @@ class Geno:
         fl = None if flip is None else np.ascontiguousarray(flip, dtype=np.uint8)
         L.check(L.load().slemm_gpu_gebv(self._h, _ptr(a), _np_ptr(fl, C.c_uint8), _ptr(out), _stream()))
         return out
+
+    def gwas_score(self, Py: torch.Tensor, skip_mac: float = 20.0):
+        """Per-SNP score terms of the GRAMMAR-gamma scan (bin/slemm_gamma.py:306-319):
+        xy_j = x_cj' Py and xx_j = ||(I - QQ') R^-1/2 x_cj||^2 with the model's rvec and Q;
+        SNPs whose minor-allele count is below skip_mac come back as NaN."""
+        if Py.dtype != torch.float32 or not Py.is_cuda or Py.numel() != self.n:
+            raise TypeError("Py must be a float32 CUDA tensor of length n")
+        p = Py.contiguous()
+        xy = torch.empty(self.m, dtype=torch.float32, device=p.device)
+        xx = torch.empty(self.m, dtype=torch.float32, device=p.device)
+        L.check(L.load().slemm_gpu_gwas_score(self._h, _ptr(p), skip_mac, _ptr(xy), _ptr(xx), _stream()))
+        return xy, xx
 
     def autotune(self, t: int, reps: int = 2) -> dict:
         """Times the block-product variants for a t-column operator and keeps the fastest."""

@@ -131,6 +131,24 @@ def test_synthesized_bed_consistent(gpu):
     assert rel_err(z, D.matvec(U)) < TOL["fp32"]
 
 
+def test_synthesized_bed_given_frequencies(gpu):
+    """Per-SNP allele frequencies passed in: dosage frequency within 5 sd of p, codes 00/10/11
+    only (no missing), and the device counts equal the oracle's on the downloaded rows."""
+    n, m = 20000, 400
+    p = np.random.default_rng(8).beta(0.8, 0.8, size=m).astype(np.float32)
+    p[:3] = [0.0, 1.0, 0.5]
+    g = gpu.Geno.synthesize_p(n, p, seed=3)
+    codes = bedio.decode_codes(g.download(), n)
+    assert not np.any(codes == 1)
+    pc, n0, n1, n2, gm = g.counts()
+    opc, *_ = oracle.counts(bedio.kmat_of_codes(codes))
+    assert np.array_equal(pc, opc)
+    f = pc / (2.0 * n)
+    sd = np.sqrt(p * (1 - p) / (2 * n)) + 1e-9
+    assert np.all(np.abs(f - p) <= 5 * sd + 1e-6)
+    assert f[0] == 0.0 and f[1] == 1.0
+
+
 def test_gebv_bit_exact(gpu):
     p = make_problem(n=3001, m=800, seed=19, missing=0.01)
     rng = np.random.default_rng(4)
