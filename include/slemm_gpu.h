@@ -33,7 +33,8 @@ enum {
   SLEMM_ERR_ALLOC = 3,   /* device allocation failed */
   SLEMM_ERR_STATE = 4,   /* handle not ready (e.g. model not set) */
   SLEMM_ERR_RANK = 5,    /* fixed-effects design not of full rank (src/reml.cpp:586,769) */
-  SLEMM_ERR_DATA = 6     /* malformed input data (e.g. truncated bed, src/input.cpp:366) */
+  SLEMM_ERR_DATA = 6,    /* malformed input data (e.g. truncated bed, src/input.cpp:366) */
+  SLEMM_ERR_NUMERIC = 7  /* numerical breakdown (e.g. an information matrix not positive definite) */
 };
 
 /* Linear operator kinds of lnr_op_select (src/reml.cpp:96-118). */
@@ -109,6 +110,11 @@ int slemm_gpu_set_model(slemm_geno* h, const float* persnp, const float* rvec,
 /* Host copies of the model terms (each may be NULL). */
 int slemm_gpu_get_model(const slemm_geno* h, float* gm, float* gw, float* r_rsqrt);
 
+/* Overrides the per-SNP weights W of the operators with gw (host, m floats, finite, >= 0) after
+ * set_model, e.g. sigma_g^2-scaled or per-window weights under differentiation.  Replaces the
+ * gw line of src/reml.cpp:622 for callers that form the weights themselves. */
+int slemm_gpu_set_gw(slemm_geno* h, const float* gw);
+
 /* ---------------------------------------------------------------- operators ---------------- */
 
 /* lnr_op_select (src/reml.cpp:96-118) over all t columns in one batch.
@@ -124,6 +130,16 @@ int slemm_gpu_op(slemm_geno* h, int kind, const int* col_kind, float tau0,
  * V device n x t (ldv).  If scale_rr != 0 the rows of V are first multiplied by r_rsqrt. */
 int slemm_gpu_zt(slemm_geno* h, const float* V, int64_t ldv, int t, float* out, int64_t ldo,
                  int weighted, int scale_rr, int precision, void* stream);
+
+/* Adjoint product: out_j = sum_k coef_k (Zc'A)_jk (Zc'B)_jk, j < m (device doubles).
+ * A, B device n x t (lda, ldb); with scale_rr != 0 their rows are first multiplied by r_rsqrt.
+ * coef: t host doubles.  Both blocks go through one pass over the bed as a 2t-column Zc'
+ * product (the rmatvec of src/reml.cpp:159-195), followed by an FP64 row reduction.  With
+ * A = P V, B = V for probe columns V and A = B = Py this gives dl/dgw_j, the derivative of the
+ * REML log-likelihood with respect to the SNP weights (no reference counterpart; the reference
+ * has no derivatives). */
+int slemm_gpu_adjoint(slemm_geno* h, const float* A, int64_t lda, const float* B, int64_t ldb, int t,
+                      const double* coef, int scale_rr, double* out, int precision, void* stream);
 
 /* matvec (src/reml.cpp:197-233): out (n x t, ldo) = Zc U with U device m x t (ldu). */
 int slemm_gpu_z(slemm_geno* h, const float* U, int64_t ldu, int t, float* out, int64_t ldo,
@@ -205,6 +221,27 @@ int slemm_gpu_lmc_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const 
                        int rng_seed, int n_MC, float tol_L, float s2max, float s2min,
                        int precision, int verbose,
                        float* Py, float* snp_blup, float* blue, slemm_reml_stats* stats);
+
+/* REML score and average-information matrix at (vg, ve) for the model of SLDF_REML (no
+ * reference counterpart): score = (dl/dvg, dl/dve), ai = (AI_gg, AI_ge, AI_ee) with
+ *   dl/dvg = -1/2 [tr(PA) - y'PAPy], dl/dve = -1/2 [tr(P) - y'PPy], AI_kl = 1/2 y'P V_k P V_l P y
+ * in the R^-1/2-whitened model (A = R^-1/2 Zc W Zc' R^-1/2, V_g = A, V_e = I).  Traces are
+ * n/n_V sum_v v'Mv over the probe columns: `probes` (device n x n_V, ldp; E[v v'] = I/n) or, when
+ * NULL, unit-norm Rademacher probes from mt19937(rng_seed+54321) as SLDF_REML draws them. */
+int slemm_gpu_reml_score(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y, double vg,
+                         double ve, const float* probes, int64_t ldp, int n_V, int rng_seed, float tol_L,
+                         int precision, double* score, double* ai);
+
+/* AI-REML: Newton steps theta += AI^-1 score on theta = (vg, ve) from (vg0, ve0) (both <= 0:
+ * h2 = 1/2 of the projected phenotypic variance) until the largest relative change is below
+ * tol_VC or max_iter steps; h2 is kept at or below s2max, which also fixes the seed shift
+ * (1-s2max)/s2max.  Outputs as SLDF_REML; history (may be NULL): max_iter rows of
+ * (vg, ve, score_g, score_e, AI_gg, AI_ge, AI_ee) at each iterate.  No reference counterpart
+ * (the reference maximises the same likelihood by golden-section search, src/reml.cpp:493-547). */
+int slemm_gpu_ai_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y, int rng_seed,
+                      int n_V, float tol_L, float s2max, double vg0, double ve0, double tol_VC,
+                      int max_iter, int precision, int verbose, float* Py, float* snp_blup, float* blue,
+                      float* var_blue, double* history, slemm_reml_stats* stats);
 
 /* ---------------------------------------------------------------- tuning ------------------- */
 

@@ -11,7 +11,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libslemm_gpu.so"
 
-OK, ERR_ARG, ERR_HIP, ERR_ALLOC, ERR_STATE, ERR_RANK, ERR_DATA = range(7)
+OK, ERR_ARG, ERR_HIP, ERR_ALLOC, ERR_STATE, ERR_RANK, ERR_DATA, ERR_NUMERIC = range(8)
 OP_LDET, OP_QUADFORM = 0, 1
 PREC_FP32, PREC_BF16X2 = 0, 1
 
@@ -71,6 +71,13 @@ SIGNATURES = {
     "slemm_gpu_timing_reset": (C.c_int, []),
     "slemm_gpu_lmc_reml": (C.c_int, [c_vp, c_fp, c_i64, C.c_int, c_fp, C.c_int, C.c_int, C.c_float, C.c_float,
                                      C.c_float, C.c_int, C.c_int, c_fp, c_fp, c_fp, C.POINTER(RemlStats)]),
+    "slemm_gpu_set_gw": (C.c_int, [c_vp, c_fp]),
+    "slemm_gpu_adjoint": (C.c_int, [c_vp, c_vp, c_i64, c_vp, c_i64, C.c_int, c_dp, C.c_int, c_vp, C.c_int, c_vp]),
+    "slemm_gpu_reml_score": (C.c_int, [c_vp, c_fp, c_i64, C.c_int, c_fp, C.c_double, C.c_double, c_vp, c_i64,
+                                       C.c_int, C.c_int, C.c_float, C.c_int, c_dp, c_dp]),
+    "slemm_gpu_ai_reml": (C.c_int, [c_vp, c_fp, c_i64, C.c_int, c_fp, C.c_int, C.c_int, C.c_float, C.c_float,
+                                    C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, c_fp, c_fp,
+                                    c_fp, c_fp, c_dp, C.POINTER(RemlStats)]),
 }
 
 _lib = None

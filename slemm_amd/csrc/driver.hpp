@@ -1,0 +1,92 @@
+// Host-side pieces shared by the REML drivers (reml.cpp, aireml.hip): device scratch vectors,
+// the covariate basis, projections and the Py / SNP BLUP / BLUE finish of src/reml.cpp:655-677.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "hostla.hpp"
+
+namespace slemm {
+
+inline void abi(int st) {
+  if (st != SLEMM_OK) fail(st, slemm_gpu_last_error());
+}
+
+struct DevF {
+  float* p = nullptr;
+  explicit DevF(size_t count) { HIPCHECK(hipMalloc(&p, sizeof(float) * std::max<size_t>(count, 1))); }
+  ~DevF() {
+    if (p) (void)hipFree(p);
+  }
+  DevF(const DevF&) = delete;
+  DevF& operator=(const DevF&) = delete;
+};
+
+inline void upload(float* d, const std::vector<float>& h) {
+  HIPCHECK(hipMemcpy(d, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+}
+inline void download(std::vector<float>& h, const float* d, size_t count) {
+  h.resize(count);
+  HIPCHECK(hipMemcpy(h.data(), d, sizeof(float) * count, hipMemcpyDeviceToHost));
+}
+
+struct SeedGuard {
+  slemm_lseed* s = nullptr;
+  ~SeedGuard() {
+    if (s) slemm_gpu_lseed_destroy(s);
+  }
+};
+
+// Q R = diag(rr) X (n x c), Q stored in the handle for the operators.  Returns R.
+inline MatD covariate_basis(slemm_geno* h, const float* X, int64_t ldx, int c, MatD& Q) {
+  const int64_t n = h->n;
+  MatD A(n, c), R;
+  for (int j = 0; j < c; ++j)
+    for (int64_t i = 0; i < n; ++i) A(i, j) = (double)h->rr_host[i] * (double)X[i + (size_t)j * ldx];
+  if (!thin_qr(A, Q, R, 1e-6)) fail(SLEMM_ERR_RANK, "Fixed-effects design matrix is not of full rank.");
+  std::vector<float> qf((size_t)n * c);
+  for (size_t e = 0; e < qf.size(); ++e) qf[e] = (float)Q.a[e];
+  set_Q(h, qf.data(), c);
+  return R;
+}
+
+// X - Q (Q' X) in place on an n x t float block (FP64 accumulation)
+inline void project_out(const MatD& Q, std::vector<float>& V, int64_t n, int t) {
+  for (int k = 0; k < t; ++k) {
+    float* v = &V[(size_t)k * n];
+    for (int q = 0; q < Q.c; ++q) {
+      double s = 0.0;
+      for (int64_t i = 0; i < n; ++i) s += Q(i, q) * v[i];
+      for (int64_t i = 0; i < n; ++i) v[i] = (float)((double)v[i] - Q(i, q) * s);
+    }
+  }
+}
+
+inline double sq_norm(const float* v, int64_t n) {
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += (double)v[i] * v[i];
+  return s;
+}
+
+inline double seconds_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Rademacher probes of unit norm from mt19937(seed+54321), written to columns of B (ld n),
+// draw order and normalisation as src/reml.cpp:597-605.
+void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B);
+
+// Final estimates at tau1 = ve/vg from a seed whose column 0 is the projected phenotype and
+// columns 1..c the covariate basis (src/reml.cpp:655-677): Py = R^-1/2 (I-QQ') H^-1 y~,
+// snp_blup = W Zc' Py, BLUE and (optionally) its covariance scaled by vg.
+void finish_estimates(slemm_geno* h, slemm_lseed* seed, const float* dB, float tau0, float tau1, const MatD& Q,
+                      const MatD& qrR, const float* X, int64_t ldx, int c, const float* y, float vg,
+                      int precision, float* Py, float* snp_blup, float* blue, float* var_blue);
+
+}  // namespace slemm

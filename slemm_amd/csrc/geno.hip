@@ -513,6 +513,17 @@ extern "C" int slemm_gpu_set_model(slemm_geno* h, const float* persnp, const flo
   });
 }
 
+extern "C" int slemm_gpu_set_gw(slemm_geno* h, const float* gw) {
+  return guard([&] {
+    REQUIRE(h && gw, "null argument");
+    require_model(h);
+    for (int64_t j = 0; j < h->m; ++j)
+      REQUIRE(std::isfinite(gw[j]) && gw[j] >= 0, "SNP weights must be finite and non-negative");
+    HIPCHECK(hipSetDevice(h->device));
+    HIPCHECK(hipMemcpy(h->gw, gw, sizeof(float) * h->m, hipMemcpyHostToDevice));
+  });
+}
+
 extern "C" int slemm_gpu_get_model(const slemm_geno* h, float* gm, float* gw, float* r_rsqrt) {
   return guard([&] {
     REQUIRE(h, "null handle");

@@ -300,6 +300,47 @@ __global__ void k_gwas_combine(const float* __restrict__ A, int64_t lda, int c, 
   xx[j] = (float)((double)S2[j] - 2.0 * mu * C1 - mu * mu * R2 - proj);
 }
 
+// Column dots, stage 1: block (k, b) sums A[i,k] B[i,k] over rows [b*rpb, (b+1)*rpb) in FP64.
+__global__ __launch_bounds__(256) void k_col_dots(const float* __restrict__ A, int64_t lda,
+                                                  const float* __restrict__ B, int64_t ldb, int64_t n,
+                                                  int64_t rpb, int t, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int k = blockIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.y * rpb;
+  const int64_t i1 = min(n, i0 + rpb);
+  double s = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256)
+    s += (double)A[i + (int64_t)k * lda] * (double)B[i + (int64_t)k * ldb];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(int64_t)blockIdx.y * t + k] = red[0];
+}
+
+// stage 2: res[k] = sum_b part[b][k], fixed order (deterministic).
+__global__ void k_col_dots_final(const double* __restrict__ part, int64_t nblk, int t, double* __restrict__ res) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= t) return;
+  double s = 0.0;
+  for (int64_t b = 0; b < nblk; ++b) s += part[b * t + k];
+  res[k] = s;
+}
+
+void col_dots(slemm_geno* h, const float* A, int64_t lda, const float* B, int64_t ldb, int64_t n, int t,
+              double* res_dev, hipStream_t st) {
+  if (t <= 0) return;
+  const int64_t rpb = 16384;
+  const int64_t nblk = (n + rpb - 1) / rpb;
+  h->redpart.reserve(sizeof(double) * nblk * t);
+  k_col_dots<<<dim3((unsigned)t, (unsigned)nblk), 256, 0, st>>>(A, lda, B, ldb, n, rpb, t, h->redpart.as<double>());
+  HIPCHECK(hipGetLastError());
+  k_col_dots_final<<<(t + 255) / 256, 256, 0, st>>>(h->redpart.as<double>(), nblk, t, res_dev);
+  HIPCHECK(hipGetLastError());
+}
+
 }  // namespace slemm
 
 using namespace slemm;
@@ -405,29 +446,76 @@ extern "C" int slemm_gpu_op(slemm_geno* h, int kind, const int* col_kind, float 
   });
 }
 
+namespace slemm {
+namespace {
+
+// (W) Zc' (R^-1/2) V -> out (m x t), one pass over the SNP-major copy.
+void run_zt(slemm_geno* h, const float* V, int64_t ldv, int t, float* out, int64_t ldo, bool weighted,
+            bool scale_rr, int precision, hipStream_t st) {
+  check_precision(precision);
+  check_dense(V, ldv, h->n, t, "V");
+  check_dense(out, ldo, h->m, t, "out");
+  ensure_counts(h, st);
+  if (weighted || scale_rr) require_model(h);
+  const Layout L = layout_for(h, t, precision);
+  pack_pass1(h, L, V, ldv, t, scale_rr ? h->rr : nullptr, nullptr, nullptr, precision, st);
+  if (use_lds(precision)) {
+    pass1_lds(h, L, t, weighted, nullptr, nullptr, out, ldo, st);
+    return;
+  }
+  BlockProdArgs a;
+  a.G = h->gs; a.ldg = h->ld_gs; a.nsuper = h->n_pad / kPad; a.X = h->x1.p; a.lo_offset16 = L.lo1_16;
+  a.KP = L.KP; a.t = t; a.rows = h->m; a.colsum = h->colsum.as<double>();
+  a.gm = h->gm; a.gw = weighted ? h->gw : nullptr; a.out = out; a.ldo = ldo;
+  centring_args(h, L, a);
+  launch_blockprod(a, precision, EPI_PASS1, L.NT, h->m_pad, st);
+}
+
+// out_j = sum_k coef_k T_jk T_j(t+k), FP64 accumulation over the k columns.
+__global__ void k_adjoint_combine(const float* __restrict__ T, int64_t ldt, int t, const double* __restrict__ coef,
+                                  int64_t m, double* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  double acc = 0.0;
+  for (int k = 0; k < t; ++k) acc += coef[k] * ((double)T[j + k * ldt] * (double)T[j + (k + t) * ldt]);
+  out[j] = acc;
+}
+
+}  // namespace
+}  // namespace slemm
+
 extern "C" int slemm_gpu_zt(slemm_geno* h, const float* V, int64_t ldv, int t, float* out, int64_t ldo,
                             int weighted, int scale_rr, int precision, void* stream) {
   return guard([&] {
     REQUIRE(h, "null handle");
     HIPCHECK(hipSetDevice(h->device));
+    run_zt(h, V, ldv, t, out, ldo, weighted != 0, scale_rr != 0, precision, (hipStream_t)stream);
+  });
+}
+
+extern "C" int slemm_gpu_adjoint(slemm_geno* h, const float* A, int64_t lda, const float* B, int64_t ldb, int t,
+                                 const double* coef, int scale_rr, double* out, int precision, void* stream) {
+  return guard([&] {
+    REQUIRE(h && coef && out, "null argument");
+    HIPCHECK(hipSetDevice(h->device));
     hipStream_t st = (hipStream_t)stream;
-    check_precision(precision);
-    check_dense(V, ldv, h->n, t, "V");
-    check_dense(out, ldo, h->m, t, "out");
-    ensure_counts(h, st);
-    if (weighted || scale_rr) require_model(h);
-    const Layout L = layout_for(h, t, precision);
-    pack_pass1(h, L, V, ldv, t, scale_rr ? h->rr : nullptr, nullptr, nullptr, precision, st);
-    if (use_lds(precision)) {
-      pass1_lds(h, L, t, weighted != 0, nullptr, nullptr, out, ldo, st);
-      return;
-    }
-    BlockProdArgs a;
-    a.G = h->gs; a.ldg = h->ld_gs; a.nsuper = h->n_pad / kPad; a.X = h->x1.p; a.lo_offset16 = L.lo1_16;
-    a.KP = L.KP; a.t = t; a.rows = h->m; a.colsum = h->colsum.as<double>();
-    a.gm = h->gm; a.gw = weighted ? h->gw : nullptr; a.out = out; a.ldo = ldo;
-    centring_args(h, L, a);
-    launch_blockprod(a, precision, EPI_PASS1, L.NT, h->m_pad, st);
+    check_dense(A, lda, h->n, t, "A");
+    check_dense(B, ldb, h->n, t, "B");
+    const int64_t n = h->n, m = h->m;
+    // [A | B] as one n x 2t operand: both sides of every product share the pass over the bed
+    DevBuf ab, zt, dc;
+    ab.reserve(sizeof(float) * n * 2 * t);
+    HIPCHECK(hipMemcpy2DAsync(ab.p, sizeof(float) * n, A, sizeof(float) * lda, sizeof(float) * n, t,
+                              hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpy2DAsync(ab.as<float>() + (size_t)n * t, sizeof(float) * n, B, sizeof(float) * ldb,
+                              sizeof(float) * n, t, hipMemcpyDeviceToDevice, st));
+    zt.reserve(sizeof(float) * m * 2 * t);
+    run_zt(h, ab.as<float>(), n, 2 * t, zt.as<float>(), m, false, scale_rr != 0, precision, st);
+    dc.reserve(sizeof(double) * t);
+    HIPCHECK(hipMemcpyAsync(dc.p, coef, sizeof(double) * t, hipMemcpyHostToDevice, st));
+    k_adjoint_combine<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(zt.as<float>(), m, t, dc.as<double>(), m, out);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
   });
 }
 

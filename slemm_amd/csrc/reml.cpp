@@ -18,78 +18,78 @@
 #include <string>
 #include <vector>
 
-#include "common.hpp"
-#include "hostla.hpp"
+#include "driver.hpp"
 
 using namespace slemm;
 
-namespace {
+namespace slemm {
 
-void abi(int st) {
-  if (st != SLEMM_OK) fail(st, slemm_gpu_last_error());
-}
-
-struct DevF {
-  float* p = nullptr;
-  explicit DevF(size_t count) { HIPCHECK(hipMalloc(&p, sizeof(float) * std::max<size_t>(count, 1))); }
-  ~DevF() {
-    if (p) (void)hipFree(p);
+void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B) {
+  std::mt19937 rng(rng_seed + 54321);
+  std::uniform_real_distribution<float> runif(0.0, 1.0);
+  for (int j = 0; j < n_V; ++j) {
+    float* v = B + (size_t)j * n;
+    for (int64_t i = 0; i < n; ++i) v[i] = runif(rng) < 0.5 ? -1.0f : 1.0f;
   }
-  DevF(const DevF&) = delete;
-  DevF& operator=(const DevF&) = delete;
-};
-
-void upload(float* d, const std::vector<float>& h) {
-  HIPCHECK(hipMemcpy(d, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
-}
-void download(std::vector<float>& h, const float* d, size_t count) {
-  h.resize(count);
-  HIPCHECK(hipMemcpy(h.data(), d, sizeof(float) * count, hipMemcpyDeviceToHost));
-}
-
-struct SeedGuard {
-  slemm_lseed* s = nullptr;
-  ~SeedGuard() {
-    if (s) slemm_gpu_lseed_destroy(s);
+  for (int j = 0; j < n_V; ++j) {
+    float* v = B + (size_t)j * n;
+    float ss = 0.f;
+    for (int64_t i = 0; i < n; ++i) ss += v[i] * v[i];
+    const float nr = std::sqrt(ss);
+    for (int64_t i = 0; i < n; ++i) v[i] /= nr;
   }
-};
+}
 
-// Q R = diag(rr) X (n x c), Q stored in the handle for the operators.  Returns R.
-MatD covariate_basis(slemm_geno* h, const float* X, int64_t ldx, int c, MatD& Q) {
-  const int64_t n = h->n;
-  MatD A(n, c), R;
+void finish_estimates(slemm_geno* h, slemm_lseed* seed, const float* dB, float tau0, float tau1, const MatD& Q,
+                      const MatD& qrR, const float* X, int64_t ldx, int c, const float* y, float vg,
+                      int precision, float* Py, float* snp_blup, float* blue, float* var_blue) {
+  const int64_t n = h->n, m = h->m;
+  DevF dX((size_t)n * std::max(c, 1)), dv(n);
+  // Py, SNP BLUP (:658-662)
+  abi(slemm_gpu_lsolve(seed, 0, 1, dB, n, tau1 - tau0, 1e-6f, dX.p, n, nullptr));
+  std::vector<float> py;
+  download(py, dX.p, n);
+  project_out(Q, py, n, 1);
+  for (int64_t i = 0; i < n; ++i) py[i] *= h->rr_host[i];
+  std::copy(py.begin(), py.end(), Py);
+  upload(dv.p, py);
+  {
+    DevF dbl(m);
+    abi(slemm_gpu_zt(h, dv.p, n, 1, dbl.p, m, 1, 0, precision, nullptr));
+    HIPCHECK(hipDeviceSynchronize());
+    HIPCHECK(hipMemcpy(snp_blup, dbl.p, sizeof(float) * m, hipMemcpyDeviceToHost));
+  }
+  // BLUE of the fixed effects (:674-677)
+  std::vector<float> hx;
+  abi(slemm_gpu_lsolve(seed, 1, c, dB + n, n, tau1 - tau0, 1e-6f, dX.p, n, nullptr));
+  download(hx, dX.p, (size_t)n * c);
+  MatD HinvX(n, c);
   for (int j = 0; j < c; ++j)
-    for (int64_t i = 0; i < n; ++i) A(i, j) = (double)h->rr_host[i] * (double)X[i + (size_t)j * ldx];
-  if (!thin_qr(A, Q, R, 1e-6)) fail(SLEMM_ERR_RANK, "Fixed-effects design matrix is not of full rank.");
-  std::vector<float> qf((size_t)n * c);
-  for (size_t e = 0; e < qf.size(); ++e) qf[e] = (float)Q.a[e];
-  set_Q(h, qf.data(), c);
-  return R;
-}
-
-// X - Q (Q' X) in place on an n x t float block (FP64 accumulation)
-void project_out(const MatD& Q, std::vector<float>& V, int64_t n, int t) {
-  for (int k = 0; k < t; ++k) {
-    float* v = &V[(size_t)k * n];
-    for (int q = 0; q < Q.c; ++q) {
+    for (int64_t i = 0; i < n; ++i) {
       double s = 0.0;
-      for (int64_t i = 0; i < n; ++i) s += Q(i, q) * v[i];
-      for (int64_t i = 0; i < n; ++i) v[i] = (float)((double)v[i] - Q(i, q) * s);
+      for (int q = 0; q < c; ++q) s += (double)hx[i + (size_t)q * n] * qrR(q, j);
+      HinvX(i, j) = h->rr_host[i] * s;
+    }
+  std::vector<double> XtHX((size_t)c * c), rhs(c);
+  for (int a2 = 0; a2 < c; ++a2) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += HinvX(i, a2) * y[i];
+    rhs[a2] = s;
+    for (int b2 = 0; b2 < c; ++b2) {
+      double t = 0.0;
+      for (int64_t i = 0; i < n; ++i) t += (double)X[i + (size_t)a2 * ldx] * HinvX(i, b2);
+      XtHX[a2 + (size_t)b2 * c] = t;
     }
   }
+  std::vector<double> bl = rhs, inv;
+  if (!chol_solve(XtHX, c, bl)) fail(SLEMM_ERR_RANK, "X' H^-1 X is not positive definite");
+  if (!invert(XtHX, c, inv)) fail(SLEMM_ERR_RANK, "X' H^-1 X is singular");
+  for (int q = 0; q < c; ++q) blue[q] = (float)bl[q];
+  if (var_blue)
+    for (int e = 0; e < c * c; ++e) var_blue[e] = (float)(inv[e] * vg);
 }
 
-double sq_norm(const float* v, int64_t n) {
-  double s = 0.0;
-  for (int64_t i = 0; i < n; ++i) s += (double)v[i] * v[i];
-  return s;
-}
-
-double seconds_since(std::chrono::steady_clock::time_point t0) {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-}
-
-}  // namespace
+}  // namespace slemm
 
 extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y,
                                    int rng_seed, int n_V, float tol_L, float s2max, float s2min, float tol_VC,
@@ -120,23 +120,9 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
     std::copy(yp.begin(), yp.end(), B.begin());
     for (int q = 0; q < c; ++q)
       for (int64_t i = 0; i < n; ++i) B[(size_t)(1 + q) * n + i] = (float)Q(i, q);
-    {
-      std::mt19937 rng(rng_seed + 54321);
-      std::uniform_real_distribution<float> runif(0.0, 1.0);
-      for (int j = 0; j < n_V; ++j) {
-        float* v = &B[(size_t)(1 + c + j) * n];
-        for (int64_t i = 0; i < n; ++i) v[i] = runif(rng) < 0.5 ? -1.0f : 1.0f;
-      }
-      for (int j = 0; j < n_V; ++j) {
-        float* v = &B[(size_t)(1 + c + j) * n];
-        float ss = 0.f;
-        for (int64_t i = 0; i < n; ++i) ss += v[i] * v[i];
-        const float nr = std::sqrt(ss);
-        for (int64_t i = 0; i < n; ++i) v[i] /= nr;
-      }
-    }
+    rademacher_probes(rng_seed, n, n_V, &B[(size_t)(1 + c) * n]);
     if (verbose) printf("Generated %d Rademacher probing vectors.\n", n_V);
-    DevF dB((size_t)n * T), dX((size_t)n * std::max(c, 1)), dv(n);
+    DevF dB((size_t)n * T), dX((size_t)n * std::max(c, 1));
     upload(dB.p, B);
 
     // seed systems y (QUADFORM), Q (LDET), probes (LDET) in one batch (:625-631)
@@ -224,50 +210,11 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
       if (verbose) printf("Itr %d: h2 in (%g, %g) and 2logLL = %g\n", i + 1, a, b, fm);
     }
 
-    // Py, SNP BLUP (:658-662)
+    // Py, SNP BLUP, BLUE (:655-677)
     const float tau1 = ve / vg;
-    abi(slemm_gpu_lsolve(sg.s, 0, 1, dB.p, n, tau1 - tau0, 1e-6f, dX.p, n, nullptr));
-    std::vector<float> py;
-    download(py, dX.p, n);
-    project_out(Q, py, n, 1);
-    for (int64_t i = 0; i < n; ++i) py[i] *= h->rr_host[i];
-    std::copy(py.begin(), py.end(), Py);
-    upload(dv.p, py);
-    {
-      DevF dbl(m);
-      abi(slemm_gpu_zt(h, dv.p, n, 1, dbl.p, m, 1, 0, precision, nullptr));
-      HIPCHECK(hipDeviceSynchronize());
-      HIPCHECK(hipMemcpy(snp_blup, dbl.p, sizeof(float) * m, hipMemcpyDeviceToHost));
-    }
+    finish_estimates(h, sg.s, dB.p, tau0, tau1, Q, qrR, X, ldx, c, y, vg, precision, Py, snp_blup, blue,
+                     var_blue);
     if (verbose) printf("\nVg,%g\nVe,%g\nh2,%g\nLRT,%g\n", vg, ve, vg / (vg + ve), lrt);
-
-    // BLUE of the fixed effects (:674-677)
-    abi(slemm_gpu_lsolve(sg.s, 1, c, dB.p + n, n, tau1 - tau0, 1e-6f, dX.p, n, nullptr));
-    download(hx, dX.p, (size_t)n * c);
-    MatD HinvX(n, c);
-    for (int j = 0; j < c; ++j)
-      for (int64_t i = 0; i < n; ++i) {
-        double s = 0.0;
-        for (int q = 0; q < c; ++q) s += (double)hx[i + (size_t)q * n] * qrR(q, j);
-        HinvX(i, j) = h->rr_host[i] * s;
-      }
-    std::vector<double> XtHX((size_t)c * c), rhs(c);
-    for (int a2 = 0; a2 < c; ++a2) {
-      double s = 0.0;
-      for (int64_t i = 0; i < n; ++i) s += HinvX(i, a2) * y[i];
-      rhs[a2] = s;
-      for (int b2 = 0; b2 < c; ++b2) {
-        double t = 0.0;
-        for (int64_t i = 0; i < n; ++i) t += (double)X[i + (size_t)a2 * ldx] * HinvX(i, b2);
-        XtHX[a2 + (size_t)b2 * c] = t;
-      }
-    }
-    std::vector<double> bl = rhs, inv;
-    if (!chol_solve(XtHX, c, bl)) fail(SLEMM_ERR_RANK, "X' H^-1 X is not positive definite");
-    if (!invert(XtHX, c, inv)) fail(SLEMM_ERR_RANK, "X' H^-1 X is singular");
-    for (int q = 0; q < c; ++q) blue[q] = (float)bl[q];
-    if (var_blue)
-      for (int e = 0; e < c * c; ++e) var_blue[e] = (float)(inv[e] * vg);
     if (verbose) printf("\nCompleted stochastic Lanczos DF-REML.\n");
     if (s2max - vg / (vg + ve) < tol_VC && verbose)
       printf("\nWarning: the h2 estimate is virtually the same as the max you specified.\n"

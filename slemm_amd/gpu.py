@@ -162,6 +162,13 @@ class Geno:
                                              _np_ptr(rr, C.c_float)))
         return gm, gw, rr
 
+    def set_gw(self, gw) -> None:
+        """Replaces the operator's per-SNP weights W (after set_model) with gw (m,)."""
+        g = np.ascontiguousarray(gw.detach().cpu().numpy() if torch.is_tensor(gw) else gw, dtype=np.float32)
+        if g.shape != (self.m,):
+            raise ValueError(f"gw must have shape ({self.m},)")
+        L.check(L.load().slemm_gpu_set_gw(self._h, _np_ptr(g, C.c_float)))
+
     # ------------------------------------------------------------------ operators
     def lnr_op_select(self, lnr, tau0: float, v: torch.Tensor, precision="bf16x2",
                       col_kind: Optional[Sequence] = None) -> torch.Tensor:
@@ -213,6 +220,24 @@ class Geno:
         out = torch.empty(self.n, dtype=torch.float64, device=a.device)
         fl = None if flip is None else np.ascontiguousarray(flip, dtype=np.uint8)
         L.check(L.load().slemm_gpu_gebv(self._h, _ptr(a), _np_ptr(fl, C.c_uint8), _ptr(out), _stream()))
+        return out
+
+    def adjoint(self, A: torch.Tensor, B: torch.Tensor, coef, scale_rr: bool = True,
+                precision="bf16x2") -> torch.Tensor:
+        """out_j = sum_k coef_k (Zc'A)_jk (Zc'B)_jk (rows of A, B scaled by r_rsqrt first when
+        scale_rr); A, B (n, t) float32 CUDA tensors.  Returns (m,) float64."""
+        Ac = _colmajor(A, self.n)
+        Bc = _colmajor(B, self.n)
+        t = Ac.shape[0]
+        if Bc.shape[0] != t:
+            raise ValueError("A and B must have the same number of columns")
+        cf = np.ascontiguousarray(coef, dtype=np.float64).reshape(-1)
+        if cf.size != t:
+            raise ValueError("need one coefficient per column")
+        out = torch.empty(self.m, dtype=torch.float64, device=Ac.device)
+        L.check(L.load().slemm_gpu_adjoint(self._h, _ptr(Ac), self.n, _ptr(Bc), self.n, t,
+                                           _np_ptr(cf, C.c_double), int(scale_rr), _ptr(out), _prec(precision),
+                                           _stream()))
         return out
 
     def gwas_score(self, Py: torch.Tensor, skip_mac: float = 20.0):
@@ -280,6 +305,48 @@ class Geno:
         return dict(vg=st.vg, ve=st.ve, h2=st.h2, Py=Py, snp_blup=blup, blue=blue, steps=st.lanczos_steps[0],
                     evals=st.criterion_evals, n_MC=n_MC, seconds_seed=st.seconds_seed,
                     seconds_total=st.seconds_total)
+
+    def reml_score(self, X: np.ndarray, y: np.ndarray, vg: float, ve: float,
+                   probes: Optional[torch.Tensor] = None, n_V: int = 30, rng_seed: int = 0,
+                   tol_L: float = 5e-4, precision="bf16x2"):
+        """(dl/dvg, dl/dve) and the AI matrix [[gg, ge], [ge, ee]] at (vg, ve); probes (n, n_V)
+        float32 CUDA with E[vv'] = I/n, or Rademacher probes as SLDF_REML draws them."""
+        Xf = np.asfortranarray(np.asarray(X, np.float32).reshape(self.n, -1))
+        c = Xf.shape[1]
+        yv = np.ascontiguousarray(y, np.float32)
+        pc = None
+        if probes is not None:
+            pc = _colmajor(probes, self.n)
+            n_V = pc.shape[0]
+        score = np.zeros(2)
+        ai = np.zeros(3)
+        L.check(L.load().slemm_gpu_reml_score(
+            self._h, _np_ptr(Xf, C.c_float), self.n, c, _np_ptr(yv, C.c_float), vg, ve, _ptr(pc), self.n, n_V,
+            rng_seed, tol_L, _prec(precision), _np_ptr(score, C.c_double), _np_ptr(ai, C.c_double)))
+        return score, np.array([[ai[0], ai[1]], [ai[1], ai[2]]])
+
+    def AI_REML(self, X: np.ndarray, y: np.ndarray, rng_seed: int = 0, n_V: int = 30, tol_L: float = 5e-4,
+                s2max: float = 0.7, vg0: float = 0.0, ve0: float = 0.0, tol_VC: float = 1e-4,
+                max_iter: int = 20, precision="bf16x2", verbose: bool = False) -> dict:
+        """Average-information REML on (vg, ve) for the SLDF_REML model (no reference counterpart)."""
+        Xf = np.asfortranarray(np.asarray(X, np.float32).reshape(self.n, -1))
+        c = Xf.shape[1]
+        yv = np.ascontiguousarray(y, np.float32)
+        Py = np.zeros(self.n, np.float32)
+        blup = np.zeros(self.m, np.float32)
+        blue = np.zeros(c, np.float32)
+        vb = np.zeros((c, c), np.float32, order="F")
+        hist = np.zeros((max_iter, 7))
+        st = L.RemlStats()
+        L.check(L.load().slemm_gpu_ai_reml(
+            self._h, _np_ptr(Xf, C.c_float), self.n, c, _np_ptr(yv, C.c_float), rng_seed, n_V, tol_L, s2max, vg0,
+            ve0, tol_VC, max_iter, _prec(precision), int(verbose), _np_ptr(Py, C.c_float),
+            _np_ptr(blup, C.c_float), _np_ptr(blue, C.c_float), _np_ptr(vb, C.c_float),
+            _np_ptr(hist, C.c_double), C.byref(st)))
+        it = st.criterion_evals
+        return dict(vg=st.vg, ve=st.ve, h2=st.h2, Py=Py, snp_blup=blup, blue=blue, var_blue=vb,
+                    iterations=it, history=hist[:it], steps=tuple(st.lanczos_steps),
+                    seconds_seed=st.seconds_seed, seconds_total=st.seconds_total)
 
     # ------------------------------------------------------------------ Krylov engine
     def L_Seed(self, lnr, tau0: float, B: torch.Tensor, tol: float = 5e-4, maxit: int = 600,
