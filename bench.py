@@ -123,22 +123,30 @@ def small_k_passes(g, lib, L, n, m, ks=(1, 3, 7, 16), reps=3):
     return out
 
 
-def reml_time_to_converge(g, n, m, nv, n_causal, h2):
-    """SLDF-REML (src/reml.cpp:549-690) on the resident bed for a phenotype simulated from it:
-    y = Zc beta + e, beta ~ N(0,1) on n_causal random SNPs, var(e) set for the target h2.
-    Time from the resident bed to the final variance components, split-BF16 and FP32."""
+def simulate_phenotype(g, n, m, n_causal, h2, seed):
+    """y = Zc beta + e + 10, beta ~ N(0,1) on n_causal random SNPs, var(e) set for h2."""
     import numpy as np
     import torch
 
-    rng = np.random.default_rng(11)
+    rng = np.random.default_rng(seed)
     beta = np.zeros(m, np.float32)
     beta[rng.choice(m, size=n_causal, replace=False)] = rng.normal(size=n_causal).astype(np.float32)
     gv = g.matvec(torch.from_numpy(beta).cuda(), precision="fp32")[:, 0].double().cpu().numpy()
     ve = gv.var() * (1 - h2) / h2
     y = (gv + rng.normal(scale=np.sqrt(ve), size=n) + 10.0).astype(np.float32)
+    return y, beta, gv, float(gv.var() / (gv.var() + ve))
+
+
+def reml_time_to_converge(g, n, m, nv, y, realised_h2, n_causal, h2):
+    """SLDF-REML (src/reml.cpp:549-690) on the resident bed for a phenotype simulated from it.
+    Time from the resident bed to the final variance components, split-BF16 and FP32; plus the
+    AI-REML driver (average-information Newton on the same model and probes) in split-BF16."""
+    import numpy as np
+    import torch
+
     X = np.ones((n, 1), np.float32)
     out = {"workload": f"SLDF-REML, {nv} Rademacher probes, simulated h2={h2} with {n_causal} causal SNPs",
-           "realised_h2": round(float(gv.var() / (gv.var() + ve)), 5)}
+           "realised_h2": round(realised_h2, 5)}
     res = {}
     for prec in ("bf16x2", "fp32"):
         torch.cuda.synchronize()
@@ -153,7 +161,65 @@ def reml_time_to_converge(g, n, m, nv, n_causal, h2):
     out["speedup_bf16x2_over_fp32"] = round(out["fp32"]["seconds"] / out["bf16x2"]["seconds"], 2)
     out["vc_rel_diff_bf16x2_vs_fp32"] = {
         k: float(abs(res["bf16x2"][k] - res["fp32"][k]) / abs(res["fp32"][k])) for k in ("vg", "ve", "h2")}
-    return out
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = g.AI_REML(X, y, rng_seed=0, n_V=nv, precision="bf16x2")
+    torch.cuda.synchronize()
+    out["ai_reml_bf16x2"] = {"seconds": round(time.perf_counter() - t0, 3), "iterations": int(r["iterations"]),
+                             "lanczos_steps": list(r["steps"]), "h2": round(float(r["h2"]), 6),
+                             "vg": float(r["vg"]), "ve": float(r["ve"])}
+    return out, res["bf16x2"]
+
+
+def differentiation_run(g, n, m, y, vg, ve, nv=64, win_size=100):
+    """dl/dsigma_g^2, dl/dsigma_e^2 and dl/d(window weights) of the REML log-likelihood through the
+    autograd harness: window weights = windowed mean of a marginal chi-square scan (100-SNP
+    windows) normalised to mean 1; forward = Lanczos seed + CG solves + SLQ on [y, nv probes],
+    backward = one adjoint pass over the bed.  The harness's dl/dsigma^2 is checked against the
+    device REML score on the same probes."""
+    import numpy as np
+    import torch
+    from slemm_amd.autodiff import RemlProblem, loglik
+
+    dev = torch.device("cuda")
+    X = np.ones((n, 1), np.float32)
+    # seeded marginal scan: chi2_j = (x_c'y_c)^2 / (x_c'x_c var(y))
+    g.set_model(None, None, np.full((n, 1), 1 / np.sqrt(n), np.float32))
+    yc = torch.from_numpy(y - y.mean()).to(dev)
+    xy, xx = g.gwas_score(yc, skip_mac=0.0)
+    chi2 = (xy.double() ** 2 / xx.double() / float(np.var(y))).nan_to_num(0.0)
+    nwin = -(-m // win_size)
+    win = torch.arange(m, device=dev) // win_size
+    wsum = torch.zeros(nwin, dtype=torch.float64, device=dev).index_add_(0, win, chi2)
+    wcnt = torch.bincount(win, minlength=nwin).double()
+    wmean = wsum / wcnt
+    w0 = (wmean / wmean.mean()).clamp_min(1e-3)
+    gen = torch.Generator(device=dev).manual_seed(17)
+    V = torch.where(torch.rand((n, nv), generator=gen, device=dev) < 0.5, -1.0, 1.0) / np.sqrt(n)
+    prob = RemlProblem(g, X, y, probes=V, tol=1e-5)
+    w = w0.clone().requires_grad_(True)
+    a = torch.tensor(vg, dtype=torch.float64, device=dev, requires_grad=True)
+    b = torch.tensor(ve, dtype=torch.float64, device=dev, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ll = loglik(prob, w, a, b, win)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ll.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    # the device score on the same probes, with the same window weights as persnp
+    g.set_model(w0[win].float().cpu().numpy(), None, None)
+    score, _ = g.reml_score(X, y, vg, ve, probes=V, tol_L=1e-5)
+    got = np.array([float(a.grad), float(b.grad)])
+    g.set_model()
+    return {"workload": f"n={n} m={m}, {nwin} windows of {win_size} SNPs, {nv} probes + y",
+            "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
+            "loglik": float(ll.detach()), "dl_dsigma_g2": got[0], "dl_dsigma_e2": got[1],
+            "dl_dw_norm": float(torch.linalg.vector_norm(w.grad)),
+            "device_score": [float(score[0]), float(score[1])],
+            "score_rel_diff": [float(abs(got[k] - score[k]) / max(abs(score[k]), 1e-30)) for k in range(2)],
+            "euler_check_w_dot_grad": float(w.grad @ w0)}
 
 
 def scale_run(n, m, nv, n_causal, h2, seed=5):
@@ -178,13 +244,9 @@ def scale_run(n, m, nv, n_causal, h2, seed=5):
     torch.cuda.synchronize()
     out["generate_s"] = round(time.perf_counter() - t_all, 3)
     g.set_model(None, None, None)
-    beta = np.zeros(m, np.float32)
-    beta[rng.choice(m, size=n_causal, replace=False)] = rng.normal(size=n_causal).astype(np.float32)
-    gv = g.matvec(torch.from_numpy(beta).cuda(), precision="fp32")[:, 0].double().cpu().numpy()
-    ve = gv.var() * (1 - h2) / h2
-    y = (gv + rng.normal(scale=np.sqrt(ve), size=n) + 10.0).astype(np.float32)
+    y, beta, gv, rh2 = simulate_phenotype(g, n, m, n_causal, h2, seed + 1)
     X = np.ones((n, 1), np.float32)
-    out["realised_h2"] = round(float(gv.var() / (gv.var() + ve)), 5)
+    out["realised_h2"] = round(rh2, 5)
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -236,6 +298,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-reml", action="store_true")
     ap.add_argument("--no-smallk", action="store_true")
+    ap.add_argument("--no-diff", action="store_true", help="skip the autograd/adjoint run")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM byte counters)")
@@ -319,7 +382,12 @@ def main():
     f2, _ = kernel_ms(lib, 1, L.PREC_FP32)
     lib.slemm_gpu_timing_reset()
 
-    reml = None if args.no_reml else reml_time_to_converge(g, n, m, nv, args.causal, args.h2)
+    reml = diff = None
+    if not args.no_reml:
+        y, _, _, rh2 = simulate_phenotype(g, n, m, args.causal, args.h2, 11)
+        reml, sl = reml_time_to_converge(g, n, m, nv, y, rh2, args.causal, args.h2)
+        if not args.no_diff:
+            diff = differentiation_run(g, n, m, y, float(sl["vg"]), float(sl["ve"]), nv=nv)
     smallk = None if args.no_smallk else small_k_passes(g, lib, L, n, m)
 
     pb = pass_bytes(n, m, K)
@@ -351,6 +419,7 @@ def main():
         "bed_generate_s": round(t_load, 3),
         "autotune": tune,
         "reml_time_to_converge": reml,
+        "differentiation": diff,
         "small_k_passes": smallk,
         "roofline": {
             "bound": "mfma",
