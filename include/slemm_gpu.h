@@ -247,11 +247,12 @@ int slemm_gpu_ai_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const f
 
 /* Autotuner of the split-BF16 block products (no reference counterpart; the reference's only
  * blocking knob is --subset_size, src/option.cpp:75,297).  Times every kernel variant (B operand
- * from global memory with 4 or 8 row tiles per wave or with table-driven code unpacking, or
- * LDS-staged in 4- or 2-step groups) at three grid sizes for both passes of a t-column operator
- * on this handle's shape, and keeps the fastest per pass for later calls.  Host outputs (any may
- * be NULL): variant[2], target[2] (workgroups the grid aims for), ms[32] = best per pass, then
- * pass*15 + variant*3 + grid-size index. */
+ * from global memory with 4 or 8 row tiles per wave, with table-driven code unpacking or
+ * software-pipelined, or LDS-staged in 4- or 2-step groups) at three grid sizes for both passes of
+ * a t-column operator on this handle's shape, and keeps the fastest per pass for later calls.
+ * Host outputs (any may be NULL): variant[2], target[2] (workgroups the grid aims for),
+ * ms[SLEMM_AUTOTUNE_MS] = best per pass, then 2 + pass*18 + variant*3 + grid-size index. */
+#define SLEMM_AUTOTUNE_MS 38
 int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, int* target, double* ms);
 
 /* ---------------------------------------------------------------- instrumentation ---------- */

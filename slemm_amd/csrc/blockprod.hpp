@@ -117,7 +117,10 @@ struct Fin2Args {
 //  BP_REG8: as BP_REG with 8 row tiles per wave (half the B traffic per MFMA, 1 wave/SIMD)
 //  BP_REGL: as BP_REG with the codes unpacked through a 16-entry LDS table (fewer VALU ops;
 //           pays off when few MFMAs share each unpacked operand, i.e. small K)
-enum { BP_REG = 0, BP_LDS4 = 1, BP_LDS2 = 2, BP_REG8 = 3, BP_REGL = 4, BP_NVARIANTS = 5 };
+//  BP_PIPE: as BP_REG, software-pipelined: B fragments requested one step ahead, codes half a
+//           super-chunk ahead (NT <= 5; wider tiles run BP_REG)
+enum { BP_REG = 0, BP_LDS4 = 1, BP_LDS2 = 2, BP_REG8 = 3, BP_REGL = 4, BP_PIPE = 5, BP_NVARIANTS = 6 };
+static_assert(BP_PIPE == 5, "slemm_geno's default block-product variant is BP_PIPE");
 
 int lds_nt(int t);
 // number of reduction splits so that the grid has about target_wgs workgroups

@@ -246,6 +246,98 @@ __global__ __launch_bounds__(256) void k_bp_reg(LdsArgs a) {
     }
 }
 
+// Software-pipelined register variant.  Same tiles and partial output as k_bp_reg, but loads
+// are requested well before their data is used, without a second register set for B:
+//  * B: the fragment register of column tile j (hi, then lo) is refilled with step T+1's
+//    fragment as soon as the RT MFMAs of step T that read it have been issued, so each load has
+//    about one step of matrix work (2*RT*NT MFMAs) to arrive;
+//  * codes: the 512-element super-chunk is consumed in two halves of 8 steps (16 bytes of codes
+//    per lane and row tile) with two register sets; the next half is requested when the current
+//    one starts.
+// Loads past the split's last step re-read that step (valid addresses, no branches).
+template <int RT, int NT, bool SQ>
+__global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t row0 = (int64_t)blockIdx.x * (64 * RT) + wave * (16 * RT);
+  const int col0 = blockIdx.y * (16 * NT);
+  const int64_t s0 = (int64_t)blockIdx.z * a.ns_per_split;
+  const int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
+  const uint4* xh = reinterpret_cast<const uint4*>(a.X);
+  const uint4* xl = xh + a.lo_offset16;
+  const uint8_t* gbase = a.G + (row0 + r) * a.ldg + g * 32;
+  f32x4_t acc[RT][NT];
+#pragma unroll
+  for (int i = 0; i < RT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+
+  // codes of half-chunk hc = 2s + h (8 steps): a super-chunk s spans row bytes [128 s, 128 s + 128),
+  // lane group g owns 32 of them and half h is its bytes [16 h, 16 h + 16)
+  auto load_codes = [&](int64_t hc, uint32_t (&dst)[RT][4]) {
+#pragma unroll
+    for (int i = 0; i < RT; ++i) {
+      const u32x4_t q = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(
+          gbase + (int64_t)i * 16 * a.ldg + (hc >> 1) * 128 + (hc & 1) * 16));
+      dst[i][0] = q.x; dst[i][1] = q.y; dst[i][2] = q.z; dst[i][3] = q.w;
+    }
+  };
+  // B fragments of global step T (32 reduction elements) sit in block row s*64 + g*16 + T%16
+  auto boff_of = [&](int64_t T) { return ((T >> 4) * 64 + g * 16 + (T & 15)) * a.KP + col0 + r; };
+  const int64_t Tlast = s1 * 16 - 1;
+  bf16x8_t bh[NT], bl[NT];
+  auto step = [&](const uint32_t (&cc)[RT][4], int t8, int64_t T) {
+    bf16x8_t av[RT];
+#pragma unroll
+    for (int i = 0; i < RT; ++i) av[i] = unpack8<SQ>((cc[i][t8 >> 1] >> (16 * (t8 & 1))) & 0xFFFFu);
+    const int64_t bn = boff_of(T < Tlast ? T + 1 : T);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+#pragma unroll
+      for (int i = 0; i < RT; ++i)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bh[j], acc[i][j], 0, 0, 0);
+      bh[j] = __builtin_bit_cast(bf16x8_t, xh[bn + j * 16]);
+      __builtin_amdgcn_sched_barrier(0);  // keep the refill after the MFMAs that read bh[j]
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+#pragma unroll
+      for (int i = 0; i < RT; ++i)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl[j], acc[i][j], 0, 0, 0);
+      bl[j] = __builtin_bit_cast(bf16x8_t, xl[bn + j * 16]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+
+  uint32_t cA[RT][4], cB[RT][4];
+  if (s0 < s1) {
+    load_codes(2 * s0, cA);
+    const int64_t b0 = boff_of(s0 * 16);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      bh[j] = __builtin_bit_cast(bf16x8_t, xh[b0 + j * 16]);
+      bl[j] = __builtin_bit_cast(bf16x8_t, xl[b0 + j * 16]);
+    }
+  }
+  for (int64_t s = s0; s < s1; ++s) {
+    load_codes(2 * s + 1, cB);
+#pragma unroll
+    for (int t8 = 0; t8 < 8; ++t8) step(cA, t8, s * 16 + t8);
+    load_codes(s + 1 < s1 ? 2 * s + 2 : 2 * s + 1, cA);
+#pragma unroll
+    for (int t8 = 0; t8 < 8; ++t8) step(cB, t8, s * 16 + 8 + t8);
+  }
+#pragma unroll
+  for (int i = 0; i < RT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int64_t col = col0 + 16 * j + r;
+      const int64_t row = row0 + 16 * i + 4 * g;
+      float4* dst = reinterpret_cast<float4*>(a.part + ((int64_t)blockIdx.z * a.KP + col) * a.rows_pad + row);
+      *dst = make_float4(acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]);
+    }
+}
+
 // Pass-1 epilogue from the split partials: T_jk = w_j (P_jk - gm_j s_k + xbar_k (pc_j - n gm_j));
 // writes the BF16 hi/lo pass-2 operand, optionally the column-major result, and per-block
 // partial sums of gm_j * represented(T_jk).  Thread (tx = column, ty): 8 blocks of 8 rows.
@@ -343,6 +435,10 @@ static void launch_variant(const LdsArgs& a, int variant, dim3 grid, hipStream_t
       grid.x /= 2;  // 8 row tiles per wave: 512 rows per workgroup
       if constexpr (NT <= 5) k_bp_reg<8, NT, SQ><<<grid, 256, 0, st>>>(a);
       else k_bp_reg<kLdsRT, NT, SQ><<<dim3(grid.x * 2, grid.y, grid.z), 256, 0, st>>>(a);
+      break;
+    case BP_PIPE:  // wider column tiles exceed two waves' register budget: plain register variant
+      if constexpr (NT <= 5) k_bp_pipe<kLdsRT, NT, SQ><<<grid, 256, 0, st>>>(a);
+      else k_bp_reg<kLdsRT, NT, SQ><<<grid, 256, 0, st>>>(a);
       break;
     case BP_LDS4: k_bp_lds<kLdsRT, NT, 4, true, SQ><<<grid, 256, 0, st>>>(a); break;
     case BP_LDS2: k_bp_lds<kLdsRT, NT, 2, false, SQ><<<grid, 256, 0, st>>>(a); break;

@@ -95,10 +95,11 @@ struct slemm_geno {
   slemm::DevBuf redpart;       // partials of column reductions (double)
   slemm::DevBuf partbuf;       // split-reduction partial products of the block products
   std::vector<int> kinds_host;
-  // block-product variant and grid target per (pass, column tiles), set by the autotuner
-  int bp_variant[2][9] = {{0, 0, 0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0, 0, 0, 0}};
-  int bp_target[2][9] = {{1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024},
-                         {1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024}};
+  // block-product variant and grid target per (pass, column tiles), set by the autotuner; the
+  // default is the software-pipelined register kernel (5 = BP_PIPE) at a 2048-workgroup grid
+  int bp_variant[2][9] = {{5, 5, 5, 5, 5, 5, 5, 5, 5}, {5, 5, 5, 5, 5, 5, 5, 5, 5}};
+  int bp_target[2][9] = {{2048, 2048, 2048, 2048, 2048, 2048, 2048, 2048, 2048},
+                         {2048, 2048, 2048, 2048, 2048, 2048, 2048, 2048, 2048}};
   std::vector<float> rr_host;    // r_rsqrt (n)
   std::vector<float> rvec_host;  // R (n)
 };

@@ -345,6 +345,21 @@ void col_dots(slemm_geno* h, const float* A, int64_t lda, const float* B, int64_
 
 using namespace slemm;
 
+static_assert(2 + 2 * BP_NVARIANTS * 3 == SLEMM_AUTOTUNE_MS, "autotune timing table size");
+
+// Pseudo-random BF16 pairs for autotuning: each 32-bit word holds two BF16 values in (-1, 1)
+// (hi halves of the operand) or about 2^-9 of that (lo halves: the upper half of the buffer).
+static __global__ void k_fill_split(uint32_t* __restrict__ w, size_t nwords, uint32_t salt) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x) {
+    uint32_t x = (uint32_t)i * 0x9E3779B9u ^ (salt * 0x85EBCA6Bu);
+    x ^= x >> 15; x *= 0x2C1B3C6Du; x ^= x >> 12; x *= 0x297A2D39u; x ^= x >> 15;
+    const uint32_t ex = i < nwords / 2 ? 0x3F00u : 0x3B00u;  // BF16 exponents of [0.5,1) / 2^-9 scale
+    const uint32_t a = ex | (x & 0x7Fu) | ((x & 0x100u) << 7);
+    const uint32_t b = ex | ((x >> 16) & 0x7Fu) | ((x & 0x200u) << 6);
+    w[i] = a | (b << 16);
+  }
+}
+
 extern "C" int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, int* target, double* best_ms) {
   return guard([&] {
     REQUIRE(h, "null handle");
@@ -353,8 +368,11 @@ extern "C" int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, 
     const Layout L = layout_for(h, t, PREC_BF16X2);
     h->x1.reserve(L.x1_bytes);
     h->x2.reserve(L.x2_bytes);
-    HIPCHECK(hipMemset(h->x1.p, 0, L.x1_bytes));
-    HIPCHECK(hipMemset(h->x2.p, 0, L.x2_bytes));
+    // operands with the value spread of real split-BF16 blocks (hi ~ U(-1,1), lo ~ 2^-9 hi): zero
+    // operands draw less power and would favour variants that only win at boosted clocks
+    k_fill_split<<<4096, 256>>>(reinterpret_cast<uint32_t*>(h->x1.p), L.x1_bytes / 4, 1);
+    k_fill_split<<<4096, 256>>>(reinterpret_cast<uint32_t*>(h->x2.p), L.x2_bytes / 4, 2);
+    HIPCHECK(hipGetLastError());
     const int64_t targets[] = {512, 1024, 2048};
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));

@@ -256,11 +256,12 @@ class Geno:
         """Times the block-product variants for a t-column operator and keeps the fastest."""
         v = (C.c_int * 2)()
         tg = (C.c_int * 2)()
-        ms = (C.c_double * 32)(*([float("nan")] * 32))
+        names = ["reg", "lds4", "lds2", "reg8", "regl", "pipe"]
+        nv = len(names)
+        ms = (C.c_double * (2 + 6 * nv))(*([float("nan")] * (2 + 6 * nv)))
         L.check(L.load().slemm_gpu_autotune(self._h, t, reps, v, tg, ms))
-        names = ["reg", "lds4", "lds2", "reg8", "regl"]
-        table = {f"pass{p + 1}/{names[vv]}/{[512, 1024, 2048][ti]}": round(ms[2 + p * 15 + vv * 3 + ti], 3)
-                 for p in range(2) for vv in range(5) for ti in range(3)}
+        table = {f"pass{p + 1}/{names[vv]}/{[512, 1024, 2048][ti]}": round(ms[2 + p * 3 * nv + vv * 3 + ti], 3)
+                 for p in range(2) for vv in range(nv) for ti in range(3)}
         return {"pass1": (names[v[0]], tg[0], round(ms[0], 3)), "pass2": (names[v[1]], tg[1], round(ms[1], 3)),
                 "table_ms": table}
 
