@@ -51,6 +51,7 @@ enum {
 
 typedef struct slemm_geno slemm_geno;   /* device-resident packed genotypes + model terms */
 typedef struct slemm_lseed slemm_lseed; /* device-resident Lanczos seed system (struct lseed) */
+typedef struct slemm_traits slemm_traits; /* batch of traits on individual subsets of one handle */
 
 const char* slemm_gpu_last_error(void);
 int slemm_gpu_version(void);
@@ -242,6 +243,31 @@ int slemm_gpu_ai_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const f
                       int n_V, float tol_L, float s2max, double vg0, double ve0, double tol_VC,
                       int max_iter, int precision, int verbose, float* Py, float* snp_blup, float* blue,
                       float* var_blue, double* history, slemm_reml_stats* stats);
+
+/* ---------------------------------------------------------------- batched traits ----------- */
+/* Several traits on individual subsets of one handle, run as one batch (no reference
+ * counterpart: the reference analyses one trait per run on its own individual set,
+ * src/input.cpp:389-454).  masks: T x n host bytes (row t = trait t, 1 = the individual belongs
+ * to the trait's subset).  Each trait gets gm and gw of src/reml.cpp:612-622 over its subset (a
+ * SNP without variation in a subset gets weight 0 and is counted in `monomorphic`) and an
+ * intercept as its only covariate; R = I. */
+int slemm_gpu_traits_create(slemm_geno* h, int T, const uint8_t* masks, slemm_traits** out);
+int slemm_gpu_traits_destroy(slemm_traits* tr);
+/* Host outputs (T each, may be NULL): subset sizes and per-subset monomorphic SNP counts. */
+int slemm_gpu_traits_info(const slemm_traits* tr, int64_t* n_t, int64_t* monomorphic);
+/* Host copies (m each, may be NULL) of one trait's gm, gw and dosage sums over its subset. */
+int slemm_gpu_traits_model(const slemm_traits* tr, int trait, float* gm, float* gw, int32_t* popcnt);
+/* Batched lnr_op_select: column k (device V, n x t) is a vector of trait col_trait[k] (zero
+ * outside its subset) and gets that trait's operator of kind col_kind[k] (NULL: all LDET), the
+ * QUADFORM projection being off the trait's intercept.  All columns share one split-BF16 block
+ * product per pass (SLEMM_PREC_BF16X2 only).  col_trait, col_kind: host, t entries. */
+int slemm_gpu_traits_op(slemm_traits* tr, const int* col_trait, const int* col_kind, float tau0,
+                        const float* V, int64_t ldv, int t, float* out, int64_t ldo, int precision,
+                        void* stream);
+/* L_Seed over the batched operator; arguments as slemm_gpu_lseed plus col_trait. */
+int slemm_gpu_traits_lseed(slemm_traits* tr, const int* col_trait, const int* col_kind, const int* col_group,
+                           float tau0, const float* B, int64_t ldb, int t, float tol, int maxit, int keep_cols,
+                           int precision, int verbose, void* stream, slemm_lseed** out);
 
 /* ---------------------------------------------------------------- tuning ------------------- */
 

@@ -72,6 +72,14 @@ SIGNATURES = {
     "slemm_gpu_lmc_reml": (C.c_int, [c_vp, c_fp, c_i64, C.c_int, c_fp, C.c_int, C.c_int, C.c_float, C.c_float,
                                      C.c_float, C.c_int, C.c_int, c_fp, c_fp, c_fp, C.POINTER(RemlStats)]),
     "slemm_gpu_set_gw": (C.c_int, [c_vp, c_fp]),
+    "slemm_gpu_traits_create": (C.c_int, [c_vp, C.c_int, c_u8p, C.POINTER(c_vp)]),
+    "slemm_gpu_traits_destroy": (C.c_int, [c_vp]),
+    "slemm_gpu_traits_info": (C.c_int, [c_vp, c_i64p, c_i64p]),
+    "slemm_gpu_traits_model": (C.c_int, [c_vp, C.c_int, c_fp, c_fp, c_i32p]),
+    "slemm_gpu_traits_op": (C.c_int, [c_vp, c_intp, c_intp, C.c_float, c_vp, c_i64, C.c_int, c_vp, c_i64, C.c_int,
+                                      c_vp]),
+    "slemm_gpu_traits_lseed": (C.c_int, [c_vp, c_intp, c_intp, c_intp, C.c_float, c_vp, c_i64, C.c_int, C.c_float,
+                                         C.c_int, C.c_int, C.c_int, C.c_int, c_vp, C.POINTER(c_vp)]),
     "slemm_gpu_adjoint": (C.c_int, [c_vp, c_vp, c_i64, c_vp, c_i64, C.c_int, c_dp, C.c_int, c_vp, C.c_int, c_vp]),
     "slemm_gpu_reml_score": (C.c_int, [c_vp, c_fp, c_i64, C.c_int, c_fp, C.c_double, C.c_double, c_vp, c_i64,
                                        C.c_int, C.c_int, C.c_float, C.c_int, c_dp, c_dp]),

@@ -352,9 +352,14 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
   double csum = 0.0;
   const bool live = k < a.t;
   const bool proj = live && a.qtv && (a.kinds == nullptr || a.kinds[k] == 1) && a.c > 0;
+  // batched traits: the column's subset mask, intercept basis and subset size
+  const int trait = (live && a.tc.col_trait) ? a.tc.col_trait[k] : -1;
+  const float* tmask = trait >= 0 ? a.tc.mask + (int64_t)trait * a.tc.ld_n : nullptr;
+  const float* Qk = trait >= 0 ? a.tc.q + (int64_t)trait * a.tc.ld_n : a.Q;
+  const double nk = trait >= 0 ? a.tc.n_t[trait] : (double)a.n;
   // column mean removed before the product (the exact remainder is added back in the pass-1
   // epilogue), so the MFMA accumulates a centred sum and a constant column costs no accuracy
-  const float xbar = (live && a.xsum) ? (float)(a.xsum[k] / (double)a.n) : 0.f;
+  const float xbar = (live && a.xsum) ? (float)(a.xsum[k] / nk) : 0.f;
   if (k < a.KP) {
     for (int b8 = 0; b8 < 8; ++b8) {
       const int64_t i0 = rbase + b8 * 8;
@@ -363,11 +368,11 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
       for (int e = 0; e < 8; ++e) {
         const int64_t i = i0 + e;
         float val = 0.f;
-        if (live && i < a.n) {
+        if (live && i < a.n && (!tmask || tmask[i] != 0.f)) {
           float v = a.V[i + (int64_t)k * a.ldv];
           if (proj) {
             double acc = 0.0;
-            for (int q = 0; q < a.c; ++q) acc += (double)a.Q[i + (int64_t)q * a.ldq] * a.qtv[q * a.KP + k];
+            for (int q = 0; q < a.c; ++q) acc += (double)Qk[i + (int64_t)q * a.ldq] * a.qtv[q * a.KP + k];
             v = (float)((double)v - acc);
           }
           val = a.rr ? a.rr[i] * v : v;

@@ -39,6 +39,20 @@ struct BlockProdArgs {
   const int* kinds = nullptr;  // per-column kinds (device); LDET columns get tau0 v here
 };
 
+// Per-column model terms of a batch of traits on individual subsets (all null: one model).
+// Column k belongs to trait col_trait[k]; per-trait arrays are trait-major with the leading
+// dimension given (n_pad for individual-indexed, m_pad for SNP-indexed arrays).
+struct TraitCols {
+  const int* col_trait = nullptr;  // K (device)
+  const float* mask = nullptr;     // T x ld_n: 1 for members of the trait's subset, else 0
+  const float* q = nullptr;        // T x ld_n: intercept basis mask / sqrt(n_t)
+  const double* n_t = nullptr;     // T subset sizes
+  const float* gm = nullptr;       // T x ld_m: 2 x A2 frequency over the subset
+  const float* gw = nullptr;       // T x ld_m: SNP weights over the subset
+  const int32_t* pc = nullptr;     // T x ld_m: dosage sums over the subset
+  int64_t ld_n = 0, ld_m = 0;
+};
+
 struct PackArgs {
   const float* V = nullptr;
   int64_t ldv = 0;
@@ -58,6 +72,7 @@ struct PackArgs {
   const double* xsum = nullptr; // if set, x_ik -= float(xsum_k / n) before packing
   double* part = nullptr;       // (rows_pad/256) x KP partial column sums
   const float* wsum = nullptr;  // optional row weights of the column sums (sum_i wsum_i x_ik)
+  TraitCols tc;                 // batched traits: per-column subset, intercept and de-meaning
 };
 
 // ---- split-BF16 block product with LDS-staged vector block (blockprod_lds.hip)
@@ -93,6 +108,7 @@ struct Fin1Args {
   float* out = nullptr;            // optional column-major rows x t
   int64_t ldo = 0;
   bool raw = false;                // out = summed partial products, no centring or weighting
+  TraitCols tc;                    // batched traits: per-column gm, gw, popcounts, subset size
 };
 
 struct Fin2Args {
@@ -108,6 +124,7 @@ struct Fin2Args {
   const int* kinds = nullptr;
   float* out = nullptr;
   int64_t ldo = 0;
+  TraitCols tc;                    // batched traits: rows outside the column's subset are zeroed
 };
 
 // Block-product variants (selected per shape by the autotuner, slemm_gpu_autotune):

@@ -347,9 +347,15 @@ __global__ __launch_bounds__(256) void k_fin1(Fin1Args a) {
   const int k = blockIdx.x * 64 + tx;
   const int64_t rbase = (int64_t)blockIdx.y * 256 + ty * 64;
   double usum = 0.0;
+  // batched traits: this column's per-SNP model terms and subset size
+  const int trait = (a.tc.col_trait && k < a.t) ? a.tc.col_trait[k] : -1;
+  const float* gmk = trait >= 0 ? a.tc.gm + (int64_t)trait * a.tc.ld_m : a.gm;
+  const float* gwk = trait >= 0 ? (a.gw ? a.tc.gw + (int64_t)trait * a.tc.ld_m : nullptr) : a.gw;
+  const int32_t* pck = trait >= 0 ? a.tc.pc + (int64_t)trait * a.tc.ld_m : a.popcnt;
+  const double nk = trait >= 0 ? a.tc.n_t[trait] : a.xsum_n;
   if (k < a.KP) {
     const double sk = a.colsum[k];
-    const double xb = (a.xsum && k < a.t) ? (double)(float)(a.xsum[k] / a.xsum_n) : 0.0;
+    const double xb = (a.xsum && k < a.t) ? (double)(float)(a.xsum[k] / nk) : 0.0;
     for (int b8 = 0; b8 < 8; ++b8) {
       const int64_t i0 = rbase + b8 * 8;
       float P[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -363,9 +369,9 @@ __global__ __launch_bounds__(256) void k_fin1(Fin1Args a) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int64_t row = i0 + e;
-        const float gmv = a.gm[row];
-        const float wv = a.gw ? a.gw[row] : 1.0f;
-        const double rem = xb * ((double)a.popcnt[row] - a.xsum_n * (double)gmv);
+        const float gmv = gmk[row];
+        const float wv = gwk ? gwk[row] : 1.0f;
+        const double rem = xb * ((double)pck[row] - nk * (double)gmv);
         const float tv = a.raw ? P[e] : wv * (float)((double)P[e] - (double)gmv * sk + rem);
         hb[e] = bf16_bits_rn(tv);
         lb[e] = bf16_bits_rn(tv - bf16_to_float(hb[e]));
@@ -400,12 +406,14 @@ __global__ __launch_bounds__(256) void k_fin2(Fin2Args a) {
   const float Pv[4] = {P.x, P.y, P.z, P.w};
   const double uk = a.colsum[k];
   const bool add_tau = a.v && (a.kinds == nullptr || a.kinds[k] == 0);
+  const float* tmask = a.tc.col_trait ? a.tc.mask + (int64_t)a.tc.col_trait[k] * a.tc.ld_n : nullptr;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const int64_t row = i + e;
     if (row >= a.rows) break;
     float zv = (float)((double)Pv[e] - uk);
     if (a.rr) zv *= a.rr[row];
+    if (tmask) zv *= tmask[row];
     if (add_tau) zv += a.tau0 * a.v[row + (int64_t)k * a.ldv];
     a.out[row + (int64_t)k * a.ldo] = zv;
   }

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "blockprod.hpp"
+#include "lanczos.hpp"
 #include "tridiag.hpp"
 
 struct slemm_lseed {
@@ -146,20 +147,17 @@ __global__ void k_lsolve(const float* const* __restrict__ Ulist, int64_t ldu, in
 
 using namespace slemm;
 
-extern "C" int slemm_gpu_lseed(slemm_geno* h, int kind, const int* col_kind, const int* col_group, float tau0,
-                               const float* B, int64_t ldb, int t, float tol, int maxit, int keep_cols,
-                               int precision, int verbose, void* stream, slemm_lseed** out) {
-  return guard([&] {
-    REQUIRE(h && B && out, "null argument");
-    *out = nullptr;
+namespace slemm {
+
+slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_group, const float* B, int64_t ldb,
+                       int t, float tol, int maxit, int keep_cols, int verbose, hipStream_t st) {
+    REQUIRE(h && B, "null argument");
     REQUIRE(t > 0, "number of columns must be positive");
     REQUIRE(ldb >= h->n, "ldb < n");
     REQUIRE(maxit >= 2, "maxit must be at least 2");
     REQUIRE(keep_cols >= 0 && keep_cols <= t, "keep_cols out of [0, t]");
     REQUIRE(tol > 0, "tolerance must be positive");
     HIPCHECK(hipSetDevice(h->device));
-    require_model(h);
-    hipStream_t st = (hipStream_t)stream;
     const int64_t n = h->n, ld = h->n_pad;
 
     std::vector<int> group(t, 0);
@@ -249,7 +247,7 @@ extern "C" int slemm_gpu_lseed(slemm_geno* h, int kind, const int* col_kind, con
     const auto t_start = std::chrono::steady_clock::now();
     while (j < maxit - 1) {
       // tmp = A U_j
-      run_op(h, kind, col_kind, tau0, Ucur, ld, t, tmp.as<float>(), ld, precision, st);
+      apply_op(Ucur, ld, tmp.as<float>(), ld, st);
       // beta_j of each column to device slot [t, 2t)
       for (int k = 0; k < t; ++k) hd[k] = beta[(size_t)j * t + k];
       HIPCHECK(hipMemcpyAsync(d_dvec + t, hd.data(), sizeof(double) * t, hipMemcpyHostToDevice, st));
@@ -325,7 +323,23 @@ extern "C" int slemm_gpu_lseed(slemm_geno* h, int kind, const int* col_kind, con
       for (int q = 0; q < s->steps[k]; ++q) s->delta[(size_t)q * t + k] = delta[(size_t)q * t + k];
     }
     (void)t_start;
-    *out = s.release();
+    return s.release();
+}
+
+}  // namespace slemm
+
+extern "C" int slemm_gpu_lseed(slemm_geno* h, int kind, const int* col_kind, const int* col_group, float tau0,
+                               const float* B, int64_t ldb, int t, float tol, int maxit, int keep_cols,
+                               int precision, int verbose, void* stream, slemm_lseed** out) {
+  return guard([&] {
+    REQUIRE(h && B && out, "null argument");
+    *out = nullptr;
+    HIPCHECK(hipSetDevice(h->device));
+    require_model(h);
+    const LanczosOp op = [&](const float* U, int64_t ldu, float* o, int64_t ldo, hipStream_t s) {
+      run_op(h, kind, col_kind, tau0, U, ldu, t, o, ldo, precision, s);
+    };
+    *out = lseed_run(h, op, col_group, B, ldb, t, tol, maxit, keep_cols, verbose, (hipStream_t)stream);
   });
 }
 

@@ -1,0 +1,19 @@
+// Operator-generic Lanczos seed (the L_Seed recurrences of src/reml.cpp:246-339 with FP64
+// scalars and per-group stopping) for callers with their own operator.
+#pragma once
+
+#include <functional>
+
+#include "common.hpp"
+
+namespace slemm {
+
+// out (ldo) = A U (ldu) for the t columns of the seed system, on the given stream.
+using LanczosOp = std::function<void(const float* U, int64_t ldu, float* out, int64_t ldo, hipStream_t st)>;
+
+// Runs the seed systems for the columns of B (n x t, ldb) and returns the seed (owned by the
+// caller, released with slemm_gpu_lseed_destroy).  Throws slemm::Error on failure.
+slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& op, const int* col_group, const float* B, int64_t ldb,
+                       int t, float tol, int maxit, int keep_cols, int verbose, hipStream_t st);
+
+}  // namespace slemm

@@ -7,17 +7,9 @@
 #include <cstring>
 #include <vector>
 
-#include "blockprod.hpp"
+#include "pipeline.hpp"
 
 namespace slemm {
-
-namespace {
-
-struct Layout {
-  int NT, KP;
-  size_t x1_bytes, x2_bytes;
-  int64_t lo1_16, lo2_16;  // lo-half offsets (16-byte units)
-};
 
 Layout layout_for(const slemm_geno* h, int t, int precision) {
   Layout L;
@@ -57,7 +49,8 @@ const int* upload_kinds(slemm_geno* h, int kind, const int* col_kind, int t, boo
 // column means removed.  colsum layout: [0,KP) sums of the represented centred operand,
 // [KP,2KP) pass-2 centring sums, [2KP,3KP) sums of the un-centred operand.
 void pack_pass1(slemm_geno* h, const Layout& L, const float* V, int64_t ldv, int t, const float* rr,
-                const double* qtv, const int* kinds, int precision, hipStream_t st, bool center = true) {
+                const double* qtv, const int* kinds, int precision, hipStream_t st, bool center,
+                const TraitCols* tc) {
   h->x1.reserve(L.x1_bytes);
   const int64_t nparts = h->n_pad / 256;
   h->part1.reserve(sizeof(double) * nparts * L.KP);
@@ -67,6 +60,10 @@ void pack_pass1(slemm_geno* h, const Layout& L, const float* V, int64_t ldv, int
   p.V = V; p.ldv = ldv; p.t = t; p.KP = L.KP; p.n = h->n; p.rows_pad = h->n_pad;
   p.rr = rr; p.Q = h->Q; p.ldq = h->n_pad; p.c = h->c; p.qtv = qtv; p.kinds = kinds;
   p.mode = precision; p.X = nullptr; p.lo_offset16 = L.lo1_16; p.part = h->part1.as<double>();
+  if (tc) {
+    p.tc = *tc;
+    p.c = qtv ? 1 : 0;  // per-trait intercept basis
+  }
   if (center) {
     launch_pack(p, st);  // sums of the un-centred operand
     launch_sum_parts(h->part1.as<double>(), nparts, L.KP, xsum, st);
@@ -91,11 +88,6 @@ bool use_lds(int precision) {
   return precision == PREC_BF16X2 && !v1;
 }
 
-struct BpChoice {
-  int variant;
-  int64_t target;
-};
-
 BpChoice bp_choice(const slemm_geno* h, int pass, int NT) {
   static const char* ev = std::getenv("SLEMM_BP_VARIANT");
   static const char* et = std::getenv("SLEMM_BP_TARGET");
@@ -113,7 +105,7 @@ int64_t pass_splits(const slemm_geno* h, int pass, int NT, int64_t& nsp) {
 
 // Z'X1 pass, epilogue T = W (Zc'x) -> pass-2 operand x2 / out / upart.
 void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, double* upart, float* out,
-               int64_t ldo, hipStream_t st, bool raw = false, bool squared = false) {
+               int64_t ldo, hipStream_t st, bool raw, bool squared, const TraitCols* tc) {
   int64_t nsp = 0;
   const int64_t nsuper = h->n_pad / kPad;
   const int S = (int)pass_splits(h, 0, L.NT, nsp);
@@ -130,12 +122,14 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
   f.colsum = h->colsum.as<double>(); f.xsum = h->colsum.as<double>() + 2 * L.KP; f.xsum_n = (double)h->n;
   f.popcnt = h->popcnt; f.gm = h->gm; f.gw = weighted ? h->gw : nullptr; f.X2 = x2; f.lo_offset16 = L.lo2_16;
   f.upart = upart; f.out = out; f.ldo = ldo; f.raw = raw;
+  if (tc) f.tc = *tc;
   launch_fin1(f, st);
 }
 
 // Z X2 pass, epilogue out = rr (P - u) (+ tau0 v for LDET columns).
 void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const float* rr, const float* V,
-               int64_t ldv, float tau0, const int* kinds, float* out, int64_t ldo, hipStream_t st) {
+               int64_t ldv, float tau0, const int* kinds, float* out, int64_t ldo, hipStream_t st,
+               const TraitCols* tc) {
   int64_t nsp = 0;
   const int64_t nsuper = h->m_pad / kPad;
   const int S = (int)pass_splits(h, 1, L.NT, nsp);
@@ -149,6 +143,7 @@ void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const flo
   Fin2Args f;
   f.part = h->partbuf.as<float>(); f.nsplit = S; f.rows_pad = h->n_pad; f.rows = h->n; f.KP = L.KP; f.t = t;
   f.colsum = u; f.rr = rr; f.v = V; f.ldv = ldv; f.tau0 = tau0; f.kinds = kinds; f.out = out; f.ldo = ldo;
+  if (tc) f.tc = *tc;
   launch_fin2(f, st);
 }
 
@@ -157,8 +152,6 @@ void check_dense(const float* p, int64_t ld, int64_t rows, int t, const char* wh
   if (ld < rows) fail(SLEMM_ERR_ARG, std::string("leading dimension of ") + what + " is too small");
   if (t <= 0) fail(SLEMM_ERR_ARG, "number of columns must be positive");
 }
-
-}  // namespace
 
 void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const float* V, int64_t ldv, int t,
             float* out, int64_t ldo, int precision, hipStream_t st) {
@@ -465,7 +458,6 @@ extern "C" int slemm_gpu_op(slemm_geno* h, int kind, const int* col_kind, float 
 }
 
 namespace slemm {
-namespace {
 
 // (W) Zc' (R^-1/2) V -> out (m x t), one pass over the SNP-major copy.
 void run_zt(slemm_geno* h, const float* V, int64_t ldv, int t, float* out, int64_t ldo, bool weighted,
@@ -488,6 +480,8 @@ void run_zt(slemm_geno* h, const float* V, int64_t ldv, int t, float* out, int64
   centring_args(h, L, a);
   launch_blockprod(a, precision, EPI_PASS1, L.NT, h->m_pad, st);
 }
+
+namespace {
 
 // out_j = sum_k coef_k T_jk T_j(t+k), FP64 accumulation over the k columns.
 __global__ void k_adjoint_combine(const float* __restrict__ T, int64_t ldt, int t, const double* __restrict__ coef,

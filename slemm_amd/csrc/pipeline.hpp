@@ -1,0 +1,48 @@
+// Pieces of the operator pipeline shared by op.hip (single model) and traits.hip (batched traits
+// on individual subsets): operand layout, pass-1 packing, the two block-product passes with their
+// epilogues, and the Z' pass.
+#pragma once
+
+#include "blockprod.hpp"
+
+namespace slemm {
+
+struct Layout {
+  int NT, KP;
+  size_t x1_bytes, x2_bytes;
+  int64_t lo1_16, lo2_16;  // lo-half offsets (16-byte units)
+};
+
+struct BpChoice {
+  int variant;
+  int64_t target;
+};
+
+Layout layout_for(const slemm_geno* h, int t, int precision);
+void check_precision(int precision);
+void check_dense(const float* p, int64_t ld, int64_t rows, int t, const char* what);
+const int* upload_kinds(slemm_geno* h, int kind, const int* col_kind, int t, bool& any_quad, hipStream_t st);
+bool use_lds(int precision);
+BpChoice bp_choice(const slemm_geno* h, int pass, int NT);
+int64_t pass_splits(const slemm_geno* h, int pass, int NT, int64_t& nsp);
+
+// pass-1 operand from V (n x t): rows scaled by rr (optional), QUADFORM columns projected,
+// column means removed (within each column's subset when tc is given).  colsum layout:
+// [0,KP) sums of the represented centred operand, [KP,2KP) pass-2 centring sums, [2KP,3KP)
+// sums of the un-centred operand.
+void pack_pass1(slemm_geno* h, const Layout& L, const float* V, int64_t ldv, int t, const float* rr,
+                const double* qtv, const int* kinds, int precision, hipStream_t st, bool center = true,
+                const TraitCols* tc = nullptr);
+// Z'X1 pass, epilogue T = W (Zc'x) -> pass-2 operand x2 / out / upart.
+void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, double* upart, float* out,
+               int64_t ldo, hipStream_t st, bool raw = false, bool squared = false,
+               const TraitCols* tc = nullptr);
+// Z X2 pass, epilogue out = rr (P - u) (+ tau0 v for LDET columns), masked to each column's subset.
+void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const float* rr, const float* V,
+               int64_t ldv, float tau0, const int* kinds, float* out, int64_t ldo, hipStream_t st,
+               const TraitCols* tc = nullptr);
+// (W) Zc' (R^-1/2) V -> out (m x t), one pass over the SNP-major copy.
+void run_zt(slemm_geno* h, const float* V, int64_t ldv, int t, float* out, int64_t ldo, bool weighted,
+            bool scale_rr, int precision, hipStream_t st);
+
+}  // namespace slemm

@@ -1,0 +1,43 @@
+// Batched traits on individual subsets of one device-resident bed.
+#pragma once
+
+#include <vector>
+
+#include "blockprod.hpp"
+
+// T traits, each with its own subset of the handle's individuals, its own allele frequencies and
+// SNP weights over that subset (src/reml.cpp:612-622 evaluated on the subset), and an intercept
+// as its only covariate.  Vector columns of a trait are zero outside its subset.
+struct slemm_traits {
+  slemm_geno* h = nullptr;
+  int T = 0;
+  slemm::DevBuf mask, q;       // T x n_pad: membership (0/1) and intercept basis mask/sqrt(n_t)
+  slemm::DevBuf n_t;           // T doubles
+  slemm::DevBuf gm, gw, pc;    // T x m_pad: per-trait 2 x A2 frequency, weights, dosage sums
+  slemm::DevBuf col_trait, qd; // per-column trait ids of the last batch; per-column Q'v
+  std::vector<int> col_trait_host;
+  std::vector<int64_t> n_host;           // subset sizes
+  std::vector<int64_t> monomorphic;      // SNPs monomorphic within each subset (weight set to 0)
+  std::vector<std::vector<int64_t>> members;  // member individuals of each subset, ascending
+
+  slemm::TraitCols cols() const {
+    slemm::TraitCols c;
+    c.col_trait = col_trait.as<int>();
+    c.mask = mask.as<float>();
+    c.q = q.as<float>();
+    c.n_t = n_t.as<double>();
+    c.gm = gm.as<float>();
+    c.gw = gw.as<float>();
+    c.pc = pc.as<int32_t>();
+    c.ld_n = h->n_pad;
+    c.ld_m = h->m_pad;
+    return c;
+  }
+};
+
+namespace slemm {
+// Batched operator: column k gets the lnr_op_select operator (src/reml.cpp:96-118) of trait
+// col_trait[k] on that trait's subset, kind col_kind[k].
+void run_op_traits(slemm_traits* tr, const int* col_trait, const int* col_kind, float tau0, const float* V,
+                   int64_t ldv, int t, float* out, int64_t ldo, int precision, hipStream_t st);
+}  // namespace slemm

@@ -422,3 +422,76 @@ class LSeed:
             self.close()
         except Exception:
             pass
+
+
+class Traits:
+    """Several traits on individual subsets of one Geno, batched through one block product per
+    pass (no reference counterpart; the reference analyses one trait per run).  masks: (T, n)
+    booleans, row t = members of trait t's subset."""
+
+    def __init__(self, geno: Geno, masks: np.ndarray):
+        m8 = np.ascontiguousarray(np.asarray(masks, dtype=bool).astype(np.uint8))
+        if m8.ndim != 2 or m8.shape[1] != geno.n:
+            raise ValueError(f"masks must have shape (T, {geno.n})")
+        self.geno = geno
+        self.T = m8.shape[0]
+        self.masks = m8.astype(bool)
+        h = C.c_void_p()
+        L.check(L.load().slemm_gpu_traits_create(geno.handle, self.T, _np_ptr(m8, C.c_uint8), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            L.check(L.load().slemm_gpu_traits_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        nt = np.zeros(self.T, np.int64)
+        mono = np.zeros(self.T, np.int64)
+        L.check(L.load().slemm_gpu_traits_info(self._h, _np_ptr(nt, C.c_int64), _np_ptr(mono, C.c_int64)))
+        return nt, mono
+
+    def model(self, t: int):
+        m = self.geno.m
+        gm = np.zeros(m, np.float32)
+        gw = np.zeros(m, np.float32)
+        pc = np.zeros(m, np.int32)
+        L.check(L.load().slemm_gpu_traits_model(self._h, t, _np_ptr(gm, C.c_float), _np_ptr(gw, C.c_float),
+                                                _np_ptr(pc, C.c_int32)))
+        return gm, gw, pc
+
+    def lnr_op_select(self, col_trait: Sequence[int], col_kind: Optional[Sequence], tau0: float, v: torch.Tensor,
+                      precision="bf16x2") -> torch.Tensor:
+        """Batched lnr_op_select (src/reml.cpp:96-118): column k of v (n, t) belongs to trait
+        col_trait[k] and gets that trait's operator of kind col_kind[k]."""
+        vc = _colmajor(v, self.geno.n)
+        t = vc.shape[0]
+        ct = (C.c_int * t)(*[int(x) for x in col_trait])
+        ck = None
+        if col_kind is not None:
+            ck = (C.c_int * t)(*[OPS[x] if isinstance(x, str) else int(x) for x in col_kind])
+        out = torch.empty_like(vc)
+        L.check(L.load().slemm_gpu_traits_op(self._h, ct, ck, tau0, _ptr(vc), self.geno.n, t, _ptr(out),
+                                             self.geno.n, _prec(precision), _stream()))
+        return out.t()
+
+    def L_Seed(self, col_trait: Sequence[int], col_kind: Optional[Sequence], tau0: float, B: torch.Tensor,
+               tol: float = 5e-4, maxit: int = 600, keep_cols: Optional[int] = None, col_group=None,
+               precision="bf16x2", verbose: bool = False) -> "LSeed":
+        Bc = _colmajor(B, self.geno.n)
+        t = Bc.shape[0]
+        keep = t if keep_cols is None else keep_cols
+        ct = (C.c_int * t)(*[int(x) for x in col_trait])
+        ck = None if col_kind is None else (C.c_int * t)(*[OPS[x] if isinstance(x, str) else int(x)
+                                                           for x in col_kind])
+        cg = None if col_group is None else (C.c_int * t)(*[int(x) for x in col_group])
+        h = C.c_void_p()
+        L.check(L.load().slemm_gpu_traits_lseed(self._h, ct, ck, cg, tau0, _ptr(Bc), self.geno.n, t, tol, maxit,
+                                                keep, _prec(precision), int(verbose), _stream(), C.byref(h)))
+        return LSeed(h, self.geno.n, t)
