@@ -269,6 +269,17 @@ int slemm_gpu_traits_lseed(slemm_traits* tr, const int* col_trait, const int* co
                            float tau0, const float* B, int64_t ldb, int t, float tol, int maxit, int keep_cols,
                            int precision, int verbose, void* stream, slemm_lseed** out);
 
+/* SLDF_REML (src/reml.cpp:549-690) for every trait of the batch at once: one batched seed over
+ * all traits' phenotype, intercept and probe columns, then the golden-section search per trait.
+ * y: T x n host (row t = trait t's phenotype; entries outside the subset are ignored).  Probes
+ * of trait t are drawn from mt19937(rng_seed+54321) over its subset, as a single-trait run on
+ * those individuals draws them.  Host outputs: vg, ve, h2, lrt (T each; lrt may be NULL),
+ * steps (T x 3: Lanczos steps of y, intercept, probes; may be NULL), seconds[2] (seed, total;
+ * may be NULL). */
+int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int rng_seed, int n_V, float tol_L,
+                               float s2max, float s2min, float tol_VC, int precision, int verbose,
+                               float* vg, float* ve, float* h2, float* lrt, int* steps, double* seconds);
+
 /* ---------------------------------------------------------------- tuning ------------------- */
 
 /* Autotuner of the split-BF16 block products (no reference counterpart; the reference's only

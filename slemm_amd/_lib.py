@@ -78,6 +78,8 @@ SIGNATURES = {
     "slemm_gpu_traits_model": (C.c_int, [c_vp, C.c_int, c_fp, c_fp, c_i32p]),
     "slemm_gpu_traits_op": (C.c_int, [c_vp, c_intp, c_intp, C.c_float, c_vp, c_i64, C.c_int, c_vp, c_i64, C.c_int,
                                       c_vp]),
+    "slemm_gpu_traits_sldf_reml": (C.c_int, [c_vp, c_fp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                             C.c_float, C.c_int, C.c_int, c_fp, c_fp, c_fp, c_fp, c_intp, c_dp]),
     "slemm_gpu_traits_lseed": (C.c_int, [c_vp, c_intp, c_intp, c_intp, C.c_float, c_vp, c_i64, C.c_int, C.c_float,
                                          C.c_int, C.c_int, C.c_int, C.c_int, c_vp, C.POINTER(c_vp)]),
     "slemm_gpu_adjoint": (C.c_int, [c_vp, c_vp, c_i64, c_vp, c_i64, C.c_int, c_dp, C.c_int, c_vp, C.c_int, c_vp]),

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include "common.hpp"
@@ -81,6 +82,14 @@ inline double seconds_since(std::chrono::steady_clock::time_point t0) {
 // Rademacher probes of unit norm from mt19937(seed+54321), written to columns of B (ld n),
 // draw order and normalisation as src/reml.cpp:597-605.
 void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B);
+
+// REML_criterion at h2 (src/reml.cpp:457-489): returns 2 logLL, sets vg, ve and the LRT.
+using Criterion = std::function<float(float h2, float& vg, float& ve, float& lrt)>;
+
+// REML_GSS (src/reml.cpp:493-547): golden-section search on (s2min, s2max) to tol_VC; vg, ve
+// and lrt are those of the last evaluated point, as in the reference.
+void golden_section(float s2min, float s2max, float tol_VC, int verbose, const Criterion& criterion, float& vg,
+                    float& ve, float& lrt);
 
 // Final estimates at tau1 = ve/vg from a seed whose column 0 is the projected phenotype and
 // columns 1..c the covariate basis (src/reml.cpp:655-677): Py = R^-1/2 (I-QQ') H^-1 y~,

@@ -40,6 +40,40 @@ void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B) {
   }
 }
 
+void golden_section(float s2min, float s2max, float tol_VC, int verbose, const Criterion& criterion, float& vg,
+                    float& ve, float& lrt) {
+  const double invphi = 0.618033988749894848204586834365638117720309179805762862135;
+  const double invphi2 = 0.381966011250105151795413165634361882279690820194237137864;
+  float a = s2min, b = s2max;
+  float fm = 0;
+  if (verbose) printf("\nDoing golden section search in (%g, %g)\n", a, b);
+  float hh = b - a;
+  const int nitr = hh < tol_VC ? 1 : (int)std::ceil(std::log(tol_VC / hh) / std::log(invphi));
+  float cc = (float)(a + invphi2 * hh), dd = (float)(a + invphi * hh);
+  float fc = criterion(cc, vg, ve, lrt);
+  float fd = criterion(dd, vg, ve, lrt);
+  for (int i = 0; i < nitr; ++i) {
+    if (fc > fd) {
+      b = dd;
+      dd = cc;
+      fd = fc;
+      hh = (float)(hh * invphi);
+      cc = (float)(a + invphi2 * hh);
+      fc = criterion(cc, vg, ve, lrt);
+      fm = fc;
+    } else {
+      a = cc;
+      cc = dd;
+      fc = fd;
+      hh = (float)(hh * invphi);
+      dd = (float)(a + invphi * hh);
+      fd = criterion(dd, vg, ve, lrt);
+      fm = fd;
+    }
+    if (verbose) printf("Itr %d: h2 in (%g, %g) and 2logLL = %g\n", i + 1, a, b, fm);
+  }
+}
+
 void finish_estimates(slemm_geno* h, slemm_lseed* seed, const float* dB, float tau0, float tau1, const MatD& Q,
                       const MatD& qrR, const float* X, int64_t ldx, int c, const float* y, float vg,
                       int precision, float* Py, float* snp_blup, float* blue, float* var_blue) {
@@ -179,36 +213,8 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
     };
 
     // REML_GSS (:493-547): golden-section search, returns the last evaluated point
-    const double invphi = 0.618033988749894848204586834365638117720309179805762862135;
-    const double invphi2 = 0.381966011250105151795413165634361882279690820194237137864;
-    float a = s2min, b = s2max;
-    float vg = 0, ve = 0, lrt = 0, fm = 0;
-    if (verbose) printf("\nDoing golden section search in (%g, %g)\n", a, b);
-    float hh = b - a;
-    const int nitr = hh < tol_VC ? 1 : (int)std::ceil(std::log(tol_VC / hh) / std::log(invphi));
-    float cc = (float)(a + invphi2 * hh), dd = (float)(a + invphi * hh);
-    float fc = criterion(cc, vg, ve, lrt);
-    float fd = criterion(dd, vg, ve, lrt);
-    for (int i = 0; i < nitr; ++i) {
-      if (fc > fd) {
-        b = dd;
-        dd = cc;
-        fd = fc;
-        hh = (float)(hh * invphi);
-        cc = (float)(a + invphi2 * hh);
-        fc = criterion(cc, vg, ve, lrt);
-        fm = fc;
-      } else {
-        a = cc;
-        cc = dd;
-        fc = fd;
-        hh = (float)(hh * invphi);
-        dd = (float)(a + invphi * hh);
-        fd = criterion(dd, vg, ve, lrt);
-        fm = fd;
-      }
-      if (verbose) printf("Itr %d: h2 in (%g, %g) and 2logLL = %g\n", i + 1, a, b, fm);
-    }
+    float vg = 0, ve = 0, lrt = 0;
+    golden_section(s2min, s2max, tol_VC, verbose, criterion, vg, ve, lrt);
 
     // Py, SNP BLUP, BLUE (:655-677)
     const float tau1 = ve / vg;

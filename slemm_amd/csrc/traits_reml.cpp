@@ -1,0 +1,134 @@
+// SLDF_REML (src/reml.cpp:549-690) for a batch of traits on individual subsets: one batched
+// Lanczos seed carries every trait's phenotype, intercept and probe columns through the shared
+// block products, and the golden-section search then runs per trait on replays of that seed.
+// Per trait the computation is the single-trait driver's on the trait's individuals: the same
+// probes (drawn from mt19937(seed+54321) over the subset), the same criterion and search.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "driver.hpp"
+#include "traits.hpp"
+
+using namespace slemm;
+
+extern "C" int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int rng_seed, int n_V, float tol_L,
+                                          float s2max, float s2min, float tol_VC, int precision, int verbose,
+                                          float* vg_out, float* ve_out, float* h2_out, float* lrt_out,
+                                          int* steps_out, double* seconds) {
+  return guard([&] {
+    REQUIRE(tr && y && vg_out && ve_out && h2_out, "null argument");
+    REQUIRE(n_V >= 1, "number of probes must be positive");
+    REQUIRE(s2min > 0 && s2max < 1 && s2min < s2max, "heritability range must satisfy 0 < s2min < s2max < 1");
+    slemm_geno* h = tr->h;
+    HIPCHECK(hipSetDevice(h->device));
+    const auto t_start = std::chrono::steady_clock::now();
+    const int T = tr->T;
+    const int64_t n = h->n;
+    const int ncol = T * (2 + n_V);
+    // columns: [y~_0 .. y~_{T-1}, q_0 .. q_{T-1}, probes of trait 0, probes of trait 1, ...]
+    std::vector<float> B((size_t)n * ncol, 0.f);
+    std::vector<int> col_trait(ncol), kinds(ncol, SLEMM_OP_LDET), groups(ncol);
+    std::vector<std::vector<float>> yp(T);
+    std::vector<MatD> Qt(T);
+    for (int t = 0; t < T; ++t) {
+      const std::vector<int64_t>& mem = tr->members[t];
+      const int64_t nt = (int64_t)mem.size();
+      // intercept basis and projected phenotype over the subset, as the single-trait driver
+      // forms them on a handle holding only these individuals (src/reml.cpp:583-592)
+      MatD ones(nt, 1), R;
+      for (int64_t i = 0; i < nt; ++i) ones(i, 0) = 1.0;
+      if (!thin_qr(ones, Qt[t], R, 1e-6)) fail(SLEMM_ERR_RANK, "Fixed-effects design matrix is not of full rank.");
+      yp[t].resize(nt);
+      for (int64_t i = 0; i < nt; ++i) yp[t][i] = y[(size_t)t * n + mem[i]];
+      project_out(Qt[t], yp[t], nt, 1);
+      std::vector<float> probes((size_t)nt * n_V);
+      rademacher_probes(rng_seed, nt, n_V, probes.data());
+      for (int64_t i = 0; i < nt; ++i) {
+        B[(size_t)t * n + mem[i]] = yp[t][i];
+        B[(size_t)(T + t) * n + mem[i]] = (float)Qt[t](i, 0);
+        for (int j = 0; j < n_V; ++j) B[(size_t)(2 * T + t * n_V + j) * n + mem[i]] = probes[(size_t)j * nt + i];
+      }
+      col_trait[t] = t;
+      kinds[t] = SLEMM_OP_QUADFORM;
+      groups[t] = t;
+      col_trait[T + t] = t;
+      groups[T + t] = T + t;
+      for (int j = 0; j < n_V; ++j) {
+        col_trait[2 * T + t * n_V + j] = t;
+        groups[2 * T + t * n_V + j] = 2 * T + t;
+      }
+    }
+    if (verbose) printf("Generated %d Rademacher probing vectors for each of %d traits.\n", n_V, T);
+    DevF dB((size_t)n * ncol), dX(n);
+    upload(dB.p, B);
+    const float tau0 = (1 - s2max) / s2max;
+    SeedGuard sg;
+    abi(slemm_gpu_traits_lseed(tr, col_trait.data(), kinds.data(), groups.data(), tau0, dB.p, n, ncol, tol_L, 600,
+                               2 * T, precision, verbose, nullptr, &sg.s));
+    std::vector<int> csteps(ncol);
+    abi(slemm_gpu_lseed_col_steps(sg.s, csteps.data()));
+    const double t_seed = seconds_since(t_start);
+
+    DevBuf dots;
+    dots.reserve(sizeof(double));
+    auto dot_dev = [&](const float* a, const float* b) {
+      col_dots(h, a, n, b, n, n, 1, dots.as<double>(), nullptr);
+      double r = 0;
+      HIPCHECK(hipMemcpy(&r, dots.p, sizeof(double), hipMemcpyDeviceToHost));
+      return r;
+    };
+    for (int t = 0; t < T; ++t) {
+      const int64_t nt = (int64_t)tr->members[t].size();
+      std::vector<std::vector<double>> Dk(n_V), Wk(n_V);
+      for (int j = 0; j < n_V; ++j) {
+        const int col = 2 * T + t * n_V + j;
+        Dk[j].resize(csteps[col]);
+        Wk[j].resize(csteps[col]);
+        abi(slemm_gpu_lseed_spectrum(sg.s, col, Dk[j].data(), Wk[j].data()));
+      }
+      const double ySy = sq_norm(yp[t].data(), nt);
+      const float* ycol = dB.p + (size_t)t * n;
+      const float* qcol = dB.p + (size_t)(T + t) * n;
+      int evals = 0;
+      // REML_criterion (src/reml.cpp:457-489) on the trait's subset (n_t individuals, c = 1)
+      auto criterion = [&](float ss, float& vg, float& ve, float& lrt) -> float {
+        ++evals;
+        const float tau = (1 - ss) / ss;
+        const float sigma = tau - tau0;
+        abi(slemm_gpu_lsolve(sg.s, t, 1, ycol, n, sigma, 1e-6f, dX.p, n, nullptr));
+        const double yPy = dot_dev(ycol, dX.p);
+        vg = (float)(yPy / (double)(nt - 1));
+        ve = vg * tau;
+        double acc = 0.0;
+        for (int j = 0; j < n_V; ++j)
+          for (size_t q = 0; q < Dk[j].size(); ++q) acc += Wk[j][q] * Wk[j][q] * std::log(Dk[j][q] + sigma);
+        const double ldetH = acc * (double)nt / (double)n_V;
+        lrt = (float)((double)nt * std::log(ySy / yPy) - ldetH);
+        abi(slemm_gpu_lsolve(sg.s, T + t, 1, qcol, n, sigma, 1e-6f, dX.p, n, nullptr));
+        const double QtX = dot_dev(qcol, dX.p);
+        return (float)((double)(nt - 1) * std::log(ySy / yPy) - std::log(std::fabs(QtX)) - ldetH);
+      };
+      float vg = 0, ve = 0, lrt = 0;
+      golden_section(s2min, s2max, tol_VC, verbose, criterion, vg, ve, lrt);
+      vg_out[t] = vg;
+      ve_out[t] = ve;
+      h2_out[t] = vg / (vg + ve);
+      if (lrt_out) lrt_out[t] = lrt;
+      if (steps_out) {
+        steps_out[3 * t] = csteps[t];
+        steps_out[3 * t + 1] = csteps[T + t];
+        steps_out[3 * t + 2] = csteps[2 * T + t * n_V];
+      }
+      if (verbose) printf("Trait %d (n=%ld): Vg %g Ve %g h2 %g after %d evaluations\n", t, (long)nt, vg, ve,
+                          vg / (vg + ve), evals);
+    }
+    if (seconds) {
+      seconds[0] = t_seed;
+      seconds[1] = seconds_since(t_start);
+    }
+  });
+}

@@ -495,3 +495,16 @@ class Traits:
         L.check(L.load().slemm_gpu_traits_lseed(self._h, ct, ck, cg, tau0, _ptr(Bc), self.geno.n, t, tol, maxit,
                                                 keep, _prec(precision), int(verbose), _stream(), C.byref(h)))
         return LSeed(h, self.geno.n, t)
+
+    def SLDF_REML(self, y: np.ndarray, rng_seed: int = 0, n_V: int = 30, tol_L: float = 5e-4, s2max: float = 0.7,
+                  s2min: float = 1e-4, tol_VC: float = 1e-4, precision="bf16x2", verbose: bool = False) -> dict:
+        """SLDF_REML (src/reml.cpp:549-690) for all traits in one batch; y (T, n)."""
+        yv = np.ascontiguousarray(np.asarray(y, np.float32).reshape(self.T, self.geno.n))
+        vg, ve, h2, lrt = (np.zeros(self.T, np.float32) for _ in range(4))
+        steps = np.zeros((self.T, 3), np.int32)
+        secs = np.zeros(2)
+        L.check(L.load().slemm_gpu_traits_sldf_reml(
+            self._h, _np_ptr(yv, C.c_float), rng_seed, n_V, tol_L, s2max, s2min, tol_VC, _prec(precision),
+            int(verbose), _np_ptr(vg, C.c_float), _np_ptr(ve, C.c_float), _np_ptr(h2, C.c_float),
+            _np_ptr(lrt, C.c_float), _np_ptr(steps, C.c_int), _np_ptr(secs, C.c_double)))
+        return dict(vg=vg, ve=ve, h2=h2, lrt=lrt, steps=steps, seconds_seed=secs[0], seconds_total=secs[1])

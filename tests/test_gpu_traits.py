@@ -98,3 +98,87 @@ def test_batched_seed_solves(batch):
             want = D.solve(kinds[k], tau0 + sigma, Bn[keep, k][:, None])[:, 0]
             assert rel_err(X[keep, k], want) < 2e-4, (t, k)
             assert np.all(X[~masks[t], k] == 0.0)
+
+
+def test_batched_seed_equals_subset_seeds(batch):
+    """The batched Lanczos seed (every trait's y, intercept and probe columns in one system)
+    against the seeds that subset handles run on the same columns: the first Lanczos
+    coefficients agree to rounding (later ones are not comparable: without re-orthogonalisation
+    FP32 Lanczos amplifies rounding once Ritz values converge), and so do what the drivers use --
+    the replayed solves and the SLQ log-determinant."""
+    import oracle
+    p, masks, g, tr, subs = batch
+    rng = np.random.default_rng(21)
+    cols, col_trait, kinds, groups = [], [], [], []
+    per_trait = []
+    for t, (keep, gs) in enumerate(subs):
+        nt = len(keep)
+        y = rng.normal(size=nt).astype(np.float32)
+        y -= y.mean()
+        q = np.full(nt, 1 / np.sqrt(nt), np.float32)
+        V = oracle.rademacher(t, nt, 4)
+        block = np.column_stack([y, q, V]).astype(np.float32)
+        per_trait.append(block)
+        for j in range(block.shape[1]):
+            v = np.zeros(p.n, np.float32)
+            v[keep] = block[:, j]
+            cols.append(v)
+            col_trait.append(t)
+            kinds.append("quadform" if j == 0 else "ldet")
+            groups.append(3 * t + min(j, 2))
+    B = torch.from_numpy(np.stack(cols, 1)).cuda()
+    seed = tr.L_Seed(col_trait, kinds, 0.43, B, tol=5e-4, col_group=groups)
+    beta, delta = seed.coeffs()
+    X = seed.L_Solve(B, 0.6).cpu().numpy()
+    off = 0
+    for t, (keep, gs) in enumerate(subs):
+        blk = per_trait[t]
+        w = blk.shape[1]
+        Bt = torch.from_numpy(blk).cuda()
+        s1 = gs.L_Seed("ldet", 0.43, Bt, tol=5e-4, col_kind=["quadform"] + ["ldet"] * (w - 1),
+                       col_group=[0, 1] + [2] * (w - 2))
+        b1, d1 = s1.coeffs()
+        assert np.allclose(delta[off:off + w, :4], d1[:, :4], rtol=3e-5), t
+        assert np.allclose(beta[off:off + w, :5], b1[:, :5], rtol=3e-5), t
+        X1 = s1.L_Solve(Bt, 0.6).cpu().numpy()
+        assert rel_err(X[keep, off:off + w], X1) < 2e-5, t
+        l_b = seed.SLQ_LDet(0.6, col0=off + 2, nv=w - 2) * len(keep) / p.n
+        l_1 = s1.SLQ_LDet(0.6, col0=2, nv=w - 2)
+        assert abs(l_b - l_1) <= 1e-5 * abs(l_1), (t, l_b, l_1)
+        off += w
+
+
+def test_batched_sldf_equals_per_subset_runs(batch):
+    """SLDF_REML for all traits in one batch vs the single-trait driver on each subset handle
+    with the same probes (same generator over the subset), and vs the FP64 criterion.  The
+    criterion is a float, as the reference's REML_criterion returns it (src/reml.cpp:457-489):
+    at n ~ 1e3 its ulp (~6e-5 at |2logLL| ~ 1e3) over the curvature (~40) limits where the
+    golden-section search can end to about 1e-3 in h2, which is the tolerance here."""
+    import oracle
+    from oracle.dense import sldf_gss
+    p, masks, g, tr, subs = batch
+    rng = np.random.default_rng(13)
+    Y = np.zeros((3, p.n), np.float32)
+    for t in range(3):
+        keep = np.flatnonzero(masks[t])
+        idx = rng.choice(p.m, size=150, replace=False)
+        gv = p.Z[np.ix_(keep, idx)] @ rng.normal(size=150)
+        h2 = [0.2, 0.5, 0.7][t]
+        e = rng.normal(size=len(keep)) * np.sqrt(gv.var() * (1 - h2) / h2)
+        Y[t, keep] = (gv + e + 5.0).astype(np.float32)
+        Y[t, ~masks[t]] = 123.0  # ignored
+    got = tr.SLDF_REML(Y, n_V=20)
+    for t, (keep, gs) in enumerate(subs):
+        ref = gs.SLDF_REML(np.ones((len(keep), 1), np.float32), Y[t, keep], n_V=20)
+        nt = len(keep)
+        gm, gw, _ = tr.model(t)
+        Q = np.full((nt, 1), 1 / np.sqrt(nt))
+        yp = Y[t, keep].astype(np.float64)
+        yp = (yp - yp.mean()).astype(np.float32)
+        _, _, _, h2_64 = sldf_gss(Dense(p.Z[keep], gm, gw, np.ones(nt), Q), yp, Q, oracle.rademacher(0, nt, 20))
+        msg = (f"trait {t}: batch {got['h2'][t]:.6f} {got['vg'][t]:.5f} single {ref['h2']:.6f} {ref['vg']:.5f} "
+               f"fp64 {h2_64:.6f}")
+        assert abs(got["h2"][t] - ref["h2"]) <= 1e-3, msg
+        assert abs(got["h2"][t] - h2_64) <= 1e-3, msg
+        assert abs(got["vg"][t] - ref["vg"]) <= 1e-2 * ref["vg"], msg
+        assert np.all(np.abs(got["steps"][t] - np.array(ref["steps"])) <= 1), msg
