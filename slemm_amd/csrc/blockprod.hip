@@ -269,6 +269,8 @@ std::vector<TimingRec> g_timing_recs;
 TimingToken timing_begin(hipStream_t st) {
   TimingToken tok;
   if (!g_timing) return tok;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (st && hipStreamIsCapturing(st, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone) return tok;
   HIPCHECK(hipEventCreate(&tok.beg));
   HIPCHECK(hipEventRecord(tok.beg, st));
   tok.on = true;

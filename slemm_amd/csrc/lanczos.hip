@@ -11,6 +11,7 @@
 #include <memory>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "blockprod.hpp"
@@ -149,8 +150,13 @@ using namespace slemm;
 
 namespace slemm {
 
+bool lanczos_graphs() {
+  const char* e = std::getenv("SLEMM_LANCZOS_GRAPH");
+  return !(e && e[0] == '0');
+}
+
 slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_group, const float* B, int64_t ldb,
-                       int t, float tol, int maxit, int keep_cols, int verbose, hipStream_t st) {
+                       int t, float tol, int maxit, int keep_cols, int verbose, hipStream_t st, bool use_graph) {
     REQUIRE(h && B, "null argument");
     REQUIRE(t > 0, "number of columns must be positive");
     REQUIRE(ldb >= h->n, "ldb < n");
@@ -243,25 +249,71 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
     float* Unext = bufB.as<float>();
     float* spare = bufC.as<float>();
     std::vector<double> hd(2 * t);
+    // graph replay needs a capturable (non-null) stream: use a private one when none is given
+    struct Graphs {
+      bool enabled = false;
+      hipStream_t own = nullptr;
+      hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+      ~Graphs() {
+        for (auto& e : exec)
+          if (e) (void)hipGraphExecDestroy(e);
+        if (own) {
+          (void)hipStreamSynchronize(own);
+          (void)hipStreamDestroy(own);
+        }
+      }
+    } graphs;
+    graphs.enabled = use_graph;
+    if (use_graph && st == nullptr) {
+      HIPCHECK(hipStreamSynchronize(nullptr));
+      HIPCHECK(hipStreamCreate(&graphs.own));
+      st = graphs.own;
+    }
     int j = 0;
     const auto t_start = std::chrono::steady_clock::now();
+    // One Lanczos step on the device: tmp = A U_j, delta_j, beta_{j+1}, U_{j+1}.  The buffers
+    // rotate with period 3 after the first step, so steps j >= 1 replay one of three HIP graphs
+    // (captured at their first use) instead of launching every kernel again.
+    auto step_work = [&](hipStream_t s) {
+      apply_op(Ucur, ld, tmp.as<float>(), ld, s);
+      k_lz_a<<<dim3((unsigned)nparts, (unsigned)t), 256, 0, s>>>(Uprev, Ucur, tmp.as<float>(), ld, n, d_dvec + t,
+                                                                dact.as<int>(), part.as<double>(), t);
+      HIPCHECK(hipGetLastError());
+      k_lz_sum<<<(t + 63) / 64, 64, 0, s>>>(part.as<double>(), nparts, t, d_dvec, 0);  // delta_j
+      k_lz_b<<<dim3((unsigned)nparts, (unsigned)t), 256, 0, s>>>(Ucur, tmp.as<float>(), ld, n, d_dvec,
+                                                                dact.as<int>(), part.as<double>(), t);
+      HIPCHECK(hipGetLastError());
+      k_lz_sum<<<(t + 63) / 64, 64, 0, s>>>(part.as<double>(), nparts, t, d_dvec + t, 1);  // beta_{j+1}
+      k_lz_c<<<dim3((unsigned)((n + 255) / 256), (unsigned)t), 256, 0, s>>>(tmp.as<float>(), Unext, ld, n,
+                                                                           d_dvec + t, dact.as<int>());
+      HIPCHECK(hipGetLastError());
+    };
+    std::vector<double> hb(t);
     while (j < maxit - 1) {
-      // tmp = A U_j
-      apply_op(Ucur, ld, tmp.as<float>(), ld, st);
       // beta_j of each column to device slot [t, 2t)
-      for (int k = 0; k < t; ++k) hd[k] = beta[(size_t)j * t + k];
-      HIPCHECK(hipMemcpyAsync(d_dvec + t, hd.data(), sizeof(double) * t, hipMemcpyHostToDevice, st));
-      k_lz_a<<<dim3((unsigned)nparts, (unsigned)t), 256, 0, st>>>(Uprev, Ucur, tmp.as<float>(), ld, n, d_dvec + t,
-                                                                 dact.as<int>(), part.as<double>(), t);
-      HIPCHECK(hipGetLastError());
-      k_lz_sum<<<(t + 63) / 64, 64, 0, st>>>(part.as<double>(), nparts, t, d_dvec, 0);  // delta_j
-      k_lz_b<<<dim3((unsigned)nparts, (unsigned)t), 256, 0, st>>>(Ucur, tmp.as<float>(), ld, n, d_dvec,
-                                                                 dact.as<int>(), part.as<double>(), t);
-      HIPCHECK(hipGetLastError());
-      k_lz_sum<<<(t + 63) / 64, 64, 0, st>>>(part.as<double>(), nparts, t, d_dvec + t, 1);  // beta_{j+1}
-      k_lz_c<<<dim3((unsigned)((n + 255) / 256), (unsigned)t), 256, 0, st>>>(tmp.as<float>(), Unext, ld, n,
-                                                                            d_dvec + t, dact.as<int>());
-      HIPCHECK(hipGetLastError());
+      for (int k = 0; k < t; ++k) hb[k] = beta[(size_t)j * t + k];
+      HIPCHECK(hipMemcpyAsync(d_dvec + t, hb.data(), sizeof(double) * t, hipMemcpyHostToDevice, st));
+      if (graphs.enabled && j >= 1) {
+        hipGraphExec_t& ge = graphs.exec[(j - 1) % 3];
+        if (!ge) {
+          hipGraph_t gr = nullptr;
+          HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+          try {
+            step_work(st);
+          } catch (...) {
+            (void)hipStreamEndCapture(st, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
+            throw;
+          }
+          HIPCHECK(hipStreamEndCapture(st, &gr));
+          const hipError_t ie = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(gr);
+          HIPCHECK(ie);
+        }
+        HIPCHECK(hipGraphLaunch(ge, st));
+      } else {
+        step_work(st);
+      }
       keep_basis(Unext);
       HIPCHECK(hipMemcpyAsync(hd.data(), d_dvec, sizeof(double) * 2 * t, hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
@@ -339,7 +391,8 @@ extern "C" int slemm_gpu_lseed(slemm_geno* h, int kind, const int* col_kind, con
     const LanczosOp op = [&](const float* U, int64_t ldu, float* o, int64_t ldo, hipStream_t s) {
       run_op(h, kind, col_kind, tau0, U, ldu, t, o, ldo, precision, s);
     };
-    *out = lseed_run(h, op, col_group, B, ldb, t, tol, maxit, keep_cols, verbose, (hipStream_t)stream);
+    *out = lseed_run(h, op, col_group, B, ldb, t, tol, maxit, keep_cols, verbose, (hipStream_t)stream,
+                     lanczos_graphs());
   });
 }
 

@@ -12,8 +12,13 @@ namespace slemm {
 using LanczosOp = std::function<void(const float* U, int64_t ldu, float* out, int64_t ldo, hipStream_t st)>;
 
 // Runs the seed systems for the columns of B (n x t, ldb) and returns the seed (owned by the
-// caller, released with slemm_gpu_lseed_destroy).  Throws slemm::Error on failure.
+// caller, released with slemm_gpu_lseed_destroy).  With use_graph, steps after the first replay
+// HIP graphs of the operator and Lanczos kernels (op must then launch only kernels on the given
+// stream once its buffers exist).  Throws slemm::Error on failure.
 slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& op, const int* col_group, const float* B, int64_t ldb,
-                       int t, float tol, int maxit, int keep_cols, int verbose, hipStream_t st);
+                       int t, float tol, int maxit, int keep_cols, int verbose, hipStream_t st, bool use_graph);
+
+// Graph replay of Lanczos steps, on unless SLEMM_LANCZOS_GRAPH=0.
+bool lanczos_graphs();
 
 }  // namespace slemm

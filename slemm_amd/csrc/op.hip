@@ -166,8 +166,7 @@ void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const floa
 
   const double* qtv = nullptr;
   if (any_quad && h->c > 0) {
-    h->qtv.reserve(sizeof(double) * h->c * L.KP);
-    HIPCHECK(hipMemsetAsync(h->qtv.p, 0, sizeof(double) * h->c * L.KP, st));
+    h->qtv.reserve(sizeof(double) * h->c * L.KP);  // entries k < t are all written and the only ones read
     tn_product(h->Q, h->n_pad, h->c, V, ldv, t, h->n, h->qtv.as<double>(), L.KP, h->redpart, st);
     qtv = h->qtv.as<double>();
   }

@@ -274,6 +274,7 @@ extern "C" int slemm_gpu_traits_lseed(slemm_traits* tr, const int* col_trait, co
     const LanczosOp op = [&](const float* U, int64_t ldu, float* o, int64_t ldo, hipStream_t s) {
       run_op_traits(tr, ct.data(), ck.data(), tau0, U, ldu, t, o, ldo, precision, s);
     };
-    *out = lseed_run(tr->h, op, col_group, B, ldb, t, tol, maxit, keep_cols, verbose, (hipStream_t)stream);
+    *out = lseed_run(tr->h, op, col_group, B, ldb, t, tol, maxit, keep_cols, verbose, (hipStream_t)stream,
+                     lanczos_graphs());
   });
 }

@@ -182,3 +182,35 @@ def test_batched_sldf_equals_per_subset_runs(batch):
         assert abs(got["h2"][t] - h2_64) <= 1e-3, msg
         assert abs(got["vg"][t] - ref["vg"]) <= 1e-2 * ref["vg"], msg
         assert np.all(np.abs(got["steps"][t] - np.array(ref["steps"])) <= 1), msg
+
+
+def test_graph_replayed_steps_equal_eager_steps(batch, monkeypatch):
+    """Lanczos steps after the first replay HIP graphs of the operator and update kernels; every
+    replayed step (coefficients) and the replayed solves must equal the eager launches exactly,
+    for the batched operator and the single-model operator."""
+    p, masks, g, tr, subs = batch
+    rng = np.random.default_rng(4)
+    cols, col_trait = [], []
+    for t in range(3):
+        for _ in range(2):
+            v = np.zeros(p.n, np.float32)
+            v[masks[t]] = rng.normal(size=masks[t].sum())
+            cols.append(v)
+            col_trait.append(t)
+    B = torch.from_numpy(np.stack(cols, 1)).cuda()
+    keep, gs = subs[1]
+    Bs = torch.from_numpy(rng.normal(size=(len(keep), 3)).astype(np.float32)).cuda()
+    runs = {}
+    for mode in ("1", "0", "1"):
+        monkeypatch.setenv("SLEMM_LANCZOS_GRAPH", mode)
+        s1 = tr.L_Seed(col_trait, ["quadform", "ldet"] * 3, 0.4, B, tol=1e-6)
+        s2 = gs.L_Seed("quadform", 0.4, Bs, tol=1e-6)
+        res = (s1.coeffs(), s1.L_Solve(B, 0.3).cpu().numpy(), s2.coeffs(), s2.L_Solve(Bs, 0.3).cpu().numpy())
+        runs.setdefault(mode, []).append(res)
+        s1.close()
+        s2.close()
+    eager = runs["0"][0]
+    for res in runs["1"]:
+        for a, b in zip((res[0][0], res[0][1], res[1], res[2][0], res[2][1], res[3]),
+                        (eager[0][0], eager[0][1], eager[1], eager[2][0], eager[2][1], eager[3])):
+            assert np.array_equal(a, b)
