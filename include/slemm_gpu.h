@@ -73,6 +73,11 @@ int slemm_gpu_geno_create(const uint8_t* bed_rows, int64_t n_raw, int64_t m_raw,
  * group ordering of src/input.cpp:389-454 without re-reading the bed). */
 int slemm_gpu_geno_subset(const slemm_geno* h, const int64_t* snp_idx, int64_t m, slemm_geno** out);
 
+/* New handle with the individuals keep_idx (strictly increasing, host) of an existing one, all
+ * SNPs: the individual compaction of src/input.cpp:360-377 applied on the device to resident
+ * rows (analyses of individual subsets without re-reading the bed). */
+int slemm_gpu_geno_keep(const slemm_geno* h, const int64_t* keep_idx, int64_t n_keep, slemm_geno** out);
+
 /* Synthetic bed generated in HBM: per-SNP p ~ U(maf_lo, maf_hi), genotype ~ Binomial(2,p)
  * (A2 count), `missing` fraction written as code 01.  Counter-based hashing, so the content
  * depends only on (seed, n, m); no reference counterpart (test and benchmark input). */

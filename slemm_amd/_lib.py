@@ -40,6 +40,7 @@ SIGNATURES = {
     "slemm_gpu_device_count": (C.c_int, [c_intp]),
     "slemm_gpu_geno_create": (C.c_int, [c_u8p, c_i64, c_i64, c_i64, c_i64p, c_i64, c_i64p, c_i64, C.POINTER(c_vp)]),
     "slemm_gpu_geno_subset": (C.c_int, [c_vp, c_i64p, c_i64, C.POINTER(c_vp)]),
+    "slemm_gpu_geno_keep": (C.c_int, [c_vp, c_i64p, c_i64, C.POINTER(c_vp)]),
     "slemm_gpu_geno_synthesize": (C.c_int, [c_i64, c_i64, C.c_uint64, C.c_float, C.c_float, C.c_float, C.POINTER(c_vp)]),
     "slemm_gpu_geno_synthesize_p": (C.c_int, [c_i64, c_i64, C.c_uint64, C.POINTER(C.c_float), C.c_float, C.POINTER(c_vp)]),
     "slemm_gpu_geno_destroy": (C.c_int, [c_vp]),

@@ -182,7 +182,9 @@ __global__ __launch_bounds__(256) void k_bp_reg(LdsArgs a) {
   const int64_t row0 = (int64_t)blockIdx.x * (64 * RT) + wave * (16 * RT);
   const int col0 = blockIdx.y * (16 * NT);
   const int64_t s0 = (int64_t)blockIdx.z * a.ns_per_split;
-  const int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
+  int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
+  if (a.grp_nsuper) s1 = min(s1, (int64_t)a.grp_nsuper[blockIdx.y]);
+  if (a.grp_rows && row0 >= a.grp_rows[blockIdx.y]) s1 = s0;  // this wave's rows are all zero
   const uint4* xh = reinterpret_cast<const uint4*>(a.X);
   const uint4* xl = xh + a.lo_offset16;
   f32x4_t acc[RT][NT];
@@ -262,7 +264,9 @@ __global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
   const int64_t row0 = (int64_t)blockIdx.x * (64 * RT) + wave * (16 * RT);
   const int col0 = blockIdx.y * (16 * NT);
   const int64_t s0 = (int64_t)blockIdx.z * a.ns_per_split;
-  const int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
+  int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
+  if (a.grp_nsuper) s1 = min(s1, (int64_t)a.grp_nsuper[blockIdx.y]);
+  if (a.grp_rows && row0 >= a.grp_rows[blockIdx.y]) s1 = s0;  // this wave's rows are all zero
   const uint4* xh = reinterpret_cast<const uint4*>(a.X);
   const uint4* xl = xh + a.lo_offset16;
   const uint8_t* gbase = a.G + (row0 + r) * a.ldg + g * 32;

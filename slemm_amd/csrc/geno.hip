@@ -363,6 +363,42 @@ extern "C" int slemm_gpu_geno_subset(const slemm_geno* hs, const int64_t* snp_id
   });
 }
 
+extern "C" int slemm_gpu_geno_keep(const slemm_geno* hs, const int64_t* keep_idx, int64_t n_keep,
+                                   slemm_geno** out) {
+  return guard([&] {
+    REQUIRE(hs && keep_idx && out, "null argument");
+    *out = nullptr;
+    REQUIRE(n_keep > 0, "individual set retained for analysis is empty");
+    std::vector<int32_t> keep32(n_keep);
+    for (int64_t k = 0; k < n_keep; ++k) {
+      REQUIRE(keep_idx[k] >= 0 && keep_idx[k] < hs->n, "keep_idx out of range");
+      REQUIRE(k == 0 || keep_idx[k] > keep_idx[k - 1], "keep_idx must be strictly increasing");
+      keep32[k] = (int32_t)keep_idx[k];
+    }
+    HIPCHECK(hipSetDevice(hs->device));
+    slemm_geno* h = alloc_geno(n_keep, hs->m);
+    try {
+      DevBuf dkeep;
+      dkeep.reserve(sizeof(int32_t) * n_keep);
+      HIPCHECK(hipMemcpy(dkeep.p, keep32.data(), sizeof(int32_t) * n_keep, hipMemcpyHostToDevice));
+      // the same compaction as geno_create, reading the source handle's SNP-major rows in place
+      for (int64_t j0 = 0; j0 < hs->m; j0 += 65535) {
+        const int64_t rows = std::min<int64_t>(65535, hs->m - j0);
+        dim3 grid((unsigned)((h->ld_gs + 255) / 256), (unsigned)rows);
+        k_compact<<<grid, 256>>>(hs->gs + j0 * hs->ld_gs, hs->ld_gs, (int)rows, dkeep.as<int32_t>(), n_keep,
+                                 h->gs + j0 * h->ld_gs, h->ld_gs);
+        HIPCHECK(hipGetLastError());
+      }
+      build_transpose(h, nullptr);
+      HIPCHECK(hipDeviceSynchronize());
+    } catch (...) {
+      slemm_gpu_geno_destroy(h);
+      throw;
+    }
+    *out = h;
+  });
+}
+
 static void synthesize(int64_t n, int64_t m, uint64_t seed, float maf_lo, float maf_hi, float missing,
                        const float* p_host, slemm_geno** out) {
   REQUIRE(out, "out is null");

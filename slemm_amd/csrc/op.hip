@@ -114,6 +114,7 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
   la.G = h->gs; la.ldg = h->ld_gs; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x1.p;
   la.lo_offset16 = L.lo1_16; la.KP = L.KP; la.rows_pad = h->m_pad; la.part = h->partbuf.as<float>();
   la.squared = squared;
+  if (tc) la.grp_nsuper = tc->grp_nsuper;  // reduction over individuals stops at the subsets' end
   TimingToken tok = timing_begin(st);
   launch_bp(la, L.NT, S, bp_choice(h, 0, L.NT).variant, st);
   timing_end(tok, st, EPI_PASS1, PREC_BF16X2);
@@ -137,6 +138,7 @@ void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const flo
   LdsArgs la;
   la.G = h->gi; la.ldg = h->ld_gi; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x2.p;
   la.lo_offset16 = L.lo2_16; la.KP = L.KP; la.rows_pad = h->n_pad; la.part = h->partbuf.as<float>();
+  if (tc) la.grp_rows = tc->grp_rows;  // output rows (individuals) past the subsets' end are zero
   TimingToken tok = timing_begin(st);
   launch_bp(la, L.NT, S, bp_choice(h, 1, L.NT).variant, st);
   timing_end(tok, st, EPI_PASS2, PREC_BF16X2);

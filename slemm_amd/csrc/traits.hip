@@ -84,13 +84,29 @@ void trait_dots(slemm_traits* tr, const float* X, int64_t ldx, int t, double* ou
   HIPCHECK(hipGetLastError());
 }
 
-void upload_col_trait(slemm_traits* tr, const int* col_trait, int t, hipStream_t st) {
+// Uploads the per-column trait ids and, per column group of 16*NT columns, the extent of the
+// group's subsets (largest member + 1) for the work limits of the block products.
+void upload_col_trait(slemm_traits* tr, const int* col_trait, int t, int NT, hipStream_t st) {
   REQUIRE(col_trait, "col_trait is null");
   std::vector<int> ct(col_trait, col_trait + t);
   for (int k = 0; k < t; ++k) REQUIRE(ct[k] >= 0 && ct[k] < tr->T, "col_trait out of range");
+  ct.push_back(NT);  // the group limits depend on the tile width too
   if (ct != tr->col_trait_host) {
+    const int w = 16 * NT;
+    const int ngrp = (t + w - 1) / w;
+    std::vector<int32_t> gns(ngrp, 0);
+    std::vector<int64_t> grows(ngrp, 0);
+    for (int k = 0; k < t; ++k) {
+      const int64_t e = tr->extent[ct[k]];
+      grows[k / w] = std::max(grows[k / w], e);
+    }
+    for (int q = 0; q < ngrp; ++q) gns[q] = (int32_t)((grows[q] + kPad - 1) / kPad);
     tr->col_trait.reserve(sizeof(int) * t);
+    tr->grp_nsuper.reserve(sizeof(int32_t) * ngrp);
+    tr->grp_rows.reserve(sizeof(int64_t) * ngrp);
     HIPCHECK(hipMemcpyAsync(tr->col_trait.p, ct.data(), sizeof(int) * t, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(tr->grp_nsuper.p, gns.data(), sizeof(int32_t) * ngrp, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(tr->grp_rows.p, grows.data(), sizeof(int64_t) * ngrp, hipMemcpyHostToDevice, st));
     HIPCHECK(hipStreamSynchronize(st));
     tr->col_trait_host = ct;
   }
@@ -108,7 +124,7 @@ void run_op_traits(slemm_traits* tr, const int* col_trait, const int* col_kind, 
   const Layout L = layout_for(h, t, precision);
   bool any_quad = false;
   const int* kinds = upload_kinds(h, SLEMM_OP_LDET, col_kind, t, any_quad, st);
-  upload_col_trait(tr, col_trait, t, st);
+  upload_col_trait(tr, col_trait, t, L.NT, st);
   const TraitCols tc = tr->cols();
 
   const double* qtv = nullptr;
@@ -164,6 +180,8 @@ extern "C" int slemm_gpu_traits_create(slemm_geno* h, int T, const uint8_t* mask
       tr->n_host[t] = (int64_t)tr->members[t].size();
       REQUIRE(tr->n_host[t] >= 2, "every trait needs at least two individuals");
     }
+    tr->extent.assign(T, 0);
+    for (int t = 0; t < T; ++t) tr->extent[t] = tr->members[t].back() + 1;
     std::vector<double> nd(tr->n_host.begin(), tr->n_host.end());
     tr->n_t.reserve(sizeof(double) * T);
     HIPCHECK(hipMemcpy(tr->n_t.p, nd.data(), sizeof(double) * T, hipMemcpyHostToDevice));

@@ -107,6 +107,13 @@ class Geno:
         L.check(lib.slemm_gpu_geno_subset(self._h, _np_ptr(idx, C.c_int64), len(idx), C.byref(h)))
         return Geno(h)
 
+    def keep(self, keep_idx: Sequence[int]) -> "Geno":
+        """New handle with the given individuals (strictly increasing), compacted on the device."""
+        idx = np.ascontiguousarray(keep_idx, dtype=np.int64)
+        h = C.c_void_p()
+        L.check(L.load().slemm_gpu_geno_keep(self._h, _np_ptr(idx, C.c_int64), len(idx), C.byref(h)))
+        return Geno(h)
+
     def close(self) -> None:
         if self._h:
             L.check(L.load().slemm_gpu_geno_destroy(self._h))

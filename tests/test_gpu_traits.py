@@ -32,6 +32,20 @@ def batch(gpu):
     return p, masks, g, tr, subs
 
 
+def test_device_individual_subset_equals_host_compaction(batch):
+    p, masks, g, tr, subs = batch
+    keep, gs = subs[0]
+    gk = g.keep(keep)
+    assert (gk.n, gk.m) == (len(keep), p.m)
+    assert np.array_equal(gk.download(), gs.download())
+    gk.set_model()
+    gs2 = gs  # the transposed copy agrees too: same matvec
+    U = torch.from_numpy(np.random.default_rng(1).normal(size=(p.m, 2)).astype(np.float32)).cuda()
+    gs2.set_model()
+    assert np.array_equal(gk.matvec(U, precision="fp32").cpu().numpy(), gs2.matvec(U, precision="fp32").cpu().numpy())
+    gs.set_model(None, None, np.full((len(keep), 1), 1 / np.sqrt(len(keep)), np.float32))
+
+
 def test_trait_model_terms_bit_exact(batch):
     p, masks, g, tr, subs = batch
     nt, mono = tr.info()
@@ -214,3 +228,33 @@ def test_graph_replayed_steps_equal_eager_steps(batch, monkeypatch):
         for a, b in zip((res[0][0], res[0][1], res[1], res[2][0], res[2][1], res[3]),
                         (eager[0][0], eager[0][1], eager[1], eager[2][0], eager[2][1], eager[3])):
             assert np.array_equal(a, b)
+
+
+def test_prefix_subsets_skip_zero_work(gpu):
+    """Subsets that end early (individuals [0, n_t)): the block products skip reduction chunks and
+    output rows past each column group's last member; results must still equal subset handles.
+    32 columns per trait, so column groups hold one or two traits."""
+    p = make_problem(n=2500, m=1300, seed=57)
+    sizes = [700, 1300, 2500]
+    masks = np.zeros((3, p.n), bool)
+    for t, s in enumerate(sizes):
+        masks[t, :s] = True
+    g = gpu.Geno.from_bed(p.rows, p.n)
+    tr = gpu.Traits(g, masks)
+    rng = np.random.default_rng(3)
+    per = 32
+    col_trait = np.repeat(np.arange(3), per)
+    kinds = (["quadform"] + ["ldet"] * (per - 1)) * 3
+    V = np.zeros((p.n, 3 * per), np.float32)
+    for k, t in enumerate(col_trait):
+        V[masks[t], k] = rng.normal(size=masks[t].sum())
+    out = tr.lnr_op_select(col_trait, kinds, 0.3, torch.from_numpy(V).cuda()).cpu().numpy()
+    for t, s in enumerate(sizes):
+        keep = np.arange(s)
+        gs = g.keep(keep)
+        gs.set_model(None, None, np.full((s, 1), 1 / np.sqrt(s), np.float32))
+        cols = np.flatnonzero(col_trait == t)
+        ref = gs.lnr_op_select("ldet", 0.3, torch.from_numpy(np.ascontiguousarray(V[np.ix_(keep, cols)])).cuda(),
+                               col_kind=[kinds[k] for k in cols]).cpu().numpy()
+        assert rel_err(out[np.ix_(keep, cols)], ref) < 1e-5, t
+        assert np.all(out[s:, cols] == 0.0)
