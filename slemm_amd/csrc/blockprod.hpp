@@ -51,11 +51,11 @@ struct TraitCols {
   const float* gw = nullptr;       // T x ld_m: SNP weights over the subset
   const int32_t* pc = nullptr;     // T x ld_m: dosage sums over the subset
   int64_t ld_n = 0, ld_m = 0;
-  // per column group of the block product (16*NT columns): one past the last member of any of
-  // its columns' subsets, in 512-individual super-chunks (pass-1 reduction) and in rows (pass-2
-  // output rows); work beyond is skipped because the operands there are zero
-  const int32_t* grp_nsuper = nullptr;
-  const int64_t* grp_rows = nullptr;
+  // per 16-column tile of the block product: one past the last member of any of its columns'
+  // subsets, in 512-individual super-chunks (pass-1 reduction) and in rows (pass-2 output
+  // rows); the MFMAs beyond are skipped because the operands there are zero
+  const int32_t* tile_nsuper = nullptr;
+  const int64_t* tile_rows = nullptr;
 };
 
 struct PackArgs {
@@ -94,10 +94,10 @@ struct LdsArgs {
   int64_t rows_pad = 0;
   float* part = nullptr;      // nsplit x KP x rows_pad partial products (column-major per split)
   bool squared = false;       // unpack squared dosages {0,1,1,4} instead of {0,1,1,2}
-  // optional per column group (blockIdx.y) limits: reduction super-chunks that can be nonzero,
-  // and rows from which the products are known to be zero (both skipped, zeros written)
-  const int32_t* grp_nsuper = nullptr;
-  const int64_t* grp_rows = nullptr;
+  // optional per 16-column tile limits (tile = blockIdx.y * NT + j): reduction super-chunks that
+  // can be nonzero, and rows from which the tile's products are known to be zero (skipped)
+  const int32_t* tile_nsuper = nullptr;
+  const int64_t* tile_rows = nullptr;
 };
 
 struct Fin1Args {

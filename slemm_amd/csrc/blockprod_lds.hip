@@ -13,6 +13,8 @@
 //    products and the epilogue kernels below sum the splits in a fixed order (deterministic).
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "blockprod.hpp"
 
 namespace slemm {
@@ -183,8 +185,18 @@ __global__ __launch_bounds__(256) void k_bp_reg(LdsArgs a) {
   const int col0 = blockIdx.y * (16 * NT);
   const int64_t s0 = (int64_t)blockIdx.z * a.ns_per_split;
   int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
-  if (a.grp_nsuper) s1 = min(s1, (int64_t)a.grp_nsuper[blockIdx.y]);
-  if (a.grp_rows && row0 >= a.grp_rows[blockIdx.y]) s1 = s0;  // this wave's rows are all zero
+  // per-tile work limits (batched traits): MFMAs of tile j run only for super-chunks s < lim[j]
+  int32_t lim[NT];
+  int32_t limmax = 0;
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int64_t tile = (int64_t)blockIdx.y * NT + j;
+    int32_t l = a.tile_nsuper ? a.tile_nsuper[tile] : INT32_MAX;
+    if (a.tile_rows && row0 >= a.tile_rows[tile]) l = 0;  // this wave's rows of the tile are zero
+    lim[j] = l;
+    limmax = max(limmax, l);
+  }
+  s1 = min(s1, (int64_t)limmax);
   const uint4* xh = reinterpret_cast<const uint4*>(a.X);
   const uint4* xl = xh + a.lo_offset16;
   f32x4_t acc[RT][NT];
@@ -227,14 +239,16 @@ __global__ __launch_bounds__(256) void k_bp_reg(LdsArgs a) {
       // all hi products of the step, then all lo products (independent MFMAs in between)
 #pragma unroll
       for (int j = 0; j < NT; ++j)
+        if (s < lim[j])
 #pragma unroll
-        for (int i = 0; i < RT; ++i)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bh[j], acc[i][j], 0, 0, 0);
+          for (int i = 0; i < RT; ++i)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bh[j], acc[i][j], 0, 0, 0);
 #pragma unroll
       for (int j = 0; j < NT; ++j)
+        if (s < lim[j])
 #pragma unroll
-        for (int i = 0; i < RT; ++i)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl[j], acc[i][j], 0, 0, 0);
+          for (int i = 0; i < RT; ++i)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl[j], acc[i][j], 0, 0, 0);
     }
   }
 #pragma unroll
@@ -265,8 +279,18 @@ __global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
   const int col0 = blockIdx.y * (16 * NT);
   const int64_t s0 = (int64_t)blockIdx.z * a.ns_per_split;
   int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
-  if (a.grp_nsuper) s1 = min(s1, (int64_t)a.grp_nsuper[blockIdx.y]);
-  if (a.grp_rows && row0 >= a.grp_rows[blockIdx.y]) s1 = s0;  // this wave's rows are all zero
+  // per-tile work limits (batched traits): MFMAs of tile j run only for super-chunks s < lim[j]
+  int32_t lim[NT];
+  int32_t limmax = 0;
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int64_t tile = (int64_t)blockIdx.y * NT + j;
+    int32_t l = a.tile_nsuper ? a.tile_nsuper[tile] : INT32_MAX;
+    if (a.tile_rows && row0 >= a.tile_rows[tile]) l = 0;  // this wave's rows of the tile are zero
+    lim[j] = l;
+    limmax = max(limmax, l);
+  }
+  s1 = min(s1, (int64_t)limmax);
   const uint4* xh = reinterpret_cast<const uint4*>(a.X);
   const uint4* xl = xh + a.lo_offset16;
   const uint8_t* gbase = a.G + (row0 + r) * a.ldg + g * 32;
@@ -295,19 +319,22 @@ __global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
 #pragma unroll
     for (int i = 0; i < RT; ++i) av[i] = unpack8<SQ>((cc[i][t8 >> 1] >> (16 * (t8 & 1))) & 0xFFFFu);
     const int64_t bn = boff_of(T < Tlast ? T + 1 : T);
+    const int64_t sc = T >> 4;  // super-chunk of this step
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
+      if (sc < lim[j])
 #pragma unroll
-      for (int i = 0; i < RT; ++i)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bh[j], acc[i][j], 0, 0, 0);
+        for (int i = 0; i < RT; ++i)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bh[j], acc[i][j], 0, 0, 0);
       bh[j] = __builtin_bit_cast(bf16x8_t, xh[bn + j * 16]);
       __builtin_amdgcn_sched_barrier(0);  // keep the refill after the MFMAs that read bh[j]
     }
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
+      if (sc < lim[j])
 #pragma unroll
-      for (int i = 0; i < RT; ++i)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl[j], acc[i][j], 0, 0, 0);
+        for (int i = 0; i < RT; ++i)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl[j], acc[i][j], 0, 0, 0);
       bl[j] = __builtin_bit_cast(bf16x8_t, xl[bn + j * 16]);
       __builtin_amdgcn_sched_barrier(0);
     }

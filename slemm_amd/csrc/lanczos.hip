@@ -156,7 +156,8 @@ bool lanczos_graphs() {
 }
 
 slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_group, const float* B, int64_t ldb,
-                       int t, float tol, int maxit, int keep_cols, int verbose, hipStream_t st, bool use_graph) {
+                       int t, float tol, int maxit, int keep_cols, int verbose, hipStream_t st, bool use_graph,
+                       const ActiveFn& on_active) {
     REQUIRE(h && B, "null argument");
     REQUIRE(t > 0, "number of columns must be positive");
     REQUIRE(ldb >= h->n, "ldb < n");
@@ -350,6 +351,7 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
       if (changed) {
         for (int k = 0; k < t; ++k) active[k] = gdone[group[k]] ? 0 : 1;
         HIPCHECK(hipMemcpyAsync(dact.p, active.data(), sizeof(int) * t, hipMemcpyHostToDevice, st));
+        if (on_active && !all_done) on_active(active, st);
       }
       if (all_done) break;
       // rotate buffers: prev <- cur <- next

@@ -14,6 +14,13 @@ namespace slemm {
 Layout layout_for(const slemm_geno* h, int t, int precision) {
   Layout L;
   L.NT = blockprod_nt(t);
+  // Wide split-BF16 blocks (t > 128) run as 80-column groups: the pipelined kernel keeps two
+  // waves per SIMD at 5 column tiles, where 8-tile groups drop to one (SLEMM_WIDE_NT overrides).
+  if (precision == PREC_BF16X2 && use_lds(precision) && t > 128) {
+    static const char* e = std::getenv("SLEMM_WIDE_NT");
+    const int nt = e ? std::atoi(e) : 5;
+    L.NT = nt >= 1 && nt <= 8 ? nt : 5;
+  }
   L.KP = (int)round_up(t, 16 * L.NT);
   const size_t es = precision == PREC_BF16X2 ? 4 : 4;  // hi+lo bf16, or one fp32
   L.x1_bytes = (size_t)h->n_pad * L.KP * es;
@@ -114,7 +121,7 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
   la.G = h->gs; la.ldg = h->ld_gs; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x1.p;
   la.lo_offset16 = L.lo1_16; la.KP = L.KP; la.rows_pad = h->m_pad; la.part = h->partbuf.as<float>();
   la.squared = squared;
-  if (tc) la.grp_nsuper = tc->grp_nsuper;  // reduction over individuals stops at the subsets' end
+  if (tc) la.tile_nsuper = tc->tile_nsuper;  // reduction over individuals stops at the subsets' end
   TimingToken tok = timing_begin(st);
   launch_bp(la, L.NT, S, bp_choice(h, 0, L.NT).variant, st);
   timing_end(tok, st, EPI_PASS1, PREC_BF16X2);
@@ -138,7 +145,7 @@ void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const flo
   LdsArgs la;
   la.G = h->gi; la.ldg = h->ld_gi; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x2.p;
   la.lo_offset16 = L.lo2_16; la.KP = L.KP; la.rows_pad = h->n_pad; la.part = h->partbuf.as<float>();
-  if (tc) la.grp_rows = tc->grp_rows;  // output rows (individuals) past the subsets' end are zero
+  if (tc) la.tile_rows = tc->tile_rows;  // output rows (individuals) past the subsets' end are zero
   TimingToken tok = timing_begin(st);
   launch_bp(la, L.NT, S, bp_choice(h, 1, L.NT).variant, st);
   timing_end(tok, st, EPI_PASS2, PREC_BF16X2);

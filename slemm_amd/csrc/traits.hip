@@ -84,31 +84,36 @@ void trait_dots(slemm_traits* tr, const float* X, int64_t ldx, int t, double* ou
   HIPCHECK(hipGetLastError());
 }
 
-// Uploads the per-column trait ids and, per column group of 16*NT columns, the extent of the
-// group's subsets (largest member + 1) for the work limits of the block products.
-void upload_col_trait(slemm_traits* tr, const int* col_trait, int t, int NT, hipStream_t st) {
+// Per 16-column tile of the block products: the extent of the tile's subsets (largest member +
+// 1) over its live columns (tr->active, empty = all), uploaded for the kernels' work limits.
+// The device arrays keep their addresses for a given width, as replayed graphs require.
+void upload_tile_limits(slemm_traits* tr, int t, int KP, hipStream_t st) {
+  const int ntile = KP / 16;
+  std::vector<int32_t> tns(ntile, 0);
+  std::vector<int64_t> trows(ntile, 0);
+  for (int k = 0; k < t; ++k)
+    if (tr->active.empty() || tr->active[k])
+      trows[k / 16] = std::max(trows[k / 16], tr->extent[tr->col_trait_host[k]]);
+  for (int q = 0; q < ntile; ++q) tns[q] = (int32_t)((trows[q] + kPad - 1) / kPad);
+  tr->tile_nsuper.reserve(sizeof(int32_t) * ntile);
+  tr->tile_rows.reserve(sizeof(int64_t) * ntile);
+  HIPCHECK(hipMemcpyAsync(tr->tile_nsuper.p, tns.data(), sizeof(int32_t) * ntile, hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(tr->tile_rows.p, trows.data(), sizeof(int64_t) * ntile, hipMemcpyHostToDevice, st));
+  HIPCHECK(hipStreamSynchronize(st));
+}
+
+// Uploads the per-column trait ids (cached) and the tile limits of a new batch.
+void upload_col_trait(slemm_traits* tr, const int* col_trait, int t, int KP, hipStream_t st) {
   REQUIRE(col_trait, "col_trait is null");
   std::vector<int> ct(col_trait, col_trait + t);
   for (int k = 0; k < t; ++k) REQUIRE(ct[k] >= 0 && ct[k] < tr->T, "col_trait out of range");
-  ct.push_back(NT);  // the group limits depend on the tile width too
+  ct.push_back(KP);  // the tile arrays cover the padded width too
   if (ct != tr->col_trait_host) {
-    const int w = 16 * NT;
-    const int ngrp = (t + w - 1) / w;
-    std::vector<int32_t> gns(ngrp, 0);
-    std::vector<int64_t> grows(ngrp, 0);
-    for (int k = 0; k < t; ++k) {
-      const int64_t e = tr->extent[ct[k]];
-      grows[k / w] = std::max(grows[k / w], e);
-    }
-    for (int q = 0; q < ngrp; ++q) gns[q] = (int32_t)((grows[q] + kPad - 1) / kPad);
     tr->col_trait.reserve(sizeof(int) * t);
-    tr->grp_nsuper.reserve(sizeof(int32_t) * ngrp);
-    tr->grp_rows.reserve(sizeof(int64_t) * ngrp);
     HIPCHECK(hipMemcpyAsync(tr->col_trait.p, ct.data(), sizeof(int) * t, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(tr->grp_nsuper.p, gns.data(), sizeof(int32_t) * ngrp, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(tr->grp_rows.p, grows.data(), sizeof(int64_t) * ngrp, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipStreamSynchronize(st));
     tr->col_trait_host = ct;
+    tr->active.clear();
+    upload_tile_limits(tr, t, KP, st);
   }
 }
 
@@ -124,7 +129,7 @@ void run_op_traits(slemm_traits* tr, const int* col_trait, const int* col_kind, 
   const Layout L = layout_for(h, t, precision);
   bool any_quad = false;
   const int* kinds = upload_kinds(h, SLEMM_OP_LDET, col_kind, t, any_quad, st);
-  upload_col_trait(tr, col_trait, t, L.NT, st);
+  upload_col_trait(tr, col_trait, t, L.KP, st);
   const TraitCols tc = tr->cols();
 
   const double* qtv = nullptr;
@@ -292,7 +297,21 @@ extern "C" int slemm_gpu_traits_lseed(slemm_traits* tr, const int* col_trait, co
     const LanczosOp op = [&](const float* U, int64_t ldu, float* o, int64_t ldo, hipStream_t s) {
       run_op_traits(tr, ct.data(), ck.data(), tau0, U, ldu, t, o, ldo, precision, s);
     };
-    *out = lseed_run(tr->h, op, col_group, B, ldb, t, tol, maxit, keep_cols, verbose, (hipStream_t)stream,
-                     lanczos_graphs());
+    // converged groups stop costing block-product work: their columns leave the tile limits
+    const ActiveFn on_active = [&](const std::vector<int>& active, hipStream_t s) {
+      tr->active = active;
+      upload_tile_limits(tr, t, layout_for(tr->h, t, precision).KP, s);
+    };
+    try {
+      *out = lseed_run(tr->h, op, col_group, B, ldb, t, tol, maxit, keep_cols, verbose, (hipStream_t)stream,
+                       lanczos_graphs(), on_active);
+    } catch (...) {
+      tr->active.clear();
+      tr->col_trait_host.clear();
+      throw;
+    }
+    // later calls on this batch see every column again
+    tr->active.clear();
+    tr->col_trait_host.clear();
   });
 }

@@ -15,9 +15,10 @@ struct slemm_traits {
   slemm::DevBuf n_t;           // T doubles
   slemm::DevBuf gm, gw, pc;    // T x m_pad: per-trait 2 x A2 frequency, weights, dosage sums
   slemm::DevBuf col_trait, qd; // per-column trait ids of the last batch; per-column Q'v
-  slemm::DevBuf grp_nsuper, grp_rows;  // per column group of the last batch: extent of its subsets
+  slemm::DevBuf tile_nsuper, tile_rows;  // per 16-column tile of the last batch: extent of its subsets
   std::vector<int> col_trait_host;
   std::vector<int64_t> extent;           // one past the last member of each subset
+  std::vector<int> active;               // live columns of the running seed (empty: all)
   std::vector<int64_t> n_host;           // subset sizes
   std::vector<int64_t> monomorphic;      // SNPs monomorphic within each subset (weight set to 0)
   std::vector<std::vector<int64_t>> members;  // member individuals of each subset, ascending
@@ -33,8 +34,8 @@ struct slemm_traits {
     c.pc = pc.as<int32_t>();
     c.ld_n = h->n_pad;
     c.ld_m = h->m_pad;
-    c.grp_nsuper = grp_nsuper.as<int32_t>();
-    c.grp_rows = grp_rows.as<int64_t>();
+    c.tile_nsuper = tile_nsuper.as<int32_t>();
+    c.tile_rows = tile_rows.as<int64_t>();
     return c;
   }
 };
