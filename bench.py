@@ -29,10 +29,11 @@ BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA
 FP32_PEAK_TFLOPS = 157.3
 
 
-# HBM bytes per launch of the split-BF16 block product measured with rocprofv3 FETCH_SIZE and
-# WRITE_SIZE on this workload (fetch counter scaled x2.0, write x1.0 against a 1 GiB device copy);
-# recorded with the commands and calibration in profiles/blockprod_c2_k65_hbm_traffic.md.
-MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 14.441e9 + 0.160e9, "pass2": 15.010e9 + 0.096e9}}
+# Memory-side bytes per launch of the split-BF16 block product (pipelined variant, 2048-workgroup
+# grid) measured with rocprofv3 FETCH_SIZE and WRITE_SIZE on this workload (fetch counter scaled
+# x2.0, write x1.0 against a 1 GiB device copy); commands and calibration in
+# profiles/blockprod_c2_k65_hbm_traffic.md.  FETCH_SIZE includes Infinity Cache hits.
+MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 26.76e9 + 0.320e9, "pass2": 27.90e9 + 0.193e9}}
 
 
 def pass_bytes(n, m, K):
