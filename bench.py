@@ -33,7 +33,7 @@ FP32_PEAK_TFLOPS = 157.3
 # grid) measured with rocprofv3 FETCH_SIZE and WRITE_SIZE on this workload (fetch counter scaled
 # x2.0, write x1.0 against a 1 GiB device copy); commands and calibration in
 # profiles/blockprod_c2_k65_hbm_traffic.md.  FETCH_SIZE includes Infinity Cache hits.
-MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 26.76e9 + 0.320e9, "pass2": 27.90e9 + 0.193e9}}
+MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 15.68e9 + 0.320e9, "pass2": 22.08e9 + 0.193e9}}
 
 
 def pass_bytes(n, m, K):

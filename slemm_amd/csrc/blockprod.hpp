@@ -145,7 +145,10 @@ struct Fin2Args {
 //           pays off when few MFMAs share each unpacked operand, i.e. small K)
 //  BP_PIPE: as BP_REG, software-pipelined: B fragments requested one step ahead, codes half a
 //           super-chunk ahead (NT <= 5; wider tiles run BP_REG)
-enum { BP_REG = 0, BP_LDS4 = 1, BP_LDS2 = 2, BP_REG8 = 3, BP_REGL = 4, BP_PIPE = 5, BP_NVARIANTS = 6 };
+//  BP_PERSIST: BP_PIPE as a persistent grid (two workgroups per CU walking the tiles in order,
+//           so resident workgroups share the vector block in L2)
+enum { BP_REG = 0, BP_LDS4 = 1, BP_LDS2 = 2, BP_REG8 = 3, BP_REGL = 4, BP_PIPE = 5, BP_PERSIST = 6,
+       BP_NVARIANTS = 7 };
 static_assert(BP_PIPE == 5, "slemm_geno's default block-product variant is BP_PIPE");
 
 int lds_nt(int t);

@@ -13,6 +13,7 @@
 //    products and the epilogue kernels below sum the splits in a fixed order (deterministic).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 
 #include "blockprod.hpp"
@@ -271,13 +272,14 @@ __global__ __launch_bounds__(256) void k_bp_reg(LdsArgs a) {
 //    per lane and row tile) with two register sets; the next half is requested when the current
 //    one starts.
 // Loads past the split's last step re-read that step (valid addresses, no branches).
+// One (row block bx, reduction split bz) tile of the pipelined product for column group blockIdx.y.
 template <int RT, int NT, bool SQ>
-__global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
+__device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64_t bz) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 15, g = lane >> 4;
-  const int64_t row0 = (int64_t)blockIdx.x * (64 * RT) + wave * (16 * RT);
+  const int64_t row0 = bx * (64 * RT) + wave * (16 * RT);
   const int col0 = blockIdx.y * (16 * NT);
-  const int64_t s0 = (int64_t)blockIdx.z * a.ns_per_split;
+  const int64_t s0 = bz * a.ns_per_split;
   int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
   // per-tile work limits (batched traits): MFMAs of tile j run only for super-chunks s < lim[j]
   int32_t lim[NT];
@@ -293,25 +295,32 @@ __global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
   s1 = min(s1, (int64_t)limmax);
   const uint4* xh = reinterpret_cast<const uint4*>(a.X);
   const uint4* xl = xh + a.lo_offset16;
-  const uint8_t* gbase = a.G + (row0 + r) * a.ldg + g * 32;
+  const uint8_t* gbase = a.G + (row0 + r) * a.ldg + g * 16;
   f32x4_t acc[RT][NT];
 #pragma unroll
   for (int i = 0; i < RT; ++i)
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 
-  // codes of half-chunk hc = 2s + h (8 steps): a super-chunk s spans row bytes [128 s, 128 s + 128),
-  // lane group g owns 32 of them and half h is its bytes [16 h, 16 h + 16)
+  // codes of half-chunk hc = 2s + h (8 steps): a super-chunk s spans row bytes [128 s, 128 s + 128)
+  // and half h its bytes [64 h, 64 h + 64), of which lane group g reads [16 g, 16 g + 16).  Each
+  // half is one contiguous 64-byte sector per row, so no sector is fetched twice (two halves
+  // interleaved within a line would make the second load re-fetch the streamed line).  Element
+  // (h, g, step t8, e) of the super-chunk is 256 h + 64 g + 8 t8 + e, i.e. vector block row
+  // 32 h + 8 g + t8 (boff_of below).
   auto load_codes = [&](int64_t hc, uint32_t (&dst)[RT][4]) {
 #pragma unroll
     for (int i = 0; i < RT; ++i) {
-      const u32x4_t q = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(
-          gbase + (int64_t)i * 16 * a.ldg + (hc >> 1) * 128 + (hc & 1) * 16));
+      // a plain (not streaming) load: the line must survive in L2 until the other half is read
+      const u32x4_t q = *reinterpret_cast<const u32x4_t*>(
+          gbase + (int64_t)i * 16 * a.ldg + (hc >> 1) * 128 + (hc & 1) * 64);
       dst[i][0] = q.x; dst[i][1] = q.y; dst[i][2] = q.z; dst[i][3] = q.w;
     }
   };
-  // B fragments of global step T (32 reduction elements) sit in block row s*64 + g*16 + T%16
-  auto boff_of = [&](int64_t T) { return ((T >> 4) * 64 + g * 16 + (T & 15)) * a.KP + col0 + r; };
+  // B fragments of global step T = 16 s + 8 h + t8 sit in block row 64 s + 32 h + 8 g + t8
+  auto boff_of = [&](int64_t T) {
+    return ((T >> 4) * 64 + ((T >> 3) & 1) * 32 + g * 8 + (T & 7)) * a.KP + col0 + r;
+  };
   const int64_t Tlast = s1 * 16 - 1;
   bf16x8_t bh[NT], bl[NT];
   auto step = [&](const uint32_t (&cc)[RT][4], int t8, int64_t T) {
@@ -364,9 +373,23 @@ __global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
     for (int j = 0; j < NT; ++j) {
       const int64_t col = col0 + 16 * j + r;
       const int64_t row = row0 + 16 * i + 4 * g;
-      float4* dst = reinterpret_cast<float4*>(a.part + ((int64_t)blockIdx.z * a.KP + col) * a.rows_pad + row);
+      float4* dst = reinterpret_cast<float4*>(a.part + (bz * a.KP + col) * a.rows_pad + row);
       *dst = make_float4(acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]);
     }
+}
+
+template <int RT, int NT, bool SQ>
+__global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
+  bp_pipe_tile<RT, NT, SQ>(a, blockIdx.x, blockIdx.z);
+}
+
+// Persistent form: a grid sized to what the GPU holds at once (two workgroups per CU) walks the
+// (row block, split) tiles in order, tile v = split * nbx + row block.  All resident workgroups
+// then sweep the same reduction range at the same pace, so each vector-block line fetched into
+// an XCD's L2 serves every workgroup of that XCD instead of being re-fetched by late starters.
+template <int RT, int NT, bool SQ>
+__global__ __launch_bounds__(256, 2) void k_bp_persist(LdsArgs a, int64_t nbx, int64_t ntiles) {
+  for (int64_t v = blockIdx.x; v < ntiles; v += gridDim.x) bp_pipe_tile<RT, NT, SQ>(a, v % nbx, v / nbx);
 }
 
 // Pass-1 epilogue from the split partials: T_jk = w_j (P_jk - gm_j s_k + xbar_k (pc_j - n gm_j));
@@ -478,6 +501,16 @@ static void launch_variant(const LdsArgs& a, int variant, dim3 grid, hipStream_t
     case BP_PIPE:  // wider column tiles exceed two waves' register budget: plain register variant
       if constexpr (NT <= 5) k_bp_pipe<kLdsRT, NT, SQ><<<grid, 256, 0, st>>>(a);
       else k_bp_reg<kLdsRT, NT, SQ><<<grid, 256, 0, st>>>(a);
+      break;
+    case BP_PERSIST:
+      if constexpr (NT <= 5) {
+        const int64_t nbx = grid.x, ntiles = (int64_t)grid.x * grid.z;
+        const int64_t resident = std::max<int64_t>(64, 512 / (int64_t)grid.y);  // 2 workgroups per CU
+        k_bp_persist<kLdsRT, NT, SQ><<<dim3((unsigned)std::min(ntiles, resident), grid.y, 1), 256, 0, st>>>(
+            a, nbx, ntiles);
+      } else {
+        k_bp_reg<kLdsRT, NT, SQ><<<grid, 256, 0, st>>>(a);
+      }
       break;
     case BP_LDS4: k_bp_lds<kLdsRT, NT, 4, true, SQ><<<grid, 256, 0, st>>>(a); break;
     case BP_LDS2: k_bp_lds<kLdsRT, NT, 2, false, SQ><<<grid, 256, 0, st>>>(a); break;
