@@ -147,8 +147,10 @@ struct Fin2Args {
 //           super-chunk ahead (NT <= 5; wider tiles run BP_REG)
 //  BP_PERSIST: BP_PIPE as a persistent grid (two workgroups per CU walking the tiles in order,
 //           so resident workgroups share the vector block in L2)
+//  BP_PIPEL: BP_PIPE with the codes unpacked through a 256-entry LDS byte table (a third of the
+//           VALU work; for narrow blocks, where the unpack and not the MFMA issue is the limit)
 enum { BP_REG = 0, BP_LDS4 = 1, BP_LDS2 = 2, BP_REG8 = 3, BP_REGL = 4, BP_PIPE = 5, BP_PERSIST = 6,
-       BP_NVARIANTS = 7 };
+       BP_PIPEL = 7, BP_NVARIANTS = 8 };
 static_assert(BP_PIPE == 5, "slemm_geno's default block-product variant is BP_PIPE");
 
 int lds_nt(int t);

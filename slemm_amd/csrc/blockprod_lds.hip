@@ -38,11 +38,13 @@ __device__ __forceinline__ float bf16_to_float(uint32_t b) { return __uint_as_fl
 template <bool SQ = false>
 __device__ __forceinline__ bf16x8_t unpack8(uint32_t p16) {
   constexpr uint32_t lo_tab = SQ ? 0x80808000u : 0x00808000u;
+  // sp holds code 2q at bits 4q..4q+1 and code 2q+1 at bits 4q+16..4q+17 (bits 14-15 mix two
+  // codes but are never read), so each pair needs one shift and one mask
+  const uint32_t sp = p16 | (p16 << 14);
   uint32_t w[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const uint32_t t = (p16 >> (4 * q)) & 0xFu;
-    const uint32_t x = (t | (t << 14)) & 0x00030003u;  // c0 -> byte 0, c1 -> byte 2
+    const uint32_t x = (sp >> (4 * q)) & 0x00030003u;  // c_2q -> byte 0, c_2q+1 -> byte 2
     w[q] = __builtin_amdgcn_perm(0x403F3F00u, lo_tab, x * 0x0101u + 0x04000400u);
   }
   return __builtin_bit_cast(bf16x8_t, make_uint4(w[0], w[1], w[2], w[3]));
@@ -272,9 +274,20 @@ __global__ __launch_bounds__(256) void k_bp_reg(LdsArgs a) {
 //    per lane and row tile) with two register sets; the next half is requested when the current
 //    one starts.
 // Loads past the split's last step re-read that step (valid addresses, no branches).
+// 256-entry LDS table for the table-driven unpack: entry b holds the four BF16 dosages (or
+// squared dosages) of the four codes of byte b, two per 32-bit word.
+template <bool SQ>
+__device__ __forceinline__ void fill_byte_table(uint2* lut8) {
+  const uint32_t t = threadIdx.x;  // 256 threads, one entry each
+  const uint32_t bf[4] = {0x0000u, 0x3F80u, 0x3F80u, SQ ? 0x4080u : 0x4000u};
+  lut8[t] = make_uint2(bf[t & 3] | (bf[(t >> 2) & 3] << 16), bf[(t >> 4) & 3] | (bf[t >> 6] << 16));
+}
+
 // One (row block bx, reduction split bz) tile of the pipelined product for column group blockIdx.y.
-template <int RT, int NT, bool SQ>
-__device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64_t bz) {
+// LUT: codes are unpacked by two 8-byte LDS reads per 8 codes (lut8 filled by the caller)
+// instead of the permute sequence: a third of the VALU work, for blocks too narrow to hide it.
+template <int RT, int NT, bool SQ, bool LUT = false>
+__device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64_t bz, const uint2* lut8 = nullptr) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 15, g = lane >> 4;
   const int64_t row0 = bx * (64 * RT) + wave * (16 * RT);
@@ -322,29 +335,41 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
     return ((T >> 4) * 64 + ((T >> 3) & 1) * 32 + g * 8 + (T & 7)) * a.KP + col0 + r;
   };
   const int64_t Tlast = s1 * 16 - 1;
-  bf16x8_t bh[NT], bl[NT];
+  // B prefetch distance in steps: a step with few column tiles is short, so narrow blocks keep
+  // D register sets in flight (D divides 8, so step t8 of a half always uses set t8 % D)
+  constexpr int D = NT == 1 ? 8 : (NT == 2 ? 4 : (NT == 3 ? 2 : 1));
+  bf16x8_t bh[D][NT], bl[D][NT];
   auto step = [&](const uint32_t (&cc)[RT][4], int t8, int64_t T) {
+    const int slot = t8 % D;
     bf16x8_t av[RT];
 #pragma unroll
-    for (int i = 0; i < RT; ++i) av[i] = unpack8<SQ>((cc[i][t8 >> 1] >> (16 * (t8 & 1))) & 0xFFFFu);
-    const int64_t bn = boff_of(T < Tlast ? T + 1 : T);
+    for (int i = 0; i < RT; ++i) {
+      if constexpr (LUT) {
+        const uint32_t w = cc[i][t8 >> 1] >> (16 * (t8 & 1));
+        const uint2 lo = lut8[w & 0xFFu], hi = lut8[(w >> 8) & 0xFFu];
+        av[i] = __builtin_bit_cast(bf16x8_t, make_uint4(lo.x, lo.y, hi.x, hi.y));
+      } else {
+        av[i] = unpack8<SQ>((cc[i][t8 >> 1] >> (16 * (t8 & 1))) & 0xFFFFu);
+      }
+    }
+    const int64_t bn = boff_of(T + D <= Tlast ? T + D : Tlast);  // refill with step T + D
     const int64_t sc = T >> 4;  // super-chunk of this step
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
       if (sc < lim[j])
 #pragma unroll
         for (int i = 0; i < RT; ++i)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bh[j], acc[i][j], 0, 0, 0);
-      bh[j] = __builtin_bit_cast(bf16x8_t, xh[bn + j * 16]);
-      __builtin_amdgcn_sched_barrier(0);  // keep the refill after the MFMAs that read bh[j]
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bh[slot][j], acc[i][j], 0, 0, 0);
+      bh[slot][j] = __builtin_bit_cast(bf16x8_t, xh[bn + j * 16]);
+      __builtin_amdgcn_sched_barrier(0);  // keep the refill after the MFMAs that read the set
     }
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
       if (sc < lim[j])
 #pragma unroll
         for (int i = 0; i < RT; ++i)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl[j], acc[i][j], 0, 0, 0);
-      bl[j] = __builtin_bit_cast(bf16x8_t, xl[bn + j * 16]);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl[slot][j], acc[i][j], 0, 0, 0);
+      bl[slot][j] = __builtin_bit_cast(bf16x8_t, xl[bn + j * 16]);
       __builtin_amdgcn_sched_barrier(0);
     }
   };
@@ -352,11 +377,14 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
   uint32_t cA[RT][4], cB[RT][4];
   if (s0 < s1) {
     load_codes(2 * s0, cA);
-    const int64_t b0 = boff_of(s0 * 16);
 #pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      bh[j] = __builtin_bit_cast(bf16x8_t, xh[b0 + j * 16]);
-      bl[j] = __builtin_bit_cast(bf16x8_t, xl[b0 + j * 16]);
+    for (int d = 0; d < D; ++d) {
+      const int64_t b0 = boff_of(s0 * 16 + d);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        bh[d][j] = __builtin_bit_cast(bf16x8_t, xh[b0 + j * 16]);
+        bl[d][j] = __builtin_bit_cast(bf16x8_t, xl[b0 + j * 16]);
+      }
     }
   }
   for (int64_t s = s0; s < s1; ++s) {
@@ -381,6 +409,14 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
 template <int RT, int NT, bool SQ>
 __global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
   bp_pipe_tile<RT, NT, SQ>(a, blockIdx.x, blockIdx.z);
+}
+
+template <int RT, int NT, bool SQ>
+__global__ __launch_bounds__(256, 2) void k_bp_pipel(LdsArgs a) {
+  __shared__ uint2 lut8[256];
+  fill_byte_table<SQ>(lut8);
+  __syncthreads();
+  bp_pipe_tile<RT, NT, SQ, true>(a, blockIdx.x, blockIdx.z, lut8);
 }
 
 // Persistent form: a grid sized to what the GPU holds at once (two workgroups per CU) walks the
@@ -501,6 +537,10 @@ static void launch_variant(const LdsArgs& a, int variant, dim3 grid, hipStream_t
     case BP_PIPE:  // wider column tiles exceed two waves' register budget: plain register variant
       if constexpr (NT <= 5) k_bp_pipe<kLdsRT, NT, SQ><<<grid, 256, 0, st>>>(a);
       else k_bp_reg<kLdsRT, NT, SQ><<<grid, 256, 0, st>>>(a);
+      break;
+    case BP_PIPEL:
+      if constexpr (NT <= 5) k_bp_pipel<kLdsRT, NT, SQ><<<grid, 256, 0, st>>>(a);
+      else k_bp_reg<kLdsRT, NT, SQ, true><<<grid, 256, 0, st>>>(a);
       break;
     case BP_PERSIST:
       if constexpr (NT <= 5) {
