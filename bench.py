@@ -91,7 +91,7 @@ def small_k_passes(g, lib, L, n, m, ks=(1, 3, 7, 16), reps=3):
     for K in ks:
         V = torch.randn(n, K, device="cuda")
         U = torch.randn(m, K, device="cuda")
-        g.autotune(K, reps=2)
+        at = g.autotune(K, reps=2)
         g.rmatvec(V)
         g.matvec(U)
         torch.cuda.synchronize()
@@ -109,7 +109,8 @@ def small_k_passes(g, lib, L, n, m, ks=(1, 3, 7, 16), reps=3):
         out[f"K={K}"] = {"pass1_ms": round(p1, 3), "pass2_ms": round(p2, 3),
                          "pass1_GB/s": round(b / p1 / 1e6, 1), "pass2_GB/s": round(b / p2 / 1e6, 1),
                          "pass1_hbm_frac": round(b / p1 / 1e6 / HBM_PEAK_GBS, 3),
-                         "pass2_hbm_frac": round(b / p2 / 1e6 / HBM_PEAK_GBS, 3)}
+                         "pass2_hbm_frac": round(b / p2 / 1e6 / HBM_PEAK_GBS, 3),
+                         "kernels": [at["pass1"][:2], at["pass2"][:2]]}
     alpha = torch.randn(m, dtype=torch.float64, device="cuda")
     g.gebv(alpha)
     torch.cuda.synchronize()

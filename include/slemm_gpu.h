@@ -290,13 +290,14 @@ int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int rng_seed, i
 /* Autotuner of the split-BF16 block products (no reference counterpart; the reference's only
  * blocking knob is --subset_size, src/option.cpp:75,297).  Times every kernel variant (B operand
  * from global memory with 4 or 8 row tiles per wave, with table-driven code unpacking, or
- * software-pipelined as an ordinary or a persistent grid or with a byte-table unpack; or
- * LDS-staged in 4- or 2-step groups)
+ * software-pipelined as an ordinary or a persistent grid or with a byte-table unpack from one
+ * shared or a 16- or 32-fold replicated table; or LDS-staged in 4- or 2-step groups)
  * at three grid sizes for both passes of a t-column operator on this handle's shape, and keeps
- * the fastest per pass for later calls.  Host outputs (any may be NULL): variant[2], target[2]
- * (workgroups the grid aims for), ms[SLEMM_AUTOTUNE_MS] = best per pass, then
- * 2 + pass*24 + variant*3 + grid-size index. */
-#define SLEMM_AUTOTUNE_MS 50
+ * the fastest per pass for later calls.  Blocks of t <= 8 vectors (hi and lo halves in one
+ * 16-column tile) have only the pipelined variants; the others read -1 in ms.  Host outputs
+ * (any may be NULL): variant[2], target[2] (workgroups the grid aims for),
+ * ms[SLEMM_AUTOTUNE_MS] = best per pass, then 2 + pass*30 + variant*3 + grid-size index. */
+#define SLEMM_AUTOTUNE_MS 62
 int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, int* target, double* ms);
 
 /* ---------------------------------------------------------------- instrumentation ---------- */

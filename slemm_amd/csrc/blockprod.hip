@@ -396,6 +396,7 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
           const double rep = (double)bf16_to_float(hb[e]) + (double)bf16_to_float(lb[e]);
           csum += a.wsum ? (i0 + e < a.n ? (double)a.wsum[i0 + e] * rep : 0.0) : rep;
         }
+        if (a.hl && !live) continue;  // hi/lo in columns: column k >= t holds the lo half of k - t
         uint4* xh = reinterpret_cast<uint4*>(a.X);
         const int64_t idx = (i0 >> 3) * a.KP + k;
         xh[idx] = make_uint4(hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16), hb[6] | (hb[7] << 16));

@@ -74,6 +74,8 @@ struct PackArgs {
   int mode = 0;
   void* X = nullptr;            // null: only the column sums of x are produced
   int64_t lo_offset16 = 0;
+  bool hl = false;              // hi/lo in columns: lo half of column k is column t + k (lo_offset16
+                                // = t); pad columns k >= t are not written
   const double* xsum = nullptr; // if set, x_ik -= float(xsum_k / n) before packing
   double* part = nullptr;       // (rows_pad/256) x KP partial column sums
   const float* wsum = nullptr;  // optional row weights of the column sums (sum_i wsum_i x_ik)
@@ -94,6 +96,11 @@ struct LdsArgs {
   int64_t rows_pad = 0;
   float* part = nullptr;      // nsplit x KP x rows_pad partial products (column-major per split)
   bool squared = false;       // unpack squared dosages {0,1,1,4} instead of {0,1,1,2}
+  int ncols = 0;              // live (nonzero) columns of the block, 0: all; the pipelined variants
+                              // skip B loads of the others
+  bool hl = false;            // hi/lo in columns (one column tile): the block's columns are the hi
+                              // halves of the t vectors followed by their lo halves, and each
+                              // step runs one MFMA per row tile; X's lo block is not read
   // optional per 16-column tile limits (tile = blockIdx.y * NT + j): reduction super-chunks that
   // can be nonzero, and rows from which the tile's products are known to be zero (skipped)
   const int32_t* tile_nsuper = nullptr;
@@ -117,6 +124,8 @@ struct Fin1Args {
   float* out = nullptr;            // optional column-major rows x t
   int64_t ldo = 0;
   bool raw = false;                // out = summed partial products, no centring or weighting
+  bool hl = false;                 // partials of column k's lo half are column t + k; the pass-2
+                                   // operand is written in the same layout (lo_offset16 = t)
   TraitCols tc;                    // batched traits: per-column gm, gw, popcounts, subset size
 };
 
@@ -133,6 +142,7 @@ struct Fin2Args {
   const int* kinds = nullptr;
   float* out = nullptr;
   int64_t ldo = 0;
+  bool hl = false;                 // partials of column k's lo half are column t + k
   TraitCols tc;                    // batched traits: rows outside the column's subset are zeroed
 };
 
@@ -149,8 +159,19 @@ struct Fin2Args {
 //           so resident workgroups share the vector block in L2)
 //  BP_PIPEL: BP_PIPE with the codes unpacked through a 256-entry LDS byte table (a third of the
 //           VALU work; for narrow blocks, where the unpack and not the MFMA issue is the limit)
+//  BP_PIPER16 / BP_PIPER32: BP_PIPEL with the table replicated 16 / 32 times (32 / 64 KB), lane l
+//           reading copy l % R: the copies of an entry sit in consecutive banks, so the random
+//           byte lookups of a wave collide less (16) or not at all (32) in LDS banks; BP_PIPER32
+//           runs 512-thread workgroups on 512 rows so that the 64 KB table still allows four
+//           waves per SIMD
+// Under the hi/lo-in-columns layout (LdsArgs::hl) only the pipelined variants exist; the
+// others run BP_PIPE.
 enum { BP_REG = 0, BP_LDS4 = 1, BP_LDS2 = 2, BP_REG8 = 3, BP_REGL = 4, BP_PIPE = 5, BP_PERSIST = 6,
-       BP_PIPEL = 7, BP_NVARIANTS = 8 };
+       BP_PIPEL = 7, BP_PIPER16 = 8, BP_PIPER32 = 9, BP_NVARIANTS = 10 };
+// variants that implement the hi/lo-in-columns layout
+constexpr bool bp_hl_variant(int v) {
+  return v == BP_PIPE || v == BP_PERSIST || v == BP_PIPEL || v == BP_PIPER16 || v == BP_PIPER32;
+}
 static_assert(BP_PIPE == 5, "slemm_geno's default block-product variant is BP_PIPE");
 
 int lds_nt(int t);

@@ -275,35 +275,58 @@ __global__ __launch_bounds__(256) void k_bp_reg(LdsArgs a) {
 //    one starts.
 // Loads past the split's last step re-read that step (valid addresses, no branches).
 // 256-entry LDS table for the table-driven unpack: entry b holds the four BF16 dosages (or
-// squared dosages) of the four codes of byte b, two per 32-bit word.
-template <bool SQ>
+// squared dosages) of the four codes of byte b, two per 32-bit word.  R copies of each entry
+// are stored side by side (index b * R + copy) so that lanes reading different copies hit
+// different banks.
+template <bool SQ, int R = 1, int NTH = 256>
 __device__ __forceinline__ void fill_byte_table(uint2* lut8) {
-  const uint32_t t = threadIdx.x;  // 256 threads, one entry each
+  static_assert((256 * R) % NTH == 0, "table size must be a multiple of the workgroup size");
   const uint32_t bf[4] = {0x0000u, 0x3F80u, 0x3F80u, SQ ? 0x4080u : 0x4000u};
-  lut8[t] = make_uint2(bf[t & 3] | (bf[(t >> 2) & 3] << 16), bf[(t >> 4) & 3] | (bf[t >> 6] << 16));
+#pragma unroll
+  for (int k = 0; k < 256 * R / NTH; ++k) {
+    const uint32_t idx = threadIdx.x + (uint32_t)NTH * k;  // NTH threads
+    const uint32_t t = idx / R;
+    lut8[idx] = make_uint2(bf[t & 3] | (bf[(t >> 2) & 3] << 16), bf[(t >> 4) & 3] | (bf[t >> 6] << 16));
+  }
 }
 
 // One (row block bx, reduction split bz) tile of the pipelined product for column group blockIdx.y.
-// LUT: codes are unpacked by two 8-byte LDS reads per 8 codes (lut8 filled by the caller)
-// instead of the permute sequence: a third of the VALU work, for blocks too narrow to hide it.
-template <int RT, int NT, bool SQ, bool LUT = false>
-__device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64_t bz, const uint2* lut8 = nullptr) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// LUT > 0: codes are unpacked by two 8-byte LDS reads per 8 codes (lut8, LUT copies per entry,
+// filled by the caller) instead of the permute sequence: a third of the VALU work, for blocks
+// too narrow to hide it.  HL: hi/lo-in-columns block, one MFMA per row tile and step.  WPG waves
+// per workgroup, each on its own 16 * RT rows.  FULL: each row's 128 code bytes of a super-chunk
+// are requested at once (lane group g reads bytes [32 g, 32 g + 32)), a super-chunk ahead, so
+// that the DRAM sees whole lines per row instead of two 64-byte halves eight steps apart.
+// SBC > 0 (hi/lo-in-columns blocks with at most SBC live columns, FULL): the workgroup stages
+// each super-chunk's live B columns in LDS (sb, 2 x 64 x SBC 16-byte pieces), loaded with the
+// codes a super-chunk ahead.  Per-step B loads from global memory would share the in-order
+// vector-memory counter with the code loads, so every wait for a B fragment would also wait for
+// the next super-chunk's codes and the code prefetch would shrink to a few steps.
+template <int RT, int NT, bool SQ, int LUT = 0, bool HL = false, int WPG = 4, bool FULL = false, int SBC = 0>
+__device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64_t bz, const uint2* lut8 = nullptr,
+                                             uint4* sb = nullptr) {
+  static_assert(SBC == 0 || (HL && FULL), "staged B is for hi/lo-in-columns blocks with whole-line codes");
+  static_assert(!HL || NT == 1, "hi/lo-in-columns blocks have one column tile");
+  // the wave index is uniform across the wave: keep it (and the tile limits) in scalar registers
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = lane & 15, g = lane >> 4;
-  const int64_t row0 = bx * (64 * RT) + wave * (16 * RT);
+  const int64_t row0 = bx * (WPG * 16 * RT) + wave * (16 * RT);
   const int col0 = blockIdx.y * (16 * NT);
   const int64_t s0 = bz * a.ns_per_split;
   int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
   // per-tile work limits (batched traits): MFMAs of tile j run only for super-chunks s < lim[j]
+  // (with staged B the loop bound must be the same for all waves of the workgroup, which meet at
+  // a barrier every super-chunk: it is taken over the workgroup's rows)
   int32_t lim[NT];
   int32_t limmax = 0;
+  const int64_t wg_row0 = bx * (WPG * 16 * RT);
 #pragma unroll
   for (int j = 0; j < NT; ++j) {
     const int64_t tile = (int64_t)blockIdx.y * NT + j;
-    int32_t l = a.tile_nsuper ? a.tile_nsuper[tile] : INT32_MAX;
-    if (a.tile_rows && row0 >= a.tile_rows[tile]) l = 0;  // this wave's rows of the tile are zero
-    lim[j] = l;
-    limmax = max(limmax, l);
+    const int32_t ln = a.tile_nsuper ? a.tile_nsuper[tile] : INT32_MAX;
+    // this wave's (workgroup's) rows of the tile are zero from tile_rows on
+    lim[j] = (a.tile_rows && row0 >= a.tile_rows[tile]) ? 0 : ln;
+    limmax = max(limmax, SBC > 0 ? ((a.tile_rows && wg_row0 >= a.tile_rows[tile]) ? 0 : ln) : lim[j]);
   }
   s1 = min(s1, (int64_t)limmax);
   const uint4* xh = reinterpret_cast<const uint4*>(a.X);
@@ -331,29 +354,55 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
     }
   };
   // B fragments of global step T = 16 s + 8 h + t8 sit in block row 64 s + 32 h + 8 g + t8
+  // (FULL: step T = 16 s + t uses codes 128 g + 8 t + e of the super-chunk, block row 64 s + 16 g + t)
   auto boff_of = [&](int64_t T) {
+    if constexpr (FULL) return ((T >> 4) * 64 + g * 16 + (T & 15)) * a.KP + col0 + r;
     return ((T >> 4) * 64 + ((T >> 3) & 1) * 32 + g * 8 + (T & 7)) * a.KP + col0 + r;
   };
   const int64_t Tlast = s1 * 16 - 1;
   // B prefetch distance in steps: a step with few column tiles is short, so narrow blocks keep
   // D register sets in flight (D divides 8, so step t8 of a half always uses set t8 % D)
-  constexpr int D = NT == 1 ? 8 : (NT == 2 ? 4 : (NT == 3 ? 2 : 1));
+  // (4 with whole-line code sets, which take the room)
+  constexpr int D = NT == 1 ? (FULL ? 4 : 8) : (NT == 2 ? 4 : (NT == 3 ? 2 : 1));
   bf16x8_t bh[D][NT], bl[D][NT];
-  auto step = [&](const uint32_t (&cc)[RT][4], int t8, int64_t T) {
-    const int slot = t8 % D;
+  constexpr int LS = LUT > 1 ? LUT : 1;                    // entry stride of the table
+  const uint2* lutl = LUT > 1 ? lut8 + (lane % LS) : lut8;  // this lane's copy
+  // 32 copies: entry b of this lane's copy is at byte b * 256 + (lane % 32) * 8, so each lookup
+  // address is one mask-and-or of the code word (bits 8-15 select the entry)
+  const char* lutb = reinterpret_cast<const char*>(lut8);
+  const uint32_t lane_off = (uint32_t)(lane % 32) * 8u;
+  // step t8 (0..7 of a half, or 0..15 of a whole line) of the code set cc
+  auto step = [&](const auto& cc, int t8, int64_t T) {
     bf16x8_t av[RT];
 #pragma unroll
     for (int i = 0; i < RT; ++i) {
-      if constexpr (LUT) {
+      if constexpr (LUT == 32) {
+        const uint32_t W = cc[i][t8 >> 1];
+        const uint32_t olo = ((t8 & 1) ? (W >> 8) : (W << 8)) & 0xFF00u;
+        const uint32_t ohi = ((t8 & 1) ? (W >> 16) : W) & 0xFF00u;
+        const uint2 lo = *reinterpret_cast<const uint2*>(lutb + (olo | lane_off));
+        const uint2 hi = *reinterpret_cast<const uint2*>(lutb + (ohi | lane_off));
+        av[i] = __builtin_bit_cast(bf16x8_t, make_uint4(lo.x, lo.y, hi.x, hi.y));
+      } else if constexpr (LUT > 0) {
         const uint32_t w = cc[i][t8 >> 1] >> (16 * (t8 & 1));
-        const uint2 lo = lut8[w & 0xFFu], hi = lut8[(w >> 8) & 0xFFu];
+        const uint2 lo = lutl[(w & 0xFFu) * LS], hi = lutl[((w >> 8) & 0xFFu) * LS];
         av[i] = __builtin_bit_cast(bf16x8_t, make_uint4(lo.x, lo.y, hi.x, hi.y));
       } else {
         av[i] = unpack8<SQ>((cc[i][t8 >> 1] >> (16 * (t8 & 1))) & 0xFFFFu);
       }
     }
-    const int64_t bn = boff_of(T + D <= Tlast ? T + D : Tlast);  // refill with step T + D
     const int64_t sc = T >> 4;  // super-chunk of this step
+    if constexpr (SBC > 0) {
+      // staged B: piece (k8 row 16 g + t, live column r % SBC) of this super-chunk's buffer;
+      // lanes of dead columns read a live column's piece, which only reaches unread outputs
+      const bf16x8_t bv = __builtin_bit_cast(bf16x8_t, sb[(sc & 1) * 64 * SBC + (g * 16 + (T & 15)) * SBC + r % SBC]);
+      if (sc < lim[0])
+#pragma unroll
+        for (int i = 0; i < RT; ++i) acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv, acc[i][0], 0, 0, 0);
+      return;
+    }
+    const int slot = t8 % D;
+    const int64_t bn = boff_of(T + D <= Tlast ? T + D : Tlast);  // refill with step T + D
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
       if (sc < lim[j])
@@ -363,17 +412,94 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
       bh[slot][j] = __builtin_bit_cast(bf16x8_t, xh[bn + j * 16]);
       __builtin_amdgcn_sched_barrier(0);  // keep the refill after the MFMAs that read the set
     }
+    if constexpr (!HL) {
 #pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      if (sc < lim[j])
+      for (int j = 0; j < NT; ++j) {
+        if (sc < lim[j])
 #pragma unroll
-        for (int i = 0; i < RT; ++i)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl[slot][j], acc[i][j], 0, 0, 0);
-      bl[slot][j] = __builtin_bit_cast(bf16x8_t, xl[bn + j * 16]);
-      __builtin_amdgcn_sched_barrier(0);
+          for (int i = 0; i < RT; ++i)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl[slot][j], acc[i][j], 0, 0, 0);
+        bl[slot][j] = __builtin_bit_cast(bf16x8_t, xl[bn + j * 16]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
   };
 
+  if constexpr (FULL) {
+    // two whole-line sets; the loop walks pairs of super-chunks so that set indices are constant
+    uint32_t cA[RT][8], cB[RT][8];
+    const uint8_t* lbase = a.G + (row0 + r) * a.ldg + g * 32;
+    auto ld = [&](int64_t sc, uint32_t (&dst)[RT][8]) {
+      sc = sc < s1 ? sc : s1 - 1;  // loads past the range re-read the last line
+#pragma unroll
+      for (int i = 0; i < RT; ++i) {
+        const u32x4_t* q = reinterpret_cast<const u32x4_t*>(lbase + (int64_t)i * 16 * a.ldg + sc * 128);
+        const u32x4_t q0 = q[0], q1 = q[1];
+        dst[i][0] = q0.x; dst[i][1] = q0.y; dst[i][2] = q0.z; dst[i][3] = q0.w;
+        dst[i][4] = q1.x; dst[i][5] = q1.y; dst[i][6] = q1.z; dst[i][7] = q1.w;
+      }
+    };
+    // staged B (SBC > 0): piece p < 64 * ncols of super-chunk sc is block row 64 sc + p / ncols,
+    // column p % ncols; thread threadIdx.x holds pieces threadIdx.x + 64 * WPG * q
+    constexpr int NP = SBC > 0 ? (64 * SBC + 64 * WPG - 1) / (64 * WPG) : 1;
+    const int nc = a.ncols;
+    uint4 stg[NP];
+    auto bload = [&](int64_t sc) {
+      sc = sc < s1 ? sc : s1 - 1;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const int p = threadIdx.x + 64 * WPG * q;
+        if (p < 64 * nc) stg[q] = xh[(sc * 64 + p / nc) * a.KP + p % nc];
+      }
+    };
+    auto bstore = [&](int64_t sc) {
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const int p = threadIdx.x + 64 * WPG * q;
+        if (p < 64 * nc) sb[(sc & 1) * 64 * SBC + (p / nc) * SBC + p % nc] = stg[q];
+      }
+    };
+    if (s0 < s1) {
+      ld(s0, cA);
+      if constexpr (SBC > 0) {
+        bload(s0);
+        bstore(s0);
+        __syncthreads();
+      } else {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const int64_t b0 = boff_of(s0 * 16 + d);
+#pragma unroll
+          for (int j = 0; j < NT; ++j) {
+            bh[d][j] = __builtin_bit_cast(bf16x8_t, xh[b0 + j * 16]);
+            if constexpr (!HL) bl[d][j] = __builtin_bit_cast(bf16x8_t, xl[b0 + j * 16]);
+          }
+        }
+      }
+    }
+    // with staged B, the buffer of super-chunk s + 1 is written after every wave has finished
+    // super-chunk s - 1 (the barrier that ended it), so one barrier per super-chunk suffices
+    for (int64_t s = s0; s < s1; s += 2) {
+      ld(s + 1, cB);
+      if constexpr (SBC > 0) bload(s + 1);
+#pragma unroll
+      for (int t = 0; t < 16; ++t) step(cA, t, s * 16 + t);
+      if constexpr (SBC > 0) {
+        bstore(s + 1);
+        __syncthreads();
+      }
+      if (s + 1 < s1) {
+        ld(s + 2, cA);
+        if constexpr (SBC > 0) bload(s + 2);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) step(cB, t, s * 16 + 16 + t);
+        if constexpr (SBC > 0) {
+          bstore(s + 2);
+          __syncthreads();
+        }
+      }
+    }
+  } else {
   uint32_t cA[RT][4], cB[RT][4];
   if (s0 < s1) {
     load_codes(2 * s0, cA);
@@ -383,7 +509,7 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
         bh[d][j] = __builtin_bit_cast(bf16x8_t, xh[b0 + j * 16]);
-        bl[d][j] = __builtin_bit_cast(bf16x8_t, xl[b0 + j * 16]);
+        if constexpr (!HL) bl[d][j] = __builtin_bit_cast(bf16x8_t, xl[b0 + j * 16]);
       }
     }
   }
@@ -394,6 +520,7 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
     load_codes(s + 1 < s1 ? 2 * s + 2 : 2 * s + 1, cA);
 #pragma unroll
     for (int t8 = 0; t8 < 8; ++t8) step(cB, t8, s * 16 + 8 + t8);
+  }
   }
 #pragma unroll
   for (int i = 0; i < RT; ++i)
@@ -406,31 +533,48 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
     }
 }
 
-template <int RT, int NT, bool SQ>
+// hi/lo-in-columns blocks (HL) run with whole-line codes and B staged in LDS (SBC live columns)
+template <int RT, int NT, bool SQ, bool HL = false, int SBC = 0>
 __global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
-  bp_pipe_tile<RT, NT, SQ>(a, blockIdx.x, blockIdx.z);
+  __shared__ uint4 sb[SBC > 0 ? 2 * 64 * SBC : 1];
+  bp_pipe_tile<RT, NT, SQ, 0, HL, 4, (SBC > 0), SBC>(a, blockIdx.x, blockIdx.z, nullptr, sb);
 }
 
-template <int RT, int NT, bool SQ>
+// byte-table unpack with R copies of the table (R = 1: one shared 2 KB table)
+template <int RT, int NT, bool SQ, int R = 1, bool HL = false, int SBC = 0>
 __global__ __launch_bounds__(256, 2) void k_bp_pipel(LdsArgs a) {
-  __shared__ uint2 lut8[256];
-  fill_byte_table<SQ>(lut8);
+  __shared__ uint2 lut8[256 * R];
+  __shared__ uint4 sb[SBC > 0 ? 2 * 64 * SBC : 1];
+  fill_byte_table<SQ, R>(lut8);
   __syncthreads();
-  bp_pipe_tile<RT, NT, SQ, true>(a, blockIdx.x, blockIdx.z, lut8);
+  bp_pipe_tile<RT, NT, SQ, R, HL, 4, NT == 1, SBC>(a, blockIdx.x, blockIdx.z, lut8, sb);
+}
+
+// 32 table copies (64 KB) shared by a 512-thread workgroup of 8 waves on 512 rows: two such
+// workgroups fit a CU's LDS, i.e. four waves per SIMD
+template <int RT, int NT, bool SQ, bool HL = false, int SBC = 0>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(HL ? 4 : 2))) void k_bp_pipew(LdsArgs a) {
+  __shared__ uint2 lut8[256 * 32];
+  __shared__ uint4 sb[SBC > 0 ? 2 * 64 * SBC : 1];
+  fill_byte_table<SQ, 32, 512>(lut8);
+  __syncthreads();
+  bp_pipe_tile<RT, NT, SQ, 32, HL, 8, (SBC > 0), SBC>(a, blockIdx.x, blockIdx.z, lut8, sb);
 }
 
 // Persistent form: a grid sized to what the GPU holds at once (two workgroups per CU) walks the
 // (row block, split) tiles in order, tile v = split * nbx + row block.  All resident workgroups
 // then sweep the same reduction range at the same pace, so each vector-block line fetched into
 // an XCD's L2 serves every workgroup of that XCD instead of being re-fetched by late starters.
-template <int RT, int NT, bool SQ>
+template <int RT, int NT, bool SQ, bool HL = false>
 __global__ __launch_bounds__(256, 2) void k_bp_persist(LdsArgs a, int64_t nbx, int64_t ntiles) {
-  for (int64_t v = blockIdx.x; v < ntiles; v += gridDim.x) bp_pipe_tile<RT, NT, SQ>(a, v % nbx, v / nbx);
+  for (int64_t v = blockIdx.x; v < ntiles; v += gridDim.x) bp_pipe_tile<RT, NT, SQ, 0, HL>(a, v % nbx, v / nbx);
 }
 
 // Pass-1 epilogue from the split partials: T_jk = w_j (P_jk - gm_j s_k + xbar_k (pc_j - n gm_j));
 // writes the BF16 hi/lo pass-2 operand, optionally the column-major result, and per-block
 // partial sums of gm_j * represented(T_jk).  Thread (tx = column, ty): 8 blocks of 8 rows.
+// Hi/lo-in-columns blocks: P_jk is the sum of the partials of columns k and t + k, and threads
+// of columns k >= t only write their (zero) workgroup sums.
 __global__ __launch_bounds__(256) void k_fin1(Fin1Args a) {
   __shared__ double red[4][64];
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -443,18 +587,22 @@ __global__ __launch_bounds__(256) void k_fin1(Fin1Args a) {
   const float* gwk = trait >= 0 ? (a.gw ? a.tc.gw + (int64_t)trait * a.tc.ld_m : nullptr) : a.gw;
   const int32_t* pck = trait >= 0 ? a.tc.pc + (int64_t)trait * a.tc.ld_m : a.popcnt;
   const double nk = trait >= 0 ? a.tc.n_t[trait] : a.xsum_n;
-  if (k < a.KP) {
+  if (k < a.KP && !(a.hl && k >= a.t)) {
     const double sk = a.colsum[k];
     const double xb = (a.xsum && k < a.t) ? (double)(float)(a.xsum[k] / nk) : 0.0;
     for (int b8 = 0; b8 < 8; ++b8) {
       const int64_t i0 = rbase + b8 * 8;
       float P[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      for (int z = 0; z < a.nsplit; ++z) {
-        const float4* src = reinterpret_cast<const float4*>(a.part + ((int64_t)z * a.KP + k) * a.rows_pad + i0);
-        const float4 p0 = src[0], p1 = src[1];
-        P[0] += p0.x; P[1] += p0.y; P[2] += p0.z; P[3] += p0.w;
-        P[4] += p1.x; P[5] += p1.y; P[6] += p1.z; P[7] += p1.w;
-      }
+      // padding columns (k >= t) are zero; their partials are not read (the pipelined kernels
+      // leave them undefined)
+      for (int h = 0; h < (k < a.t ? (a.hl ? 2 : 1) : 0); ++h)
+        for (int z = 0; z < a.nsplit; ++z) {
+          const float4* src = reinterpret_cast<const float4*>(
+              a.part + ((int64_t)z * a.KP + k + h * a.t) * a.rows_pad + i0);
+          const float4 p0 = src[0], p1 = src[1];
+          P[0] += p0.x; P[1] += p0.y; P[2] += p0.z; P[3] += p0.w;
+          P[4] += p1.x; P[5] += p1.y; P[6] += p1.z; P[7] += p1.w;
+        }
       uint32_t hb[8], lb[8];
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
@@ -489,10 +637,11 @@ __global__ __launch_bounds__(256) void k_fin2(Fin2Args a) {
   const int64_t i = 4 * ((int64_t)blockIdx.x * 256 + threadIdx.x);
   if (i >= a.rows || k >= a.t) return;
   float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int z = 0; z < a.nsplit; ++z) {
-    const float4 p = *reinterpret_cast<const float4*>(a.part + ((int64_t)z * a.KP + k) * a.rows_pad + i);
-    P.x += p.x; P.y += p.y; P.z += p.z; P.w += p.w;
-  }
+  for (int h = 0; h < (a.hl ? 2 : 1); ++h)  // hi/lo in columns: lo partials in column t + k
+    for (int z = 0; z < a.nsplit; ++z) {
+      const float4 p = *reinterpret_cast<const float4*>(a.part + ((int64_t)z * a.KP + k + h * a.t) * a.rows_pad + i);
+      P.x += p.x; P.y += p.y; P.z += p.z; P.w += p.w;
+    }
   const float Pv[4] = {P.x, P.y, P.z, P.w};
   const double uk = a.colsum[k];
   const bool add_tau = a.v && (a.kinds == nullptr || a.kinds[k] == 0);
@@ -552,9 +701,37 @@ static void launch_variant(const LdsArgs& a, int variant, dim3 grid, hipStream_t
         k_bp_reg<kLdsRT, NT, SQ><<<grid, 256, 0, st>>>(a);
       }
       break;
+    case BP_PIPER16:
+      if constexpr (NT <= 5) k_bp_pipel<kLdsRT, NT, SQ, 16><<<grid, 256, 0, st>>>(a);
+      else k_bp_reg<kLdsRT, NT, SQ, true><<<grid, 256, 0, st>>>(a);
+      break;
+    case BP_PIPER32:
+      if constexpr (NT <= 5) k_bp_pipew<kLdsRT, NT, SQ><<<dim3(grid.x / 2, grid.y, grid.z), 512, 0, st>>>(a);
+      else k_bp_reg<kLdsRT, NT, SQ, true><<<grid, 256, 0, st>>>(a);
+      break;
     case BP_LDS4: k_bp_lds<kLdsRT, NT, 4, true, SQ><<<grid, 256, 0, st>>>(a); break;
     case BP_LDS2: k_bp_lds<kLdsRT, NT, 2, false, SQ><<<grid, 256, 0, st>>>(a); break;
     default: fail(SLEMM_ERR_ARG, "unknown block-product variant");
+  }
+}
+
+// hi/lo-in-columns blocks (one column tile, at most SBC live columns): pipelined variants only
+template <bool SQ, int SBC>
+static void launch_variant_hl(const LdsArgs& a, int variant, dim3 grid, hipStream_t st) {
+  switch (variant) {
+    case BP_PIPEL: k_bp_pipel<kLdsRT, 1, SQ, 1, true, SBC><<<grid, 256, 0, st>>>(a); break;
+    case BP_PIPER16: k_bp_pipel<kLdsRT, 1, SQ, 16, true, SBC><<<grid, 256, 0, st>>>(a); break;
+    case BP_PIPER32: k_bp_pipew<kLdsRT, 1, SQ, true, SBC><<<dim3(grid.x / 2, grid.y, grid.z), 512, 0, st>>>(a); break;
+    case BP_PERSIST: {  // B from global memory, per step
+      const int64_t nbx = grid.x, ntiles = (int64_t)grid.x * grid.z;
+      const int64_t resident = std::max<int64_t>(64, 512 / (int64_t)grid.y);
+      k_bp_persist<kLdsRT, 1, SQ, true><<<dim3((unsigned)std::min(ntiles, resident), grid.y, 1), 256, 0, st>>>(
+          a, nbx, ntiles);
+      break;
+    }
+    default:
+      if (variant < 0 || variant >= BP_NVARIANTS) fail(SLEMM_ERR_ARG, "unknown block-product variant");
+      k_bp_pipe<kLdsRT, 1, SQ, true, SBC><<<grid, 256, 0, st>>>(a);
   }
 }
 
@@ -562,7 +739,20 @@ void launch_bp(const LdsArgs& a, int NT, int nsplit, int variant, hipStream_t st
   if (a.rows_pad % (64 * kLdsRT) != 0) fail(SLEMM_ERR_ARG, "row padding mismatch");
   if (a.KP % (16 * NT) != 0) fail(SLEMM_ERR_ARG, "column padding mismatch");
   dim3 grid((unsigned)(a.rows_pad / (64 * kLdsRT)), (unsigned)(a.KP / (16 * NT)), (unsigned)nsplit);
-  if (a.squared) {
+  if (a.hl) {
+    if (NT != 1) fail(SLEMM_ERR_ARG, "hi/lo-in-columns blocks have one column tile");
+    if (a.ncols < 1 || a.ncols > 16) fail(SLEMM_ERR_ARG, "hi/lo-in-columns blocks have 1 to 16 live columns");
+    if (a.squared) {
+      if (a.ncols > 2) fail(SLEMM_ERR_ARG, "squared-dosage products take one vector");
+      launch_variant_hl<true, 2>(a, variant, grid, st);
+    } else if (a.ncols <= 2) {
+      launch_variant_hl<false, 2>(a, variant, grid, st);
+    } else if (a.ncols <= 8) {
+      launch_variant_hl<false, 8>(a, variant, grid, st);
+    } else {
+      launch_variant_hl<false, 16>(a, variant, grid, st);
+    }
+  } else if (a.squared) {
     if (NT != 1) fail(SLEMM_ERR_ARG, "squared-dosage products support one column tile");
     launch_variant<1, true>(a, variant, grid, st);
   } else {

@@ -22,7 +22,16 @@ Layout layout_for(const slemm_geno* h, int t, int precision) {
     L.NT = nt >= 1 && nt <= 8 ? nt : 5;
   }
   L.KP = (int)round_up(t, 16 * L.NT);
-  const size_t es = precision == PREC_BF16X2 ? 4 : 4;  // hi+lo bf16, or one fp32
+  const char* ehl = std::getenv("SLEMM_HL_COLUMNS");  // "0": hi and lo as separate blocks
+  L.hl = precision == PREC_BF16X2 && use_lds(precision) && 2 * t <= 16 && !(ehl && ehl[0] == '0');
+  L.slot = L.hl ? 0 : L.NT;
+  if (L.hl) {
+    L.x1_bytes = (size_t)h->n_pad * L.KP * 2;
+    L.x2_bytes = (size_t)h->m_pad * L.KP * 2;
+    L.lo1_16 = L.lo2_16 = t;  // lo half of column k is column t + k
+    return L;
+  }
+  const size_t es = 4;  // hi+lo bf16, or one fp32
   L.x1_bytes = (size_t)h->n_pad * L.KP * es;
   L.x2_bytes = (size_t)h->m_pad * L.KP * es;
   L.lo1_16 = precision == PREC_BF16X2 ? (int64_t)h->n_pad * L.KP / 8 : 0;
@@ -67,6 +76,7 @@ void pack_pass1(slemm_geno* h, const Layout& L, const float* V, int64_t ldv, int
   p.V = V; p.ldv = ldv; p.t = t; p.KP = L.KP; p.n = h->n; p.rows_pad = h->n_pad;
   p.rr = rr; p.Q = h->Q; p.ldq = h->n_pad; p.c = h->c; p.qtv = qtv; p.kinds = kinds;
   p.mode = precision; p.X = nullptr; p.lo_offset16 = L.lo1_16; p.part = h->part1.as<double>();
+  p.hl = L.hl;
   if (tc) {
     p.tc = *tc;
     p.c = qtv ? 1 : 0;  // per-trait intercept basis
@@ -95,17 +105,17 @@ bool use_lds(int precision) {
   return precision == PREC_BF16X2 && !v1;
 }
 
-BpChoice bp_choice(const slemm_geno* h, int pass, int NT) {
-  static const char* ev = std::getenv("SLEMM_BP_VARIANT");
-  static const char* et = std::getenv("SLEMM_BP_TARGET");
-  BpChoice c{h->bp_variant[pass][NT], h->bp_target[pass][NT]};
+BpChoice bp_choice(const slemm_geno* h, int pass, int slot) {
+  const char* ev = std::getenv("SLEMM_BP_VARIANT");  // read per call: tests switch variants in-process
+  const char* et = std::getenv("SLEMM_BP_TARGET");
+  BpChoice c{h->bp_variant[pass][slot], h->bp_target[pass][slot]};
   if (ev) c.variant = std::atoi(ev);
   if (et) c.target = std::atoll(et);
   return c;
 }
 
-int64_t pass_splits(const slemm_geno* h, int pass, int NT, int64_t& nsp) {
-  const BpChoice c = bp_choice(h, pass, NT);
+int64_t pass_splits(const slemm_geno* h, int pass, int slot, int64_t& nsp) {
+  const BpChoice c = bp_choice(h, pass, slot);
   return pass == 0 ? lds_splits(h->m_pad, h->n_pad / kPad, c.target, nsp)
                    : lds_splits(h->n_pad, h->m_pad / kPad, c.target, nsp);
 }
@@ -115,21 +125,23 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
                int64_t ldo, hipStream_t st, bool raw, bool squared, const TraitCols* tc) {
   int64_t nsp = 0;
   const int64_t nsuper = h->n_pad / kPad;
-  const int S = (int)pass_splits(h, 0, L.NT, nsp);
+  const int S = (int)pass_splits(h, 0, L.slot, nsp);
   h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * h->m_pad);
   LdsArgs la;
   la.G = h->gs; la.ldg = h->ld_gs; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x1.p;
   la.lo_offset16 = L.lo1_16; la.KP = L.KP; la.rows_pad = h->m_pad; la.part = h->partbuf.as<float>();
   la.squared = squared;
+  la.hl = L.hl;
+  la.ncols = L.hl ? 2 * t : t;
   if (tc) la.tile_nsuper = tc->tile_nsuper;  // reduction over individuals stops at the subsets' end
   TimingToken tok = timing_begin(st);
-  launch_bp(la, L.NT, S, bp_choice(h, 0, L.NT).variant, st);
+  launch_bp(la, L.NT, S, bp_choice(h, 0, L.slot).variant, st);
   timing_end(tok, st, EPI_PASS1, PREC_BF16X2);
   Fin1Args f;
   f.part = h->partbuf.as<float>(); f.nsplit = S; f.rows_pad = h->m_pad; f.rows = h->m; f.KP = L.KP; f.t = t;
   f.colsum = h->colsum.as<double>(); f.xsum = h->colsum.as<double>() + 2 * L.KP; f.xsum_n = (double)h->n;
   f.popcnt = h->popcnt; f.gm = h->gm; f.gw = weighted ? h->gw : nullptr; f.X2 = x2; f.lo_offset16 = L.lo2_16;
-  f.upart = upart; f.out = out; f.ldo = ldo; f.raw = raw;
+  f.upart = upart; f.out = out; f.ldo = ldo; f.raw = raw; f.hl = L.hl;
   if (tc) f.tc = *tc;
   launch_fin1(f, st);
 }
@@ -140,18 +152,21 @@ void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const flo
                const TraitCols* tc) {
   int64_t nsp = 0;
   const int64_t nsuper = h->m_pad / kPad;
-  const int S = (int)pass_splits(h, 1, L.NT, nsp);
+  const int S = (int)pass_splits(h, 1, L.slot, nsp);
   h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * h->n_pad);
   LdsArgs la;
   la.G = h->gi; la.ldg = h->ld_gi; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x2.p;
   la.lo_offset16 = L.lo2_16; la.KP = L.KP; la.rows_pad = h->n_pad; la.part = h->partbuf.as<float>();
+  la.hl = L.hl;
+  la.ncols = L.hl ? 2 * t : t;
   if (tc) la.tile_rows = tc->tile_rows;  // output rows (individuals) past the subsets' end are zero
   TimingToken tok = timing_begin(st);
-  launch_bp(la, L.NT, S, bp_choice(h, 1, L.NT).variant, st);
+  launch_bp(la, L.NT, S, bp_choice(h, 1, L.slot).variant, st);
   timing_end(tok, st, EPI_PASS2, PREC_BF16X2);
   Fin2Args f;
   f.part = h->partbuf.as<float>(); f.nsplit = S; f.rows_pad = h->n_pad; f.rows = h->n; f.KP = L.KP; f.t = t;
   f.colsum = u; f.rr = rr; f.v = V; f.ldv = ldv; f.tau0 = tau0; f.kinds = kinds; f.out = out; f.ldo = ldo;
+  f.hl = L.hl;
   if (tc) f.tc = *tc;
   launch_fin2(f, st);
 }
@@ -188,8 +203,8 @@ void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const floa
     h->part2.reserve(sizeof(double) * nparts * L.KP);
     {  // size the split-partial buffer for both passes before anything is in flight
       int64_t nsp = 0;
-      const int64_t S1 = pass_splits(h, 0, L.NT, nsp);
-      const int64_t S2 = pass_splits(h, 1, L.NT, nsp);
+      const int64_t S1 = pass_splits(h, 0, L.slot, nsp);
+      const int64_t S2 = pass_splits(h, 1, L.slot, nsp);
       h->partbuf.reserve(sizeof(float) * L.KP *
                          std::max<size_t>((size_t)S1 * h->m_pad, (size_t)S2 * h->n_pad));
     }
@@ -387,6 +402,10 @@ extern "C" int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, 
       for (int v = 0; v < BP_NVARIANTS; ++v)
         for (int ti = 0; ti < 3; ++ti) {
           const int64_t tg = targets[ti];
+          if (L.hl && !bp_hl_variant(v)) {  // not implemented for hi/lo-in-columns blocks
+            if (best_ms) best_ms[2 + pass * (BP_NVARIANTS * 3) + v * 3 + ti] = -1.0;
+            continue;
+          }
           int64_t nsp = 0;
           const int S = lds_splits(rows_pad, nsuper, tg, nsp);
           h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * rows_pad);
@@ -396,6 +415,8 @@ extern "C" int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, 
           la.nsuper = nsuper; la.ns_per_split = nsp; la.X = pass == 0 ? h->x1.p : h->x2.p;
           la.lo_offset16 = pass == 0 ? L.lo1_16 : L.lo2_16; la.KP = L.KP; la.rows_pad = rows_pad;
           la.part = h->partbuf.as<float>();
+          la.hl = L.hl;
+          la.ncols = L.hl ? 2 * t : t;
           launch_bp(la, L.NT, S, v, nullptr);  // warm-up
           HIPCHECK(hipEventRecord(e0, nullptr));
           for (int r = 0; r < reps; ++r) launch_bp(la, L.NT, S, v, nullptr);
@@ -411,8 +432,8 @@ extern "C" int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, 
             bt = tg;
           }
         }
-      h->bp_variant[pass][L.NT] = bv;
-      h->bp_target[pass][L.NT] = (int)bt;
+      h->bp_variant[pass][L.slot] = bv;
+      h->bp_target[pass][L.slot] = (int)bt;
       if (variant) variant[pass] = bv;
       if (target) target[pass] = (int)bt;
       if (best_ms) best_ms[pass] = best;
@@ -559,6 +580,7 @@ extern "C" int slemm_gpu_z(slemm_geno* h, const float* U, int64_t ldu, int t, fl
     p.V = U; p.ldv = ldu; p.t = t; p.KP = L.KP; p.n = h->m; p.rows_pad = h->m_pad;
     p.mode = precision; p.X = h->x2.p; p.lo_offset16 = L.lo2_16; p.part = h->part2.as<double>();
     p.wsum = h->gm;
+    p.hl = L.hl;
     launch_pack(p, st);
     double* u = h->colsum.as<double>() + L.KP;
     launch_sum_parts(h->part2.as<double>(), nparts, L.KP, u, st);

@@ -11,6 +11,10 @@ struct Layout {
   int NT, KP;
   size_t x1_bytes, x2_bytes;
   int64_t lo1_16, lo2_16;  // lo-half offsets (16-byte units)
+  // split-BF16 blocks of at most 8 vectors keep hi and lo halves as columns [0,t) and [t,2t) of
+  // one 16-column block: one MFMA per step instead of a hi and a lo MFMA on mostly empty tiles
+  bool hl;
+  int slot;                // autotuner table entry: NT, or 0 for hi/lo-in-columns blocks
 };
 
 struct BpChoice {
@@ -23,8 +27,8 @@ void check_precision(int precision);
 void check_dense(const float* p, int64_t ld, int64_t rows, int t, const char* what);
 const int* upload_kinds(slemm_geno* h, int kind, const int* col_kind, int t, bool& any_quad, hipStream_t st);
 bool use_lds(int precision);
-BpChoice bp_choice(const slemm_geno* h, int pass, int NT);
-int64_t pass_splits(const slemm_geno* h, int pass, int NT, int64_t& nsp);
+BpChoice bp_choice(const slemm_geno* h, int pass, int slot);
+int64_t pass_splits(const slemm_geno* h, int pass, int slot, int64_t& nsp);
 
 // pass-1 operand from V (n x t): rows scaled by rr (optional), QUADFORM columns projected,
 // column means removed (within each column's subset when tc is given).  colsum layout:

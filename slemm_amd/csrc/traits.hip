@@ -144,8 +144,8 @@ void run_op_traits(slemm_traits* tr, const int* col_trait, const int* col_kind, 
   h->part2.reserve(sizeof(double) * nparts * L.KP);
   {
     int64_t nsp = 0;
-    const int64_t S1 = pass_splits(h, 0, L.NT, nsp);
-    const int64_t S2 = pass_splits(h, 1, L.NT, nsp);
+    const int64_t S1 = pass_splits(h, 0, L.slot, nsp);
+    const int64_t S2 = pass_splits(h, 1, L.slot, nsp);
     h->partbuf.reserve(sizeof(float) * L.KP * std::max<size_t>((size_t)S1 * h->m_pad, (size_t)S2 * h->n_pad));
   }
   pass1_lds(h, L, t, true, h->x2.p, h->part2.as<double>(), nullptr, 0, st, false, false, &tc);

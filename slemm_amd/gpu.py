@@ -263,7 +263,7 @@ class Geno:
         """Times the block-product variants for a t-column operator and keeps the fastest."""
         v = (C.c_int * 2)()
         tg = (C.c_int * 2)()
-        names = ["reg", "lds4", "lds2", "reg8", "regl", "pipe", "persist", "pipel"]
+        names = ["reg", "lds4", "lds2", "reg8", "regl", "pipe", "persist", "pipel", "piper16", "piper32"]
         nv = len(names)
         ms = (C.c_double * (2 + 6 * nv))(*([float("nan")] * (2 + 6 * nv)))
         L.check(L.load().slemm_gpu_autotune(self._h, t, reps, v, tg, ms))

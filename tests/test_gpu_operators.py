@@ -77,6 +77,36 @@ def test_op_ldet_and_quadform(gpu, prec, t):
         assert rel_err(out[:, k], ref) < TOL[prec]
 
 
+BP_VARIANTS = 10  # block-product kernel variants selectable with SLEMM_BP_VARIANT
+
+
+@pytest.mark.parametrize("t", [1, 3, 5, 12, 33, 66])
+def test_block_product_variants(gpu, t, monkeypatch):
+    """Every split-BF16 block-product variant at one reduction split and at several, for both
+    split layouts of narrow blocks (t <= 8: hi and lo halves as columns of one tile, or as two
+    blocks), against the FP64 dense operator and the transposed product."""
+    p = make_problem(n=2600, m=1700, seed=19, weights=True, c=2)
+    g = _geno(gpu, p)
+    g.set_model(None, p.rvec, p.Q)
+    D = p.dense()
+    rng = np.random.default_rng(100 + t)
+    V = rng.normal(size=(p.n, t)).astype(np.float32)
+    Vd = torch.from_numpy(V).cuda()
+    tau0 = 0.37
+    ref = D.op("ldet", tau0, V)
+    zref = D.rmatvec(V, weighted=True)
+    for hl in (("1", "0") if t <= 8 else ("1",)):
+        monkeypatch.setenv("SLEMM_HL_COLUMNS", hl)
+        for v in range(BP_VARIANTS):
+            for target in ("1", "2048"):
+                monkeypatch.setenv("SLEMM_BP_VARIANT", str(v))
+                monkeypatch.setenv("SLEMM_BP_TARGET", target)
+                out = g.lnr_op_select("ldet", tau0, Vd, precision="bf16x2").cpu().numpy()
+                assert rel_err(out, ref) < TOL["bf16x2"], (hl, v, target, rel_err(out, ref))
+                zt = g.weighted_rmatvec(Vd, precision="bf16x2").cpu().numpy()
+                assert rel_err(zt, zref) < TOL["bf16x2"], (hl, v, target, rel_err(zt, zref))
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
 def test_rmatvec_matvec(gpu, prec):
     p = make_problem(n=1500, m=2100, seed=13)
