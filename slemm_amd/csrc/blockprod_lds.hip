@@ -13,6 +13,8 @@
 //    products and the epilogue kernels below sum the splits in a fixed order (deterministic).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <climits>
 
@@ -298,10 +300,10 @@ __device__ __forceinline__ void fill_byte_table(uint2* lut8) {
 // are requested at once (lane group g reads bytes [32 g, 32 g + 32)), a super-chunk ahead, so
 // that the DRAM sees whole lines per row instead of two 64-byte halves eight steps apart.
 // SBC > 0 (hi/lo-in-columns blocks with at most SBC live columns, FULL): the workgroup stages
-// each super-chunk's live B columns in LDS (sb, 2 x 64 x SBC 16-byte pieces), loaded with the
-// codes a super-chunk ahead.  Per-step B loads from global memory would share the in-order
-// vector-memory counter with the code loads, so every wait for a B fragment would also wait for
-// the next super-chunk's codes and the code prefetch would shrink to a few steps.
+// the live B columns of each half super-chunk in LDS (sb, 2 x 32 x SBC 16-byte pieces), loaded
+// a half ahead and before the codes.  Per-step B loads from global memory would share the
+// in-order vector-memory counter with the code loads, so every wait for a B fragment would also
+// wait for the next super-chunk's codes and the code prefetch would shrink to a few steps.
 template <int RT, int NT, bool SQ, int LUT = 0, bool HL = false, int WPG = 4, bool FULL = false, int SBC = 0>
 __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64_t bz, const uint2* lut8 = nullptr,
                                              uint4* sb = nullptr) {
@@ -393,9 +395,10 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
     }
     const int64_t sc = T >> 4;  // super-chunk of this step
     if constexpr (SBC > 0) {
-      // staged B: piece (k8 row 16 g + t, live column r % SBC) of this super-chunk's buffer;
-      // lanes of dead columns read a live column's piece, which only reaches unread outputs
-      const bf16x8_t bv = __builtin_bit_cast(bf16x8_t, sb[(sc & 1) * 64 * SBC + (g * 16 + (T & 15)) * SBC + r % SBC]);
+      // staged B: piece (g, step t8 & 7, live column r % SBC) of this half's buffer; lanes of
+      // dead columns read a live column's piece, which only reaches unread outputs
+      const bf16x8_t bv =
+          __builtin_bit_cast(bf16x8_t, sb[((T >> 3) & 1) * 32 * SBC + (g * 8 + (T & 7)) * SBC + r % SBC]);
       if (sc < lim[0])
 #pragma unroll
         for (int i = 0; i < RT; ++i) acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv, acc[i][0], 0, 0, 0);
@@ -439,31 +442,35 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
         dst[i][4] = q1.x; dst[i][5] = q1.y; dst[i][6] = q1.z; dst[i][7] = q1.w;
       }
     };
-    // staged B (SBC > 0): piece p < 64 * ncols of super-chunk sc is block row 64 sc + p / ncols,
-    // column p % ncols; thread threadIdx.x holds pieces threadIdx.x + 64 * WPG * q
-    constexpr int NP = SBC > 0 ? (64 * SBC + 64 * WPG - 1) / (64 * WPG) : 1;
+    // staged B (SBC > 0): piece p < 32 * ncols of half hc = 2 s + h is column p % ncols of block
+    // row 64 s + 16 g + 8 h + u with (g, u) = (p / ncols) / 8, % 8; thread threadIdx.x holds
+    // pieces threadIdx.x + 64 * WPG * q
+    constexpr int NP = SBC > 0 ? (32 * SBC + 64 * WPG - 1) / (64 * WPG) : 1;
     const int nc = a.ncols;
+    const int64_t hlast = 2 * s1 - 1;
     uint4 stg[NP];
-    auto bload = [&](int64_t sc) {
-      sc = sc < s1 ? sc : s1 - 1;
+    auto bload = [&](int64_t hc) {
+      hc = hc <= hlast ? hc : hlast;
 #pragma unroll
       for (int q = 0; q < NP; ++q) {
         const int p = threadIdx.x + 64 * WPG * q;
-        if (p < 64 * nc) stg[q] = xh[(sc * 64 + p / nc) * a.KP + p % nc];
+        const int row = p / nc;
+        if (p < 32 * nc)
+          stg[q] = xh[((hc >> 1) * 64 + (row >> 3) * 16 + (hc & 1) * 8 + (row & 7)) * a.KP + p % nc];
       }
     };
-    auto bstore = [&](int64_t sc) {
+    auto bstore = [&](int64_t hc) {
 #pragma unroll
       for (int q = 0; q < NP; ++q) {
         const int p = threadIdx.x + 64 * WPG * q;
-        if (p < 64 * nc) sb[(sc & 1) * 64 * SBC + (p / nc) * SBC + p % nc] = stg[q];
+        if (p < 32 * nc) sb[(hc & 1) * 32 * SBC + (p / nc) * SBC + p % nc] = stg[q];
       }
     };
     if (s0 < s1) {
+      if constexpr (SBC > 0) bload(2 * s0);
       ld(s0, cA);
       if constexpr (SBC > 0) {
-        bload(s0);
-        bstore(s0);
+        bstore(2 * s0);
         __syncthreads();
       } else {
 #pragma unroll
@@ -477,26 +484,29 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
         }
       }
     }
-    // with staged B, the buffer of super-chunk s + 1 is written after every wave has finished
-    // super-chunk s - 1 (the barrier that ended it), so one barrier per super-chunk suffices
-    for (int64_t s = s0; s < s1; s += 2) {
-      ld(s + 1, cB);
-      if constexpr (SBC > 0) bload(s + 1);
+    // staged B: the buffer of half hc + 1 is written after every wave has finished half hc - 1
+    // (the barrier that ended it), so one barrier per half suffices; B of the next half is
+    // requested before the codes, so waiting for it never waits for the codes
+    // half H of super-chunk s from the codes cc; its first half requests super-chunk s + 1 into nx
+    auto half = [&](auto hconst, const uint32_t (&cc)[RT][8], uint32_t (&nx)[RT][8], int64_t s) {
+      constexpr int H = decltype(hconst)::value;
+      if constexpr (SBC > 0) bload(2 * s + H + 1);
+      if constexpr (H == 0) ld(s + 1, nx);
 #pragma unroll
-      for (int t = 0; t < 16; ++t) step(cA, t, s * 16 + t);
+      for (int t = 0; t < 8; ++t) step(cc, 8 * H + t, s * 16 + 8 * H + t);
       if constexpr (SBC > 0) {
-        bstore(s + 1);
+        bstore(2 * s + H + 1);
         __syncthreads();
       }
+    };
+    using H0 = std::integral_constant<int, 0>;
+    using H1 = std::integral_constant<int, 1>;
+    for (int64_t s = s0; s < s1; s += 2) {
+      half(H0{}, cA, cB, s);
+      half(H1{}, cA, cB, s);
       if (s + 1 < s1) {
-        ld(s + 2, cA);
-        if constexpr (SBC > 0) bload(s + 2);
-#pragma unroll
-        for (int t = 0; t < 16; ++t) step(cB, t, s * 16 + 16 + t);
-        if constexpr (SBC > 0) {
-          bstore(s + 2);
-          __syncthreads();
-        }
+        half(H0{}, cB, cA, s + 1);
+        half(H1{}, cB, cA, s + 1);
       }
     }
   } else {
@@ -536,7 +546,7 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
 // hi/lo-in-columns blocks (HL) run with whole-line codes and B staged in LDS (SBC live columns)
 template <int RT, int NT, bool SQ, bool HL = false, int SBC = 0>
 __global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
-  __shared__ uint4 sb[SBC > 0 ? 2 * 64 * SBC : 1];
+  __shared__ uint4 sb[SBC > 0 ? 2 * 32 * SBC : 1];
   bp_pipe_tile<RT, NT, SQ, 0, HL, 4, (SBC > 0), SBC>(a, blockIdx.x, blockIdx.z, nullptr, sb);
 }
 
@@ -544,7 +554,7 @@ __global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
 template <int RT, int NT, bool SQ, int R = 1, bool HL = false, int SBC = 0>
 __global__ __launch_bounds__(256, 2) void k_bp_pipel(LdsArgs a) {
   __shared__ uint2 lut8[256 * R];
-  __shared__ uint4 sb[SBC > 0 ? 2 * 64 * SBC : 1];
+  __shared__ uint4 sb[SBC > 0 ? 2 * 32 * SBC : 1];
   fill_byte_table<SQ, R>(lut8);
   __syncthreads();
   bp_pipe_tile<RT, NT, SQ, R, HL, 4, NT == 1, SBC>(a, blockIdx.x, blockIdx.z, lut8, sb);
@@ -555,7 +565,7 @@ __global__ __launch_bounds__(256, 2) void k_bp_pipel(LdsArgs a) {
 template <int RT, int NT, bool SQ, bool HL = false, int SBC = 0>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(HL ? 4 : 2))) void k_bp_pipew(LdsArgs a) {
   __shared__ uint2 lut8[256 * 32];
-  __shared__ uint4 sb[SBC > 0 ? 2 * 64 * SBC : 1];
+  __shared__ uint4 sb[SBC > 0 ? 2 * 32 * SBC : 1];
   fill_byte_table<SQ, 32, 512>(lut8);
   __syncthreads();
   bp_pipe_tile<RT, NT, SQ, 32, HL, 8, (SBC > 0), SBC>(a, blockIdx.x, blockIdx.z, lut8, sb);
