@@ -87,13 +87,16 @@ void trait_dots(slemm_traits* tr, const float* X, int64_t ldx, int t, double* ou
 // Per 16-column tile of the block products: the extent of the tile's subsets (largest member +
 // 1) over its live columns (tr->active, empty = all), uploaded for the kernels' work limits.
 // The device arrays keep their addresses for a given width, as replayed graphs require.
-void upload_tile_limits(slemm_traits* tr, int t, int KP, hipStream_t st) {
-  const int ntile = KP / 16;
+void upload_tile_limits(slemm_traits* tr, int t, const Layout& L, hipStream_t st) {
+  const int ntile = L.KP / 16;
   std::vector<int32_t> tns(ntile, 0);
   std::vector<int64_t> trows(ntile, 0);
   for (int k = 0; k < t; ++k)
-    if (tr->active.empty() || tr->active[k])
-      trows[k / 16] = std::max(trows[k / 16], tr->extent[tr->col_trait_host[k]]);
+    if (tr->active.empty() || tr->active[k]) {
+      const int64_t e = tr->extent[tr->col_trait_host[k]];
+      trows[k / 16] = std::max(trows[k / 16], e);
+      if (L.hl) trows[(t + k) / 16] = std::max(trows[(t + k) / 16], e);  // its lo half, column t + k
+    }
   for (int q = 0; q < ntile; ++q) tns[q] = (int32_t)((trows[q] + kPad - 1) / kPad);
   tr->tile_nsuper.reserve(sizeof(int32_t) * ntile);
   tr->tile_rows.reserve(sizeof(int64_t) * ntile);
@@ -103,17 +106,17 @@ void upload_tile_limits(slemm_traits* tr, int t, int KP, hipStream_t st) {
 }
 
 // Uploads the per-column trait ids (cached) and the tile limits of a new batch.
-void upload_col_trait(slemm_traits* tr, const int* col_trait, int t, int KP, hipStream_t st) {
+void upload_col_trait(slemm_traits* tr, const int* col_trait, int t, const Layout& L, hipStream_t st) {
   REQUIRE(col_trait, "col_trait is null");
   std::vector<int> ct(col_trait, col_trait + t);
   for (int k = 0; k < t; ++k) REQUIRE(ct[k] >= 0 && ct[k] < tr->T, "col_trait out of range");
-  ct.push_back(KP);  // the tile arrays cover the padded width too
+  ct.push_back(2 * L.KP + (L.hl ? 1 : 0));  // the tile arrays cover the padded width and layout too
   if (ct != tr->col_trait_host) {
     tr->col_trait.reserve(sizeof(int) * t);
     HIPCHECK(hipMemcpyAsync(tr->col_trait.p, ct.data(), sizeof(int) * t, hipMemcpyHostToDevice, st));
     tr->col_trait_host = ct;
     tr->active.clear();
-    upload_tile_limits(tr, t, KP, st);
+    upload_tile_limits(tr, t, L, st);
   }
 }
 
@@ -129,7 +132,7 @@ void run_op_traits(slemm_traits* tr, const int* col_trait, const int* col_kind, 
   const Layout L = layout_for(h, t, precision);
   bool any_quad = false;
   const int* kinds = upload_kinds(h, SLEMM_OP_LDET, col_kind, t, any_quad, st);
-  upload_col_trait(tr, col_trait, t, L.KP, st);
+  upload_col_trait(tr, col_trait, t, L, st);
   const TraitCols tc = tr->cols();
 
   const double* qtv = nullptr;
@@ -300,7 +303,7 @@ extern "C" int slemm_gpu_traits_lseed(slemm_traits* tr, const int* col_trait, co
     // converged groups stop costing block-product work: their columns leave the tile limits
     const ActiveFn on_active = [&](const std::vector<int>& active, hipStream_t s) {
       tr->active = active;
-      upload_tile_limits(tr, t, layout_for(tr->h, t, precision).KP, s);
+      upload_tile_limits(tr, t, layout_for(tr->h, t, precision), s);
     };
     try {
       *out = lseed_run(tr->h, op, col_group, B, ldb, t, tol, maxit, keep_cols, verbose, (hipStream_t)stream,
