@@ -29,11 +29,13 @@ BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA
 FP32_PEAK_TFLOPS = 157.3
 
 
-# Memory-side bytes per launch of the split-BF16 block product (pipelined variant, 2048-workgroup
-# grid) measured with rocprofv3 FETCH_SIZE and WRITE_SIZE on this workload (fetch counter scaled
-# x2.0, write x1.0 against a 1 GiB device copy); commands and calibration in
-# profiles/blockprod_c2_k65_hbm_traffic.md.  FETCH_SIZE includes Infinity Cache hits.
-MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 15.68e9 + 0.320e9, "pass2": 22.08e9 + 0.193e9}}
+# Memory-side bytes per launch of the split-BF16 block product (byte-table pipelined kernel at
+# K=65, narrow-block kernel at K=1; 2048-workgroup grids) measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE on this workload (fetch counter scaled x2.0, write x1.0 against a 1 GiB device copy);
+# commands and calibration in profiles/blockprod_c2_k65_hbm_traffic.md.  FETCH_SIZE includes
+# Infinity Cache hits.
+MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 15.73e9 + 0.320e9, "pass2": 20.93e9 + 0.193e9},
+                    (100_000, 500_000, 1): {"pass1": 12.64e9 + 0.064e9, "pass2": 12.71e9 + 0.039e9}}
 
 
 def pass_bytes(n, m, K):
@@ -111,6 +113,8 @@ def small_k_passes(g, lib, L, n, m, ks=(1, 3, 7, 16), reps=3):
                          "pass1_hbm_frac": round(b / p1 / 1e6 / HBM_PEAK_GBS, 3),
                          "pass2_hbm_frac": round(b / p2 / 1e6 / HBM_PEAK_GBS, 3),
                          "kernels": [at["pass1"][:2], at["pass2"][:2]]}
+        if (n, m, K) in MEASURED_TRAFFIC:
+            out[f"K={K}"]["traffic_bytes"] = MEASURED_TRAFFIC[(n, m, K)]
     alpha = torch.randn(m, dtype=torch.float64, device="cuda")
     g.gebv(alpha)
     torch.cuda.synchronize()
