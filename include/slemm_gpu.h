@@ -285,6 +285,21 @@ int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int rng_seed, i
                                float s2max, float s2min, float tol_VC, int precision, int verbose,
                                float* vg, float* ve, float* h2, float* lrt, int* steps, double* seconds);
 
+/* ---------------------------------------------------------------- LMM quadratic forms ------ */
+
+/* Quadratic forms of the --lmm association approximation (src/reml.cpp:884-982, real-genotype
+ * branch): for each SNP snps[i] (host, k entries; the reference samples num_qf_markers SNPs with
+ * MAC > 20 by std::random_shuffle, the caller chooses them here), after a fit with heritability
+ * h2 (tau1 = (1-h2)/h2) whose Py (host, n) it returns: xPy = x'Py, xPx = v'H^-1 v (Lanczos seed
+ * at tau1 with tolerance tol_L, then L_Solve), xSx = v'v with v = (I-QQ') R^-1/2 (x - gm), and
+ * xBx = (v'v - z'(Zw'Zw + tau1 diag(1/gw))^-1 z)/tau1 over the window_size SNPs around the SNP
+ * (src/reml.cpp:944-968; FP64 window systems through rocBLAS/rocSOLVER).  X (n x c, ldx)
+ * covariates as given to the fit.  Host outputs xPy, xPx, xBx, xSx (k each); the gstat file's
+ * ChiSq is xPy^2 / (vg xPx) (src/output.cpp:113). */
+int slemm_gpu_lmm_gstat(slemm_geno* h, const float* X, int64_t ldx, int c, const float* Py, float h2,
+                        const int64_t* snps, int k, int window_size, float tol_L, int precision, int verbose,
+                        float* xPy, float* xPx, float* xBx, float* xSx);
+
 /* ---------------------------------------------------------------- tuning ------------------- */
 
 /* Autotuner of the split-BF16 block products (no reference counterpart; the reference's only

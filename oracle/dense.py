@@ -132,3 +132,27 @@ def sldf_gss(D: "Dense", y_proj, Q, V, s2min=1e-4, s2max=0.7, tol=1e-4):
             fd, *last = sldf_criterion(D, y_proj, Q, V, d, tau0)
     vg, ve, lrt = last
     return vg, ve, lrt, vg / (vg + ve)
+
+
+def lmm_gstat(D: "Dense", Py, tau1: float, snps, window_size: int):
+    """Quadratic forms of --lmm (src/reml.cpp:920-980, real-genotype branch) in FP64 with exact
+    solves: xPy = x'Py, xPx = v'(A_quad0 + tau1 I)^-1 v, xSx = v'v with v = P R^-1/2 (x - gm),
+    xBx = (v'v - z'(Zw'Zw + tau1 diag(1/gw))^-1 z) / tau1 with Zw the projected, scaled window of
+    window_size SNPs around the SNP and z = Zw'v."""
+    snps = np.asarray(snps, dtype=np.int64)
+    m, w = D.m, int(window_size)
+    X = D.Zc[:, snps]
+    xPy = X.T @ np.asarray(Py, dtype=np.float64)
+    V = D.P @ (D.rr[:, None] * X)
+    xPx = np.sum(V * D.solve("quadform", tau1, V), axis=0)
+    xSx = np.sum(V * V, axis=0)
+    xBx = np.zeros(len(snps))
+    for i, j in enumerate(snps):
+        wds = 0 if j < w // 2 else (m - w if j > m - w // 2 else j - w // 2)
+        Zw = D.P @ (D.rr[:, None] * D.Zc[:, wds:wds + w])
+        with np.errstate(divide="ignore"):
+            ridge = tau1 / D.gw[wds:wds + w]
+        G = Zw.T @ Zw + np.diag(ridge)
+        z = Zw.T @ V[:, i]
+        xBx[i] = (xSx[i] - z @ np.linalg.solve(G, z)) / tau1
+    return xPy, xPx, xBx, xSx

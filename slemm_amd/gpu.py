@@ -314,6 +314,22 @@ class Geno:
                     evals=st.criterion_evals, n_MC=n_MC, seconds_seed=st.seconds_seed,
                     seconds_total=st.seconds_total)
 
+    def lmm_gstat(self, X: np.ndarray, Py: np.ndarray, h2: float, snps: np.ndarray, window_size: int = 1000,
+                  tol_L: float = 5e-4, precision="bf16x2", verbose: bool = False) -> dict:
+        """Quadratic forms of --lmm (src/reml.cpp:884-982): xPy, xPx, xBx, xSx for the SNPs
+        `snps` after a fit with heritability h2 that returned Py."""
+        Xf = np.asfortranarray(np.asarray(X, np.float32).reshape(self.n, -1))
+        c = Xf.shape[1]
+        pv = np.ascontiguousarray(Py, np.float32)
+        idx = np.ascontiguousarray(snps, np.int64)
+        k = idx.size
+        out = {key: np.zeros(k, np.float32) for key in ("xPy", "xPx", "xBx", "xSx")}
+        L.check(L.load().slemm_gpu_lmm_gstat(
+            self._h, _np_ptr(Xf, C.c_float), self.n, c, _np_ptr(pv, C.c_float), h2,
+            idx.ctypes.data_as(C.POINTER(C.c_int64)), k, window_size, tol_L, _prec(precision), int(verbose),
+            *[_np_ptr(out[key], C.c_float) for key in ("xPy", "xPx", "xBx", "xSx")]))
+        return out
+
     def reml_score(self, X: np.ndarray, y: np.ndarray, vg: float, ve: float,
                    probes: Optional[torch.Tensor] = None, n_V: int = 30, rng_seed: int = 0,
                    tol_L: float = 5e-4, precision="bf16x2"):

@@ -116,3 +116,51 @@ def test_rank_deficient_covariates_raise(case):
     with pytest.raises(SlemmError) as ei:
         g.SLDF_REML(X, y, n_V=5)
     assert ei.value.code == ERR_RANK
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
+def test_lmm_gstat_against_fp64(gpu, prec):
+    """--lmm quadratic forms (src/reml.cpp:920-980) after an LMC fit, against the FP64 dense
+    evaluation with exact solves; SNPs include both ends of the chromosome (clamped windows)."""
+    from oracle.dense import Dense, lmm_gstat
+    p = make_problem(n=1200, m=1500, seed=12, weights=True, c=2)
+    y = make_phenotype(p, 0.4, seed=3)
+    g = gpu.Geno.from_bed(p.rows, p.n)
+    g.set_model(None, p.rvec)
+    fit = g.LMC_REML(p.X, y, n_MC=10, precision=prec)
+    h2 = np.float32(fit["h2"])
+    tau1 = float((np.float32(1) - h2) / h2)
+    snps = np.array([0, 7, 99, 100, 640, 1399, 1400, 1499])
+    w = 200
+    got = g.lmm_gstat(p.X, fit["Py"], float(h2), snps, window_size=w, precision=prec)
+    D = Dense(p.Z, p.gm, p.gw, p.rr, p.Q)
+    xPy, xPx, xBx, xSx = lmm_gstat(D, fit["Py"], tau1, snps, w)
+    assert rel_err(got["xPy"], xPy) < 1e-5
+    assert rel_err(got["xSx"], xSx) < 1e-5
+    assert rel_err(got["xPx"], xPx) < 1e-4, (got["xPx"], xPx)
+    assert rel_err(got["xBx"], xBx) < 1e-5, (got["xBx"], xBx)
+    assert np.all(got["xBx"] > got["xPx"])
+
+
+def test_grammar_gamma_pipeline(gpu):
+    """LMC fit -> gstat quadratic forms on 30 sampled SNPs -> correction factor -> GRAMMAR-gamma
+    tests of every SNP (bin/slemm_gamma.py:163-174, 306-325), against the dense restatement."""
+    from oracle import gwas as ogwas
+    from slemm_amd import gwas
+    p = make_problem(n=1200, m=1500, seed=12, weights=True, c=2)
+    y = make_phenotype(p, 0.4, seed=3)
+    g = gpu.Geno.from_bed(p.rows, p.n)
+    g.set_model(None, p.rvec)
+    fit = g.LMC_REML(p.X, y, n_MC=10)
+    snps = gwas.sample_gstat_snps(p.gm, p.n, 30, seed=4)
+    gst = g.lmm_gstat(p.X, fit["Py"], fit["h2"], snps, window_size=200)
+    res = gwas.grammar_gamma(g, fit["Py"], gst, snp_var=fit["vg"], mac=20)
+    xy, xx = ogwas.gamma_scores(p.Z, fit["Py"], p.rr, p.Q, mac=20)
+    cf = ogwas.mean_wo_outliers(gst["xSx"].astype(np.float64) / gst["xPx"].astype(np.float64))
+    assert abs(res["cfactor"] - cf) < 1e-12
+    beta, se, chisq, pv = ogwas.gamma_stats(xy, xx, cf, fit["vg"])
+    ok = np.isfinite(chisq)
+    assert np.array_equal(ok, np.isfinite(res["CHISQ"])) and ok.sum() > 1000
+    assert rel_err(res["BETA"][ok], beta[ok]) < 3e-4
+    assert rel_err(res["SE"][ok], se[ok]) < 3e-4
+    assert np.max(np.abs(res["P"][ok] - pv[ok])) < 1e-3

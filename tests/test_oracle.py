@@ -275,3 +275,30 @@ def test_example_maf_are_allele_fractions(ex10k):
     for f in meta["maf"]:
         k = round(f * two_n)
         assert abs(f - k / two_n) < 1e-6
+
+
+# ---------------------------------------------------------------------- LMM quadratic forms ----
+def test_gstat_window_form_is_woodbury_and_whole_genome_window_is_xPx():
+    """xBx = (v'v - z'(Zw'Zw + tau1 D^-1)^-1 z)/tau1 equals v'(Zw D Zw' + tau1 I)^-1 v (Woodbury),
+    and with the window widened to every SNP it is xPx; windows at both ends hold their SNP."""
+    from oracle.dense import lmm_gstat
+    p = make_problem(n=300, m=160, seed=21, weights=True, c=2)
+    D = p.dense()
+    tau1 = 1.7
+    Py = np.random.default_rng(5).normal(size=p.n)
+    snps = np.array([0, 3, 15, 16, 80, 143, 144, 159])
+    w = 32
+    xPy, xPx, xBx, xSx = lmm_gstat(D, Py, tau1, snps, w)
+    assert np.allclose(xPy, D.Zc[:, snps].T @ Py)
+    for i, j in enumerate(snps):
+        wds = 0 if j < w // 2 else (p.m - w if j > p.m - w // 2 else j - w // 2)
+        assert wds <= j < wds + w and 0 <= wds and wds + w <= p.m
+        Zw = D.P @ (D.rr[:, None] * D.Zc[:, wds:wds + w])
+        v = D.P @ (D.rr * D.Zc[:, j])
+        ref = v @ np.linalg.solve(Zw @ np.diag(D.gw[wds:wds + w]) @ Zw.T + tau1 * np.eye(p.n), v)
+        assert abs(xBx[i] - ref) < 1e-9 * abs(ref)
+        assert abs(xSx[i] - v @ v) < 1e-9 * (v @ v)
+    _, xPx_all, xBx_all, _ = lmm_gstat(D, Py, tau1, snps, p.m)
+    assert np.allclose(xBx_all, xPx_all, rtol=1e-9)
+    assert np.allclose(xPx_all, xPx, rtol=1e-12)
+    assert np.all(xBx > xPx)  # a window explains less of v than the whole genome
