@@ -73,6 +73,14 @@ int slemm_gpu_geno_create(const uint8_t* bed_rows, int64_t n_raw, int64_t m_raw,
  * group ordering of src/input.cpp:389-454 without re-reading the bed). */
 int slemm_gpu_geno_subset(const slemm_geno* h, const int64_t* snp_idx, int64_t m, slemm_geno** out);
 
+/* Handle from a PLINK .bed file (path; 3-byte SNP-major magic checked): the same result as
+ * slemm_gpu_geno_create on the file's rows, but the body is streamed from disk through pinned
+ * double buffers, overlapped with the upload and compaction, and only the SNPs in snp_idx (NULL:
+ * all) are read -- the replacement of the byte-at-a-time ifstream of src/input.cpp:359-384 for
+ * beds larger than host memory.  SLEMM_ERR_DATA for a missing, foreign or short file. */
+int slemm_gpu_geno_load_bed(const char* bed_path, int64_t n_raw, int64_t m_raw, const int64_t* keep_idx,
+                            int64_t n_keep, const int64_t* snp_idx, int64_t m, slemm_geno** out);
+
 /* New handle with the individuals keep_idx (strictly increasing, host) of an existing one, all
  * SNPs: the individual compaction of src/input.cpp:360-377 applied on the device to resident
  * rows (analyses of individual subsets without re-reading the bed). */
@@ -284,6 +292,14 @@ int slemm_gpu_traits_lseed(slemm_traits* tr, const int* col_trait, const int* co
 int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int rng_seed, int n_V, float tol_L,
                                float s2max, float s2min, float tol_VC, int precision, int verbose,
                                float* vg, float* ve, float* h2, float* lrt, int* steps, double* seconds);
+
+/* --pred (src/pred.cpp:4-151): reads the estimate CSV (SNP id column 0, allele column 3, effect
+ * column 9), bfile_prefix.fam/.bim/.bed, flips SNPs whose bim allele 1 differs from the
+ * estimate's allele, computes GEBV = sum_j z_ij alpha_j in FP64 on the GPU and writes
+ * "IID,GEBV" lines to output_file with the reference's stream formatting.  Host outputs (may be
+ * NULL): number of individuals, number of SNPs used. */
+int slemm_gpu_pred_plink(const char* estimate_file, const char* bfile_prefix, const char* output_file,
+                         int64_t* n_out, int64_t* m_used);
 
 /* ---------------------------------------------------------------- LMM quadratic forms ------ */
 

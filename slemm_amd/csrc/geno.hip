@@ -1,10 +1,14 @@
 // Packed genotype storage on the device: upload/compaction of PLINK bed rows, the
 // individual-major transposed copy, bit counts, synthetic beds and the model terms.
+#include <fcntl.h>
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -271,6 +275,78 @@ extern "C" int slemm_gpu_device_count(int* count) {
   });
 }
 
+// validated individual keep list (strictly increasing, in range) as int32; returns n
+static int64_t check_keep(const int64_t* keep_idx, int64_t n_keep, int64_t n_raw, std::vector<int32_t>& keep32) {
+  const int64_t n = keep_idx ? n_keep : n_raw;
+  REQUIRE(n > 0, "individual set retained for analysis is empty");
+  if (keep_idx) {
+    keep32.resize(n);
+    for (int64_t k = 0; k < n; ++k) {
+      REQUIRE(keep_idx[k] >= 0 && keep_idx[k] < n_raw, "keep_idx out of range");
+      REQUIRE(k == 0 || keep_idx[k] > keep_idx[k - 1], "keep_idx must be strictly increasing");
+      keep32[k] = (int32_t)keep_idx[k];
+    }
+  }
+  return n;
+}
+
+// writes raw bed rows [j0, j0 + rows) of the output order into dst (rows x row_bytes)
+using RowSource = std::function<void(int64_t j0, int64_t rows, uint8_t* dst)>;
+
+// Fills h->gs from raw rows in chunks of at most ~128 MB through two pinned host buffers and
+// two device staging buffers: while the GPU copies and compacts chunk c (individual keep list
+// applied, padding codes zeroed) the host fills chunk c + 1.  Then builds the individual-major
+// copy.  All work is on one private stream.
+static void stream_rows(slemm_geno* h, int64_t row_bytes, const std::vector<int32_t>& keep32,
+                        const RowSource& src) {
+  const int64_t n = h->n, m = h->m;
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>({m, (128ll << 20) / row_bytes, (int64_t)65535}));
+  const size_t cbytes = (size_t)chunk * row_bytes;
+  hipStream_t st = nullptr;
+  uint8_t* pin[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  DevBuf dkeep, stage[2];
+  auto cleanup = [&] {
+    if (st) (void)hipStreamSynchronize(st);
+    for (int b = 0; b < 2; ++b) {
+      if (pin[b]) (void)hipHostFree(pin[b]);
+      if (done[b]) (void)hipEventDestroy(done[b]);
+    }
+    if (st) (void)hipStreamDestroy(st);
+  };
+  try {
+    HIPCHECK(hipStreamCreate(&st));
+    for (int b = 0; b < 2; ++b) {
+      HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&pin[b]), cbytes));
+      HIPCHECK(hipEventCreate(&done[b]));
+      stage[b].reserve(cbytes);
+    }
+    if (!keep32.empty()) {
+      dkeep.reserve(sizeof(int32_t) * n);
+      HIPCHECK(hipMemcpyAsync(dkeep.p, keep32.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    }
+    int64_t c = 0;
+    for (int64_t j0 = 0; j0 < m; j0 += chunk, ++c) {
+      const int b = (int)(c & 1);
+      const int64_t rows = std::min(chunk, m - j0);
+      if (c >= 2) HIPCHECK(hipEventSynchronize(done[b]));  // chunk c - 2 has left pin[b] and stage[b]
+      src(j0, rows, pin[b]);
+      HIPCHECK(hipMemcpyAsync(stage[b].p, pin[b], (size_t)rows * row_bytes, hipMemcpyHostToDevice, st));
+      dim3 grid((unsigned)((h->ld_gs + 255) / 256), (unsigned)rows);
+      k_compact<<<grid, 256, 0, st>>>(stage[b].as<uint8_t>(), row_bytes, (int)rows, dkeep.as<int32_t>(), n,
+                                      h->gs + j0 * h->ld_gs, h->ld_gs);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipEventRecord(done[b], st));
+    }
+    build_transpose(h, st);
+    HIPCHECK(hipStreamSynchronize(st));
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
+}
+
 extern "C" int slemm_gpu_geno_create(const uint8_t* bed_rows, int64_t n_raw, int64_t m_raw,
                                      int64_t row_bytes, const int64_t* keep_idx, int64_t n_keep,
                                      const int64_t* snp_idx, int64_t m, slemm_geno** out) {
@@ -281,54 +357,87 @@ extern "C" int slemm_gpu_geno_create(const uint8_t* bed_rows, int64_t n_raw, int
     REQUIRE(n_raw > 0 && m_raw > 0, "empty genotype matrix");
     REQUIRE(row_bytes >= (n_raw + 3) / 4, "row_bytes < ceil(n_raw/4)");
     REQUIRE(n_raw < (int64_t)1 << 31, "n_raw too large");
-    int64_t n = keep_idx ? n_keep : n_raw;
-    REQUIRE(n > 0, "individual set retained for analysis is empty");
+    std::vector<int32_t> keep32;
+    const int64_t n = check_keep(keep_idx, n_keep, n_raw, keep32);
     if (!snp_idx) m = m_raw;
     REQUIRE(m > 0, "marker set retained for analysis is empty");
-    std::vector<int32_t> keep32;
-    if (keep_idx) {
-      keep32.resize(n);
-      for (int64_t k = 0; k < n; ++k) {
-        REQUIRE(keep_idx[k] >= 0 && keep_idx[k] < n_raw, "keep_idx out of range");
-        REQUIRE(k == 0 || keep_idx[k] > keep_idx[k - 1], "keep_idx must be strictly increasing");
-        keep32[k] = (int32_t)keep_idx[k];
-      }
-    }
     if (snp_idx)
       for (int64_t j = 0; j < m; ++j)
         REQUIRE(snp_idx[j] >= 0 && snp_idx[j] < m_raw, "snp_idx out of range");
 
     slemm_geno* h = alloc_geno(n, m);
     try {
-      DevBuf dkeep, stage;
-      if (keep_idx) {
-        dkeep.reserve(sizeof(int32_t) * n);
-        HIPCHECK(hipMemcpy(dkeep.p, keep32.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-      }
-      // stream rows through a staging buffer of at most ~256 MB
-      const int64_t chunk =
-          std::max<int64_t>(1, std::min<int64_t>({m, (256ll << 20) / row_bytes, (int64_t)65535}));
-      stage.reserve((size_t)chunk * row_bytes);
-      std::vector<uint8_t> hbuf;
-      if (snp_idx) hbuf.resize((size_t)chunk * row_bytes);
-      for (int64_t j0 = 0; j0 < m; j0 += chunk) {
-        const int64_t rows = std::min(chunk, m - j0);
-        const uint8_t* src;
-        if (snp_idx) {
-          for (int64_t r = 0; r < rows; ++r)
-            std::memcpy(hbuf.data() + r * row_bytes, bed_rows + snp_idx[j0 + r] * row_bytes, row_bytes);
-          src = hbuf.data();
-        } else {
-          src = bed_rows + j0 * row_bytes;
+      // rows of the output order, from the caller's array (gathered when snp_idx reorders)
+      const RowSource src = [&](int64_t j0, int64_t rows, uint8_t* dst) {
+        if (!snp_idx) {
+          std::memcpy(dst, bed_rows + j0 * row_bytes, (size_t)rows * row_bytes);
+          return;
         }
-        HIPCHECK(hipMemcpy(stage.p, src, (size_t)rows * row_bytes, hipMemcpyHostToDevice));
-        dim3 grid((unsigned)((h->ld_gs + 255) / 256), (unsigned)rows);
-        k_compact<<<grid, 256>>>(stage.as<uint8_t>(), row_bytes, (int)rows, dkeep.as<int32_t>(), n,
-                                 h->gs + j0 * h->ld_gs, h->ld_gs);
-        HIPCHECK(hipGetLastError());
-      }
-      build_transpose(h, nullptr);
-      HIPCHECK(hipDeviceSynchronize());
+        for (int64_t r = 0; r < rows; ++r)
+          std::memcpy(dst + r * row_bytes, bed_rows + snp_idx[j0 + r] * row_bytes, row_bytes);
+      };
+      stream_rows(h, row_bytes, keep32, src);
+    } catch (...) {
+      slemm_gpu_geno_destroy(h);
+      throw;
+    }
+    *out = h;
+  });
+}
+
+// .bed file -> handle: the body is read with pread straight into pinned buffers (runs of
+// consecutive wanted SNPs in one read), overlapped with the copies and compaction of the
+// previous chunk; the file is never held in host memory as a whole.
+extern "C" int slemm_gpu_geno_load_bed(const char* bed_path, int64_t n_raw, int64_t m_raw, const int64_t* keep_idx,
+                                       int64_t n_keep, const int64_t* snp_idx, int64_t m, slemm_geno** out) {
+  return guard([&] {
+    REQUIRE(out && bed_path, "null argument");
+    *out = nullptr;
+    REQUIRE(n_raw > 0 && m_raw > 0, "empty genotype matrix");
+    REQUIRE(n_raw < (int64_t)1 << 31, "n_raw too large");
+    const int64_t row_bytes = (n_raw + 3) / 4;
+    std::vector<int32_t> keep32;
+    const int64_t n = check_keep(keep_idx, n_keep, n_raw, keep32);
+    if (!snp_idx) m = m_raw;
+    REQUIRE(m > 0, "marker set retained for analysis is empty");
+    if (snp_idx)
+      for (int64_t j = 0; j < m; ++j)
+        REQUIRE(snp_idx[j] >= 0 && snp_idx[j] < m_raw, "snp_idx out of range");
+    const int fd = open(bed_path, O_RDONLY);
+    if (fd < 0) fail(SLEMM_ERR_DATA, std::string("Cannot open ") + bed_path);
+    struct FdGuard {
+      int fd;
+      ~FdGuard() { close(fd); }
+    } fg{fd};
+    // PLINK magic 0x6c 0x1b and 0x01 for SNP-major order (src/input.cpp:352-358)
+    uint8_t magic[3] = {0, 0, 0};
+    if (pread(fd, magic, 3, 0) != 3 || magic[0] != 0x6c || magic[1] != 0x1b || magic[2] != 0x01)
+      fail(SLEMM_ERR_DATA, "Error: problem with the BED file: not a SNP-major PLINK .bed");
+    // the body must reach the last SNP read (the reference stops reading there, src/pred.cpp:129)
+    int64_t last = m_raw - 1;
+    if (snp_idx) last = *std::max_element(snp_idx, snp_idx + m);
+    struct stat sb;
+    if (fstat(fd, &sb) != 0 || (int64_t)sb.st_size < 3 + (last + 1) * row_bytes)
+      fail(SLEMM_ERR_DATA, "Error: problem with the BED file ... has the FAM/BIM file been changed?");
+    slemm_geno* h = alloc_geno(n, m);
+    try {
+      const RowSource src = [&](int64_t j0, int64_t rows, uint8_t* dst) {
+        int64_t r = 0;
+        while (r < rows) {
+          const int64_t first = snp_idx ? snp_idx[j0 + r] : j0 + r;
+          int64_t run = 1;
+          while (r + run < rows && (snp_idx ? snp_idx[j0 + r + run] : j0 + r + run) == first + run) ++run;
+          const size_t want = (size_t)(run * row_bytes);
+          size_t got = 0;
+          while (got < want) {
+            const ssize_t k = pread(fd, dst + r * row_bytes + got, want - got, (off_t)(3 + first * row_bytes + (int64_t)got));
+            if (k <= 0) fail(SLEMM_ERR_DATA, "Error: problem with the BED file ... has the FAM/BIM file been changed?");
+            got += (size_t)k;
+          }
+          r += run;
+        }
+      };
+      stream_rows(h, row_bytes, keep32, src);
     } catch (...) {
       slemm_gpu_geno_destroy(h);
       throw;

@@ -83,6 +83,19 @@ class Geno:
         return cls(h)
 
     @classmethod
+    def load_bed(cls, bed_path: str, n_raw: int, m_raw: int, keep_idx: Optional[Sequence[int]] = None,
+                 snp_idx: Optional[Sequence[int]] = None) -> "Geno":
+        """Handle streamed from a PLINK .bed file (src/input.cpp:343-386 without host copies)."""
+        h = C.c_void_p()
+        keep = None if keep_idx is None else np.ascontiguousarray(keep_idx, dtype=np.int64)
+        snps = None if snp_idx is None else np.ascontiguousarray(snp_idx, dtype=np.int64)
+        L.check(L.load().slemm_gpu_geno_load_bed(
+            str(bed_path).encode(), n_raw, m_raw,
+            _np_ptr(keep, C.c_int64), 0 if keep is None else len(keep),
+            _np_ptr(snps, C.c_int64), 0 if snps is None else len(snps), C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def synthesize(cls, n: int, m: int, seed: int = 1, maf_lo: float = 0.05, maf_hi: float = 0.5,
                    missing: float = 0.0) -> "Geno":
         lib = L.load()
@@ -531,3 +544,12 @@ class Traits:
             int(verbose), _np_ptr(vg, C.c_float), _np_ptr(ve, C.c_float), _np_ptr(h2, C.c_float),
             _np_ptr(lrt, C.c_float), _np_ptr(steps, C.c_int), _np_ptr(secs, C.c_double)))
         return dict(vg=vg, ve=ve, h2=h2, lrt=lrt, steps=steps, seconds_seed=secs[0], seconds_total=secs[1])
+
+
+def pred_plink(estimate_file: str, bfile_prefix: str, output_file: str) -> tuple:
+    """--pred (src/pred.cpp:4-151): writes IID,GEBV for the fileset; returns (n, SNPs used)."""
+    n = C.c_int64()
+    m = C.c_int64()
+    L.check(L.load().slemm_gpu_pred_plink(str(estimate_file).encode(), str(bfile_prefix).encode(),
+                                          str(output_file).encode(), C.byref(n), C.byref(m)))
+    return n.value, m.value
