@@ -42,7 +42,11 @@ extern "C" int slemm_gpu_pred_plink(const char* estimate_file, const char* bfile
       if (!line.empty() && line.back() == '\r') line.pop_back();
       const std::vector<std::string> cols = split_csv(line);
       if (cols.size() < 10 || cols[0].empty() || cols[9].empty()) continue;
-      eff[cols[0]] = std::stod(cols[9]);
+      try {
+        eff[cols[0]] = std::stod(cols[9]);
+      } catch (const std::exception&) {
+        fail(SLEMM_ERR_DATA, "bad effect value for SNP " + cols[0] + " in " + estimate_file);
+      }
       allele[cols[0]] = cols[3];
     }
     ifs.close();

@@ -62,7 +62,7 @@ def test_pred_writes_reference_gebv(gpu, tmp_path):
     alpha = rng.normal(size=p.m)
     lines = ["SNP,Chr,Pos,Allele1,Allele2,MAF,HWE_Pval,Group,Weight,Effect"]
     for j in used:
-        lines.append(f"s{j},1,{1000 + j},{'C' if flip[j] else 'A'},x,0.1,0.5,NULL,1,{alpha[j]!r}")
+        lines.append(f"s{j},1,{1000 + j},{'C' if flip[j] else 'A'},x,0.1,0.5,NULL,1,{float(alpha[j])!r}")
     lines.append("s0,1,1000,A,C,0.1,0.5,NULL,1,")  # empty effect: skipped
     lines.append("short,row")
     est = tmp_path / "est.csv"
