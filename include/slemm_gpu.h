@@ -301,6 +301,24 @@ int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int rng_seed, i
 int slemm_gpu_pred_plink(const char* estimate_file, const char* bfile_prefix, const char* output_file,
                          int64_t* n_out, int64_t* m_used);
 
+/* --iter_weighting (src/option.cpp:417-528): equal-weight fit (SLDF when lrt and n < 5000,
+ * else LMC with n_MC samples); per-SNP variances 2 maf (1-maf) b^2 normalised to mean 1;
+ * clumping -- the largest remaining variance above sig_chisq becomes a fixed effect (centred
+ * dosages) if more than indep_r2 of its sum of squares survives regression on those already
+ * taken; weights persnp * 0.001/(0.001 + 0.999 exp(-mean_window/2)) over window_size+1 SNPs,
+ * clumped SNPs 0, normalised; refit (SLDF when lrt, else LMC) with the extra covariates, whose
+ * estimates replace those SNPs' BLUPs.  When m < window_size+1 the weighting is disabled and
+ * one fit with persnp/mean(persnp) runs, as in the reference.  Host inputs: X (n x c), y,
+ * persnp (m, NULL = ones), rvec (n, NULL = ones), maf (m).  Host outputs: Py (n), snp_blup (m),
+ * blue (c: the original covariates), persnp_out (m, the final weights), sig_idx (m capacity) and
+ * n_sig (the clumped SNPs in the order taken), stats of the final fit. */
+int slemm_gpu_iter_weighting_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y,
+                                  const float* persnp, const float* rvec, const float* maf, int window_size,
+                                  float sig_chisq, float indep_r2, int lrt, int rng_seed, int n_V, int n_MC,
+                                  float tol_L, float s2max, float s2min, float tol_VC, int precision, int verbose,
+                                  float* Py, float* snp_blup, float* blue, float* persnp_out, int64_t* sig_idx,
+                                  int* n_sig, slemm_reml_stats* stats);
+
 /* ---------------------------------------------------------------- LMM quadratic forms ------ */
 
 /* Quadratic forms of the --lmm association approximation (src/reml.cpp:884-982, real-genotype

@@ -186,3 +186,65 @@ def LMC_REML(kmat, gm, gw, rvec, X, y, rng_seed=0, n_MC=None, tol_L=5e-4, s2max=
     blue = np.linalg.solve(X.T @ X, X.T @ resid)
     return dict(vg=vg, ve=ve, h2=float(s2), Py=Py, snp_blup=snp_blup, blue=blue, steps=seed.steps,
                 evals=evals[0], n_MC=n_MC)
+
+
+def iter_weighting_reml(kmat, gm, rvec, X, y, maf, persnp=None, window_size=20, sig_chisq=1e6, indep_r2=0.5,
+                        lrt=False, rng_seed=0, n_V=30, n_MC=None, tol_L=5e-4, s2max=0.7):
+    """--iter_weighting (src/option.cpp:417-528) over the restated REML drivers: equal-weight fit,
+    2 maf (1-maf) b^2 normalised, clumping by the largest variance above sig_chisq with the
+    indep_r2 residual rule, window-mean logistic weights, refit with the clumped SNPs as
+    covariates.  Returns the final fit's dict plus persnp and sig_snps."""
+    n = len(y)
+    m = len(gm)
+    Z = (np.asarray(kmat)[:, 0::2].astype(np.float32) + np.asarray(kmat)[:, 1::2]).T  # n x m dosages
+    base = np.ones(m, np.float32) if persnp is None else np.asarray(persnp, np.float32)
+
+    def fit(sldf, pw, Xf):
+        gw = oracle.gw(gm, pw)
+        if sldf:
+            return SLDF_REML(kmat, gm, gw, rvec, Xf, y, rng_seed=rng_seed, n_V=n_V, tol_L=tol_L, s2max=s2max)
+        return LMC_REML(kmat, gm, gw, rvec, Xf, y, rng_seed=rng_seed, n_MC=n_MC, tol_L=tol_L, s2max=s2max)
+
+    if m < window_size + 1:
+        pw = base / np.float32(base.mean())
+        r = fit(lrt, pw, X)
+        return dict(r, persnp=pw, sig_snps=np.zeros(0, np.int64))
+    r0 = fit(lrt and n < 5000, np.ones(m, np.float32), X)
+    b = np.asarray(r0["snp_blup"], np.float32)
+    mf = np.asarray(maf, np.float32)
+    snpvar = np.float32(2.0) * mf * (np.float32(1.0) - mf) * b * b
+    snpvar = snpvar / np.float32(snpvar.astype(np.float64).mean())
+    tmp = snpvar.copy()
+    sig, cols = [], []
+    while True:
+        j = int(np.argmax(tmp))
+        if not tmp[j] > sig_chisq:
+            break
+        tmp[j] = 0
+        z = Z[:, j].astype(np.float64)
+        yc = z - float(np.float32(z.sum() / n))
+        if not sig:
+            take = True
+        else:
+            S = np.stack(cols, axis=1)
+            res = yc - S @ np.linalg.lstsq(S, yc, rcond=None)[0]
+            take = (res @ res) / (yc @ yc) > indep_r2
+        if take:
+            sig.append(j)
+            cols.append(yc)
+    w, half = window_size, window_size // 2
+    pre = np.concatenate([[0.0], np.cumsum(snpvar.astype(np.float64))])
+    for i in range(m):
+        a = 0 if i < half else (m - w - 1 if i > m - half - 1 else i - half)
+        v = np.float32((pre[a + w + 1] - pre[a]) / (w + 1))
+        tmp[i] = np.float32(0.001) / (np.float32(0.001) + np.float32(0.999) * np.exp(v * np.float32(-0.5)))
+    tmp[sig] = 0
+    pw = base * tmp
+    pw = pw / np.float32(pw.astype(np.float64).mean())
+    Xn = np.concatenate([np.asarray(X, np.float64).reshape(n, -1)] + [c[:, None] for c in cols], axis=1)
+    r = fit(lrt, pw, Xn)
+    blup = np.array(r["snp_blup"], np.float32)
+    c = np.asarray(X).reshape(n, -1).shape[1]
+    for a, j in enumerate(sig):
+        blup[j] = r["blue"][c + a]
+    return dict(r, snp_blup=blup, blue=np.asarray(r["blue"])[:c], persnp=pw, sig_snps=np.array(sig, np.int64))

@@ -75,6 +75,11 @@ SIGNATURES = {
     "slemm_gpu_set_gw": (C.c_int, [c_vp, c_fp]),
     "slemm_gpu_geno_load_bed": (C.c_int, [C.c_char_p, c_i64, c_i64, c_i64p, c_i64, c_i64p, c_i64, C.POINTER(c_vp)]),
     "slemm_gpu_pred_plink": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, c_i64p, c_i64p]),
+    "slemm_gpu_iter_weighting_reml": (C.c_int, [c_vp, c_fp, c_i64, C.c_int, c_fp, c_fp, c_fp, c_fp, C.c_int,
+                                                C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
+                                                c_fp, c_fp, c_fp, c_fp, c_i64p, C.POINTER(C.c_int),
+                                                C.POINTER(RemlStats)]),
     "slemm_gpu_lmm_gstat": (C.c_int, [c_vp, c_fp, c_i64, C.c_int, c_fp, C.c_float, C.POINTER(C.c_int64), C.c_int,
                                       C.c_int, C.c_float, C.c_int, C.c_int, c_fp, c_fp, c_fp, c_fp]),
     "slemm_gpu_traits_create": (C.c_int, [c_vp, C.c_int, c_u8p, C.POINTER(c_vp)]),

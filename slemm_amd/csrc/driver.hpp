@@ -79,6 +79,9 @@ inline double seconds_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Dosages (0, 1, 2 as float) of the SNPs snps, n x k column-major on the host (gstat.hip).
+void extract_dosage_cols(slemm_geno* h, const std::vector<int64_t>& snps, std::vector<float>& out);
+
 // Rademacher probes of unit norm from mt19937(seed+54321), written to columns of B (ld n),
 // draw order and normalisation as src/reml.cpp:597-605.
 void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B);

@@ -183,3 +183,23 @@ extern "C" int slemm_gpu_lmm_gstat(slemm_geno* h, const float* X, int64_t ldx, i
     }
   });
 }
+
+namespace slemm {
+
+// Dosages (0, 1, 2 as float) of the SNPs snps, n x k column-major on the host.
+void extract_dosage_cols(slemm_geno* h, const std::vector<int64_t>& snps, std::vector<float>& out) {
+  const int64_t n = h->n, k = (int64_t)snps.size();
+  out.assign((size_t)n * k, 0.f);
+  if (k == 0) return;
+  DevT<int64_t> ds(k);
+  HIPCHECK(hipMemcpy(ds.p, snps.data(), sizeof(int64_t) * k, hipMemcpyHostToDevice));
+  DevT<float> zero((size_t)h->m);
+  HIPCHECK(hipMemset(zero.p, 0, sizeof(float) * h->m));
+  DevF dout((size_t)n * k);
+  k_extract_centered<<<dim3((unsigned)((n + 255) / 256), (unsigned)k), 256>>>(h->gs, h->ld_gs, ds.p, n, zero.p,
+                                                                              dout.p, n);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(out.data(), dout.p, sizeof(float) * n * k, hipMemcpyDeviceToHost));
+}
+
+}  // namespace slemm

@@ -327,6 +327,36 @@ class Geno:
                     evals=st.criterion_evals, n_MC=n_MC, seconds_seed=st.seconds_seed,
                     seconds_total=st.seconds_total)
 
+    def iter_weighting_REML(self, X: np.ndarray, y: np.ndarray, maf: np.ndarray, persnp: Optional[np.ndarray] = None,
+                            rvec: Optional[np.ndarray] = None, window_size: int = 20, sig_chisq: float = 1e6,
+                            indep_r2: float = 0.5, lrt: bool = False, rng_seed: int = 0, n_V: int = 30,
+                            n_MC: Optional[int] = None, tol_L: float = 5e-4, s2max: float = 0.7, s2min: float = 1e-4,
+                            tol_VC: float = 1e-4, precision="bf16x2", verbose: bool = False) -> dict:
+        """--iter_weighting (src/option.cpp:417-528); defaults as src/option.cpp:52-76, 303, 414."""
+        if n_MC is None:
+            n_MC = max(min(int(4e9 / self.n / self.n), 30), 3)
+        Xf = np.asfortranarray(np.asarray(X, np.float32).reshape(self.n, -1))
+        c = Xf.shape[1]
+        yv = np.ascontiguousarray(y, np.float32)
+        mf = np.ascontiguousarray(maf, np.float32)
+        ps = None if persnp is None else np.ascontiguousarray(persnp, np.float32)
+        rv = None if rvec is None else np.ascontiguousarray(rvec, np.float32)
+        Py = np.zeros(self.n, np.float32)
+        blup = np.zeros(self.m, np.float32)
+        blue = np.zeros(c, np.float32)
+        w = np.zeros(self.m, np.float32)
+        sig = np.zeros(self.m, np.int64)
+        nsig = C.c_int()
+        st = L.RemlStats()
+        L.check(L.load().slemm_gpu_iter_weighting_reml(
+            self._h, _np_ptr(Xf, C.c_float), self.n, c, _np_ptr(yv, C.c_float), _np_ptr(ps, C.c_float),
+            _np_ptr(rv, C.c_float), _np_ptr(mf, C.c_float), window_size, sig_chisq, indep_r2, int(lrt), rng_seed,
+            n_V, n_MC, tol_L, s2max, s2min, tol_VC, _prec(precision), int(verbose), _np_ptr(Py, C.c_float),
+            _np_ptr(blup, C.c_float), _np_ptr(blue, C.c_float), _np_ptr(w, C.c_float), _np_ptr(sig, C.c_int64),
+            C.byref(nsig), C.byref(st)))
+        return dict(vg=st.vg, ve=st.ve, h2=st.h2, Py=Py, snp_blup=blup, blue=blue, persnp=w,
+                    sig_snps=sig[:nsig.value].copy())
+
     def lmm_gstat(self, X: np.ndarray, Py: np.ndarray, h2: float, snps: np.ndarray, window_size: int = 1000,
                   tol_L: float = 5e-4, precision="bf16x2", verbose: bool = False) -> dict:
         """Quadratic forms of --lmm (src/reml.cpp:884-982): xPy, xPx, xBx, xSx for the SNPs

@@ -164,3 +164,32 @@ def test_grammar_gamma_pipeline(gpu):
     assert rel_err(res["BETA"][ok], beta[ok]) < 3e-4
     assert rel_err(res["SE"][ok], se[ok]) < 3e-4
     assert np.max(np.abs(res["P"][ok] - pv[ok])) < 1e-3
+
+
+def test_iter_weighting_against_restatement(gpu):
+    """--iter_weighting (src/option.cpp:417-528) vs the restated driver over the CPU REML: same
+    clumped SNPs (a duplicated large-effect SNP enters once), weights and final estimates."""
+    import oracle
+    from oracle import bedio
+    from oracle.dense import dosage_of_kmat
+    p = make_problem(n=1500, m=1200, seed=31)
+    codes = p.codes.copy()
+    codes[41] = codes[40]
+    rows = bedio.encode_codes(codes)
+    kmat = bedio.kmat_of_codes(codes)
+    _, _, _, _, gm = oracle.counts(kmat)
+    Z = dosage_of_kmat(kmat)
+    rng = np.random.default_rng(8)
+    y = (Z[:, 40] * 3.0 + Z[:, 700] * 2.5 + Z @ rng.normal(scale=0.05, size=p.m) + rng.normal(size=p.n))
+    y = y.astype(np.float32)
+    maf = oracle.maf(kmat)
+    X = np.ones((p.n, 1), np.float32)
+    ref = oreml.iter_weighting_reml(kmat, gm, None, X, y, maf, window_size=20, sig_chisq=8.0, n_MC=8)
+    g = gpu.Geno.from_bed(rows, p.n)
+    got = g.iter_weighting_REML(X, y, maf, window_size=20, sig_chisq=8.0, n_MC=8)
+    assert list(got["sig_snps"]) == list(ref["sig_snps"]) and 700 in got["sig_snps"]
+    assert (40 in got["sig_snps"]) != (41 in got["sig_snps"])
+    assert rel_err(got["persnp"], ref["persnp"]) < 2e-2
+    assert abs(got["h2"] - ref["h2"]) < 2e-3
+    sig = got["sig_snps"]
+    assert rel_err(got["snp_blup"][sig], ref["snp_blup"][sig]) < 1e-2

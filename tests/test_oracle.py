@@ -302,3 +302,33 @@ def test_gstat_window_form_is_woodbury_and_whole_genome_window_is_xPx():
     assert np.allclose(xBx_all, xPx_all, rtol=1e-9)
     assert np.allclose(xPx_all, xPx, rtol=1e-12)
     assert np.all(xBx > xPx)  # a window explains less of v than the whole genome
+
+
+# ----------------------------------------------------------------------- iterative weighting ----
+def _iterw_problem():
+    p = make_problem(n=500, m=240, seed=31)
+    codes = p.codes.copy()
+    codes[41] = codes[40]  # SNP 41 duplicates SNP 40
+    rows = bedio.encode_codes(codes)
+    kmat = bedio.kmat_of_codes(codes)
+    pc, _, _, _, gm = oracle.counts(kmat)
+    Z = dosage_of_kmat(kmat)
+    rng = np.random.default_rng(8)
+    y = (Z[:, 40] * 3.0 + Z[:, 150] * 2.5 + Z @ rng.normal(scale=0.05, size=p.m) + rng.normal(size=p.n))
+    return p, codes, rows, kmat, gm, y.astype(np.float32)
+
+
+def test_iter_weighting_restatement_rules():
+    p, codes, rows, kmat, gm, y = _iterw_problem()
+    maf = oracle.maf(kmat)
+    X = np.ones((p.n, 1))
+    r = oreml.iter_weighting_reml(kmat, gm, None, X, y, maf, window_size=20, sig_chisq=8.0, n_MC=8)
+    sig = list(r["sig_snps"])
+    assert 150 in sig and (40 in sig) != (41 in sig)  # one copy of the duplicated SNP, not both
+    assert np.all(r["persnp"][sig] == 0) and abs(float(r["persnp"].mean()) - 1) < 1e-6
+    assert np.all(r["persnp"] >= 0)
+    # disabled when the window is wider than the genome: one fit with persnp / mean(persnp)
+    ps = np.random.default_rng(2).uniform(0.5, 2.0, p.m).astype(np.float32)
+    r2 = oreml.iter_weighting_reml(kmat, gm, None, X, y, maf, persnp=ps, window_size=p.m, n_MC=8)
+    r3 = oreml.LMC_REML(kmat, gm, oracle.gw(gm, ps / np.float32(ps.mean())), None, X, y, n_MC=8)
+    assert r2["h2"] == r3["h2"] and len(r2["sig_snps"]) == 0
