@@ -334,6 +334,19 @@ int slemm_gpu_lmm_gstat(slemm_geno* h, const float* X, int64_t ldx, int c, const
                         const int64_t* snps, int k, int window_size, float tol_L, int precision, int verbose,
                         float* xPy, float* xPx, float* xBx, float* xSx);
 
+/* Windowed score tests of slemm_gwa step 2 (bin/slemm_gwa.py:322-431): model SNPs model_snps
+ * (host, K, increasing) form blocks of block_size (reduced to an even number <= K) overlapping by
+ * half; block b tests the SNPs from the previous block's bound to its own (model SNP
+ * block_size - block_size/4 of the block), the first from test_begin and the last to test_end
+ * (inclusive, handle SNP indices).  Per block H = Zb'Zb + model_snp_ct tau diag((2-p)p/2) in
+ * FP64 (rocBLAS DSYRK, rocSOLVER Cholesky); per tested SNP x (centred by its mean dosage)
+ * xy = x'y and, with x scaled by R^-1/2 and projected off Q(X), xx = (x'x - z'H^-1 z)/tau,
+ * z = Zb'x.  SNPs whose allele count is <= mac either way are NaN.  Host inputs X (n x c), y (n:
+ * the phenotype column of the fit's output); host outputs xy, xx (test_end - test_begin + 1). */
+int slemm_gpu_gwa_window(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y,
+                         const int64_t* model_snps, int K, int block_size, int64_t test_begin, int64_t test_end,
+                         float tau, int64_t model_snp_ct, float mac, float* xy, float* xx);
+
 /* ---------------------------------------------------------------- tuning ------------------- */
 
 /* Autotuner of the split-BF16 block products (no reference counterpart; the reference's only

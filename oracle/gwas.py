@@ -73,3 +73,61 @@ def gamma_stats(xy, xx, cfactor: float, snp_var: float):
     serr = np.sqrt((snp_var * cfactor) / xx)
     chisq = np.square(beta / serr)
     return beta, serr, chisq, chi2.sf(chisq, 1)
+
+
+def gwa_blocks(model_snps, block_size: int, test_begin: int, test_end: int):
+    """Blocks of model SNPs and the tested range [lo, hi) of each (bin/slemm_gwa.py:322-345)."""
+    ksnp = np.asarray(model_snps, np.int64)
+    K = len(ksnp)
+    b = block_size if block_size <= K else int(K / 2) * 2
+    step = int(b / 2)
+    ext = int(step / 2)
+    nb = int((K - step) / step)
+    blocks = [ksnp[i * step:i * step + b] for i in range(nb)]
+    if (K - step) % step != 0:
+        blocks.append(ksnp[K - b:K])
+    bounds = np.empty((len(blocks), 2), np.int64)
+    bounds[0, 0] = test_begin
+    bounds[0, 1] = blocks[0][b - ext]
+    for i in range(1, len(blocks)):
+        bounds[i, 0] = bounds[i - 1, 1]
+        bounds[i, 1] = blocks[i][b - ext]
+    bounds[-1, 1] = test_end + 1
+    return blocks, bounds
+
+
+def gwa_window_scores(Z, y, rr, Q, model_snps, block_size, test_begin, test_end, tau, model_snp_ct, mac=0.0):
+    """Score terms of the windowed scan (bin/slemm_gwa.py:347-407) in FP64: per block
+    H = Zb'Zb + model_snp_ct tau diag((2 - p) p / 2); per tested SNP xy = x'y and
+    xx = (x'x - z'H^-1 z) / tau, x centred (float32 mean), scaled by rr, projected off Q."""
+    Z = np.asarray(Z, np.float64)
+    n = Z.shape[0]
+    rr = np.asarray(rr, np.float64)
+    Q = np.asarray(Q, np.float64).reshape(n, -1)
+    y = np.asarray(y, np.float64)
+    blocks, bounds = gwa_blocks(model_snps, block_size, test_begin, test_end)
+    t = test_end - test_begin + 1
+    xy = np.full(t, np.nan)
+    xx = np.full(t, np.nan)
+
+    def prep(cols):
+        mu = np.float32(cols.sum(axis=0)) / np.float32(n)  # float32 mean of exact integer sums
+        A = (cols - mu.astype(np.float64)[None, :]) * rr[:, None]
+        return A - Q @ (Q.T @ A), mu
+
+    for blk, (lo, hi) in zip(blocks, bounds):
+        Zb, mu = prep(Z[:, blk])
+        rv = ((np.float32(2.0) - mu) * mu / np.float32(2.0)).astype(np.float64)
+        H = Zb.T @ Zb + np.diag(model_snp_ct * tau * rv)
+        for j in range(max(lo, test_begin), min(hi, test_end + 1)):
+            x = Z[:, j]
+            s = x.sum()
+            if s <= mac or 2 * n - s <= mac:
+                continue
+            xc = x - float(np.float32(s) / np.float32(n))
+            xy[j - test_begin] = xc @ y
+            v = xc * rr
+            v -= Q @ (Q.T @ v)
+            z = Zb.T @ v
+            xx[j - test_begin] = (v @ v - z @ np.linalg.solve(H, z)) / tau
+    return xy, xx

@@ -203,3 +203,188 @@ void extract_dosage_cols(slemm_geno* h, const std::vector<int64_t>& snps, std::v
 }
 
 }  // namespace slemm
+
+// ------------------------------------------------------------------------------------------
+// Windowed score tests of slemm_gwa step 2 (bin/slemm_gwa.py:322-431): blocks of block_size
+// model SNPs overlapping by half, each with H = Zb'Zb + model_snp_ct tau diag((2 - p) p / 2) (p the
+// SNP's mean dosage), and every tested SNP of the block's range scored as
+// xx = (x'x - z'H^-1 z) / tau with z = Zb'x; x centred, scaled by R^-1/2 and projected off Q.
+
+namespace slemm {
+namespace {
+
+// out[i + c n] = dosage(snps[c], i) - gm[snps[c]] in FP64 (the FP32 mean, as the script's
+// float32 vec_x -= sum / n)
+__global__ void k_extract_f64(const uint8_t* __restrict__ gs, int64_t ld_gs, const int64_t* __restrict__ snps,
+                              int64_t n, const float* __restrict__ gm, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n) return;
+  const int64_t j = snps[c];
+  const uint32_t code = (gs[j * ld_gs + (i >> 2)] >> (2 * (i & 3))) & 3u;
+  out[i + (int64_t)c * n] = (double)((float)((code & 1u) + (code >> 1)) - gm[j]);
+}
+
+__global__ void k_scale_rows(double* __restrict__ a, int64_t n, int t, const float* __restrict__ rr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i + (int64_t)blockIdx.y * n] *= (double)rr[i];
+}
+
+// xx_c = (sum_i x_ic^2 - sum_k z_kc s_kc) / tau for the t columns; one block per column
+__global__ void k_window_xx(const double* __restrict__ X, int64_t n, const double* __restrict__ Z,
+                            const double* __restrict__ S, int b, double tau, double* __restrict__ xx) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) {
+    const double v = X[i + (int64_t)c * n];
+    s += v * v;
+  }
+  for (int k = threadIdx.x; k < b; k += 256) s -= Z[k + (int64_t)c * b] * S[k + (int64_t)c * b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) xx[c] = red[0] / tau;
+}
+
+// H_kk += model_snp_ct tau rvar_k
+__global__ void k_add_diag(double* __restrict__ H, int b, const double* __restrict__ d) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < b) H[k + (int64_t)k * b] += d[k];
+}
+
+}  // namespace
+}  // namespace slemm
+
+extern "C" int slemm_gpu_gwa_window(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y,
+                                    const int64_t* model_snps, int K, int block_size, int64_t test_begin,
+                                    int64_t test_end, float tau, int64_t model_snp_ct, float mac, float* xy,
+                                    float* xx) {
+  return guard([&] {
+    REQUIRE(h && X && y && model_snps && xy && xx, "null argument");
+    REQUIRE(c >= 1 && ldx >= h->n, "covariates: need c >= 1 and ldx >= n");
+    REQUIRE(K >= 2, "need at least two model SNPs");
+    REQUIRE(tau > 0, "tau must be positive");
+    HIPCHECK(hipSetDevice(h->device));
+    const int64_t n = h->n, m = h->m;
+    REQUIRE(test_begin >= 0 && test_end >= test_begin && test_end < m, "tested SNP range out of bounds");
+    for (int i = 0; i < K; ++i) {
+      REQUIRE(model_snps[i] >= 0 && model_snps[i] < m, "model SNP index out of range");
+      REQUIRE(i == 0 || model_snps[i] > model_snps[i - 1], "model SNPs must be strictly increasing");
+    }
+    ensure_counts(h, nullptr);
+    std::vector<int32_t> pc(m);
+    std::vector<float> gmh(m);
+    HIPCHECK(hipMemcpy(pc.data(), h->popcnt, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(gmh.data(), h->gm, sizeof(float) * m, hipMemcpyDeviceToHost));
+    // the model's R^-1/2 and Q of the covariates
+    if (h->rr_host.empty()) abi(slemm_gpu_set_model(h, nullptr, nullptr, nullptr, n, 0));
+    MatD Q;
+    covariate_basis(h, X, ldx, c, Q);
+
+    // blocks of model SNPs and the tested range of each (bin/slemm_gwa.py:322-345)
+    int b = block_size;
+    if (b > K) b = K / 2 * 2;
+    REQUIRE(b >= 2, "block size must be at least 2");
+    const int step = b / 2, ext = step / 2;
+    std::vector<std::vector<int64_t>> blocks;
+    const int nb = (K - step) / step;
+    for (int i = 0; i < nb; ++i) blocks.emplace_back(model_snps + (size_t)i * step, model_snps + (size_t)i * step + b);
+    if ((K - step) % step != 0) blocks.emplace_back(model_snps + (K - b), model_snps + K);
+    const int nblk = (int)blocks.size();
+    std::vector<int64_t> lo(nblk), hi(nblk);
+    lo[0] = test_begin;
+    hi[0] = blocks[0][b - ext];
+    for (int i = 1; i < nblk; ++i) {
+      lo[i] = hi[i - 1];
+      hi[i] = blocks[i][b - ext];
+    }
+    hi[nblk - 1] = test_end + 1;
+    for (int i = 0; i < nblk; ++i) {  // outputs cover [test_begin, test_end] only
+      lo[i] = std::max(lo[i], test_begin);
+      hi[i] = std::min(hi[i], test_end + 1);
+    }
+
+    Blas blas;
+    const double one = 1.0, zero = 0.0, minus_one = -1.0;
+    const int cq = Q.c;
+    DevT<double> qd((size_t)n * cq), Zb((size_t)n * b), H((size_t)b * b), qz((size_t)cq * std::max<int64_t>(b, 4096)),
+        dy((size_t)n), diag((size_t)b);
+    {
+      std::vector<double> qh((size_t)n * cq), yh(n);
+      for (size_t e = 0; e < qh.size(); ++e) qh[e] = (double)(float)Q.a[e];
+      for (int64_t i = 0; i < n; ++i) yh[i] = (double)y[i];
+      HIPCHECK(hipMemcpy(qd.p, qh.data(), sizeof(double) * qh.size(), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(dy.p, yh.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    const int64_t TCH = 4096;  // tested SNPs per batch
+    DevT<int64_t> didx((size_t)std::max<int64_t>(b, TCH));
+    DevT<double> Xt((size_t)n * TCH), Zx((size_t)b * TCH), Sx((size_t)b * TCH), dxy(TCH), dxx(TCH);
+    DevT<rocblas_int> info(1);
+    // centre, scale by R^-1/2 and project out Q (in place on an n x t FP64 block)
+    auto prepare = [&](double* A, int t) {
+      k_scale_rows<<<dim3((unsigned)((n + 255) / 256), (unsigned)t), 256>>>(A, n, t, h->rr);
+      HIPCHECK(hipGetLastError());
+      rb(rocblas_dgemm(blas.h, rocblas_operation_transpose, rocblas_operation_none, cq, t, (rocblas_int)n, &one, qd.p,
+                       (rocblas_int)n, A, (rocblas_int)n, &zero, qz.p, cq), "dgemm Q'A");
+      rb(rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, t, cq, &minus_one,
+                       qd.p, (rocblas_int)n, qz.p, cq, &one, A, (rocblas_int)n), "dgemm A - Q(Q'A)");
+    };
+    for (int blk = 0; blk < nblk; ++blk) {
+      // Zb and H = Zb'Zb + model_snp_ct tau rvars (:363-383, :387)
+      HIPCHECK(hipMemcpy(didx.p, blocks[blk].data(), sizeof(int64_t) * b, hipMemcpyHostToDevice));
+      k_extract_f64<<<dim3((unsigned)((n + 255) / 256), (unsigned)b), 256>>>(h->gs, h->ld_gs, didx.p, n, h->gm, Zb.p);
+      HIPCHECK(hipGetLastError());
+      prepare(Zb.p, b);
+      rb(rocblas_dsyrk(blas.h, rocblas_fill_lower, rocblas_operation_transpose, b, (rocblas_int)n, &one, Zb.p,
+                       (rocblas_int)n, &zero, H.p, b), "dsyrk");
+      std::vector<double> dh(b);
+      for (int k = 0; k < b; ++k) {
+        const float mu = gmh[blocks[blk][k]];
+        const float rv = (2.0f - mu) * mu / 2.0f;
+        dh[k] = (double)model_snp_ct * (double)tau * (double)rv;
+      }
+      HIPCHECK(hipMemcpy(diag.p, dh.data(), sizeof(double) * b, hipMemcpyHostToDevice));
+      k_add_diag<<<(b + 255) / 256, 256>>>(H.p, b, diag.p);
+      rb(rocsolver_dpotrf(blas.h, rocblas_fill_lower, b, H.p, b, info.p), "dpotrf");
+      rocblas_int inf = 0;
+      HIPCHECK(hipMemcpy(&inf, info.p, sizeof(rocblas_int), hipMemcpyDeviceToHost));
+      if (inf != 0) fail(SLEMM_ERR_NUMERIC, "block system of the windowed scan is not positive definite");
+      // tested SNPs of the block's range, in batches (:388-407)
+      for (int64_t j0 = lo[blk]; j0 < hi[blk]; j0 += TCH) {
+        const int t = (int)std::min<int64_t>(TCH, hi[blk] - j0);
+        std::vector<int64_t> idx(t);
+        for (int q = 0; q < t; ++q) idx[q] = j0 + q;
+        HIPCHECK(hipMemcpy(didx.p, idx.data(), sizeof(int64_t) * t, hipMemcpyHostToDevice));
+        k_extract_f64<<<dim3((unsigned)((n + 255) / 256), (unsigned)t), 256>>>(h->gs, h->ld_gs, didx.p, n, h->gm,
+                                                                               Xt.p);
+        HIPCHECK(hipGetLastError());
+        rb(rocblas_dgemv(blas.h, rocblas_operation_transpose, (rocblas_int)n, t, &one, Xt.p, (rocblas_int)n, dy.p, 1,
+                         &zero, dxy.p, 1), "dgemv X'y");
+        prepare(Xt.p, t);
+        rb(rocblas_dgemm(blas.h, rocblas_operation_transpose, rocblas_operation_none, b, t, (rocblas_int)n, &one,
+                         Zb.p, (rocblas_int)n, Xt.p, (rocblas_int)n, &zero, Zx.p, b), "dgemm Zb'X");
+        HIPCHECK(hipMemcpy(Sx.p, Zx.p, sizeof(double) * b * t, hipMemcpyDeviceToDevice));
+        rb(rocsolver_dpotrs(blas.h, rocblas_fill_lower, b, t, H.p, b, Sx.p, b), "dpotrs");
+        k_window_xx<<<t, 256>>>(Xt.p, n, Zx.p, Sx.p, b, (double)tau, dxx.p);
+        HIPCHECK(hipGetLastError());
+        std::vector<double> hxy(t), hxx(t);
+        HIPCHECK(hipMemcpy(hxy.data(), dxy.p, sizeof(double) * t, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(hxx.data(), dxx.p, sizeof(double) * t, hipMemcpyDeviceToHost));
+        for (int q = 0; q < t; ++q) {
+          const int64_t j = j0 + q, o = j - test_begin;
+          const float s = (float)pc[j];
+          if (s <= mac || 2.0f * (float)n - s <= mac) {  // MAC filter (:396-400)
+            xy[o] = xx[o] = __builtin_nanf("");
+          } else {
+            xy[o] = (float)hxy[q];
+            xx[o] = (float)hxx[q];
+          }
+        }
+      }
+    }
+  });
+}

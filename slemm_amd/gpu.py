@@ -357,6 +357,23 @@ class Geno:
         return dict(vg=st.vg, ve=st.ve, h2=st.h2, Py=Py, snp_blup=blup, blue=blue, persnp=w,
                     sig_snps=sig[:nsig.value].copy())
 
+    def gwa_window(self, X: np.ndarray, y: np.ndarray, model_snps: np.ndarray, block_size: int, test_begin: int,
+                   test_end: int, tau: float, model_snp_ct: int, mac: float = 0.0):
+        """Windowed score terms of slemm_gwa step 2 (bin/slemm_gwa.py:322-407): xy, xx for the
+        SNPs test_begin..test_end (inclusive)."""
+        Xf = np.asfortranarray(np.asarray(X, np.float32).reshape(self.n, -1))
+        c = Xf.shape[1]
+        yv = np.ascontiguousarray(y, np.float32)
+        ms = np.ascontiguousarray(model_snps, np.int64)
+        t = test_end - test_begin + 1
+        xy = np.zeros(t, np.float32)
+        xx = np.zeros(t, np.float32)
+        L.check(L.load().slemm_gpu_gwa_window(
+            self._h, _np_ptr(Xf, C.c_float), self.n, c, _np_ptr(yv, C.c_float), _np_ptr(ms, C.c_int64), ms.size,
+            block_size, test_begin, test_end, tau, model_snp_ct, mac, _np_ptr(xy, C.c_float),
+            _np_ptr(xx, C.c_float)))
+        return xy, xx
+
     def lmm_gstat(self, X: np.ndarray, Py: np.ndarray, h2: float, snps: np.ndarray, window_size: int = 1000,
                   tol_L: float = 5e-4, precision="bf16x2", verbose: bool = False) -> dict:
         """Quadratic forms of --lmm (src/reml.cpp:884-982): xPy, xPx, xBx, xSx for the SNPs

@@ -47,6 +47,26 @@ def correction_factor(gst: dict) -> float:
     return mean_wo_outliers(np.asarray(gst["xSx"], np.float64) / np.asarray(gst["xPx"], np.float64))
 
 
+def gwa_correction_factor(gst: dict) -> float:
+    """Correction factor of the windowed scan: the plain mean of xBx / xPx over the sampled SNPs
+    (bin/slemm_gwa.py:141-157)."""
+    r = np.asarray(gst["xBx"], np.float64) / np.asarray(gst["xPx"], np.float64)
+    return float(r.sum() / r.size)
+
+
+def gwa_window_stats(xy, xx, cfactor: float, snp_var: float) -> dict:
+    """BETA, SE, CHISQ, P of the windowed scan from its score terms (bin/slemm_gwa.py:408-412)."""
+    xy = np.asarray(xy, np.float64)
+    xx = np.asarray(xx, np.float64)
+    beta = xy / xx * cfactor
+    se = np.sqrt((snp_var * cfactor) / xx)
+    chisq = np.square(beta / se)
+    pval = np.full_like(chisq, np.nan)
+    ok = np.isfinite(chisq)
+    pval[ok] = chisq_pval_1df(chisq[ok])
+    return dict(BETA=beta, SE=se, CHISQ=chisq, P=pval)
+
+
 def grammar_gamma(geno: "gpu.Geno", Py: np.ndarray, gst: dict, snp_var: float, mac: float = 0.0,
                   cfactor: Optional[float] = None) -> dict:
     """Single-SNP GRAMMAR-gamma tests of every SNP of the handle (bin/slemm_gamma.py:306-325):

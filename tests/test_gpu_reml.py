@@ -193,3 +193,26 @@ def test_iter_weighting_against_restatement(gpu):
     assert abs(got["h2"] - ref["h2"]) < 2e-3
     sig = got["sig_snps"]
     assert rel_err(got["snp_blup"][sig], ref["snp_blup"][sig]) < 1e-2
+
+
+def test_gwa_window_scan_against_fp64(gpu):
+    """Windowed score tests of slemm_gwa step 2 (bin/slemm_gwa.py:322-407) vs the FP64
+    restatement: overlapping blocks of model SNPs, every SNP of the range tested, MAC filter."""
+    from oracle import gwas as ogwas
+    from slemm_amd import gwas
+    p = make_problem(n=900, m=1300, seed=15, weights=True, c=2, missing=0.0)
+    g = gpu.Geno.from_bed(p.rows, p.n)
+    g.set_model(None, p.rvec)
+    rng = np.random.default_rng(3)
+    y = rng.normal(size=p.n).astype(np.float32)
+    ms = np.sort(rng.choice(p.m, size=500, replace=False))
+    tau, ct, mac = 1.8, 500, 15.0
+    for bs, (t0, t1) in ((100, (0, p.m - 1)), (64, (200, 900))):
+        xy, xx = g.gwa_window(p.X, y, ms, bs, t0, t1, tau, ct, mac)
+        rxy, rxx = ogwas.gwa_window_scores(p.Z, y, p.rr, p.Q, ms, bs, t0, t1, tau, ct, mac)
+        assert np.array_equal(np.isnan(xx), np.isnan(rxx)) and np.isfinite(rxx).sum() > 0.8 * (t1 - t0 + 1)
+        ok = np.isfinite(rxx)
+        assert rel_err(xy[ok], rxy[ok]) < 1e-5
+        assert rel_err(xx[ok], rxx[ok]) < 1e-6
+    st = gwas.gwa_window_stats(xy, xx, 1.1, 0.5)
+    assert np.all(np.isfinite(st["P"][ok])) and np.all((st["P"][ok] >= 0) & (st["P"][ok] <= 1))

@@ -332,3 +332,32 @@ def test_iter_weighting_restatement_rules():
     r2 = oreml.iter_weighting_reml(kmat, gm, None, X, y, maf, persnp=ps, window_size=p.m, n_MC=8)
     r3 = oreml.LMC_REML(kmat, gm, oracle.gw(gm, ps / np.float32(ps.mean())), None, X, y, n_MC=8)
     assert r2["h2"] == r3["h2"] and len(r2["sig_snps"]) == 0
+
+
+# ------------------------------------------------------------------------ windowed GWA scan ----
+def test_gwa_blocks_known_layout():
+    """Blocks of 8 over 20 model SNPs at odd positions: step 4, bound at block position 6."""
+    ms = np.arange(1, 40, 2)
+    blocks, bounds = gwas.gwa_blocks(ms, 8, 0, 40)
+    assert [list(b) for b in blocks] == [list(ms[0:8]), list(ms[4:12]), list(ms[8:16]), list(ms[12:20])]
+    assert bounds.tolist() == [[0, 13], [13, 21], [21, 29], [29, 41]]
+    b2, _ = gwas.gwa_blocks(ms[:18], 8, 0, 40)  # (18 - 4) % 4 != 0: a last block ending at the end
+    assert list(b2[-1]) == list(ms[10:18])
+
+
+def test_gwa_window_score_is_woodbury_form():
+    """x'x - z'(Zb'Zb + D)^-1 z = x'(I + Zb D^-1 Zb')^-1 x for one block holding every model SNP."""
+    p = make_problem(n=220, m=60, seed=41, weights=True, c=2)
+    ms = np.arange(0, 60, 2)
+    tau, ct = 1.3, 30
+    xy, xx = gwas.gwa_window_scores(p.Z, np.ones(p.n), p.rr, p.Q, ms, len(ms), 0, 59, tau, ct)
+    mu = (p.Z[:, ms].sum(axis=0) / p.n).astype(np.float32)
+    Zb = (p.Z[:, ms] - mu) * p.rr[:, None]
+    Zb -= p.Q @ (p.Q.T @ Zb)
+    D = ct * tau * ((2 - mu) * mu / 2).astype(np.float64)
+    M = np.eye(p.n) + Zb @ np.diag(1 / D) @ Zb.T
+    for j in (1, 7, 30, 59):
+        x = p.Z[:, j] - np.float32(p.Z[:, j].sum()) / np.float32(p.n)
+        v = x * p.rr
+        v -= p.Q @ (p.Q.T @ v)
+        assert abs(xx[j] - v @ np.linalg.solve(M, v) / tau) < 1e-9 * abs(xx[j])
