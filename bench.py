@@ -38,6 +38,15 @@ MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 15.73e9 + 0.320e9, "pass2"
                     (100_000, 500_000, 1): {"pass1": 12.64e9 + 0.064e9, "pass2": 12.71e9 + 0.039e9}}
 
 
+def issue_factor(K: int) -> float:
+    """MFMA columns issued per algorithmic column of a K-vector split-BF16 block: a hi and a lo
+    tile per 16 vectors, except that a last tile of at most 8 vectors holds its hi and lo halves
+    as columns (one tile), as do blocks of at most 8 vectors."""
+    q, r = divmod(K, 16)
+    tiles = 2 * q + (0 if r == 0 else (1 if r <= 8 else 2))
+    return 16 * tiles / K
+
+
 def pass_bytes(n, m, K):
     return m * ((n + 3) // 4) + n * K * 4 + m * K * 4
 
@@ -535,8 +544,8 @@ def main():
             "traffic": (MEASURED_TRAFFIC.get((n, m, K), {}).get("pass2" if k2 >= k1 else "pass1")),
             "algorithmic_per_launch": {"flop": flops, "bytes": pb},
             "hbm_GB/s_of_kernel": round(pb / (dom_ms * 1e-3) / 1e9, 1),
-            "mfma_issue_factor": round(2 * (16 * -(-K // 16)) / K, 3),
-            "issued_TFLOP/s": round(ach_tf * 2 * (16 * -(-K // 16)) / K, 1),
+            "mfma_issue_factor": round(issue_factor(K), 3),
+            "issued_TFLOP/s": round(ach_tf * issue_factor(K), 1),
             "practical_mfma_ceiling": ceiling,
         },
     }

@@ -396,12 +396,16 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
           const double rep = (double)bf16_to_float(hb[e]) + (double)bf16_to_float(lb[e]);
           csum += a.wsum ? (i0 + e < a.n ? (double)a.wsum[i0 + e] * rep : 0.0) : rep;
         }
-        if (a.hl && !live) continue;  // hi/lo in columns: column k >= t holds the lo half of k - t
+        // tail hi/lo: columns [t, tail0 + 2 tail_r) hold lo halves of the tail vectors
+        if (a.tail_r > 0 && !live && k < a.tail0 + 2 * a.tail_r) continue;
         uint4* xh = reinterpret_cast<uint4*>(a.X);
         const int64_t idx = (i0 >> 3) * a.KP + k;
         xh[idx] = make_uint4(hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16), hb[6] | (hb[7] << 16));
-        xh[idx + a.lo_offset16] =
+        const uint4 lo4 =
             make_uint4(lb[0] | (lb[1] << 16), lb[2] | (lb[3] << 16), lb[4] | (lb[5] << 16), lb[6] | (lb[7] << 16));
+        if (a.tail_r > 0 && live && k >= a.tail0) xh[idx + a.tail_r] = lo4;
+        else if (a.tail_r == 0) xh[idx + a.lo_offset16] = lo4;
+        else if (k < a.tail0) xh[idx + a.lo_offset16] = lo4;  // pad columns past the tail: hi only
       } else {
         float* xf = reinterpret_cast<float*>(a.X);
 #pragma unroll

@@ -74,8 +74,9 @@ struct PackArgs {
   int mode = 0;
   void* X = nullptr;            // null: only the column sums of x are produced
   int64_t lo_offset16 = 0;
-  bool hl = false;              // hi/lo in columns: lo half of column k is column t + k (lo_offset16
-                                // = t); pad columns k >= t are not written
+  // tail hi/lo (tail_r > 0): the lo half of vector k >= tail0 is column k + tail_r of the hi
+  // block (columns [t, tail0 + 2 tail_r) hold those halves); pad columns beyond get their hi only
+  int tail0 = 0, tail_r = 0;
   const double* xsum = nullptr; // if set, x_ik -= float(xsum_k / n) before packing
   double* part = nullptr;       // (rows_pad/256) x KP partial column sums
   const float* wsum = nullptr;  // optional row weights of the column sums (sum_i wsum_i x_ik)
@@ -101,6 +102,9 @@ struct LdsArgs {
   bool hl = false;            // hi/lo in columns (one column tile): the block's columns are the hi
                               // halves of the t vectors followed by their lo halves, and each
                               // step runs one MFMA per row tile; X's lo block is not read
+  int tail_tile = -1;         // pipelined variants: this tile holds hi and lo halves as columns
+                              // (no lo MFMA), tiles after last_tile are all zero (no MFMA)
+  int last_tile = -1;         // -1: all tiles live
   // optional per 16-column tile limits (tile = blockIdx.y * NT + j): reduction super-chunks that
   // can be nonzero, and rows from which the tile's products are known to be zero (skipped)
   const int32_t* tile_nsuper = nullptr;
@@ -124,8 +128,8 @@ struct Fin1Args {
   float* out = nullptr;            // optional column-major rows x t
   int64_t ldo = 0;
   bool raw = false;                // out = summed partial products, no centring or weighting
-  bool hl = false;                 // partials of column k's lo half are column t + k; the pass-2
-                                   // operand is written in the same layout (lo_offset16 = t)
+  int tail0 = 0, tail_r = 0;       // tail hi/lo: partials of vector k >= tail0's lo half are column
+                                   // k + tail_r; the pass-2 operand is written in the same layout
   TraitCols tc;                    // batched traits: per-column gm, gw, popcounts, subset size
 };
 
@@ -142,7 +146,7 @@ struct Fin2Args {
   const int* kinds = nullptr;
   float* out = nullptr;
   int64_t ldo = 0;
-  bool hl = false;                 // partials of column k's lo half are column t + k
+  int tail0 = 0, tail_r = 0;       // tail hi/lo: partials of vector k >= tail0's lo half are column k + tail_r
   TraitCols tc;                    // batched traits: rows outside the column's subset are zeroed
 };
 

@@ -319,13 +319,18 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
   // per-tile work limits (batched traits): MFMAs of tile j run only for super-chunks s < lim[j]
   // (with staged B the loop bound must be the same for all waves of the workgroup, which meet at
   // a barrier every super-chunk: it is taken over the workgroup's rows)
+  // Tiles past last_tile hold only zero columns and run no MFMA; the tail tile holds hi and lo
+  // halves as columns and runs no lo MFMA (both flags are uniform across the workgroup).
   int32_t lim[NT];
+  bool lo_on[NT];
   int32_t limmax = 0;
   const int64_t wg_row0 = bx * (WPG * 16 * RT);
 #pragma unroll
   for (int j = 0; j < NT; ++j) {
     const int64_t tile = (int64_t)blockIdx.y * NT + j;
-    const int32_t ln = a.tile_nsuper ? a.tile_nsuper[tile] : INT32_MAX;
+    lo_on[j] = tile != a.tail_tile;
+    const int32_t ln =
+        (a.last_tile >= 0 && tile > a.last_tile) ? 0 : (a.tile_nsuper ? a.tile_nsuper[tile] : INT32_MAX);
     // this wave's (workgroup's) rows of the tile are zero from tile_rows on
     lim[j] = (a.tile_rows && row0 >= a.tile_rows[tile]) ? 0 : ln;
     limmax = max(limmax, SBC > 0 ? ((a.tile_rows && wg_row0 >= a.tile_rows[tile]) ? 0 : ln) : lim[j]);
@@ -418,6 +423,7 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
     if constexpr (!HL) {
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
+        if (!lo_on[j]) continue;
         if (sc < lim[j])
 #pragma unroll
           for (int i = 0; i < RT; ++i)
@@ -479,7 +485,7 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
 #pragma unroll
           for (int j = 0; j < NT; ++j) {
             bh[d][j] = __builtin_bit_cast(bf16x8_t, xh[b0 + j * 16]);
-            if constexpr (!HL) bl[d][j] = __builtin_bit_cast(bf16x8_t, xl[b0 + j * 16]);
+            if constexpr (!HL) if (lo_on[j]) bl[d][j] = __builtin_bit_cast(bf16x8_t, xl[b0 + j * 16]);
           }
         }
       }
@@ -519,7 +525,7 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
         bh[d][j] = __builtin_bit_cast(bf16x8_t, xh[b0 + j * 16]);
-        if constexpr (!HL) bl[d][j] = __builtin_bit_cast(bf16x8_t, xl[b0 + j * 16]);
+        if constexpr (!HL) if (lo_on[j]) bl[d][j] = __builtin_bit_cast(bf16x8_t, xl[b0 + j * 16]);
       }
     }
   }
@@ -583,8 +589,9 @@ __global__ __launch_bounds__(256, 2) void k_bp_persist(LdsArgs a, int64_t nbx, i
 // Pass-1 epilogue from the split partials: T_jk = w_j (P_jk - gm_j s_k + xbar_k (pc_j - n gm_j));
 // writes the BF16 hi/lo pass-2 operand, optionally the column-major result, and per-block
 // partial sums of gm_j * represented(T_jk).  Thread (tx = column, ty): 8 blocks of 8 rows.
-// Hi/lo-in-columns blocks: P_jk is the sum of the partials of columns k and t + k, and threads
-// of columns k >= t only write their (zero) workgroup sums.
+// Tail hi/lo layouts: P_jk of a tail vector is the sum of the partials of columns k and
+// k + tail_r, and threads of the columns holding lo halves only write their (zero) workgroup sums;
+// pad columns past them write their hi halves (zero) only.
 __global__ __launch_bounds__(256) void k_fin1(Fin1Args a) {
   __shared__ double red[4][64];
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -597,7 +604,9 @@ __global__ __launch_bounds__(256) void k_fin1(Fin1Args a) {
   const float* gwk = trait >= 0 ? (a.gw ? a.tc.gw + (int64_t)trait * a.tc.ld_m : nullptr) : a.gw;
   const int32_t* pck = trait >= 0 ? a.tc.pc + (int64_t)trait * a.tc.ld_m : a.popcnt;
   const double nk = trait >= 0 ? a.tc.n_t[trait] : a.xsum_n;
-  if (k < a.KP && !(a.hl && k >= a.t)) {
+  const bool tail = a.tail_r > 0 && k >= a.tail0 && k < a.t;
+  const bool lo_col = a.tail_r > 0 && k >= a.t && k < a.tail0 + 2 * a.tail_r;
+  if (k < a.KP && !lo_col) {
     const double sk = a.colsum[k];
     const double xb = (a.xsum && k < a.t) ? (double)(float)(a.xsum[k] / nk) : 0.0;
     for (int b8 = 0; b8 < 8; ++b8) {
@@ -605,10 +614,10 @@ __global__ __launch_bounds__(256) void k_fin1(Fin1Args a) {
       float P[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       // padding columns (k >= t) are zero; their partials are not read (the pipelined kernels
       // leave them undefined)
-      for (int h = 0; h < (k < a.t ? (a.hl ? 2 : 1) : 0); ++h)
+      for (int h = 0; h < (k < a.t ? (tail ? 2 : 1) : 0); ++h)
         for (int z = 0; z < a.nsplit; ++z) {
           const float4* src = reinterpret_cast<const float4*>(
-              a.part + ((int64_t)z * a.KP + k + h * a.t) * a.rows_pad + i0);
+              a.part + ((int64_t)z * a.KP + k + h * a.tail_r) * a.rows_pad + i0);
           const float4 p0 = src[0], p1 = src[1];
           P[0] += p0.x; P[1] += p0.y; P[2] += p0.z; P[3] += p0.w;
           P[4] += p1.x; P[5] += p1.y; P[6] += p1.z; P[7] += p1.w;
@@ -630,8 +639,10 @@ __global__ __launch_bounds__(256) void k_fin1(Fin1Args a) {
         uint4* x2 = reinterpret_cast<uint4*>(a.X2);
         const int64_t idx = (i0 >> 3) * a.KP + k;
         x2[idx] = make_uint4(hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16), hb[6] | (hb[7] << 16));
-        x2[idx + a.lo_offset16] =
+        const uint4 lo4 =
             make_uint4(lb[0] | (lb[1] << 16), lb[2] | (lb[3] << 16), lb[4] | (lb[5] << 16), lb[6] | (lb[7] << 16));
+        if (tail) x2[idx + a.tail_r] = lo4;
+        else if (a.tail_r == 0 || k < a.tail0) x2[idx + a.lo_offset16] = lo4;
       }
     }
   }
@@ -647,9 +658,11 @@ __global__ __launch_bounds__(256) void k_fin2(Fin2Args a) {
   const int64_t i = 4 * ((int64_t)blockIdx.x * 256 + threadIdx.x);
   if (i >= a.rows || k >= a.t) return;
   float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int h = 0; h < (a.hl ? 2 : 1); ++h)  // hi/lo in columns: lo partials in column t + k
+  const bool tail = a.tail_r > 0 && k >= a.tail0;  // tail hi/lo: lo partials in column k + tail_r
+  for (int h = 0; h < (tail ? 2 : 1); ++h)
     for (int z = 0; z < a.nsplit; ++z) {
-      const float4 p = *reinterpret_cast<const float4*>(a.part + ((int64_t)z * a.KP + k + h * a.t) * a.rows_pad + i);
+      const float4 p =
+          *reinterpret_cast<const float4*>(a.part + ((int64_t)z * a.KP + k + h * a.tail_r) * a.rows_pad + i);
       P.x += p.x; P.y += p.y; P.z += p.z; P.w += p.w;
     }
   const float Pv[4] = {P.x, P.y, P.z, P.w};
@@ -747,6 +760,11 @@ static void launch_variant_hl(const LdsArgs& a, int variant, dim3 grid, hipStrea
 
 void launch_bp(const LdsArgs& a, int NT, int nsplit, int variant, hipStream_t st) {
   if (a.rows_pad % (64 * kLdsRT) != 0) fail(SLEMM_ERR_ARG, "row padding mismatch");
+  // a tail hi/lo tile is only implemented by the pipelined kernels (NT <= 5)
+  if (a.tail_tile >= 0) {
+    if (NT > 5) fail(SLEMM_ERR_ARG, "tail hi/lo blocks need at most 5 column tiles per group");
+    if (!bp_hl_variant(variant)) variant = BP_PIPE;
+  }
   if (a.KP % (16 * NT) != 0) fail(SLEMM_ERR_ARG, "column padding mismatch");
   dim3 grid((unsigned)(a.rows_pad / (64 * kLdsRT)), (unsigned)(a.KP / (16 * NT)), (unsigned)nsplit);
   if (a.hl) {

@@ -23,13 +23,20 @@ Layout layout_for(const slemm_geno* h, int t, int precision) {
   }
   L.KP = (int)round_up(t, 16 * L.NT);
   const char* ehl = std::getenv("SLEMM_HL_COLUMNS");  // "0": hi and lo as separate blocks
-  L.hl = precision == PREC_BF16X2 && use_lds(precision) && 2 * t <= 16 && !(ehl && ehl[0] == '0');
+  const bool hl_ok = precision == PREC_BF16X2 && use_lds(precision) && !(ehl && ehl[0] == '0');
+  L.hl = hl_ok && 2 * t <= 16;
   L.slot = L.hl ? 0 : L.NT;
+  L.tail0 = L.tail_r = 0;
   if (L.hl) {
+    L.tail_r = t;
     L.x1_bytes = (size_t)h->n_pad * L.KP * 2;
     L.x2_bytes = (size_t)h->m_pad * L.KP * 2;
     L.lo1_16 = L.lo2_16 = t;  // lo half of column k is column t + k
     return L;
+  }
+  if (hl_ok && t > 16 && t % 16 >= 1 && t % 16 <= 8 && L.NT <= 5) {
+    L.tail0 = t / 16 * 16;
+    L.tail_r = t % 16;
   }
   const size_t es = 4;  // hi+lo bf16, or one fp32
   L.x1_bytes = (size_t)h->n_pad * L.KP * es;
@@ -76,7 +83,7 @@ void pack_pass1(slemm_geno* h, const Layout& L, const float* V, int64_t ldv, int
   p.V = V; p.ldv = ldv; p.t = t; p.KP = L.KP; p.n = h->n; p.rows_pad = h->n_pad;
   p.rr = rr; p.Q = h->Q; p.ldq = h->n_pad; p.c = h->c; p.qtv = qtv; p.kinds = kinds;
   p.mode = precision; p.X = nullptr; p.lo_offset16 = L.lo1_16; p.part = h->part1.as<double>();
-  p.hl = L.hl;
+  p.tail0 = L.tail0; p.tail_r = L.tail_r;
   if (tc) {
     p.tc = *tc;
     p.c = qtv ? 1 : 0;  // per-trait intercept basis
@@ -114,6 +121,15 @@ BpChoice bp_choice(const slemm_geno* h, int pass, int slot) {
   return c;
 }
 
+// block-product arguments of the layout's hi/lo arrangement: the staged-B narrow kernels (hl),
+// the tail tile without lo MFMAs, and the last tile holding live columns
+void tail_args(const Layout& L, int t, LdsArgs& la) {
+  la.hl = L.hl;
+  la.ncols = L.hl ? 2 * t : t;
+  la.tail_tile = (L.tail_r > 0 && !L.hl) ? L.tail0 / 16 : -1;
+  la.last_tile = L.hl ? 0 : (L.tail_r > 0 ? L.tail0 / 16 : (t + 15) / 16 - 1);
+}
+
 int64_t pass_splits(const slemm_geno* h, int pass, int slot, int64_t& nsp) {
   const BpChoice c = bp_choice(h, pass, slot);
   return pass == 0 ? lds_splits(h->m_pad, h->n_pad / kPad, c.target, nsp)
@@ -131,8 +147,7 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
   la.G = h->gs; la.ldg = h->ld_gs; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x1.p;
   la.lo_offset16 = L.lo1_16; la.KP = L.KP; la.rows_pad = h->m_pad; la.part = h->partbuf.as<float>();
   la.squared = squared;
-  la.hl = L.hl;
-  la.ncols = L.hl ? 2 * t : t;
+  tail_args(L, t, la);
   if (tc) la.tile_nsuper = tc->tile_nsuper;  // reduction over individuals stops at the subsets' end
   TimingToken tok = timing_begin(st);
   launch_bp(la, L.NT, S, bp_choice(h, 0, L.slot).variant, st);
@@ -141,7 +156,7 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
   f.part = h->partbuf.as<float>(); f.nsplit = S; f.rows_pad = h->m_pad; f.rows = h->m; f.KP = L.KP; f.t = t;
   f.colsum = h->colsum.as<double>(); f.xsum = h->colsum.as<double>() + 2 * L.KP; f.xsum_n = (double)h->n;
   f.popcnt = h->popcnt; f.gm = h->gm; f.gw = weighted ? h->gw : nullptr; f.X2 = x2; f.lo_offset16 = L.lo2_16;
-  f.upart = upart; f.out = out; f.ldo = ldo; f.raw = raw; f.hl = L.hl;
+  f.upart = upart; f.out = out; f.ldo = ldo; f.raw = raw; f.tail0 = L.tail0; f.tail_r = L.tail_r;
   if (tc) f.tc = *tc;
   launch_fin1(f, st);
 }
@@ -157,8 +172,7 @@ void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const flo
   LdsArgs la;
   la.G = h->gi; la.ldg = h->ld_gi; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x2.p;
   la.lo_offset16 = L.lo2_16; la.KP = L.KP; la.rows_pad = h->n_pad; la.part = h->partbuf.as<float>();
-  la.hl = L.hl;
-  la.ncols = L.hl ? 2 * t : t;
+  tail_args(L, t, la);
   if (tc) la.tile_rows = tc->tile_rows;  // output rows (individuals) past the subsets' end are zero
   TimingToken tok = timing_begin(st);
   launch_bp(la, L.NT, S, bp_choice(h, 1, L.slot).variant, st);
@@ -166,7 +180,7 @@ void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const flo
   Fin2Args f;
   f.part = h->partbuf.as<float>(); f.nsplit = S; f.rows_pad = h->n_pad; f.rows = h->n; f.KP = L.KP; f.t = t;
   f.colsum = u; f.rr = rr; f.v = V; f.ldv = ldv; f.tau0 = tau0; f.kinds = kinds; f.out = out; f.ldo = ldo;
-  f.hl = L.hl;
+  f.tail0 = L.tail0; f.tail_r = L.tail_r;
   if (tc) f.tc = *tc;
   launch_fin2(f, st);
 }
@@ -402,7 +416,7 @@ extern "C" int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, 
       for (int v = 0; v < BP_NVARIANTS; ++v)
         for (int ti = 0; ti < 3; ++ti) {
           const int64_t tg = targets[ti];
-          if (L.hl && !bp_hl_variant(v)) {  // not implemented for hi/lo-in-columns blocks
+          if (L.tail_r > 0 && !bp_hl_variant(v)) {  // not implemented for hi/lo-in-columns blocks
             if (best_ms) best_ms[2 + pass * (BP_NVARIANTS * 3) + v * 3 + ti] = -1.0;
             continue;
           }
@@ -415,8 +429,7 @@ extern "C" int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, 
           la.nsuper = nsuper; la.ns_per_split = nsp; la.X = pass == 0 ? h->x1.p : h->x2.p;
           la.lo_offset16 = pass == 0 ? L.lo1_16 : L.lo2_16; la.KP = L.KP; la.rows_pad = rows_pad;
           la.part = h->partbuf.as<float>();
-          la.hl = L.hl;
-          la.ncols = L.hl ? 2 * t : t;
+          tail_args(L, t, la);
           launch_bp(la, L.NT, S, v, nullptr);  // warm-up
           HIPCHECK(hipEventRecord(e0, nullptr));
           for (int r = 0; r < reps; ++r) launch_bp(la, L.NT, S, v, nullptr);
@@ -580,7 +593,7 @@ extern "C" int slemm_gpu_z(slemm_geno* h, const float* U, int64_t ldu, int t, fl
     p.V = U; p.ldv = ldu; p.t = t; p.KP = L.KP; p.n = h->m; p.rows_pad = h->m_pad;
     p.mode = precision; p.X = h->x2.p; p.lo_offset16 = L.lo2_16; p.part = h->part2.as<double>();
     p.wsum = h->gm;
-    p.hl = L.hl;
+    p.tail0 = L.tail0; p.tail_r = L.tail_r;
     launch_pack(p, st);
     double* u = h->colsum.as<double>() + L.KP;
     launch_sum_parts(h->part2.as<double>(), nparts, L.KP, u, st);

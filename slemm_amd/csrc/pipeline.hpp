@@ -14,6 +14,10 @@ struct Layout {
   // split-BF16 blocks of at most 8 vectors keep hi and lo halves as columns [0,t) and [t,2t) of
   // one 16-column block: one MFMA per step instead of a hi and a lo MFMA on mostly empty tiles
   bool hl;
+  // tail hi/lo: when the last tile holds r <= 8 vectors (t = 16 q + r, q >= 1), their lo halves
+  // are columns [t, t + r) of the hi block and that tile runs no lo MFMA (t = 65: 9 MFMA tiles per
+  // step instead of 10); hl is the case q = 0.  tail_r = 0: none.
+  int tail0, tail_r;
   int slot;                // autotuner table entry: NT, or 0 for hi/lo-in-columns blocks
 };
 
@@ -29,6 +33,7 @@ const int* upload_kinds(slemm_geno* h, int kind, const int* col_kind, int t, boo
 bool use_lds(int precision);
 BpChoice bp_choice(const slemm_geno* h, int pass, int slot);
 int64_t pass_splits(const slemm_geno* h, int pass, int slot, int64_t& nsp);
+void tail_args(const Layout& L, int t, LdsArgs& la);
 
 // pass-1 operand from V (n x t): rows scaled by rr (optional), QUADFORM columns projected,
 // column means removed (within each column's subset when tc is given).  colsum layout:

@@ -95,7 +95,8 @@ void upload_tile_limits(slemm_traits* tr, int t, const Layout& L, hipStream_t st
     if (tr->active.empty() || tr->active[k]) {
       const int64_t e = tr->extent[tr->col_trait_host[k]];
       trows[k / 16] = std::max(trows[k / 16], e);
-      if (L.hl) trows[(t + k) / 16] = std::max(trows[(t + k) / 16], e);  // its lo half, column t + k
+      if (L.tail_r > 0 && k >= L.tail0)  // its lo half, column k + tail_r
+        trows[(k + L.tail_r) / 16] = std::max(trows[(k + L.tail_r) / 16], e);
     }
   for (int q = 0; q < ntile; ++q) tns[q] = (int32_t)((trows[q] + kPad - 1) / kPad);
   tr->tile_nsuper.reserve(sizeof(int32_t) * ntile);
@@ -110,7 +111,7 @@ void upload_col_trait(slemm_traits* tr, const int* col_trait, int t, const Layou
   REQUIRE(col_trait, "col_trait is null");
   std::vector<int> ct(col_trait, col_trait + t);
   for (int k = 0; k < t; ++k) REQUIRE(ct[k] >= 0 && ct[k] < tr->T, "col_trait out of range");
-  ct.push_back(2 * L.KP + (L.hl ? 1 : 0));  // the tile arrays cover the padded width and layout too
+  ct.push_back(2 * L.KP + (L.tail_r > 0 ? 1 : 0));  // the tile arrays cover the padded width and layout too
   if (ct != tr->col_trait_host) {
     tr->col_trait.reserve(sizeof(int) * t);
     HIPCHECK(hipMemcpyAsync(tr->col_trait.p, ct.data(), sizeof(int) * t, hipMemcpyHostToDevice, st));

@@ -80,10 +80,10 @@ def test_op_ldet_and_quadform(gpu, prec, t):
 BP_VARIANTS = 10  # block-product kernel variants selectable with SLEMM_BP_VARIANT
 
 
-@pytest.mark.parametrize("t", [1, 3, 5, 12, 16, 33, 66])
+@pytest.mark.parametrize("t", [1, 3, 5, 12, 16, 33, 65, 66, 161])
 def test_block_product_variants(gpu, t, monkeypatch):
     """Every split-BF16 block-product variant at one reduction split and at several, for both
-    split layouts of narrow blocks (t <= 8: hi and lo halves as columns of one tile, or as two
+    split layouts (hi and lo halves of narrow blocks or of the last tile as columns, or as two
     blocks), against the FP64 dense operator and the transposed product."""
     p = make_problem(n=2600, m=1700, seed=19, weights=True, c=2)
     g = _geno(gpu, p)
@@ -95,7 +95,7 @@ def test_block_product_variants(gpu, t, monkeypatch):
     tau0 = 0.37
     ref = D.op("ldet", tau0, V)
     zref = D.rmatvec(V, weighted=True)
-    for hl in (("1", "0") if t <= 8 else ("1",)):
+    for hl in ("1", "0"):
         monkeypatch.setenv("SLEMM_HL_COLUMNS", hl)
         for v in range(BP_VARIANTS):
             for target in ("1", "2048"):
