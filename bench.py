@@ -34,7 +34,7 @@ FP32_PEAK_TFLOPS = 157.3
 # WRITE_SIZE on this workload (fetch counter scaled x2.0, write x1.0 against a 1 GiB device copy);
 # commands and calibration in profiles/blockprod_c2_k65_hbm_traffic.md.  FETCH_SIZE includes
 # Infinity Cache hits.
-MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 15.73e9 + 0.320e9, "pass2": 20.93e9 + 0.193e9},
+MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 15.14e9 + 0.321e9, "pass2": 17.44e9 + 0.193e9},
                     (100_000, 500_000, 1): {"pass1": 12.64e9 + 0.064e9, "pass2": 12.71e9 + 0.039e9}}
 
 
