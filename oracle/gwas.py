@@ -20,8 +20,10 @@ def gstat_chisq(xPy: float, xPx: float, vg: float) -> float:
 
 
 def mac_of_maf(maf: float, pop_size: int) -> int:
-    """MAC column of .gstat.txt (src/output.cpp:112)."""
-    return int(np.float32(maf) * 2 * pop_size + 0.5)
+    """MAC column of .gstat.txt (src/output.cpp:112): maf*2*pop_size is a float product, the
+    + 0.5 a double sum."""
+    prod = np.float32(np.float32(maf) * np.float32(2)) * np.float32(pop_size)
+    return int(float(prod) + 0.5)
 
 
 def mean_wo_outliers(x) -> float:
@@ -64,14 +66,18 @@ def gamma_scores(Z: np.ndarray, Py: np.ndarray, rr: np.ndarray, Q: np.ndarray, m
 
 
 def gamma_stats(xy, xx, cfactor: float, snp_var: float):
-    """BETA, SE, CHISQ, P of bin/slemm_gamma.py:321-325."""
+    """BETA, SE, CHISQ, P of bin/slemm_gamma.py:321-325: xy and xx are float32 arrays there and
+    cfactor, snp_var Python floats, so every step but the double product snp_var*cfactor is a
+    float32 operation; P = chi2.sf in float64."""
     from scipy.stats import chi2
 
-    xy = np.asarray(xy, np.float64)
-    xx = np.asarray(xx, np.float64)
-    beta = xy / xx * cfactor
-    serr = np.sqrt((snp_var * cfactor) / xx)
-    chisq = np.square(beta / serr)
+    xy = np.asarray(xy, np.float32)
+    xx = np.asarray(xx, np.float32)
+    cf32 = np.float32(cfactor)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        beta = xy / xx * cf32
+        serr = np.sqrt(np.float32(float(snp_var) * float(cfactor)) / xx)
+        chisq = np.square(beta / serr)
     return beta, serr, chisq, chi2.sf(chisq, 1)
 
 

@@ -8,7 +8,9 @@ a rerun:
     h2 = Vg/(Vg+Ve), sum(Py) ~ 0 (projection, src/reml.cpp:718), BLUE intercept;
   * 10k.gstat.txt: ChiSq = xPy^2/(Vg xPx) (src/output.cpp:105,115), MAC = int(MAF*2n+0.5) (:112),
     Pval = erfc(sqrt(ChiSq/2)) (src/chisq.h:14-18);
-  * 10k.gamma.txt: CHISQ = (BETA/SE)^2 (bin/slemm_gamma.py:325);
+  * 10k.gamma.txt: CHISQ = (BETA/SE)^2 in float32 and P = chi2.sf(CHISQ, 1) (bin/slemm_gamma.py:
+    321-325), the text of each line (:327-361);
+  * 10k.gstat.txt lines as written (src/output.cpp:85-120);
   * 10k.reml.snp.csv: MAF values are k/(2n) (src/weight.cpp:12-14).
 Run from the repository root:  python tests/golden/make_fixtures.py [reference_root]
 """
@@ -51,6 +53,7 @@ def main(ref_root: str = "/root/reference") -> None:
     col = {h: i for i, h in enumerate(hdr)}
     meta["gstat"] = [{k: r[col[k]] for k in ("variant", "MAC", "MAF", "xPy", "xPx", "xBx", "xSx", "ChiSq", "Pval")}
                      for r in rows]
+    meta["gstat_lines"] = (exp / "10k.gstat.txt").read_text().splitlines()
 
     # first rows of the GRAMMAR-gamma output
     gam = (exp / "10k.gamma.txt").read_text().splitlines()
