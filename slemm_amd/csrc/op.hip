@@ -261,38 +261,72 @@ void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const floa
 // --------------------------------------------------------------------------- GEBV ----------
 // One thread per individual over the individual-major copy: each thread walks its own row of
 // codes SNP by SNP, so every individual's FP64 sum is accumulated in exactly the reference's
-// order (tgv += z*alpha_j, src/pred.cpp:128); z*alpha is exact for z in {0,1,2}, so the fma
-// rounds like mul+add.  Chunks of 256 SNPs: 64 bytes of codes per thread (full 64-byte sectors
-// of its row), alpha and the flip flags of the chunk staged in LDS and read as broadcasts.
-__global__ __launch_bounds__(256) void k_gebv(const uint8_t* __restrict__ gi, int64_t ld_gi, int64_t m,
-                                              int64_t n, const double* __restrict__ alpha,
-                                              const uint8_t* __restrict__ flip, double* __restrict__ out) {
-  __shared__ double sa[256];
-  __shared__ uint32_t sf[256];
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;  // < n_pad: rows exist in gi
+// order (tgv += z*alpha_j, src/pred.cpp:128).  The chain of dependent FP64 adds is the bound (n
+// chains of m steps), so a step is kept to a bit-field extract of the 2-bit code, its LDS
+// address, one 8-byte LDS read and the add: a table of the four values z(code) * alpha_j per SNP
+// (z = popcount(code), 2 - z for a flipped SNP; the products are exact and 0 * alpha keeps its
+// sign, as the reference's z*alpha does) is staged in LDS, 128 SNPs (4 KB) per chunk,
+// double-buffered and filled one chunk ahead with in-order vector loads.  Codes come in chunks
+// of 128 SNPs (32 bytes of the row per thread): each iteration takes the chunk loaded during
+// the previous one and at once issues the load of the next (clamped to the last chunk, so it is
+// unconditional).  Table padding past m is zero, so padding SNPs add +0 (the sum never becomes
+// -0: it starts at +0 and x + (-x) is +0).  Staging the alphas in LDS rather than reading them
+// with scalar loads matters at low occupancy (100k individuals = 1.5 waves per SIMD): scalar
+// loads can only be waited for all together, which left an L2 round trip per few steps.
+__global__ __launch_bounds__(256) void k_gebv(const uint8_t* __restrict__ gi, int64_t ld_gi, int64_t nchunk,
+                                              int64_t n, const double* __restrict__ table,
+                                              double* __restrict__ out) {
+  __shared__ double tab[2][128 * 4];
+  const int tid = threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * 256 + tid;  // < n_pad: rows exist in gi
   const uint4* row = reinterpret_cast<const uint4*>(gi + i * ld_gi);
+  const uint4* tsrc = reinterpret_cast<const uint4*>(table);  // 4 KB = 256 x 16 bytes per chunk
+  const int64_t last = nchunk - 1;
   double acc = 0.0;
-  for (int64_t j0 = 0; j0 < m; j0 += 256) {
-    __syncthreads();
-    {
-      const int64_t j = j0 + threadIdx.x;
-      sa[threadIdx.x] = j < m ? alpha[j] : 0.0;
-      sf[threadIdx.x] = (j < m && flip && flip[j]) ? 1u : 0u;
+  reinterpret_cast<uint4*>(tab[0])[tid] = tsrc[tid];
+  uint4 n0 = row[0], n1 = row[1];
+  uint4 tn = tsrc[min<int64_t>(1, last) * 256 + tid];
+  __syncthreads();
+  for (int64_t c = 0; c < nchunk; ++c) {
+    const uint4 c0 = n0, c1 = n1;
+    const uint4 tc = tn;
+    const int64_t cn = min(c + 1, last), cn2 = min(c + 2, last);
+    n0 = row[cn * 2];
+    n1 = row[cn * 2 + 1];
+    __builtin_amdgcn_sched_barrier(0);
+    // chunk c+1's table, loaded during chunk c-1, into the buffer chunk c-1 used
+    if (c + 1 < nchunk) reinterpret_cast<uint4*>(tab[(c + 1) & 1])[tid] = tc;
+    tn = tsrc[cn2 * 256 + tid];
+    const double* t = tab[c & 1];
+    const uint32_t w[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) acc += t[(q * 16 + k) * 4 + __builtin_amdgcn_ubfe(w[q], 2 * k, 2)];
     }
-    const uint4* src = row + j0 / 64;
-    const uint4 c0 = src[0], c1 = src[1], c2 = src[2], c3 = src[3];
+    __builtin_amdgcn_sched_barrier(0);
+    // no instruction: a use of the next chunk here keeps its load in this iteration (left to
+    // itself the compiler moves it into the next one, right before the first step)
+    asm volatile("" ::"v"(n0.x), "v"(n0.y), "v"(n0.z), "v"(n0.w), "v"(n1.x), "v"(n1.y), "v"(n1.z), "v"(n1.w));
     __syncthreads();
-    const uint32_t w[16] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w,
-                            c2.x, c2.y, c2.z, c2.w, c3.x, c3.y, c3.z, c3.w};
-    const int jn = (int)min<int64_t>(256, m - j0);
-    for (int jj = 0; jj < jn; ++jj) {
-      const uint32_t code = (w[jj >> 4] >> (2 * (jj & 15))) & 3u;
-      uint32_t d = (code & 1u) + (code >> 1);
-      if (sf[jj]) d = 2u - d;
-      acc = fma((double)d, sa[jj], acc);
-    }
   }
   if (i < n) out[i] = acc;
+}
+
+// table[4j + code] = z(code) * alpha_j with z = popcount(code), 2 - z for a flipped SNP (flip
+// words: bits 2k, 2k+1 of word j/16); zeros past m
+__global__ void k_gebv_table(const double* __restrict__ alpha, const uint32_t* __restrict__ flipw, int64_t m,
+                             int64_t len4, double* __restrict__ table) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= len4) return;
+  const int64_t j = e >> 2;
+  const uint32_t code = (uint32_t)(e & 3);
+  if (j >= m) {
+    table[e] = 0.0;
+    return;
+  }
+  const uint32_t f = (flipw[j >> 4] >> (2 * (j & 15))) & 3u;
+  table[e] = (double)__builtin_popcount(code ^ f) * alpha[j];
 }
 
 // GRAMMAR-gamma score columns: [Py, rr o Q_1..c, rr^2] (n x (2+c), column-major, ld n).
@@ -614,13 +648,24 @@ extern "C" int slemm_gpu_gebv(slemm_geno* h, const double* alpha, const uint8_t*
     REQUIRE(h && alpha && gebv, "null argument");
     HIPCHECK(hipSetDevice(h->device));
     hipStream_t st = (hipStream_t)stream;
-    DevBuf dflip;
-    if (flip) {
-      dflip.reserve(h->m);
-      HIPCHECK(hipMemcpyAsync(dflip.p, flip, h->m, hipMemcpyHostToDevice, st));
-    }
-    k_gebv<<<(unsigned)(h->n_pad / 256), 256, 0, st>>>(h->gi, h->ld_gi, h->m, h->n, alpha,
-                                                      dflip.as<uint8_t>(), gebv);
+    REQUIRE(h->m_pad % 512 == 0 && h->ld_gi % 16 == 0 && h->ld_gi * 4 >= h->m_pad && h->n_pad % 256 == 0,
+            "GEBV: unexpected padding");
+    const int64_t ngroup = h->m_pad / 16;
+    // flip words: bits 2k, 2k+1 of word j/16 set when SNP j (< m) is flipped
+    std::vector<uint32_t> fw((size_t)ngroup, 0u);
+    if (flip)
+      for (int64_t j = 0; j < h->m; ++j)
+        if (flip[j]) fw[(size_t)(j >> 4)] |= 3u << (2 * (j & 15));
+    DevBuf dfw, dtab;
+    dfw.reserve(sizeof(uint32_t) * fw.size());
+    HIPCHECK(hipMemcpyAsync(dfw.p, fw.data(), sizeof(uint32_t) * fw.size(), hipMemcpyHostToDevice, st));
+    const int64_t len4 = h->m_pad * 4;
+    dtab.reserve(sizeof(double) * (size_t)len4);
+    k_gebv_table<<<(unsigned)((len4 + 255) / 256), 256, 0, st>>>(alpha, dfw.as<uint32_t>(), h->m, len4,
+                                                                   dtab.as<double>());
+    HIPCHECK(hipGetLastError());
+    k_gebv<<<(unsigned)(h->n_pad / 256), 256, 0, st>>>(h->gi, h->ld_gi, h->m_pad / 128, h->n, dtab.as<double>(),
+                                                      gebv);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(st));
   });

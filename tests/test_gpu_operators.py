@@ -179,15 +179,21 @@ def test_synthesized_bed_given_frequencies(gpu):
     assert f[0] == 0.0 and f[1] == 1.0
 
 
-def test_gebv_bit_exact(gpu):
-    p = make_problem(n=3001, m=800, seed=19, missing=0.01)
+@pytest.mark.parametrize("n,m,flips", [(3001, 800, True), (257, 1, False), (1024, 1537, False), (700, 2048, True)])
+def test_gebv_bit_exact(gpu, n, m, flips):
+    """FP64 GEBV in the reference's SNP order, compared bit for bit (signed zeros included):
+    alphas with exact cancellations, -0.0 and a wide exponent range; flips on a third of the SNPs."""
+    p = make_problem(n=n, m=m, seed=19 + m, missing=0.01)
     rng = np.random.default_rng(4)
-    alpha = rng.normal(size=p.m)
-    flip = (rng.random(p.m) < 0.3).astype(np.uint8)
+    alpha = rng.normal(size=p.m) * np.exp2(rng.integers(-30, 30, size=p.m))
+    alpha[1::7] = -alpha[0:-1:7][: alpha[1::7].size]
+    alpha[3::11] = -0.0
+    flip = (rng.random(p.m) < 0.3).astype(np.uint8) if flips else None
     g = _geno(gpu, p)
     got = g.gebv(torch.from_numpy(alpha).cuda(), flip).cpu().numpy()
-    want = oracle.gebv(bedio.bed_bytes(p.rows), p.n, p.m, np.ones(p.m, np.uint8), flip, alpha)
-    assert np.array_equal(got, want)
+    fl = flip if flips else np.zeros(p.m, np.uint8)
+    want = oracle.gebv(bedio.bed_bytes(p.rows), p.n, p.m, np.ones(p.m, np.uint8), fl, alpha)
+    assert np.array_equal(got.view(np.uint64), np.asarray(want, np.float64).view(np.uint64))
 
 
 @pytest.mark.parametrize("weights", [False, True])
