@@ -334,6 +334,16 @@ int slemm_gpu_lmm_gstat(slemm_geno* h, const float* X, int64_t ldx, int c, const
                         const int64_t* snps, int k, int window_size, float tol_L, int precision, int verbose,
                         float* xPy, float* xPx, float* xBx, float* xSx);
 
+/* --fake_geno branch of the --lmm quadratic forms (src/reml.cpp:886-918), after an LMC_REML fit
+ * with seed rng_seed and n_MC samples: k simulated markers continue the fit's random stream
+ * (mt19937(rng_seed+54321) after its n_MC*m + n_MC*n normal draws), each with allele frequency
+ * uniform_real_distribution<float>(0.01, 0.99) and n binomial(2, p) dosages, centred by their
+ * float mean; xPy, xPx, xSx (host, k each) as slemm_gpu_lmm_gstat computes them (no window
+ * term: the reference reports xBx as NA and names the markers fake-1, fake-2, ...). */
+int slemm_gpu_lmm_gstat_fake(slemm_geno* h, const float* X, int64_t ldx, int c, const float* Py, float h2, int k,
+                             int rng_seed, int n_MC, float tol_L, int precision, int verbose, float* xPy,
+                             float* xPx, float* xSx);
+
 /* Windowed score tests of slemm_gwa step 2 (bin/slemm_gwa.py:322-431): model SNPs model_snps
  * (host, K, increasing) form blocks of block_size (reduced to an even number <= K) overlapping by
  * half; block b tests the SNPs from the previous block's bound to its own (model SNP

@@ -63,6 +63,7 @@ def load():
         "orc_rademacher": (None, [C.c_int, i64, C.c_int, f32p]),
         "orc_lmc_samples": (None, [C.c_int, i64, i64, C.c_int, f32p, f32p]),
         "orc_gebv": (C.c_int, [u8p, i64, i64, i64, u8p, u8p, f64p, f64p]),
+        "orc_fake_geno": (None, [C.c_int, i64, i64, C.c_int, C.c_int, f32p, f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -238,6 +239,14 @@ def lmc_samples(seed: int, n: int, m: int, nmc: int):
     e = np.zeros((n, nmc), np.float32, order="F")
     load().orc_lmc_samples(seed, n, m, nmc, _p(u, C.c_float), _p(e, C.c_float))
     return u, e
+
+
+def fake_geno(seed: int, n: int, m: int, nmc: int, k: int):
+    """Uncentred fake-marker dosages (n x k) and their frequencies (k) of --fake_geno."""
+    V = np.zeros((n, k), np.float32, order="F")
+    f = np.zeros(k, np.float32)
+    load().orc_fake_geno(seed, n, m, nmc, k, _p(V, C.c_float), _p(f, C.c_float))
+    return V, f
 
 
 # ------------------------------------------------------------------------------ GEBV ------

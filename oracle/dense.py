@@ -134,6 +134,22 @@ def sldf_gss(D: "Dense", y_proj, Q, V, s2min=1e-4, s2max=0.7, tol=1e-4):
     return vg, ve, lrt, vg / (vg + ve)
 
 
+def lmm_gstat_fake(D: "Dense", Py, tau1: float, V0):
+    """Quadratic forms of the --fake_geno branch (src/reml.cpp:900-917) in FP64 with exact solves,
+    for uncentred fake dosages V0 (n x k): the columns centred by their float mean (one float
+    divide of the exact dosage sum), xPy = x'Py, v = P R^-1/2 x, xPx = v'(A_quad0 + tau1 I)^-1 v,
+    xSx = v'v."""
+    V0 = np.asarray(V0, np.float32)
+    n = V0.shape[0]
+    mu = (V0.astype(np.float64).sum(axis=0).astype(np.float32) / np.float32(n)).astype(np.float32)
+    X = (V0 - mu[None, :]).astype(np.float64)
+    xPy = X.T @ np.asarray(Py, dtype=np.float64)
+    V = D.P @ (D.rr[:, None] * X)
+    xPx = np.sum(V * D.solve("quadform", tau1, V), axis=0)
+    xSx = np.sum(V * V, axis=0)
+    return xPy, xPx, xSx
+
+
 def lmm_gstat(D: "Dense", Py, tau1: float, snps, window_size: int):
     """Quadratic forms of --lmm (src/reml.cpp:920-980, real-genotype branch) in FP64 with exact
     solves: xPy = x'Py, xPx = v'(A_quad0 + tau1 I)^-1 v, xSx = v'v with v = P R^-1/2 (x - gm),

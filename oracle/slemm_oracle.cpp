@@ -442,6 +442,23 @@ void orc_lmc_samples(int seed, int64_t n, int64_t m, int nmc, float* u, float* e
     for (int64_t i = 0; i < n; ++i) e[i + (size_t)j * n] = rnorm(rng);
 }
 
+// Fake markers of --lmm --fake_geno (src/reml.cpp:886-899): the LMC stream continued after its
+// n_MC*m + n_MC*n normal draws; per marker j a frequency freq[j] = uniform_real<float>(0.01,
+// 0.99) and n binomial(2, freq[j]) dosages (V, n x k col-major, uncentred).
+void orc_fake_geno(int seed, int64_t n, int64_t m, int nmc, int k, float* V, float* freq) {
+  std::mt19937 rng(seed + 54321);
+  std::normal_distribution<float> rnorm(0.0, 1.0);
+  for (int64_t e = 0; e < (int64_t)nmc * m; ++e) (void)rnorm(rng);
+  for (int64_t e = 0; e < (int64_t)nmc * n; ++e) (void)rnorm(rng);
+  std::uniform_real_distribution<float> runif(0.01, 0.99);
+  for (int j = 0; j < k; ++j) {
+    const float a1_freq = runif(rng);
+    freq[j] = a1_freq;
+    std::binomial_distribution<int> rbiom(2, a1_freq);
+    for (int64_t i = 0; i < n; ++i) V[i + (size_t)j * n] = (float)rbiom(rng);
+  }
+}
+
 // ------------------------------------------------------------------------ GEBV ------------
 // computeTotalGeneticValuePLINK bed loop (src/pred.cpp:104-130): tgv (FP64) over all fam
 // individuals, SNPs in bim order, z = low+high bit, z -> 2-z where bflip, tgv += z*alpha[j].

@@ -84,6 +84,8 @@ SIGNATURES = {
                                                 C.POINTER(RemlStats)]),
     "slemm_gpu_lmm_gstat": (C.c_int, [c_vp, c_fp, c_i64, C.c_int, c_fp, C.c_float, C.POINTER(C.c_int64), C.c_int,
                                       C.c_int, C.c_float, C.c_int, C.c_int, c_fp, c_fp, c_fp, c_fp]),
+    "slemm_gpu_lmm_gstat_fake": (C.c_int, [c_vp, c_fp, c_i64, C.c_int, c_fp, C.c_float, C.c_int, C.c_int, C.c_int,
+                                           C.c_float, C.c_int, C.c_int, c_fp, c_fp, c_fp]),
     "slemm_gpu_traits_create": (C.c_int, [c_vp, C.c_int, c_u8p, C.POINTER(c_vp)]),
     "slemm_gpu_traits_destroy": (C.c_int, [c_vp]),
     "slemm_gpu_traits_info": (C.c_int, [c_vp, c_i64p, c_i64p]),

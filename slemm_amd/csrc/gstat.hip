@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <random>
 #include <vector>
 
 #include "driver.hpp"
@@ -72,6 +73,37 @@ struct DevT {
   DevT& operator=(const DevT&) = delete;
 };
 
+// xPy = V' Py (V: centred markers, n x k, before scaling; :936), then v = (I - QQ') R^-1/2 V
+// (:938-939), xSx = ||v||^2 (:942) and xPx = v' H^-1 v with one Lanczos seed over all columns
+// at tau1 and L_Solve at shift 0 (:940-941).  V is overwritten by v; dV holds v on exit.
+void quad_forms(slemm_geno* h, const MatD& Q, std::vector<float>& V, int k, const float* Py, float tau1,
+                float tol_L, int precision, int verbose, DevF& dV, float* xPy, float* xPx, float* xSx) {
+  const int64_t n = h->n;
+  for (int i = 0; i < k; ++i) {
+    double s = 0.0;
+    for (int64_t r = 0; r < n; ++r) s += (double)V[(size_t)i * n + r] * Py[r];
+    xPy[i] = (float)s;
+  }
+  for (int i = 0; i < k; ++i)
+    for (int64_t r = 0; r < n; ++r) V[(size_t)i * n + r] *= h->rr_host[r];
+  project_out(Q, V, n, k);
+  for (int i = 0; i < k; ++i) xSx[i] = (float)sq_norm(&V[(size_t)i * n], n);
+  upload(dV.p, V);
+  DevF dW((size_t)n * k);
+  SeedGuard sg;
+  abi(slemm_gpu_lseed(h, SLEMM_OP_QUADFORM, nullptr, nullptr, tau1, dV.p, n, k, tol_L, 600, k, precision, verbose,
+                      nullptr, &sg.s));
+  abi(slemm_gpu_lsolve(sg.s, 0, k, dV.p, n, 0.0f, 1e-6f, dW.p, n, nullptr));
+  HIPCHECK(hipDeviceSynchronize());
+  std::vector<float> W;
+  download(W, dW.p, (size_t)n * k);
+  for (int i = 0; i < k; ++i) {
+    double s = 0.0;
+    for (int64_t r = 0; r < n; ++r) s += (double)V[(size_t)i * n + r] * W[(size_t)i * n + r];
+    xPx[i] = (float)s;
+  }
+}
+
 }  // namespace
 }  // namespace slemm
 
@@ -95,42 +127,16 @@ extern "C" int slemm_gpu_lmm_gstat(slemm_geno* h, const float* X, int64_t ldx, i
     MatD Q;
     covariate_basis(h, X, ldx, c, Q);
 
-    // V = centred dosages of the sampled SNPs; xPy = V' Py before scaling (:936)
+    // V = centred dosages of the sampled SNPs, then xPy, xSx, xPx
     DevT<int64_t> dsnps(k);
     HIPCHECK(hipMemcpy(dsnps.p, snps, sizeof(int64_t) * k, hipMemcpyHostToDevice));
-    DevF dV((size_t)n * k), dW((size_t)n * k);
+    DevF dV((size_t)n * k);
     k_extract_centered<<<dim3((unsigned)((n + 255) / 256), (unsigned)k), 256>>>(h->gs, h->ld_gs, dsnps.p, n, h->gm,
                                                                                 dV.p, n);
     HIPCHECK(hipGetLastError());
     std::vector<float> V;
     download(V, dV.p, (size_t)n * k);
-    for (int i = 0; i < k; ++i) {
-      double s = 0.0;
-      for (int64_t r = 0; r < n; ++r) s += (double)V[(size_t)i * n + r] * Py[r];
-      xPy[i] = (float)s;
-    }
-    // v = (I - QQ') R^-1/2 x (:938-939); xSx = ||v||^2 (:942)
-    for (int i = 0; i < k; ++i)
-      for (int64_t r = 0; r < n; ++r) V[(size_t)i * n + r] *= h->rr_host[r];
-    project_out(Q, V, n, k);
-    for (int i = 0; i < k; ++i) xSx[i] = (float)sq_norm(&V[(size_t)i * n], n);
-    upload(dV.p, V);
-
-    // xPx = v' H^-1 v: one seed over all sampled SNPs at tau1, L_Solve at shift 0 (:940-941)
-    {
-      SeedGuard sg;
-      abi(slemm_gpu_lseed(h, SLEMM_OP_QUADFORM, nullptr, nullptr, tau1, dV.p, n, k, tol_L, 600, k, precision,
-                          verbose, nullptr, &sg.s));
-      abi(slemm_gpu_lsolve(sg.s, 0, k, dV.p, n, 0.0f, 1e-6f, dW.p, n, nullptr));
-      HIPCHECK(hipDeviceSynchronize());
-      std::vector<float> W;
-      download(W, dW.p, (size_t)n * k);
-      for (int i = 0; i < k; ++i) {
-        double s = 0.0;
-        for (int64_t r = 0; r < n; ++r) s += (double)V[(size_t)i * n + r] * W[(size_t)i * n + r];
-        xPx[i] = (float)s;
-      }
-    }
+    quad_forms(h, Q, V, k, Py, tau1, tol_L, precision, verbose, dV, xPy, xPx, xSx);
 
     // xBx over the window around each SNP (:944-968), FP64 on the device
     const int w = window_size;
@@ -181,6 +187,54 @@ extern "C" int slemm_gpu_lmm_gstat(slemm_geno* h, const float* X, int64_t ldx, i
       xBx[i] = (float)((zh[jj] - q) / (double)tau1);  // zh[jj] = v'v
       if (verbose && (i + 1) % 10 == 0) printf("  gstat: %d of %d SNPs\n", i + 1, k);
     }
+  });
+}
+
+// --fake_geno branch of --lmm (src/reml.cpp:886-918): k simulated markers drawn from the fit's
+// random stream after the MC samples of LMC_REML (:795-809, replayed here: mt19937(seed+54321),
+// n_MC*m then n_MC*n normal_distribution<float> draws), each with allele frequency
+// uniform_real_distribution<float>(0.01, 0.99) and n binomial_distribution<int>(2, p) dosages;
+// centred by their float mean, then the same xPy, xPx, xSx as for real SNPs (no window term).
+extern "C" int slemm_gpu_lmm_gstat_fake(slemm_geno* h, const float* X, int64_t ldx, int c, const float* Py, float h2,
+                                        int k, int rng_seed, int n_MC, float tol_L, int precision, int verbose,
+                                        float* xPy, float* xPx, float* xSx) {
+  return guard([&] {
+    REQUIRE(h && X && Py && xPy && xPx && xSx, "null argument");
+    REQUIRE(c >= 1 && ldx >= h->n, "covariates: need c >= 1 and ldx >= n");
+    REQUIRE(k >= 1, "need at least one marker");
+    REQUIRE(n_MC >= 1, "n_MC must be positive");
+    REQUIRE(h2 > 0 && h2 < 1, "h2 must be in (0, 1)");
+    HIPCHECK(hipSetDevice(h->device));
+    require_model(h);
+    const int64_t n = h->n, m = h->m;
+    const float tau1 = (1 - h2) / h2;
+    MatD Q;
+    covariate_basis(h, X, ldx, c, Q);
+    std::vector<float> V((size_t)n * k);
+    {
+      std::mt19937 rng(rng_seed + 54321);
+      std::normal_distribution<float> rnorm(0.0, 1.0);
+      for (int64_t e = 0; e < (int64_t)n_MC * m; ++e) (void)rnorm(rng);
+      for (int64_t e = 0; e < (int64_t)n_MC * n; ++e) (void)rnorm(rng);
+      std::uniform_real_distribution<float> runif(0.01, 0.99);
+      for (int j = 0; j < k; ++j) {
+        const float a1_freq = runif(rng);
+        std::binomial_distribution<int> rbiom(2, a1_freq);
+        for (int64_t i = 0; i < n; ++i) V[(size_t)j * n + i] = (float)rbiom(rng);
+      }
+    }
+    // column means as one float divide of the dosage sum (Eigen's colwise mean); the sum is exact
+    // in any order up to n = 2^23, beyond which the reference's float summation order decides
+    // its last bit
+    for (int j = 0; j < k; ++j) {
+      float* v = &V[(size_t)j * n];
+      int64_t s = 0;
+      for (int64_t i = 0; i < n; ++i) s += (int64_t)v[i];
+      const float mu = (float)s / (float)n;
+      for (int64_t i = 0; i < n; ++i) v[i] -= mu;
+    }
+    DevF dV((size_t)n * k);
+    quad_forms(h, Q, V, k, Py, tau1, tol_L, precision, verbose, dV, xPy, xPx, xSx);
   });
 }
 

@@ -374,6 +374,19 @@ class Geno:
             _np_ptr(xx, C.c_float)))
         return xy, xx
 
+    def lmm_gstat_fake(self, X: np.ndarray, Py: np.ndarray, h2: float, k: int, rng_seed: int = 0, n_MC: int = 10,
+                       tol_L: float = 5e-4, precision="bf16x2", verbose: bool = False) -> dict:
+        """--fake_geno quadratic forms (src/reml.cpp:886-918) after an LMC_REML fit with the same
+        rng_seed and n_MC: xPy, xPx, xSx of k simulated markers (xBx is not defined for them)."""
+        Xf = np.asfortranarray(np.asarray(X, np.float32).reshape(self.n, -1))
+        c = Xf.shape[1]
+        pv = np.ascontiguousarray(Py, np.float32)
+        out = {key: np.zeros(int(k), np.float32) for key in ("xPy", "xPx", "xSx")}
+        L.check(L.load().slemm_gpu_lmm_gstat_fake(
+            self._h, _np_ptr(Xf, C.c_float), self.n, c, _np_ptr(pv, C.c_float), h2, int(k), int(rng_seed), int(n_MC),
+            tol_L, _prec(precision), int(verbose), *[_np_ptr(out[key], C.c_float) for key in ("xPy", "xPx", "xSx")]))
+        return out
+
     def lmm_gstat(self, X: np.ndarray, Py: np.ndarray, h2: float, snps: np.ndarray, window_size: int = 1000,
                   tol_L: float = 5e-4, precision="bf16x2", verbose: bool = False) -> dict:
         """Quadratic forms of --lmm (src/reml.cpp:884-982): xPy, xPx, xBx, xSx for the SNPs

@@ -145,6 +145,22 @@ def write_gstat(path, variants: Sequence[str], chrom: Sequence[str], pos: Sequen
                                _fmt_g(xSx[j]), str(int(window_size)), _fmt_g(chisq[j]), _fmt_g(pval[j])]) + "\n")
 
 
+def write_gstat_fake(path, vg: float, gst: dict) -> None:
+    """The .gstat.txt of --lmm --fake_geno (src/output.cpp:101-106): rows fake-1, fake-2, ... with
+    NA for Chr, Pos, MAC, MAF, HWEP, xBx and WindowSize."""
+    xPy = np.asarray(gst["xPy"], np.float32)
+    xPx = np.asarray(gst["xPx"], np.float32)
+    xSx = np.asarray(gst["xSx"], np.float32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        chisq = (xPy * xPy) / (np.float32(vg) * xPx)
+    pval = chisq_pval_1df(chisq)
+    with open(path, "w") as f:
+        f.write("variant\tChr\tPos\tMAC\tMAF\tHWEP\txPy\txPx\txBx\txSx\tWindowSize\tChiSq\tPval\n")
+        for j in range(xPy.size):
+            f.write(f"fake-{j + 1}\tNA\tNA\tNA\tNA\tNA\t" + "\t".join(
+                [_fmt_g(xPy[j]), _fmt_g(xPx[j]), "NA", _fmt_g(xSx[j]), "NA", _fmt_g(chisq[j]), _fmt_g(pval[j])]) + "\n")
+
+
 def read_gstat(path) -> dict:
     """Columns of a .gstat.txt as the scripts read them: whitespace-split fields, numbers as
     Python floats (bin/slemm_gamma.py:163-172, bin/slemm_gwa.py:141-156)."""

@@ -142,6 +142,29 @@ def test_lmm_gstat_against_fp64(gpu, prec):
     assert np.all(got["xBx"] > got["xPx"])
 
 
+def test_lmm_gstat_fake_against_fp64(gpu):
+    """--fake_geno quadratic forms (src/reml.cpp:886-918): the fit's random stream continued after
+    its MC samples gives the same fake markers as the oracle's draws (xSx to 1e-5 would fail for
+    any other marker), and xPy, xPx, xSx agree with the FP64 dense evaluation."""
+    import oracle
+    from oracle.dense import Dense, lmm_gstat_fake
+    p = make_problem(n=1200, m=1500, seed=12, weights=True, c=2)
+    y = make_phenotype(p, 0.4, seed=3)
+    g = gpu.Geno.from_bed(p.rows, p.n)
+    g.set_model(None, p.rvec)
+    seed, nmc, k = 7, 10, 9
+    fit = g.LMC_REML(p.X, y, rng_seed=seed, n_MC=nmc)
+    h2 = np.float32(fit["h2"])
+    tau1 = float((np.float32(1) - h2) / h2)
+    got = g.lmm_gstat_fake(p.X, fit["Py"], float(h2), k, rng_seed=seed, n_MC=nmc)
+    V0, _ = oracle.fake_geno(seed, p.n, p.m, nmc, k)
+    D = Dense(p.Z, p.gm, p.gw, p.rr, p.Q)
+    xPy, xPx, xSx = lmm_gstat_fake(D, fit["Py"], tau1, V0)
+    assert rel_err(got["xSx"], xSx) < 1e-5
+    assert rel_err(got["xPy"], xPy) < 1e-5
+    assert rel_err(got["xPx"], xPx) < 1e-4
+
+
 def test_grammar_gamma_pipeline(gpu):
     """LMC fit -> gstat quadratic forms on 30 sampled SNPs -> correction factor -> GRAMMAR-gamma
     tests of every SNP (bin/slemm_gamma.py:163-174, 306-325), against the dense restatement."""

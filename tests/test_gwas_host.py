@@ -161,3 +161,17 @@ def test_mac_of_maf_float_product_double_sum():
     assert gwas.mac_of_maf(0.5, 16777215) == 16777215 == ogwas.mac_of_maf(0.5, 16777215)
     for row in _meta()["gstat"]:
         assert gwas.mac_of_maf(float(row["MAF"]), _meta()["pop_size"]) == int(row["MAC"])
+
+
+def test_write_gstat_fake_rows(tmp_path):
+    """--fake_geno rows of .gstat.txt (src/output.cpp:101-106): the marker index -(j+1) printed
+    after "fake", NA in the columns a fake marker does not have."""
+    gst = dict(xPy=np.float32([-70.8334, 12.5]), xPx=np.float32([172.206, 99.0]), xSx=np.float32([1727.29, 800.0]))
+    out = tmp_path / "f.gstat.txt"
+    gwas.write_gstat_fake(out, 1.5, gst)
+    lines = out.read_text().splitlines()
+    assert lines[1].split("\t")[:6] == ["fake-1", "NA", "NA", "NA", "NA", "NA"]
+    f = lines[2].split("\t")
+    assert f[0] == "fake-2" and f[8] == "NA" and f[10] == "NA" and f[6] == "12.5" and f[9] == "800"
+    chisq = np.float32(12.5) * np.float32(12.5) / (np.float32(1.5) * np.float32(99.0))
+    assert f[11] == "%g" % float(chisq)

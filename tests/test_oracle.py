@@ -191,6 +191,86 @@ def test_lmc_samples_are_mt19937_marsaglia_polar():
     assert np.allclose(got, want, rtol=2e-6, atol=1e-7)
 
 
+class _Libstdcxx:
+    """libstdc++'s distributions over a raw mt19937 stream (numpy's MT19937 with the legacy
+    init_genrand seeding): generate_canonical<float, 24> takes one 32-bit draw, <double, 53> two
+    (low word first); normal_distribution<float> is Marsaglia's polar method keeping the second
+    deviate; uniform_real_distribution<float>(a, b) is U * (b - a) + a in float;
+    binomial_distribution<int>(t, p) with t * min(p, 1-p) < 8 is Devroye's second waiting-time
+    method (Non-Uniform Random Variate Generation, 1986, ch. X.4) on min(p, 1-p), flipped to t - x
+    when p > 0.5."""
+
+    def __init__(self, seed, count):
+        self.raw = _mt_raw(seed, count)
+        self.pos = 0
+        self.saved = None
+
+    def u32(self):
+        v = int(self.raw[self.pos])
+        self.pos += 1
+        return v
+
+    def canon_f(self):
+        v = np.float32(self.u32()) / np.float32(4294967296.0)
+        return np.nextafter(np.float32(1), np.float32(0)) if v >= 1 else v
+
+    def canon_d(self):
+        x0 = float(self.u32())
+        s = x0 + float(self.u32()) * 4294967296.0
+        v = s / 18446744073709551616.0
+        return math.nextafter(1.0, 0.0) if v >= 1 else v
+
+    def normal_f(self):
+        if self.saved is not None:
+            v, self.saved = self.saved, None
+            return v
+        while True:
+            x = np.float32(2.0) * self.canon_f() - np.float32(1.0)
+            y = np.float32(2.0) * self.canon_f() - np.float32(1.0)
+            r2 = x * x + y * y
+            if r2 <= 1.0 and r2 != 0.0:
+                break
+        mult = np.sqrt(np.float32(-2.0) * np.log(r2) / r2)
+        self.saved = x * mult
+        return y * mult
+
+    def uniform_f(self, a, b):
+        a, b = np.float32(a), np.float32(b)
+        return self.canon_f() * (b - a) + a
+
+    def binomial(self, t, p):
+        p = float(p)
+        p12 = p if p <= 0.5 else 1.0 - p
+        q = -math.log(1.0 - p12)
+        x, s = 0, 0.0
+        while True:
+            if x == t:
+                break
+            s += -math.log(1.0 - self.canon_d()) / (t - x)
+            x += 1
+            if not s <= q:
+                x -= 1
+                break
+        return t - x if p12 != p else x
+
+
+def test_fake_geno_is_libstdcxx_stream():
+    """--fake_geno markers (src/reml.cpp:886-899): the LMC stream's n_MC*(m+n) normals replayed,
+    then per marker one uniform_real<float>(0.01, 0.99) frequency and n binomial(2, p) dosages."""
+    n, m, nmc, k, seed = 23, 11, 2, 14, 3
+    V, f = oracle.fake_geno(seed, n, m, nmc, k)
+    st = _Libstdcxx(seed + 54321, 8 * nmc * (m + n) + 4 * n * k + 64)
+    for _ in range(nmc * (m + n)):
+        st.normal_f()
+    for j in range(k):
+        a1 = st.uniform_f(0.01, 0.99)
+        assert f[j] == a1, (j, f[j], a1)
+        col = [st.binomial(2, a1) for _ in range(n)]
+        assert V[:, j].tolist() == col, j
+    assert np.any(f > 0.5) and np.any(f < 0.5)  # both branches of the flip
+    assert set(np.unique(V)) <= {0.0, 1.0, 2.0}
+
+
 # ------------------------------------------------------------------------------ GEBV ------------
 def test_gebv_exact():
     rng = np.random.default_rng(9)
