@@ -77,6 +77,7 @@ struct slemm_geno {
 
   // model terms (device): gm, gw padded to m_pad with zeros; rr padded to n_pad with zeros.
   int32_t* popcnt = nullptr;  // m
+  int32_t* cnt2 = nullptr;    // m: genotypes with both bits set (dosage 2)
   float* gm = nullptr;
   float* gw = nullptr;
   float* rr = nullptr;
@@ -84,6 +85,7 @@ struct slemm_geno {
   int c = 0;
   bool counts_ready = false;
   bool model_ready = false;
+  bool rr_unit = true;  // no error-variance weights: r_rsqrt = 1
 
   // workspaces of the operator pipeline
   slemm::DevBuf x1, x2;        // packed block operands of pass 1 / pass 2
@@ -95,9 +97,11 @@ struct slemm_geno {
   slemm::DevBuf redpart;       // partials of column reductions (double)
   slemm::DevBuf partbuf;       // split-reduction partial products of the block products
   std::vector<int> kinds_host;
-  // block-product variant and grid target per (pass, column tiles), set by the autotuner; the
-  // default is the software-pipelined register kernel (5 = BP_PIPE) at a 2048-workgroup grid
-  int bp_variant[2][9] = {{5, 5, 5, 5, 5, 5, 5, 5, 5}, {5, 5, 5, 5, 5, 5, 5, 5, 5}};
+  // block-product variant and grid target per (pass, slot: 0 = hi/lo-in-columns block, else column
+  // tiles), set by the autotuner; the defaults are the kernels it picks on MI355X at 100k x 500k:
+  // the 32-copy byte-table pipelined kernel (9 = BP_PIPER32) up to five tiles, the pipelined
+  // register kernel (5 = BP_PIPE, which falls back to the register kernel) beyond, 2048 workgroups
+  int bp_variant[2][9] = {{9, 9, 9, 9, 9, 9, 5, 5, 5}, {9, 9, 9, 9, 9, 9, 5, 5, 5}};
   int bp_target[2][9] = {{2048, 2048, 2048, 2048, 2048, 2048, 2048, 2048, 2048},
                          {2048, 2048, 2048, 2048, 2048, 2048, 2048, 2048, 2048}};
   std::vector<float> rr_host;    // r_rsqrt (n)

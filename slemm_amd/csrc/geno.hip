@@ -208,10 +208,12 @@ static slemm_geno* alloc_geno(int64_t n, int64_t m) {
     HIPCHECK(hipMalloc(&h->gi, (size_t)h->n_pad * h->ld_gi));
     HIPCHECK(hipMemset(h->gs, 0, (size_t)h->m_pad * h->ld_gs));
     HIPCHECK(hipMalloc(&h->popcnt, sizeof(int32_t) * h->m_pad));
+    HIPCHECK(hipMalloc(&h->cnt2, sizeof(int32_t) * h->m_pad));
     HIPCHECK(hipMalloc(&h->gm, sizeof(float) * h->m_pad));
     HIPCHECK(hipMalloc(&h->gw, sizeof(float) * h->m_pad));
     HIPCHECK(hipMalloc(&h->rr, sizeof(float) * h->n_pad));
     HIPCHECK(hipMemset(h->popcnt, 0, sizeof(int32_t) * h->m_pad));
+    HIPCHECK(hipMemset(h->cnt2, 0, sizeof(int32_t) * h->m_pad));
     HIPCHECK(hipMemset(h->gm, 0, sizeof(float) * h->m_pad));
     HIPCHECK(hipMemset(h->gw, 0, sizeof(float) * h->m_pad));
     HIPCHECK(hipMemset(h->rr, 0, sizeof(float) * h->n_pad));
@@ -230,11 +232,10 @@ static void build_transpose(slemm_geno* h, hipStream_t st) {
 
 void ensure_counts(slemm_geno* h, hipStream_t st) {
   if (h->counts_ready) return;
-  DevBuf c2, cany;
-  c2.reserve(sizeof(int32_t) * h->m);
+  DevBuf cany;
   cany.reserve(sizeof(int32_t) * h->m);
-  k_counts<<<(unsigned)((h->m + 3) / 4), 256, 0, st>>>(h->gs, h->ld_gs, h->m, h->n, h->popcnt,
-                                                      c2.as<int32_t>(), cany.as<int32_t>(), h->gm);
+  k_counts<<<(unsigned)((h->m + 3) / 4), 256, 0, st>>>(h->gs, h->ld_gs, h->m, h->n, h->popcnt, h->cnt2,
+                                                      cany.as<int32_t>(), h->gm);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(st));
   h->counts_ready = true;
@@ -559,6 +560,7 @@ extern "C" int slemm_gpu_geno_destroy(slemm_geno* h) {
     if (h->gs) (void)hipFree(h->gs);
     if (h->gi) (void)hipFree(h->gi);
     if (h->popcnt) (void)hipFree(h->popcnt);
+    if (h->cnt2) (void)hipFree(h->cnt2);
     if (h->gm) (void)hipFree(h->gm);
     if (h->gw) (void)hipFree(h->gw);
     if (h->rr) (void)hipFree(h->rr);
@@ -590,16 +592,15 @@ extern "C" int slemm_gpu_geno_counts(slemm_geno* h, int32_t* popcnt, int32_t* n0
   return guard([&] {
     REQUIRE(h, "null handle");
     HIPCHECK(hipSetDevice(h->device));
-    DevBuf c2, cany;
-    c2.reserve(sizeof(int32_t) * h->m);
+    DevBuf cany;
     cany.reserve(sizeof(int32_t) * h->m);
-    k_counts<<<(unsigned)((h->m + 3) / 4), 256>>>(h->gs, h->ld_gs, h->m, h->n, h->popcnt,
-                                                  c2.as<int32_t>(), cany.as<int32_t>(), h->gm);
+    k_counts<<<(unsigned)((h->m + 3) / 4), 256>>>(h->gs, h->ld_gs, h->m, h->n, h->popcnt, h->cnt2,
+                                                  cany.as<int32_t>(), h->gm);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipDeviceSynchronize());
     h->counts_ready = true;
     std::vector<int32_t> v2(h->m), vany(h->m);
-    HIPCHECK(hipMemcpy(v2.data(), c2.p, sizeof(int32_t) * h->m, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(v2.data(), h->cnt2, sizeof(int32_t) * h->m, hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(vany.data(), cany.p, sizeof(int32_t) * h->m, hipMemcpyDeviceToHost));
     if (popcnt) HIPCHECK(hipMemcpy(popcnt, h->popcnt, sizeof(int32_t) * h->m, hipMemcpyDeviceToHost));
     if (gm) HIPCHECK(hipMemcpy(gm, h->gm, sizeof(float) * h->m, hipMemcpyDeviceToHost));
@@ -649,6 +650,7 @@ extern "C" int slemm_gpu_set_model(slemm_geno* h, const float* persnp, const flo
     HIPCHECK(hipMemcpy(h->gw, gw.data(), sizeof(float) * m, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(h->rr, rr.data(), sizeof(float) * n, hipMemcpyHostToDevice));
     h->rr_host = rr;
+    h->rr_unit = std::all_of(rr.begin(), rr.end(), [](float v) { return v == 1.0f; });
     h->rvec_host.assign(n, 1.0f);
     if (rvec) std::copy(rvec, rvec + n, h->rvec_host.begin());
     std::vector<float> qc((size_t)n * c);

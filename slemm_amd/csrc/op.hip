@@ -342,9 +342,11 @@ __global__ void k_gwas_cols(const float* __restrict__ Py, const float* __restric
 
 // xy_j = x_c'Py and xx_j = ||(I-QQ')R^-1/2 x_c||^2 = sum rr^2 (d-mu)^2 - ||Q'(rr o x_c)||^2 with
 // sum rr^2 (d-mu)^2 = S2 - 2 mu C1 - mu^2 R2  (S2 = sum rr^2 d^2, C1 = Zc'(rr^2), R2 = sum rr^2).
+// Without error-variance weights (S2 null) S2 is the exact count sum popcnt + 2 n2.
 __global__ void k_gwas_combine(const float* __restrict__ A, int64_t lda, int c, const float* __restrict__ S2,
-                               const float* __restrict__ gm, const int32_t* __restrict__ pc, int64_t m, int64_t n,
-                               double R2, float mac, float* __restrict__ xy, float* __restrict__ xx) {
+                               const int32_t* __restrict__ cnt2, const float* __restrict__ gm,
+                               const int32_t* __restrict__ pc, int64_t m, int64_t n, double R2, float mac,
+                               float* __restrict__ xy, float* __restrict__ xx) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
   const float s = (float)pc[j];
@@ -360,8 +362,9 @@ __global__ void k_gwas_combine(const float* __restrict__ A, int64_t lda, int c, 
     proj += a * a;
   }
   const double C1 = A[j + (int64_t)(1 + c) * lda];
+  const double s2 = S2 ? (double)S2[j] : (double)pc[j] + 2.0 * (double)cnt2[j];
   xy[j] = A[j];
-  xx[j] = (float)((double)S2[j] - 2.0 * mu * C1 - mu * mu * R2 - proj);
+  xx[j] = (float)(s2 - 2.0 * mu * C1 - mu * mu * R2 - proj);
 }
 
 // Column dots, stage 1: block (k, b) sums A[i,k] B[i,k] over rows [b*rpb, (b+1)*rpb) in FP64.
@@ -510,15 +513,19 @@ extern "C" int slemm_gpu_gwas_score(slemm_geno* h, const float* Py, float skip_m
     const Layout LA = layout_for(h, t, PREC_BF16X2);
     pack_pass1(h, LA, cols.as<float>(), n, t, nullptr, nullptr, nullptr, PREC_BF16X2, st);
     pass1_lds(h, LA, t, false, nullptr, nullptr, A.as<float>(), m, st);
-    // raw squared-dosage product Z2'(rr^2)
-    const Layout LB = layout_for(h, 1, PREC_BF16X2);
-    pack_pass1(h, LB, cols.as<float>() + (size_t)(1 + c) * n, n, 1, nullptr, nullptr, nullptr, PREC_BF16X2, st,
-               false);
-    pass1_lds(h, LB, 1, false, nullptr, nullptr, B.as<float>(), m, st, true, true);
+    // raw squared-dosage product Z2'(rr^2): a second pass over the bed, needed only with
+    // error-variance weights (otherwise the dosage counts give it exactly)
+    if (!h->rr_unit) {
+      const Layout LB = layout_for(h, 1, PREC_BF16X2);
+      pack_pass1(h, LB, cols.as<float>() + (size_t)(1 + c) * n, n, 1, nullptr, nullptr, nullptr, PREC_BF16X2, st,
+                 false);
+      pass1_lds(h, LB, 1, false, nullptr, nullptr, B.as<float>(), m, st, true, true);
+    }
     double R2 = 0.0;
     for (int64_t i = 0; i < n; ++i) R2 += (double)h->rr_host[i] * h->rr_host[i];
-    k_gwas_combine<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(A.as<float>(), m, c, B.as<float>(), h->gm,
-                                                                h->popcnt, m, n, R2, skip_mac, xy, xx);
+    k_gwas_combine<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(A.as<float>(), m, c,
+                                                                h->rr_unit ? nullptr : B.as<float>(), h->cnt2,
+                                                                h->gm, h->popcnt, m, n, R2, skip_mac, xy, xx);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(st));
   });
