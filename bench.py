@@ -207,6 +207,103 @@ def reml_time_to_converge(g, n, m, nv, y, realised_h2, n_causal, h2):
     return out, res["bf16x2"]
 
 
+def next_rows_run(g, n, m, y, sl, loader_m=40_000):
+    """The rows either side of the hot path, timed on the resident bed and the split-BF16 SLDF fit
+    of the REML run: the --lmm quadratic forms of 30 sampled SNPs with the reference's 1,000-SNP
+    windows (src/reml.cpp:920-980), the windowed scan of slemm_gwa step 2 over the first 50,000 SNPs
+    with every SNP a model SNP and 1,000-SNP blocks (bin/slemm_gwa.py:322-431), --iter_weighting
+    with the reference's defaults (src/option.cpp:417-528); then, on a synthetic fileset of n
+    individuals x loader_m SNPs (random codes) written to a temporary directory, the streaming .bed
+    loader (src/input.cpp:343-386) and --pred end to end (src/pred.cpp:4-151).  FP64 rates count
+    the window systems' dominant products (Zw'Zw per window or block, Zb'x per tested SNP)."""
+    import os
+    import shutil
+    import tempfile
+
+    import numpy as np
+    import torch
+    from slemm_amd import gpu, gwas
+
+    out = {}
+    X = np.ones((n, 1), np.float32)
+    Py, h2, vg, ve = sl["Py"], float(sl["h2"]), float(sl["vg"]), float(sl["ve"])
+    pc, _, _, _, gm = g.counts()
+
+    w = 1000
+    snps = gwas.sample_gstat_snps(gm, n, 30, seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gst = g.lmm_gstat(X, Py, h2, snps, window_size=w)
+    dt = time.perf_counter() - t0
+    out["lmm_gstat"] = {"snps": int(snps.size), "window": w, "seconds": round(dt, 3),
+                        "fp64_TFLOP/s_window_systems": round(snps.size * float(n) * w * w / dt / 1e12, 2),
+                        "correction_factor_xSx_over_xPx": round(gwas.correction_factor(gst), 4)}
+    print(f"# next rows: --lmm gstat {dt:.2f} s", file=sys.stderr, flush=True)
+
+    bs, t_end = 1000, min(m, 50_000) - 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xy, xx = g.gwa_window(X, Py, np.arange(m), bs, 0, t_end, ve / vg, m, 20.0)
+    dt = time.perf_counter() - t0
+    nt = t_end + 1
+    nblk = -(-nt // bs) + 1
+    out["gwa_window"] = {"tested_snps": nt, "block_size": bs, "seconds": round(dt, 3),
+                         "snps_per_s": round(nt / dt, 1),
+                         "fp64_TFLOP/s": round((nblk * float(n) * bs * bs + 2.0 * nt * n * bs) / dt / 1e12, 2),
+                         "finite": int(np.isfinite(xx).sum())}
+    print(f"# next rows: windowed scan {dt:.2f} s", file=sys.stderr, flush=True)
+
+    f = pc.astype(np.float64) / (2.0 * n)
+    maf = np.where(f > 0.5, 1.0 - f, f).astype(np.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = g.iter_weighting_REML(X, y, maf)
+    dt = time.perf_counter() - t0
+    out["iter_weighting"] = {"seconds": round(dt, 3), "h2": round(float(r["h2"]), 6),
+                             "clumped_snps": int(r["sig_snps"].size)}
+    print(f"# next rows: --iter_weighting {dt:.2f} s", file=sys.stderr, flush=True)
+    g.set_model()
+
+    tmp = tempfile.mkdtemp(prefix="slemm_bench_")
+    try:
+        nb = (n + 3) // 4
+        prefix = os.path.join(tmp, "syn")
+        rng = np.random.default_rng(5)
+        with open(prefix + ".bed", "wb") as fh:
+            fh.write(bytes([0x6C, 0x1B, 0x01]))
+            step = max(1, (64 << 20) // nb)
+            for j0 in range(0, loader_m, step):
+                fh.write(rng.integers(0, 256, size=(min(step, loader_m - j0), nb), dtype=np.uint8).tobytes())
+        with open(prefix + ".fam", "w") as fh:
+            fh.writelines(f"f{i} i{i} 0 0 0 -9\n" for i in range(n))
+        with open(prefix + ".bim", "w") as fh:
+            fh.writelines(f"1\tsnp{j}\t0\t{j + 1}\tA\tG\n" for j in range(loader_m))
+        eff = rng.normal(size=loader_m) * 1e-3
+        with open(prefix + ".est.csv", "w") as fh:
+            fh.write("SNP,CHR,POS,A1,A2,MAF,HWEP,x,y,EFFECT\n")
+            fh.writelines(f"snp{j},1,{j + 1},{'G' if j % 3 == 0 else 'A'},x,0,0,0,0,{float(eff[j])!r}\n"
+                          for j in range(loader_m))
+        file_bytes = 3 + loader_m * nb
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h = gpu.Geno.load_bed(prefix + ".bed", n, loader_m)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        h.close()
+        out["bed_loader"] = {"n": n, "m": loader_m, "file_bytes": file_bytes, "seconds": round(dt, 3),
+                             "GB/s": round(file_bytes / dt / 1e9, 2),
+                             "note": "file just written (in the page cache): host memory -> HBM, both layouts built"}
+        print(f"# next rows: bed loader {dt:.2f} s", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        npred, mused = gpu.pred_plink(prefix + ".est.csv", prefix, prefix + ".gebv.csv")
+        dt = time.perf_counter() - t0
+        out["pred"] = {"individuals": int(npred), "snps": int(mused), "seconds": round(dt, 3),
+                       "note": "estimate/fam/bim parsing, bed streaming, FP64 GEBV and IID,GEBV output"}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return out
+
+
 def differentiation_run(g, n, m, y, vg, ve, nv=64, win_size=100):
     """dl/dsigma_g^2, dl/dsigma_e^2 and dl/d(window weights) of the REML log-likelihood through the
     autograd harness: window weights = windowed mean of a marginal chi-square scan (100-SNP
@@ -404,6 +501,8 @@ def main():
     ap.add_argument("--no-reml", action="store_true")
     ap.add_argument("--no-smallk", action="store_true")
     ap.add_argument("--no-diff", action="store_true", help="skip the autograd/adjoint run")
+    ap.add_argument("--no-next", action="store_true",
+                    help="skip the rows either side of the path (--lmm, windowed scan, --iter_weighting, loader, --pred)")
     ap.add_argument("--no-ceiling", action="store_true", help="skip the vendor-GEMM MFMA ceiling")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -493,12 +592,15 @@ def main():
     f2, _ = kernel_ms(lib, 1, L.PREC_FP32)
     lib.slemm_gpu_timing_reset()
 
-    reml = diff = None
+    reml = diff = nxt = None
     if not args.no_reml:
         y, _, _, rh2 = simulate_phenotype(g, n, m, args.causal, args.h2, 11)
         reml, sl = reml_time_to_converge(g, n, m, nv, y, rh2, args.causal, args.h2)
         if not args.no_diff:
             diff = differentiation_run(g, n, m, y, float(sl["vg"]), float(sl["ve"]), nv=nv)
+        if not args.no_next:
+            print("# next rows: --lmm, windowed scan, --iter_weighting, loader, --pred", file=sys.stderr, flush=True)
+            nxt = next_rows_run(g, n, m, y, sl)
     smallk = None if args.no_smallk else small_k_passes(g, lib, L, n, m)
 
     ceiling = None if args.no_ceiling else gemm_ceiling()
@@ -532,6 +634,7 @@ def main():
         "autotune": tune,
         "reml_time_to_converge": reml,
         "differentiation": diff,
+        "next_rows": nxt,
         "small_k_passes": smallk,
         "roofline": {
             "bound": "mfma",

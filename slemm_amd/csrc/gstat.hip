@@ -73,6 +73,91 @@ struct DevT {
   DevT& operator=(const DevT&) = delete;
 };
 
+// G[e] = sum over the S partial products W[s][e], in order
+__global__ void k_sum_slices(const double* __restrict__ W, int64_t slice, int S, double* __restrict__ G) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= slice) return;
+  double acc = 0.0;
+  for (int q = 0; q < S; ++q) acc += W[(int64_t)q * slice + e];
+  G[e] = acc;
+}
+
+// QA[q + j c] = sum_i Q[i + q n] A[i + j n] (c x w): one workgroup per column j of A, fixed-order
+// reduction (rocBLAS runs this c-row product as a GEMM with a handful of output tiles)
+__global__ __launch_bounds__(256) void k_qt_a(const double* __restrict__ Q, int c, const double* __restrict__ A,
+                                              int64_t n, double* __restrict__ QA) {
+  __shared__ double red[256];
+  const int64_t j = blockIdx.x;
+  const double* a = A + j * n;
+  for (int q = 0; q < c; ++q) {
+    const double* qq = Q + (int64_t)q * n;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) s += qq[i] * a[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+      if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) QA[q + j * c] = red[0];
+    __syncthreads();
+  }
+}
+
+// out (cols x rows, ld cols) = in' for in (rows x cols, ld rows), FP64, through 32 x 32 LDS tiles
+__global__ void k_transpose_f64(const double* __restrict__ in, int64_t rows, int cols, double* __restrict__ out) {
+  __shared__ double t[32][33];
+  const int64_t r0 = (int64_t)blockIdx.x * 32;
+  const int c0 = blockIdx.y * 32;
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const int64_t r = r0 + threadIdx.x;
+    const int c = c0 + k;
+    if (r < rows && c < cols) t[k][threadIdx.x] = in[r + (int64_t)c * rows];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const int c = c0 + threadIdx.x;
+    const int64_t r = r0 + k;
+    if (r < rows && c < cols) out[c + r * cols] = t[threadIdx.x][k];
+  }
+}
+
+void transpose_f64(const double* in, int64_t rows, int cols, double* out) {
+  k_transpose_f64<<<dim3((unsigned)((rows + 31) / 32), (unsigned)((cols + 31) / 32)), dim3(32, 8)>>>(in, rows, cols,
+                                                                                                   out);
+  HIPCHECK(hipGetLastError());
+}
+
+// C = A'B (p x q, ld p) for tall FP64 A (n x p) and B (n x q), given as their transposes AT
+// (p x n, ld p) and BT (q x n, ld q).  rocBLAS runs this shape well only as NT products on the
+// transposed operands and with the long reduction split: S row chunks as one strided-batched
+// GEMM (plus the leftover rows) into separate slices of work, summed in a fixed order.
+struct TallProduct {
+  int64_t n;
+  int S;
+  int64_t ch, rem;
+  DevT<double> work;
+  TallProduct(int64_t n_, int64_t max_pq)
+      : n(n_), S((int)std::max<int64_t>(1, std::min<int64_t>(32, n_ / 2048))), ch(n_ / S), rem(n_ - ch * S),
+        work((size_t)(S + 1) * max_pq) {}
+  void operator()(rocblas_handle bh, const double* AT, int p, const double* BT, int q, double* C) {
+    const double one = 1.0, zero = 0.0;
+    const int64_t slice = (int64_t)p * q;
+    rb(rocblas_dgemm_strided_batched(bh, rocblas_operation_none, rocblas_operation_transpose, p, q, (rocblas_int)ch,
+                                     &one, AT, p, ch * p, BT, q, ch * q, &zero, work.p, p, slice, S),
+       "dgemm_strided_batched A'B");
+    int nsl = S;
+    if (rem > 0) {
+      rb(rocblas_dgemm(bh, rocblas_operation_none, rocblas_operation_transpose, p, q, (rocblas_int)rem, &one,
+                       AT + ch * S * p, p, BT + ch * S * q, q, &zero, work.p + (int64_t)S * slice, p),
+         "dgemm A'B rest");
+      nsl = S + 1;
+    }
+    k_sum_slices<<<(unsigned)((slice + 255) / 256), 256>>>(work.p, slice, nsl, C);
+    HIPCHECK(hipGetLastError());
+  }
+};
+
 // xPy = V' Py (V: centred markers, n x k, before scaling; :936), then v = (I - QQ') R^-1/2 V
 // (:938-939), xSx = ||v||^2 (:942) and xPx = v' H^-1 v with one Lanczos seed over all columns
 // at tau1 and L_Solve at shift 0 (:940-941).  V is overwritten by v; dV holds v on exit.
@@ -144,12 +229,14 @@ extern "C" int slemm_gpu_lmm_gstat(slemm_geno* h, const float* X, int64_t ldx, i
     DevT<double> zw((size_t)n * w), G((size_t)w * w), qd((size_t)n * std::max(c, 1)), qz((size_t)c * w),
         z((size_t)w);
     DevT<rocblas_int> info(1);
+    TallProduct tall(n, (int64_t)w * w);
+    DevT<double> zwT((size_t)n * w);
     {
       std::vector<double> qh((size_t)n * c);
       for (size_t e = 0; e < qh.size(); ++e) qh[e] = (double)(float)Q.a[e];  // the handle's FP32 basis
       HIPCHECK(hipMemcpy(qd.p, qh.data(), sizeof(double) * qh.size(), hipMemcpyHostToDevice));
     }
-    const double one = 1.0, zero = 0.0, minus_one = -1.0;
+    const double one = 1.0, minus_one = -1.0;
     for (int i = 0; i < k; ++i) {
       const int64_t j = snps[i];
       int64_t wds;
@@ -160,18 +247,18 @@ extern "C" int slemm_gpu_lmm_gstat(slemm_geno* h, const float* X, int64_t ldx, i
                                                                                h->rr, zw.p, n);
       HIPCHECK(hipGetLastError());
       // Zw <- Zw - Q (Q' Zw)
-      rb(rocblas_dgemm(blas.h, rocblas_operation_transpose, rocblas_operation_none, c, w, (rocblas_int)n, &one, qd.p,
-                       (rocblas_int)n, zw.p, (rocblas_int)n, &zero, qz.p, c), "dgemm Q'Zw");
+      k_qt_a<<<(unsigned)w, 256>>>(qd.p, c, zw.p, n, qz.p);
+      HIPCHECK(hipGetLastError());
       rb(rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, w, c, &minus_one, qd.p,
                        (rocblas_int)n, qz.p, c, &one, zw.p, (rocblas_int)n), "dgemm Zw - Q(Q'Zw)");
-      // G = Zw' Zw + tau1 diag(1/gw); z = Zw' v with v the window's column of SNP j
+      // G = Zw'Zw (from the transposed window) + tau1 diag(1/gw); z = Zw'v with v = Zw's column jj
+      // (the SNP's own column) is G's column jj before the ridge
       const int64_t jj = j - wds;
-      rb(rocblas_dsyrk(blas.h, rocblas_fill_lower, rocblas_operation_transpose, w, (rocblas_int)n, &one, zw.p,
-                       (rocblas_int)n, &zero, G.p, w), "dsyrk");
+      transpose_f64(zw.p, n, w, zwT.p);
+      tall(blas.h, zwT.p, w, zwT.p, w, G.p);
+      HIPCHECK(hipMemcpy(z.p, G.p + jj * w, sizeof(double) * w, hipMemcpyDeviceToDevice));
       k_add_ridge<<<(w + 255) / 256, 256>>>(G.p, w, w, h->gw, wds, (double)tau1);
       HIPCHECK(hipGetLastError());
-      rb(rocblas_dgemv(blas.h, rocblas_operation_transpose, (rocblas_int)n, w, &one, zw.p, (rocblas_int)n,
-                       zw.p + jj * n, 1, &zero, z.p, 1), "dgemv Zw'v");
       std::vector<double> zh(w);
       HIPCHECK(hipMemcpy(zh.data(), z.p, sizeof(double) * w, hipMemcpyDeviceToHost));
       // G^-1 z by Cholesky (Eigen's llt in the reference)
@@ -374,27 +461,30 @@ extern "C" int slemm_gpu_gwa_window(slemm_geno* h, const float* X, int64_t ldx, 
       HIPCHECK(hipMemcpy(qd.p, qh.data(), sizeof(double) * qh.size(), hipMemcpyHostToDevice));
       HIPCHECK(hipMemcpy(dy.p, yh.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     }
-    const int64_t TCH = 4096;  // tested SNPs per batch
+    const int64_t TCH = 1024;  // tested SNPs per batch (a block's range holds about b / 2)
     DevT<int64_t> didx((size_t)std::max<int64_t>(b, TCH));
     DevT<double> Xt((size_t)n * TCH), Zx((size_t)b * TCH), Sx((size_t)b * TCH), dxy(TCH), dxx(TCH);
     DevT<rocblas_int> info(1);
+    TallProduct tall(n, (int64_t)b * std::max<int64_t>(b, TCH));
+    DevT<double> ZbT((size_t)n * b), XtT((size_t)n * TCH);
     // centre, scale by R^-1/2 and project out Q (in place on an n x t FP64 block)
     auto prepare = [&](double* A, int t) {
       k_scale_rows<<<dim3((unsigned)((n + 255) / 256), (unsigned)t), 256>>>(A, n, t, h->rr);
       HIPCHECK(hipGetLastError());
-      rb(rocblas_dgemm(blas.h, rocblas_operation_transpose, rocblas_operation_none, cq, t, (rocblas_int)n, &one, qd.p,
-                       (rocblas_int)n, A, (rocblas_int)n, &zero, qz.p, cq), "dgemm Q'A");
+      k_qt_a<<<(unsigned)t, 256>>>(qd.p, cq, A, n, qz.p);
+      HIPCHECK(hipGetLastError());
       rb(rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, t, cq, &minus_one,
                        qd.p, (rocblas_int)n, qz.p, cq, &one, A, (rocblas_int)n), "dgemm A - Q(Q'A)");
     };
     for (int blk = 0; blk < nblk; ++blk) {
+      if (lo[blk] >= hi[blk]) continue;  // no tested SNP in this block's range
       // Zb and H = Zb'Zb + model_snp_ct tau rvars (:363-383, :387)
       HIPCHECK(hipMemcpy(didx.p, blocks[blk].data(), sizeof(int64_t) * b, hipMemcpyHostToDevice));
       k_extract_f64<<<dim3((unsigned)((n + 255) / 256), (unsigned)b), 256>>>(h->gs, h->ld_gs, didx.p, n, h->gm, Zb.p);
       HIPCHECK(hipGetLastError());
       prepare(Zb.p, b);
-      rb(rocblas_dsyrk(blas.h, rocblas_fill_lower, rocblas_operation_transpose, b, (rocblas_int)n, &one, Zb.p,
-                       (rocblas_int)n, &zero, H.p, b), "dsyrk");
+      transpose_f64(Zb.p, n, b, ZbT.p);
+      tall(blas.h, ZbT.p, b, ZbT.p, b, H.p);
       std::vector<double> dh(b);
       for (int k = 0; k < b; ++k) {
         const float mu = gmh[blocks[blk][k]];
@@ -419,8 +509,8 @@ extern "C" int slemm_gpu_gwa_window(slemm_geno* h, const float* X, int64_t ldx, 
         rb(rocblas_dgemv(blas.h, rocblas_operation_transpose, (rocblas_int)n, t, &one, Xt.p, (rocblas_int)n, dy.p, 1,
                          &zero, dxy.p, 1), "dgemv X'y");
         prepare(Xt.p, t);
-        rb(rocblas_dgemm(blas.h, rocblas_operation_transpose, rocblas_operation_none, b, t, (rocblas_int)n, &one,
-                         Zb.p, (rocblas_int)n, Xt.p, (rocblas_int)n, &zero, Zx.p, b), "dgemm Zb'X");
+        transpose_f64(Xt.p, n, t, XtT.p);
+        tall(blas.h, ZbT.p, b, XtT.p, t, Zx.p);
         HIPCHECK(hipMemcpy(Sx.p, Zx.p, sizeof(double) * b * t, hipMemcpyDeviceToDevice));
         rb(rocsolver_dpotrs(blas.h, rocblas_fill_lower, b, t, H.p, b, Sx.p, b), "dpotrs");
         k_window_xx<<<t, 256>>>(Xt.p, n, Zx.p, Sx.p, b, (double)tau, dxx.p);

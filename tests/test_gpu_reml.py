@@ -218,19 +218,21 @@ def test_iter_weighting_against_restatement(gpu):
     assert rel_err(got["snp_blup"][sig], ref["snp_blup"][sig]) < 1e-2
 
 
-def test_gwa_window_scan_against_fp64(gpu):
+@pytest.mark.parametrize("n,m", [(900, 1300), (5001, 700)])
+def test_gwa_window_scan_against_fp64(gpu, n, m):
     """Windowed score tests of slemm_gwa step 2 (bin/slemm_gwa.py:322-407) vs the FP64
-    restatement: overlapping blocks of model SNPs, every SNP of the range tested, MAC filter."""
+    restatement: overlapping blocks of model SNPs, every SNP of the range tested, MAC filter.
+    n = 5001 splits the block Gram products into two row chunks plus one leftover row."""
     from oracle import gwas as ogwas
     from slemm_amd import gwas
-    p = make_problem(n=900, m=1300, seed=15, weights=True, c=2, missing=0.0)
+    p = make_problem(n=n, m=m, seed=15, weights=True, c=2, missing=0.0)
     g = gpu.Geno.from_bed(p.rows, p.n)
     g.set_model(None, p.rvec)
     rng = np.random.default_rng(3)
     y = rng.normal(size=p.n).astype(np.float32)
     ms = np.sort(rng.choice(p.m, size=500, replace=False))
     tau, ct, mac = 1.8, 500, 15.0
-    for bs, (t0, t1) in ((100, (0, p.m - 1)), (64, (200, 900))):
+    for bs, (t0, t1) in ((100, (0, p.m - 1)), (64, (200, min(900, p.m - 1)))):
         xy, xx = g.gwa_window(p.X, y, ms, bs, t0, t1, tau, ct, mac)
         rxy, rxx = ogwas.gwa_window_scores(p.Z, y, p.rr, p.Q, ms, bs, t0, t1, tau, ct, mac)
         assert np.array_equal(np.isnan(xx), np.isnan(rxx)) and np.isfinite(rxx).sum() > 0.8 * (t1 - t0 + 1)
