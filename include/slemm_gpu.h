@@ -349,7 +349,7 @@ int slemm_gpu_lmm_gstat_fake(slemm_geno* h, const float* X, int64_t ldx, int c, 
  * half; block b tests the SNPs from the previous block's bound to its own (model SNP
  * block_size - block_size/4 of the block), the first from test_begin and the last to test_end
  * (inclusive, handle SNP indices).  Per block H = Zb'Zb + model_snp_ct tau diag((2-p)p/2) in
- * FP64 (rocBLAS DSYRK, rocSOLVER Cholesky); per tested SNP x (centred by its mean dosage)
+ * FP64 (batched rocBLAS DGEMMs, rocSOLVER Cholesky); per tested SNP x (centred by its mean dosage)
  * xy = x'y and, with x scaled by R^-1/2 and projected off Q(X), xx = (x'x - z'H^-1 z)/tau,
  * z = Zb'x.  SNPs whose allele count is <= mac either way are NaN.  Host inputs X (n x c), y (n:
  * the phenotype column of the fit's output); host outputs xy, xx (test_end - test_begin + 1). */
