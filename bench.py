@@ -264,7 +264,11 @@ def next_rows_run(g, n, m, y, sl, loader_m=40_000):
     print(f"# next rows: --iter_weighting {dt:.2f} s", file=sys.stderr, flush=True)
     g.set_model()
 
-    tmp = tempfile.mkdtemp(prefix="slemm_bench_")
+    try:
+        tmp = tempfile.mkdtemp(prefix="slemm_bench_")
+    except OSError as e:
+        out["bed_loader"] = out["pred"] = {"error": repr(e)}
+        return out
     try:
         nb = (n + 3) // 4
         prefix = os.path.join(tmp, "syn")
@@ -299,6 +303,9 @@ def next_rows_run(g, n, m, y, sl, loader_m=40_000):
         dt = time.perf_counter() - t0
         out["pred"] = {"individuals": int(npred), "snps": int(mused), "seconds": round(dt, 3),
                        "note": "estimate/fam/bim parsing, bed streaming, FP64 GEBV and IID,GEBV output"}
+    except OSError as e:  # e.g. no room for the 1 GB file: the two file rows are reported missing
+        out.setdefault("bed_loader", {"error": repr(e)})
+        out.setdefault("pred", {"error": repr(e)})
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     return out
@@ -600,7 +607,11 @@ def main():
             diff = differentiation_run(g, n, m, y, float(sl["vg"]), float(sl["ve"]), nv=nv)
         if not args.no_next:
             print("# next rows: --lmm, windowed scan, --iter_weighting, loader, --pred", file=sys.stderr, flush=True)
-            nxt = next_rows_run(g, n, m, y, sl)
+            try:
+                nxt = next_rows_run(g, n, m, y, sl)
+            except Exception as e:  # reported, never fatal to the bench line
+                nxt = {"error": repr(e)}
+                g.set_model()
     smallk = None if args.no_smallk else small_k_passes(g, lib, L, n, m)
 
     ceiling = None if args.no_ceiling else gemm_ceiling()
@@ -664,14 +675,21 @@ def main():
         del V
         torch.cuda.empty_cache()
         print(f"# scale pipeline n={args.scale_n} m={args.scale_m}", file=sys.stderr, flush=True)
-        result["scale_pipeline"] = scale_run(args.scale_n, args.scale_m, 100, 2000, 0.4)
+        try:
+            result["scale_pipeline"] = scale_run(args.scale_n, args.scale_m, 100, 2000, 0.4)
+        except Exception as e:  # reported, never fatal to the bench line
+            result["scale_pipeline"] = {"error": repr(e)}
+            torch.cuda.empty_cache()
     if not args.no_traits:
         if args.no_scale:
             g.close()
             del V
         torch.cuda.empty_cache()
         print(f"# batched traits n={args.traits_n} m={args.traits_m} T={args.traits_T}", file=sys.stderr, flush=True)
-        result["batched_traits"] = traits_run(args.traits_n, args.traits_m, args.traits_T, 32)
+        try:
+            result["batched_traits"] = traits_run(args.traits_n, args.traits_m, args.traits_T, 32)
+        except Exception as e:  # reported, never fatal to the bench line
+            result["batched_traits"] = {"error": repr(e)}
     print(json.dumps(result), flush=True)
 
 
