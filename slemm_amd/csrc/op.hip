@@ -14,9 +14,11 @@ namespace slemm {
 Layout layout_for(const slemm_geno* h, int t, int precision) {
   Layout L;
   L.NT = blockprod_nt(t);
-  // Wide split-BF16 blocks (t > 128) run as 80-column groups: the pipelined kernel keeps two
-  // waves per SIMD at 5 column tiles, where 8-tile groups drop to one (SLEMM_WIDE_NT overrides).
-  if (precision == PREC_BF16X2 && use_lds(precision) && t > 128) {
+  // Wide split-BF16 blocks (t > 80) run as 80-column groups: the pipelined kernels take at most
+  // 5 column tiles (two waves per SIMD), and 6-8-tile groups would fall back to the unpipelined
+  // register kernel (SLEMM_WIDE_NT overrides).  Each group reads the codes again; these blocks
+  // are MFMA-bound, so the extra reads cost no time.
+  if (precision == PREC_BF16X2 && use_lds(precision) && t > 80) {
     static const char* e = std::getenv("SLEMM_WIDE_NT");
     const int nt = e ? std::atoi(e) : 5;
     L.NT = nt >= 1 && nt <= 8 ? nt : 5;

@@ -80,7 +80,7 @@ def test_op_ldet_and_quadform(gpu, prec, t):
 BP_VARIANTS = 10  # block-product kernel variants selectable with SLEMM_BP_VARIANT
 
 
-@pytest.mark.parametrize("t", [1, 3, 5, 12, 16, 33, 65, 66, 161])
+@pytest.mark.parametrize("t", [1, 3, 5, 12, 16, 33, 65, 66, 100, 161])
 def test_block_product_variants(gpu, t, monkeypatch):
     """Every split-BF16 block-product variant at one reduction split and at several, for both
     split layouts (hi and lo halves of narrow blocks or of the last tile as columns, or as two
