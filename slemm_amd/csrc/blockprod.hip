@@ -345,12 +345,23 @@ namespace slemm {
 
 // ------------------------------------------------------------------ pack (pass-1 operand) ----
 // x_ik = rr_i * (v_ik - sum_q Q_iq a_qk)   (projection only for QUADFORM columns, a = Q'V)
-// Thread (tx = column within 64, ty): 8 blocks of 8 rows; block covers 256 rows x 64 columns.
+// A block covers 256 rows x 64 columns.  The block's V tile is first staged in LDS with
+// row-contiguous reads (thread = row, one column per load), then thread (tx = column within 64,
+// ty) packs 8 blocks of 8 rows of its column, so the packed writes stay column-adjacent too.
 __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
   __shared__ double red[4][64];
+  __shared__ float vt[64][257];  // [column][row]; the pad makes the column-wise reads conflict-free
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int k = blockIdx.x * 64 + tx;
-  const int64_t rbase = (int64_t)blockIdx.y * 256 + ty * 64;
+  const int64_t rblk = (int64_t)blockIdx.y * 256;
+  const int64_t rbase = rblk + ty * 64;
+  {
+    const int64_t i = rblk + threadIdx.x;
+    const int kmax = min(64, a.t - (int)blockIdx.x * 64);
+    for (int kk = 0; kk < 64; ++kk)
+      vt[kk][threadIdx.x] = (kk < kmax && i < a.n) ? a.V[i + (int64_t)(blockIdx.x * 64 + kk) * a.ldv] : 0.f;
+  }
+  __syncthreads();
   double csum = 0.0;
   const bool live = k < a.t;
   const bool proj = live && a.qtv && (a.kinds == nullptr || a.kinds[k] == 1) && a.c > 0;
@@ -371,7 +382,7 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
         const int64_t i = i0 + e;
         float val = 0.f;
         if (live && i < a.n && (!tmask || tmask[i] != 0.f)) {
-          float v = a.V[i + (int64_t)k * a.ldv];
+          float v = vt[tx][i - rblk];
           if (proj) {
             double acc = 0.0;
             for (int q = 0; q < a.c; ++q) acc += (double)Qk[i + (int64_t)q * a.ldq] * a.qtv[q * a.KP + k];
