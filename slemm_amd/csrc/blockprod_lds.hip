@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 
 #include "blockprod.hpp"
 
@@ -307,7 +308,7 @@ __device__ __forceinline__ void fill_byte_table(uint2* lut8) {
 template <int RT, int NT, bool SQ, int LUT = 0, bool HL = false, int WPG = 4, bool FULL = false, int SBC = 0>
 __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64_t bz, const uint2* lut8 = nullptr,
                                              uint4* sb = nullptr) {
-  static_assert(SBC == 0 || (HL && FULL), "staged B is for hi/lo-in-columns blocks with whole-line codes");
+  static_assert(SBC == 0 || (FULL && (HL || NT == 1)), "staged B is for one-tile blocks with whole-line codes");
   static_assert(!HL || NT == 1, "hi/lo-in-columns blocks have one column tile");
   // the wave index is uniform across the wave: keep it (and the tile limits) in scalar registers
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -401,12 +402,20 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
     const int64_t sc = T >> 4;  // super-chunk of this step
     if constexpr (SBC > 0) {
       // staged B: piece (g, step t8 & 7, live column r % SBC) of this half's buffer; lanes of
-      // dead columns read a live column's piece, which only reaches unread outputs
-      const bf16x8_t bv =
-          __builtin_bit_cast(bf16x8_t, sb[((T >> 3) & 1) * 32 * SBC + (g * 8 + (T & 7)) * SBC + r % SBC]);
-      if (sc < lim[0])
+      // dead columns read a live column's piece, which only reaches unread outputs.  Blocks with
+      // hi and lo as two tiles keep the lo pieces 2 x 32 x SBC pieces further on.
+      const int pi = ((T >> 3) & 1) * 32 * SBC + (g * 8 + (T & 7)) * SBC + r % a.ncols;
+      const bf16x8_t bv = __builtin_bit_cast(bf16x8_t, sb[pi]);
+      if (sc < lim[0]) {
 #pragma unroll
         for (int i = 0; i < RT; ++i) acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv, acc[i][0], 0, 0, 0);
+        if constexpr (!HL) {
+          const bf16x8_t bl2 = __builtin_bit_cast(bf16x8_t, sb[2 * 32 * SBC + pi]);
+#pragma unroll
+          for (int i = 0; i < RT; ++i)
+            acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bl2, acc[i][0], 0, 0, 0);
+        }
+      }
       return;
     }
     const int slot = t8 % D;
@@ -454,22 +463,29 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
     constexpr int NP = SBC > 0 ? (32 * SBC + 64 * WPG - 1) / (64 * WPG) : 1;
     const int nc = a.ncols;
     const int64_t hlast = 2 * s1 - 1;
-    uint4 stg[NP];
+    uint4 stg[NP], stl[HL ? 1 : NP];
     auto bload = [&](int64_t hc) {
       hc = hc <= hlast ? hc : hlast;
 #pragma unroll
       for (int q = 0; q < NP; ++q) {
         const int p = threadIdx.x + 64 * WPG * q;
         const int row = p / nc;
-        if (p < 32 * nc)
-          stg[q] = xh[((hc >> 1) * 64 + (row >> 3) * 16 + (hc & 1) * 8 + (row & 7)) * a.KP + p % nc];
+        if (p < 32 * nc) {
+          const int64_t o = ((hc >> 1) * 64 + (row >> 3) * 16 + (hc & 1) * 8 + (row & 7)) * a.KP + p % nc;
+          stg[q] = xh[o];
+          if constexpr (!HL) stl[q] = xl[o];
+        }
       }
     };
     auto bstore = [&](int64_t hc) {
 #pragma unroll
       for (int q = 0; q < NP; ++q) {
         const int p = threadIdx.x + 64 * WPG * q;
-        if (p < 32 * nc) sb[(hc & 1) * 32 * SBC + (p / nc) * SBC + p % nc] = stg[q];
+        if (p < 32 * nc) {
+          const int o = (hc & 1) * 32 * SBC + (p / nc) * SBC + p % nc;
+          sb[o] = stg[q];
+          if constexpr (!HL) sb[2 * 32 * SBC + o] = stl[q];
+        }
       }
     };
     if (s0 < s1) {
@@ -552,7 +568,7 @@ __device__ __forceinline__ void bp_pipe_tile(const LdsArgs& a, int64_t bx, int64
 // hi/lo-in-columns blocks (HL) run with whole-line codes and B staged in LDS (SBC live columns)
 template <int RT, int NT, bool SQ, bool HL = false, int SBC = 0>
 __global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
-  __shared__ uint4 sb[SBC > 0 ? 2 * 32 * SBC : 1];
+  __shared__ uint4 sb[SBC > 0 ? (HL ? 2 : 4) * 32 * SBC : 1];
   bp_pipe_tile<RT, NT, SQ, 0, HL, 4, (SBC > 0), SBC>(a, blockIdx.x, blockIdx.z, nullptr, sb);
 }
 
@@ -560,7 +576,7 @@ __global__ __launch_bounds__(256, 2) void k_bp_pipe(LdsArgs a) {
 template <int RT, int NT, bool SQ, int R = 1, bool HL = false, int SBC = 0>
 __global__ __launch_bounds__(256, 2) void k_bp_pipel(LdsArgs a) {
   __shared__ uint2 lut8[256 * R];
-  __shared__ uint4 sb[SBC > 0 ? 2 * 32 * SBC : 1];
+  __shared__ uint4 sb[SBC > 0 ? (HL ? 2 : 4) * 32 * SBC : 1];
   fill_byte_table<SQ, R>(lut8);
   __syncthreads();
   bp_pipe_tile<RT, NT, SQ, R, HL, 4, NT == 1, SBC>(a, blockIdx.x, blockIdx.z, lut8, sb);
@@ -571,7 +587,7 @@ __global__ __launch_bounds__(256, 2) void k_bp_pipel(LdsArgs a) {
 template <int RT, int NT, bool SQ, bool HL = false, int SBC = 0>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(HL ? 4 : 2))) void k_bp_pipew(LdsArgs a) {
   __shared__ uint2 lut8[256 * 32];
-  __shared__ uint4 sb[SBC > 0 ? 2 * 32 * SBC : 1];
+  __shared__ uint4 sb[SBC > 0 ? (HL ? 2 : 4) * 32 * SBC : 1];
   fill_byte_table<SQ, 32, 512>(lut8);
   __syncthreads();
   bp_pipe_tile<RT, NT, SQ, 32, HL, 8, (SBC > 0), SBC>(a, blockIdx.x, blockIdx.z, lut8, sb);
@@ -758,6 +774,28 @@ static void launch_variant_hl(const LdsArgs& a, int variant, dim3 grid, hipStrea
   }
 }
 
+// SLEMM_STAGED_ONE_TILE=0 keeps one-tile hi/lo blocks on the per-step B loads (read per call)
+static bool staged_one_tile() {
+  const char* e = std::getenv("SLEMM_STAGED_ONE_TILE");
+  return !(e && e[0] == '0');
+}
+
+// one-tile blocks with hi and lo as two tiles (up to 16 live columns): the pipelined variants
+// with the block's live B columns staged in LDS per half super-chunk, as for hi/lo-in-columns
+// blocks; the other variants run their usual kernels
+template <int SBC>
+static void launch_variant_nt1s(const LdsArgs& a, int variant, dim3 grid, hipStream_t st) {
+  switch (variant) {
+    case BP_PIPE: k_bp_pipe<kLdsRT, 1, false, false, SBC><<<grid, 256, 0, st>>>(a); break;
+    case BP_PIPEL: k_bp_pipel<kLdsRT, 1, false, 1, false, SBC><<<grid, 256, 0, st>>>(a); break;
+    case BP_PIPER16: k_bp_pipel<kLdsRT, 1, false, 16, false, SBC><<<grid, 256, 0, st>>>(a); break;
+    case BP_PIPER32:
+      k_bp_pipew<kLdsRT, 1, false, false, SBC><<<dim3(grid.x / 2, grid.y, grid.z), 512, 0, st>>>(a);
+      break;
+    default: launch_variant<1, false>(a, variant, grid, st);
+  }
+}
+
 void launch_bp(const LdsArgs& a, int NT, int nsplit, int variant, hipStream_t st) {
   if (a.rows_pad % (64 * kLdsRT) != 0) fail(SLEMM_ERR_ARG, "row padding mismatch");
   // a tail hi/lo tile is only implemented by the pipelined kernels (NT <= 5)
@@ -783,6 +821,9 @@ void launch_bp(const LdsArgs& a, int NT, int nsplit, int variant, hipStream_t st
   } else if (a.squared) {
     if (NT != 1) fail(SLEMM_ERR_ARG, "squared-dosage products support one column tile");
     launch_variant<1, true>(a, variant, grid, st);
+  } else if (NT == 1 && a.tail_tile < 0 && a.ncols >= 1 && a.ncols <= 16 && staged_one_tile()) {
+    if (a.ncols <= 8) launch_variant_nt1s<8>(a, variant, grid, st);
+    else launch_variant_nt1s<16>(a, variant, grid, st);
   } else {
     switch (NT) {
       case 1: launch_variant<1, false>(a, variant, grid, st); break;
