@@ -93,12 +93,14 @@ def cpu_baseline(g, n, m, K, m_sample, reps):
     }
 
 
-def small_k_passes(g, lib, L, n, m, ks=(1, 3, 7, 16), reps=3):
+def small_k_passes(g, lib, L, n, m, ks=(1, 3, 5, 7, 16), reps=3):
     """HBM-bound regime (K*P <~ 39): Z'V and Z U passes at small K (GEBV K=1, GWAS K=1+c+1,
     LMC K=1+2*3, ...), kernel time from the library's HIP events; plus the FP64 GEBV kernel."""
     import torch
 
     out = {}
+    # blocks of 1 to 5 vectors run on the int8 kernels unless SLEMM_I8_NARROW=0
+    i8_on = os.environ.get("SLEMM_I8_NARROW", "1") != "0"
     for K in ks:
         V = torch.randn(n, K, device="cuda")
         U = torch.randn(m, K, device="cuda")
@@ -121,7 +123,8 @@ def small_k_passes(g, lib, L, n, m, ks=(1, 3, 7, 16), reps=3):
                          "pass1_GB/s": round(b / p1 / 1e6, 1), "pass2_GB/s": round(b / p2 / 1e6, 1),
                          "pass1_hbm_frac": round(b / p1 / 1e6 / HBM_PEAK_GBS, 3),
                          "pass2_hbm_frac": round(b / p2 / 1e6 / HBM_PEAK_GBS, 3),
-                         "kernels": [at["pass1"][:2], at["pass2"][:2]]}
+                         "kernels": ([["int8", "default"], ["int8", "default"]] if K <= 5 and i8_on
+                                     else [at["pass1"][:2], at["pass2"][:2]])}
         if (n, m, K) in MEASURED_TRAFFIC:
             out[f"K={K}"]["traffic_bytes"] = MEASURED_TRAFFIC[(n, m, K)]
     alpha = torch.randn(m, dtype=torch.float64, device="cuda")

@@ -182,6 +182,29 @@ int lds_nt(int t);
 // number of reduction splits so that the grid has about target_wgs workgroups
 int lds_splits(int64_t rows_pad, int64_t nsuper, int64_t target_wgs, int64_t& ns_per);
 void launch_bp(const LdsArgs& a, int NT, int nsplit, int variant, hipStream_t st);
+
+// Narrow split-BF16 blocks (1 to 5 vectors, hi and lo halves as columns of one 16-column tile)
+// on the int8 matrix cores: the operand X (hi + lo per element) is rendered per column as three
+// int8 digits of a 23-bit fixed-point value in the scratch buffer, and the block product writes
+// the split partials of column k (and zeros in the lo-half column t + k) into part, as the
+// split-BF16 kernels do.  red_pad: reduction length (rows of X, columns of G's packed rows).
+constexpr int kI8RT = 8;  // 16-row tiles per wave; a workgroup covers 512 rows
+struct I8Launch {
+  const uint8_t* G = nullptr;
+  int64_t ldg = 0;
+  int64_t red_pad = 0, rows_pad = 0;
+  const void* X = nullptr;
+  int KP = 0, t = 0;
+  bool squared = false;
+  int nsplit = 1;
+  int64_t ns_per_split = 0;
+  void* scratch = nullptr;  // i8_scratch_bytes(red_pad, t) bytes
+  float* part = nullptr;    // nsplit x KP x rows_pad
+};
+bool i8_narrow_enabled();
+size_t i8_scratch_bytes(int64_t red_pad, int t);
+int i8_splits(int64_t rows_pad, int64_t nsuper, int64_t target_wgs, int64_t& ns_per);
+void launch_bp_i8(const I8Launch& l, hipStream_t st);
 void launch_fin1(const Fin1Args& a, hipStream_t st);
 void launch_fin2(const Fin2Args& a, hipStream_t st);
 

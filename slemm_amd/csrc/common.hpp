@@ -96,6 +96,7 @@ struct slemm_geno {
   slemm::DevBuf kinds;         // per-column operator kinds
   slemm::DevBuf redpart;       // partials of column reductions (double)
   slemm::DevBuf partbuf;       // split-reduction partial products of the block products
+  slemm::DevBuf i8buf;         // digit operand and column scales of the int8 narrow blocks
   std::vector<int> kinds_host;
   // block-product variant and grid target per (pass, slot: 0 = hi/lo-in-columns block, else column
   // tiles), set by the autotuner; the defaults are the kernels it picks on MI355X at 100k x 500k:

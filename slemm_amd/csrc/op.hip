@@ -132,6 +132,31 @@ void tail_args(const Layout& L, int t, LdsArgs& la) {
   la.last_tile = L.hl ? 0 : (L.tail_r > 0 ? L.tail0 / 16 : (t + 15) / 16 - 1);
 }
 
+// Narrow blocks of 1 to 5 vectors run on the int8 kernels (SLEMM_I8_NARROW=0: the split-BF16
+// ones); batched-trait blocks, whose tiles carry per-column reduction limits, keep the latter.
+// SLEMM_I8_TARGET sets the workgroup target of their reduction split (read per call; by default
+// 512 for the SNP-major pass and 1024 for the individual-major one, the best of 512 to 8192 at
+// n = 100,000 x m = 500,000).
+static bool use_i8(const Layout& L, int t, const TraitCols* tc) {
+  return L.hl && 3 * t <= 16 && !tc && i8_narrow_enabled();
+}
+
+static int i8_pass_splits(int pass, int64_t rows_pad, int64_t nsuper, int64_t& nsp) {
+  const char* e = std::getenv("SLEMM_I8_TARGET");
+  const int64_t target = e ? std::max<int64_t>(1, std::atoll(e)) : (pass == 0 ? 512 : 1024);
+  return i8_splits(rows_pad, nsuper, target, nsp);
+}
+
+static void bp_i8(slemm_geno* h, const uint8_t* G, int64_t ldg, int64_t red_pad, int64_t rows_pad, const void* X,
+                  const Layout& L, int t, bool squared, int S, int64_t nsp, hipStream_t st) {
+  h->i8buf.reserve(i8_scratch_bytes(red_pad, t));
+  I8Launch l;
+  l.G = G; l.ldg = ldg; l.red_pad = red_pad; l.rows_pad = rows_pad; l.X = X; l.KP = L.KP; l.t = t;
+  l.squared = squared; l.nsplit = S; l.ns_per_split = nsp; l.scratch = h->i8buf.p;
+  l.part = h->partbuf.as<float>();
+  launch_bp_i8(l, st);
+}
+
 int64_t pass_splits(const slemm_geno* h, int pass, int slot, int64_t& nsp) {
   const BpChoice c = bp_choice(h, pass, slot);
   return pass == 0 ? lds_splits(h->m_pad, h->n_pad / kPad, c.target, nsp)
@@ -143,7 +168,8 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
                int64_t ldo, hipStream_t st, bool raw, bool squared, const TraitCols* tc) {
   int64_t nsp = 0;
   const int64_t nsuper = h->n_pad / kPad;
-  const int S = (int)pass_splits(h, 0, L.slot, nsp);
+  const bool i8 = use_i8(L, t, tc);
+  const int S = i8 ? i8_pass_splits(0, h->m_pad, nsuper, nsp) : (int)pass_splits(h, 0, L.slot, nsp);
   h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * h->m_pad);
   LdsArgs la;
   la.G = h->gs; la.ldg = h->ld_gs; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x1.p;
@@ -152,7 +178,8 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
   tail_args(L, t, la);
   if (tc) la.tile_nsuper = tc->tile_nsuper;  // reduction over individuals stops at the subsets' end
   TimingToken tok = timing_begin(st);
-  launch_bp(la, L.NT, S, bp_choice(h, 0, L.slot).variant, st);
+  if (i8) bp_i8(h, h->gs, h->ld_gs, h->n_pad, h->m_pad, h->x1.p, L, t, squared, S, nsp, st);
+  else launch_bp(la, L.NT, S, bp_choice(h, 0, L.slot).variant, st);
   timing_end(tok, st, EPI_PASS1, PREC_BF16X2);
   Fin1Args f;
   f.part = h->partbuf.as<float>(); f.nsplit = S; f.rows_pad = h->m_pad; f.rows = h->m; f.KP = L.KP; f.t = t;
@@ -169,7 +196,8 @@ void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const flo
                const TraitCols* tc) {
   int64_t nsp = 0;
   const int64_t nsuper = h->m_pad / kPad;
-  const int S = (int)pass_splits(h, 1, L.slot, nsp);
+  const bool i8 = use_i8(L, t, tc);
+  const int S = i8 ? i8_pass_splits(1, h->n_pad, nsuper, nsp) : (int)pass_splits(h, 1, L.slot, nsp);
   h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * h->n_pad);
   LdsArgs la;
   la.G = h->gi; la.ldg = h->ld_gi; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x2.p;
@@ -177,7 +205,8 @@ void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const flo
   tail_args(L, t, la);
   if (tc) la.tile_rows = tc->tile_rows;  // output rows (individuals) past the subsets' end are zero
   TimingToken tok = timing_begin(st);
-  launch_bp(la, L.NT, S, bp_choice(h, 1, L.slot).variant, st);
+  if (i8) bp_i8(h, h->gi, h->ld_gi, h->m_pad, h->n_pad, h->x2.p, L, t, false, S, nsp, st);
+  else launch_bp(la, L.NT, S, bp_choice(h, 1, L.slot).variant, st);
   timing_end(tok, st, EPI_PASS2, PREC_BF16X2);
   Fin2Args f;
   f.part = h->partbuf.as<float>(); f.nsplit = S; f.rows_pad = h->n_pad; f.rows = h->n; f.KP = L.KP; f.t = t;

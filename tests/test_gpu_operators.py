@@ -95,6 +95,7 @@ def test_block_product_variants(gpu, t, monkeypatch):
     tau0 = 0.37
     ref = D.op("ldet", tau0, V)
     zref = D.rmatvec(V, weighted=True)
+    monkeypatch.setenv("SLEMM_I8_NARROW", "0")  # narrow blocks on the split-BF16 kernels
     for hl in ("1", "0"):
         monkeypatch.setenv("SLEMM_HL_COLUMNS", hl)
         for v in range(BP_VARIANTS):
@@ -105,6 +106,44 @@ def test_block_product_variants(gpu, t, monkeypatch):
                 assert rel_err(out, ref) < TOL["bf16x2"], (hl, v, target, rel_err(out, ref))
                 zt = g.weighted_rmatvec(Vd, precision="bf16x2").cpu().numpy()
                 assert rel_err(zt, zref) < TOL["bf16x2"], (hl, v, target, rel_err(zt, zref))
+
+
+@pytest.mark.parametrize("t", [1, 2, 3, 4, 5, 6])
+def test_int8_narrow_blocks(gpu, t, monkeypatch):
+    """Narrow blocks (1 to 5 vectors) on the int8 kernels, at one reduction split and at several
+    (including an odd number of super-chunks per split), against the FP64 dense operator, the
+    transposed product and the product, and against the split-BF16 kernels on the same operand;
+    a zero column and a column whose largest magnitude is a power of two exercise the scale
+    choice.  t = 6 takes the split-BF16 path either way."""
+    p = make_problem(n=2600, m=1700, seed=23, weights=True, c=2)
+    g = _geno(gpu, p)
+    g.set_model(None, p.rvec, p.Q)
+    D = p.dense()
+    rng = np.random.default_rng(200 + t)
+    V = rng.normal(size=(p.n, t)).astype(np.float32)
+    if t >= 3:
+        V[:, 1] = 0.0
+        V[:, 2] *= 4.0 / np.abs(V[:, 2]).max()
+    U = rng.normal(size=(p.m, t)).astype(np.float32)
+    Vd, Ud = torch.from_numpy(V).cuda(), torch.from_numpy(U).cuda()
+    tau0 = 0.37
+    ref = D.op("ldet", tau0, V)
+    zref = D.rmatvec(V, weighted=True)
+    uref = D.matvec(U)
+    monkeypatch.setenv("SLEMM_I8_NARROW", "0")
+    base = g.lnr_op_select("ldet", tau0, Vd, precision="bf16x2").cpu().numpy()
+    monkeypatch.setenv("SLEMM_I8_NARROW", "1")
+    for target in ("1", "3", "100000"):
+        monkeypatch.setenv("SLEMM_I8_TARGET", target)
+        out = g.lnr_op_select("ldet", tau0, Vd, precision="bf16x2").cpu().numpy()
+        assert rel_err(out, ref) < TOL["bf16x2"], (target, rel_err(out, ref))
+        assert rel_err(out, base) < TOL["bf16x2"], (target, rel_err(out, base))
+        zt = g.weighted_rmatvec(Vd, precision="bf16x2").cpu().numpy()
+        assert rel_err(zt, zref) < TOL["bf16x2"], (target, rel_err(zt, zref))
+        if t >= 3:
+            assert np.all(np.abs(zt[:, 1]) <= 1e-6 * np.abs(zt).max())
+        z = g.matvec(Ud, precision="bf16x2").cpu().numpy()
+        assert rel_err(z, uref) < TOL["bf16x2"], (target, rel_err(z, uref))
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
