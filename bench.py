@@ -26,6 +26,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA
+INT8_PEAK_TOPS = 5000.0     # dense int8 MFMA
 FP32_PEAK_TFLOPS = 157.3
 
 
@@ -36,6 +37,8 @@ FP32_PEAK_TFLOPS = 157.3
 # Infinity Cache hits.
 MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 15.14e9 + 0.321e9, "pass2": 17.44e9 + 0.193e9},
                     (100_000, 500_000, 1): {"pass1": 12.64e9 + 0.064e9, "pass2": 12.71e9 + 0.039e9}}
+# The same for the int8 kernels (wide at K=65, narrow at K=1); filled in from the counter runs.
+I8_MEASURED_TRAFFIC = {}
 
 
 def issue_factor(K: int) -> float:
@@ -45,6 +48,25 @@ def issue_factor(K: int) -> float:
     q, r = divmod(K, 16)
     tiles = 2 * q + (0 if r == 0 else (1 if r <= 8 else 2))
     return 16 * tiles / K
+
+
+def i8_issue_factor(K: int) -> float:
+    """Int8 MFMA columns issued per algorithmic column: three digit columns per vector, in tiles
+    of 16 vectors (48 columns), and the up to 5 vectors after the last full tile in one 16-column
+    tail tile (blocks of 1 to 5 vectors: one tile)."""
+    if K <= 5:
+        return 16.0 / K
+    q, r = divmod(K, 16)
+    tiles = 3 * q + 1 if 1 <= r <= 5 else 3 * ((K + 15) // 16)
+    return 16.0 * tiles / K
+
+
+def i8_on(K: int) -> bool:
+    """Whether a K-vector block runs on the int8 kernels (SLEMM_I8_NARROW / SLEMM_I8_WIDE); blocks
+    of 6 to 8 vectors stay on the BF16 kernels with hi and lo halves as columns."""
+    if 6 <= K <= 8 and os.environ.get("SLEMM_HL_COLUMNS", "1") != "0":
+        return False
+    return os.environ.get("SLEMM_I8_NARROW" if K <= 5 else "SLEMM_I8_WIDE", "1") != "0"
 
 
 def pass_bytes(n, m, K):
@@ -99,12 +121,10 @@ def small_k_passes(g, lib, L, n, m, ks=(1, 3, 5, 7, 16), reps=3):
     import torch
 
     out = {}
-    # blocks of 1 to 5 vectors run on the int8 kernels unless SLEMM_I8_NARROW=0
-    i8_on = os.environ.get("SLEMM_I8_NARROW", "1") != "0"
     for K in ks:
         V = torch.randn(n, K, device="cuda")
         U = torch.randn(m, K, device="cuda")
-        at = g.autotune(K, reps=2)
+        at = None if i8_on(K) else g.autotune(K, reps=2)
         g.rmatvec(V)
         g.matvec(U)
         torch.cuda.synchronize()
@@ -123,9 +143,9 @@ def small_k_passes(g, lib, L, n, m, ks=(1, 3, 5, 7, 16), reps=3):
                          "pass1_GB/s": round(b / p1 / 1e6, 1), "pass2_GB/s": round(b / p2 / 1e6, 1),
                          "pass1_hbm_frac": round(b / p1 / 1e6 / HBM_PEAK_GBS, 3),
                          "pass2_hbm_frac": round(b / p2 / 1e6 / HBM_PEAK_GBS, 3),
-                         "kernels": ([["int8", "default"], ["int8", "default"]] if K <= 5 and i8_on
+                         "kernels": ([["int8 narrow" if K <= 5 else "int8 wide", "default"]] * 2 if i8_on(K)
                                      else [at["pass1"][:2], at["pass2"][:2]])}
-        if (n, m, K) in MEASURED_TRAFFIC:
+        if (n, m, K) in MEASURED_TRAFFIC and not i8_on(K):
             out[f"K={K}"]["traffic_bytes"] = MEASURED_TRAFFIC[(n, m, K)]
     alpha = torch.randn(m, dtype=torch.float64, device="cuda")
     g.gebv(alpha)
@@ -159,7 +179,25 @@ def gemm_ceiling(M=16384, N=16384, Kd=8192, reps=5):
     dt = (time.perf_counter() - t0) / reps
     del a, b, c
     torch.cuda.empty_cache()
-    return {"gemm": f"{M}x{N}x{Kd} bf16 torch.matmul (hipBLASLt)", "TFLOP/s": round(2.0 * M * N * Kd / dt / 1e12, 1)}
+    out = {"gemm": f"{M}x{N}x{Kd} bf16 torch.matmul (hipBLASLt)", "TFLOP/s": round(2.0 * M * N * Kd / dt / 1e12, 1)}
+    try:  # the same shape in int8 (int32 results) through torch._int_mm
+        a = torch.randint(-128, 128, (M, Kd), device="cuda", dtype=torch.int8)
+        b = torch.randint(-128, 128, (Kd, N), device="cuda", dtype=torch.int8)
+        for _ in range(2):
+            c = torch._int_mm(a, b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            c = torch._int_mm(a, b)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        del a, b, c
+        torch.cuda.empty_cache()
+        out["int8_gemm"] = f"{M}x{N}x{Kd} int8 torch._int_mm"
+        out["int8_TOP/s"] = round(2.0 * M * N * Kd / dt / 1e12, 1)
+    except Exception as e:  # reported, never fatal
+        out["int8_error"] = repr(e)
+    return out
 
 
 def simulate_phenotype(g, n, m, n_causal, h2, seed):
@@ -570,7 +608,7 @@ def main():
         dst = src.clone()
         torch.cuda.synchronize()
         del src, dst
-    tune = None if args.no_autotune else g.autotune(K, reps=2)
+    tune = None if args.no_autotune or i8_on(K) else g.autotune(K, reps=2)  # tunes the BF16 kernels
 
     for _ in range(args.warmup):
         step("bf16x2")
@@ -636,7 +674,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x2 (genotypes exact in BF16, vector block split hi+lo, FP32 accumulate)",
+        "dtype": ("i8x3 (split-BF16 operand hi+lo rendered per vector as three int8 digits of a 23-bit fixed-point "
+                  "value, genotype dosages exact in int8, exact int32 accumulation)" if i8_on(K) else
+                  "bf16x2 (genotypes exact in BF16, vector block split hi+lo, FP32 accumulate)"),
         "data": "synthetic bed generated in HBM (MAF~U(0.05,0.5), HWE), Rademacher probes",
         "config": {"workload": f"Zc W Zc' operator on a K={K} block (64 probes + 1 RHS)",
                    "n": n, "m": m, "K": K, "bed_bytes": m * ((n + 3) // 4)},
@@ -650,7 +690,21 @@ def main():
         "differentiation": diff,
         "next_rows": nxt,
         "small_k_passes": smallk,
-        "roofline": {
+        "roofline": ({
+            "bound": "mfma",
+            "kernel": f"int8 wide block product (k_bp_i8w, three fixed-point digits per vector), {dom}",
+            "achieved": round(ach_tf, 2),
+            "peak": INT8_PEAK_TOPS,
+            "peak_kind": "dense int8 MFMA, 5.0 POP/s; achieved counts the product's 2 n m K operations",
+            "unit": "TFLOP/s",
+            "frac": round(ach_tf / INT8_PEAK_TOPS, 4),
+            "traffic": I8_MEASURED_TRAFFIC.get((n, m, K), {}).get("pass2" if k2 >= k1 else "pass1"),
+            "algorithmic_per_launch": {"flop": flops, "bytes": pb},
+            "hbm_GB/s_of_kernel": round(pb / (dom_ms * 1e-3) / 1e9, 1),
+            "mfma_issue_factor": round(i8_issue_factor(K), 3),
+            "issued_TOP/s": round(ach_tf * i8_issue_factor(K), 1),
+            "practical_mfma_ceiling": ceiling,
+        } if i8_on(K) else {
             "bound": "mfma",
             "kernel": f"split-BF16 block product ({(tune or {}).get('pass2' if k2 >= k1 else 'pass1', ['pipe'])[0]} "
                       f"variant), {dom}",
@@ -664,7 +718,7 @@ def main():
             "mfma_issue_factor": round(issue_factor(K), 3),
             "issued_TFLOP/s": round(ach_tf * issue_factor(K), 1),
             "practical_mfma_ceiling": ceiling,
-        },
+        }),
     }
     if not args.no_cpu:
         try:

@@ -183,12 +183,21 @@ int lds_nt(int t);
 int lds_splits(int64_t rows_pad, int64_t nsuper, int64_t target_wgs, int64_t& ns_per);
 void launch_bp(const LdsArgs& a, int NT, int nsplit, int variant, hipStream_t st);
 
-// Narrow split-BF16 blocks (1 to 5 vectors, hi and lo halves as columns of one 16-column tile)
-// on the int8 matrix cores: the operand X (hi + lo per element) is rendered per column as three
-// int8 digits of a 23-bit fixed-point value in the scratch buffer, and the block product writes
-// the split partials of column k (and zeros in the lo-half column t + k) into part, as the
-// split-BF16 kernels do.  red_pad: reduction length (rows of X, columns of G's packed rows).
-constexpr int kI8RT = 8;  // 16-row tiles per wave; a workgroup covers 512 rows
+// Split-BF16 blocks on the int8 matrix cores: the operand X (hi + lo per element) is rendered
+// per column as three int8 digits of a 23-bit fixed-point value in the scratch buffer, and the
+// block product writes the split partials of column k (and zeros in the column of its lo half,
+// where the operand keeps one) into part, as the split-BF16 kernels do.  red_pad: reduction
+// length (rows of X, columns of G's packed rows).
+constexpr int kI8RT = 8;   // narrow blocks: 16-row tiles per wave; a workgroup covers 512 rows
+constexpr int kI8WRT = 4;  // wide blocks: 16-row tiles per wave; a workgroup covers 256 rows
+constexpr int kI8NVG = 2;  // wide blocks: 16-vector tiles per column group (6 digit tiles + tail)
+// digit columns of t vectors: nvt tiles of 16 vectors (three digit tiles each), then r8 <= 5
+// vectors in one tail tile; ndc digit columns in all (nvt = 0, ndc = 3 t for t <= 5)
+struct I8Digits {
+  int nvt = 0, r8 = 0, ndc = 0;
+};
+I8Digits i8_digits(int t);
+int i8_groups(int t);
 struct I8Launch {
   const uint8_t* G = nullptr;
   int64_t ldg = 0;
@@ -200,10 +209,17 @@ struct I8Launch {
   int64_t ns_per_split = 0;
   void* scratch = nullptr;  // i8_scratch_bytes(red_pad, t) bytes
   float* part = nullptr;    // nsplit x KP x rows_pad
+  // operand layout: hi half of vector k in column k; lo half in column k + tail_r when
+  // tail_r > 0 and k >= tail0, else in column k of the lo block at lo_off16 (16-byte units)
+  int64_t lo_off16 = 0;
+  int tail0 = 0, tail_r = 0;
+  // optional per 16-column tile limits of the operand layout (wide blocks only), as in LdsArgs
+  const int32_t* tile_nsuper = nullptr;
+  const int64_t* tile_rows = nullptr;
 };
 bool i8_narrow_enabled();
 size_t i8_scratch_bytes(int64_t red_pad, int t);
-int i8_splits(int64_t rows_pad, int64_t nsuper, int64_t target_wgs, int64_t& ns_per);
+int i8_splits(int t, int64_t rows_pad, int64_t nsuper, int64_t target_wgs, int64_t& ns_per);
 void launch_bp_i8(const I8Launch& l, hipStream_t st);
 void launch_fin1(const Fin1Args& a, hipStream_t st);
 void launch_fin2(const Fin2Args& a, hipStream_t st);

@@ -31,16 +31,36 @@ constexpr int kI8Blocks = 64;  // column-maximum partials per vector
 __device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 
+// Where the split-BF16 operand keeps vector k (8 rows of one column per 16 bytes, KP columns per
+// 8-row block): hi half in column k; lo half in column k + tail_r of the same block when
+// tail_r > 0 and k >= tail0 (hi and lo halves as columns), else in column k of the lo block at
+// lo_off (16-byte units).
+struct OperandLayout {
+  int KP, t;
+  int64_t lo_off;
+  int tail0, tail_r;
+  __device__ int64_t lo_index(int64_t b, int k) const {
+    return (tail_r > 0 && k >= tail0) ? b * KP + k + tail_r : lo_off + b * KP + k;
+  }
+};
+
+// Digit columns: vectors [0, 16 nvt) in tiles of 16, digit p of vector k in column
+// 16 (3 (k / 16) + p) + k % 16 (the three digits of a vector in the same lane of three tiles);
+// the r8 <= 5 vectors after them in one tail tile, digit p of vector 16 nvt + k' in column
+// 48 nvt + 3 k' + p.  ndc columns in all (blocks of at most 5 vectors: nvt = 0, ndc = 3 t).
+__device__ __forceinline__ int digit_col(int k, int p, int nvt) {
+  return k < 16 * nvt ? 16 * (3 * (k >> 4) + p) + (k & 15) : 48 * nvt + 3 * (k - 16 * nvt) + p;
+}
+
 // Partial maxima of |hi + lo| over 8-row blocks: block (x, k) covers blocks x, x + 64, ... of
-// column k.  X holds 8 rows of one column per 16 bytes: hi halves in columns [0, t), lo halves in
-// columns [t, 2t) of each KP-wide row block.
-__global__ __launch_bounds__(256) void k_i8_colmax(const uint4* __restrict__ X, int KP, int t, int64_t nrb,
+// vector k.
+__global__ __launch_bounds__(256) void k_i8_colmax(const uint4* __restrict__ X, OperandLayout ol, int64_t nrb,
                                                    float* __restrict__ pmax) {
   __shared__ float red[4];
   const int k = blockIdx.y;
   float mx = 0.f;
   for (int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x; b < nrb; b += (int64_t)kI8Blocks * 256) {
-    const uint4 h = X[b * KP + k], l = X[b * KP + t + k];
+    const uint4 h = X[b * ol.KP + k], l = X[ol.lo_index(b, k)];
     const uint32_t hw[4] = {h.x, h.y, h.z, h.w}, lw[4] = {l.x, l.y, l.z, l.w};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -56,13 +76,13 @@ __global__ __launch_bounds__(256) void k_i8_colmax(const uint4* __restrict__ X, 
 }
 
 // Digits in MFMA lane order.  Step T (64 reduction rows) of lane group g covers rows
-// 512 (T / 8) + 128 g + 16 (T % 8) + j, j < 16, the rows whose codes the block-product kernel's
+// 512 (T / 8) + 128 g + 16 (T % 8) + j, j < 16, the rows whose codes the block-product kernels'
 // lane group g decodes at that step; byte 4 m + b of a lane's 16 bytes holds row j = 4 b + m (the
-// order in which the codes decode).  B[(4 T + g) nc + 3 k + p] holds digit p of vector k.
+// order in which the codes decode).  B[(4 T + g) ndc + digit_col(k, p)] holds digit p of vector k.
 // Thread (x, k): one (T, g) pair of vector k.  sc[k] = s_k 2^-22 (written by block 0).
-__global__ __launch_bounds__(256) void k_i8_quant(const uint4* __restrict__ X, int KP, int t, int64_t nstep,
-                                                  const float* __restrict__ pmax, float* __restrict__ sc,
-                                                  i32x4_t* __restrict__ B) {
+__global__ __launch_bounds__(256) void k_i8_quant(const uint4* __restrict__ X, OperandLayout ol, int nvt, int ndc,
+                                                  int64_t nstep, const float* __restrict__ pmax,
+                                                  float* __restrict__ sc, i32x4_t* __restrict__ B) {
   __shared__ float smax;
   const int k = blockIdx.y;
   if (threadIdx.x < 64) {
@@ -90,7 +110,7 @@ __global__ __launch_bounds__(256) void k_i8_quant(const uint4* __restrict__ X, i
 #pragma unroll
   for (int h8 = 0; h8 < 2; ++h8) {
     const int64_t b = (r0 >> 3) + h8;
-    const uint4 hv = X[b * KP + k], lv = X[b * KP + t + k];
+    const uint4 hv = X[b * ol.KP + k], lv = X[ol.lo_index(b, k)];
     const uint32_t hw[4] = {hv.x, hv.y, hv.z, hv.w}, lw[4] = {lv.x, lv.y, lv.z, lv.w};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -111,10 +131,9 @@ __global__ __launch_bounds__(256) void k_i8_quant(const uint4* __restrict__ X, i
     d[1][m] |= (uint32_t)(d1 & 255) << (8 * b);
     d[2][m] |= (uint32_t)(d2 & 255) << (8 * b);
   }
-  const int nc = 3 * t;
 #pragma unroll
   for (int p = 0; p < 3; ++p)
-    B[(4 * T + g) * nc + 3 * k + p] = i32x4_t{(int)d[p][0], (int)d[p][1], (int)d[p][2], (int)d[p][3]};
+    B[(4 * T + g) * ndc + digit_col(k, p, nvt)] = i32x4_t{(int)d[p][0], (int)d[p][1], (int)d[p][2], (int)d[p][3]};
 }
 
 struct I8Args {
@@ -219,15 +238,210 @@ __global__ __launch_bounds__(256) void k_bp_i8(I8Args a) {
   }
 }
 
+struct I8WArgs {
+  const uint8_t* G = nullptr;
+  int64_t ldg = 0;
+  int64_t nsuper = 0, ns_per_split = 0;
+  const i32x4_t* B = nullptr;
+  int ndc = 0, nvt = 0, r8 = 0, t = 0;
+  const float* sc = nullptr;
+  float* part = nullptr;
+  int KP = 0;
+  int64_t rows_pad = 0;
+  int tail0 = 0, tail_r = 0;  // operand layout: vectors k >= tail0 have a lo column k + tail_r
+  // optional per 16-vector tile limits (batched traits): reduction super-chunks that can be
+  // nonzero, and rows from which the tile's products are zero (both skipped)
+  const int32_t* tile_nsuper = nullptr;
+  const int64_t* tile_rows = nullptr;
+  bool xcd_pairs = false;  // dispatch the column groups of a row block back to back on one XCD
+};
+
+// Wide blocks: column group blockIdx.y holds vector tiles [NVG y, NVG y + NVG) (three digit tiles
+// each, the digits of a vector in one lane) and, in the last group, the tail tile of up to 5
+// vectors.  One workgroup: 4 waves, each on 16 RT rows of one reduction slice (blockIdx.z).
+// Codes as in the narrow kernel (whole 128-byte lines per row and super-chunk, a super-chunk
+// ahead); the group's digits of each half super-chunk (4 steps) are staged in LDS, requested a
+// half ahead and before the codes, with one barrier per half.
+template <int RT, int NVG>
+__global__ __launch_bounds__(256, 2) void k_bp_i8w(I8WArgs a) {
+  constexpr int NDT = 3 * NVG + 1;  // digit tiles of a group; the last is the tail tile
+  __shared__ i32x4_t sb[2][NDT * 16 * 16];  // [half parity][tile j][4 w4 + g][column]
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = lane & 15, g = lane >> 4;
+  // (row block bx, column group by) of this workgroup.  Workgroups are dispatched round-robin
+  // over the 8 XCDs in linear order; with xcd_pairs the column groups of one row block take
+  // consecutive slots of the same XCD, so all but the first read the codes from that XCD's L2.
+  int64_t bx = blockIdx.x;
+  int by = blockIdx.y;
+  {
+    const int64_t gx = gridDim.x, gy = gridDim.y, gx8 = gx - gx % 8;
+    const int64_t L = blockIdx.x + gx * blockIdx.y;
+    if (a.xcd_pairs && gy > 1 && L < gx8 * gy) {
+      const int64_t slot = L / 8;
+      bx = (slot / gy) * 8 + L % 8;
+      by = (int)(slot % gy);
+    } else if (a.xcd_pairs && gy > 1) {  // the last gx % 8 row blocks, in any order
+      const int64_t L2 = L - gx8 * gy, rem = gx - gx8;
+      bx = gx8 + L2 % rem;
+      by = (int)(L2 / rem);
+    }
+  }
+  const int64_t row0 = bx * (64 * RT) + wave * (16 * RT);
+  const int64_t wg_row0 = bx * (64 * RT);
+  const int vt0 = by * NVG;
+  const int nv = min(NVG, a.nvt - vt0);                      // vector tiles of this group
+  const bool tail = a.r8 > 0 && by == (int)gridDim.y - 1;    // the group holds the tail tile
+  const int64_t s0 = blockIdx.z * a.ns_per_split;
+  int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
+  const uint32_t tab = 0x02010100u;
+  // per-tile limits: digit tile j of vector tile vt0 + j / 3 (the tail tile: vector tile nvt)
+  // runs MFMAs for super-chunks s < lim[j]; the staged loop bound is the largest over the
+  // workgroup's rows, since all its waves meet at a barrier every half super-chunk
+  int32_t lim[NDT];
+  int32_t limmax = 0;
+#pragma unroll
+  for (int j = 0; j < NDT; ++j) {
+    const int vtile = j < NDT - 1 ? vt0 + j / 3 : a.nvt;
+    const bool on = j < NDT - 1 ? j < 3 * nv : tail;
+    const int32_t ln = !on ? 0 : (a.tile_nsuper ? a.tile_nsuper[vtile] : INT32_MAX);
+    lim[j] = (a.tile_rows && on && row0 >= a.tile_rows[vtile]) ? 0 : ln;
+    limmax = max(limmax, (a.tile_rows && on && wg_row0 >= a.tile_rows[vtile]) ? 0 : ln);
+  }
+  s1 = min(s1, (int64_t)limmax);
+  i32x4_t acc[RT][NDT];
+#pragma unroll
+  for (int i = 0; i < RT; ++i)
+#pragma unroll
+    for (int j = 0; j < NDT; ++j) acc[i][j] = i32x4_t{0, 0, 0, 0};
+  auto live = [&](int j) { return j < 3 * nv || (j == NDT - 1 && tail); };
+  const uint8_t* lbase = a.G + (row0 + r) * a.ldg + g * 32;
+  auto ld = [&](int64_t sc, uint32_t (&dst)[RT][8]) {
+    sc = sc < s1 ? sc : s1 - 1;  // past the slice: re-read its last super-chunk (never used)
+#pragma unroll
+    for (int i = 0; i < RT; ++i) {
+      const u32x4_t* q = reinterpret_cast<const u32x4_t*>(lbase + (int64_t)i * 16 * a.ldg + sc * 128);
+      const u32x4_t q0 = q[0], q1 = q[1];
+      dst[i][0] = q0.x; dst[i][1] = q0.y; dst[i][2] = q0.z; dst[i][3] = q0.w;
+      dst[i][4] = q1.x; dst[i][5] = q1.y; dst[i][6] = q1.z; dst[i][7] = q1.w;
+    }
+  };
+  // staging: thread piece q is entry 256 q + threadIdx.x = (16 q + 4 w4 + g') 16 + c of the
+  // half's buffer, digit column c of tile q for lane group g' at step 4 hh + w4 (global step
+  // numbering); tail-tile columns past 3 r8 repeat a live one.  The tile of a piece is uniform
+  // across the workgroup, so dead tiles cost a scalar branch.
+  const int64_t hlast = 2 * s1 - 1;
+  const int wg = threadIdx.x >> 4, cc16 = threadIdx.x & 15;
+  i32x4_t stg[NDT];
+  auto bload = [&](int64_t hh) {
+    hh = hh <= hlast ? hh : hlast;
+    const int64_t brow = (4 * (4 * hh + (wg >> 2)) + (wg & 3)) * a.ndc;
+#pragma unroll
+    for (int q = 0; q < NDT; ++q) {
+      if (live(q)) {
+        const int col = q < NDT - 1 ? 16 * (3 * vt0 + q) + cc16 : 48 * a.nvt + cc16 % (3 * a.r8);
+        stg[q] = a.B[brow + col];
+      }
+    }
+  };
+  auto bstore = [&](int64_t hh) {
+#pragma unroll
+    for (int q = 0; q < NDT; ++q)
+      if (live(q)) sb[hh & 1][256 * q + threadIdx.x] = stg[q];
+  };
+  auto step = [&](const uint32_t (&cc)[RT][8], int w, int buf, int64_t s) {
+    i32x4_t av[RT];
+#pragma unroll
+    for (int i = 0; i < RT; ++i) {
+      const uint32_t W = cc[i][w];
+      av[i] = i32x4_t{(int)__builtin_amdgcn_perm(0u, tab, W & 0x03030303u),
+                      (int)__builtin_amdgcn_perm(0u, tab, (W >> 2) & 0x03030303u),
+                      (int)__builtin_amdgcn_perm(0u, tab, (W >> 4) & 0x03030303u),
+                      (int)__builtin_amdgcn_perm(0u, tab, (W >> 6) & 0x03030303u)};
+    }
+    const i32x4_t* bp = &sb[buf][(4 * (w & 3) + g) * 16 + r];
+#pragma unroll
+    for (int j = 0; j < NDT; ++j) {
+      if (!live(j) || s >= lim[j]) continue;
+      const i32x4_t bv = bp[256 * j];
+#pragma unroll
+      for (int i = 0; i < RT; ++i) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[i], bv, acc[i][j], 0, 0, 0);
+    }
+  };
+  uint32_t cA[RT][8], cB[RT][8];
+  if (s0 < s1) {
+    bload(2 * s0);
+    ld(s0, cA);
+    bstore(2 * s0);
+    __syncthreads();
+  }
+  // half H of super-chunk s from the codes cc; its first half requests super-chunk s + 1 into nx.
+  // The buffer of half hh + 1 is written after every wave has finished half hh - 1 (the barrier
+  // that ended it), so one barrier per half suffices.
+  auto half = [&](auto hconst, const uint32_t (&cc)[RT][8], uint32_t (&nx)[RT][8], int64_t s) {
+    constexpr int H = decltype(hconst)::value;
+    bload(2 * s + H + 1);
+    if constexpr (H == 0) ld(s + 1, nx);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int w4 = 0; w4 < 4; ++w4) step(cc, 4 * H + w4, H, s);
+    __builtin_amdgcn_sched_barrier(0);
+    bstore(2 * s + H + 1);
+    __syncthreads();
+  };
+  using H0 = std::integral_constant<int, 0>;
+  using H1 = std::integral_constant<int, 1>;
+  for (int64_t s = s0; s < s1; s += 2) {
+    half(H0{}, cA, cB, s);
+    half(H1{}, cA, cB, s);
+    if (s + 1 < s1) {
+      half(H0{}, cB, cA, s + 1);
+      half(H1{}, cB, cA, s + 1);
+    }
+  }
+  // lane (r, g) holds rows 4 g .. 4 g + 3 of column r of every tile
+  auto put = [&](int k, int64_t row, const float (&o)[4]) {
+    float* base = a.part + (blockIdx.z * (int64_t)a.KP + k) * a.rows_pad + row;
+    *reinterpret_cast<float4*>(base) = make_float4(o[0], o[1], o[2], o[3]);
+    if (a.tail_r > 0 && k >= a.tail0)  // the operand's lo column, which the epilogues add
+      *reinterpret_cast<float4*>(base + (int64_t)a.tail_r * a.rows_pad) = make_float4(0.f, 0.f, 0.f, 0.f);
+  };
+#pragma unroll
+  for (int i = 0; i < RT; ++i) {
+    const int64_t row = row0 + 16 * i + 4 * g;
+#pragma unroll
+    for (int jv = 0; jv < NVG; ++jv) {
+      const int k = 16 * (vt0 + jv) + r;
+      if (jv >= nv || k >= a.t) continue;
+      const double scale = (double)a.sc[k];
+      float o[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        o[q] = (float)((double)((int64_t)acc[i][3 * jv][q] * 65536 + (int64_t)acc[i][3 * jv + 1][q] * 256 +
+                                (int64_t)acc[i][3 * jv + 2][q]) * scale);
+      put(k, row, o);
+    }
+    if (tail) {
+      const int k = 16 * a.nvt + r / 3;
+      const bool first = r % 3 == 0 && r < 3 * a.r8;
+      const double scale = first ? (double)a.sc[k] : 0.0;
+      float o[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int v0 = acc[i][NDT - 1][q];
+        const int v1 = __shfl(v0, lane + 1);
+        const int v2 = __shfl(v0, lane + 2);
+        o[q] = (float)((double)((int64_t)v0 * 65536 + (int64_t)v1 * 256 + (int64_t)v2) * scale);
+      }
+      if (first) put(k, row, o);
+    }
+  }
+}
+
 }  // namespace
 
 bool i8_narrow_enabled() {
   const char* e = std::getenv("SLEMM_I8_NARROW");  // read per call: "0" keeps the BF16 kernels
   return !(e && e[0] == '0');
-}
-
-size_t i8_scratch_bytes(int64_t red_pad, int t) {
-  return sizeof(float) * (64 + (size_t)t * kI8Blocks) + (size_t)red_pad * 3 * t;
 }
 
 // 16-row tiles per wave (2, 4 or 8; SLEMM_I8_RT, read per call).  Eight measured fastest at
@@ -238,43 +452,89 @@ static int i8_rt() {
   return rt == 2 || rt == 8 ? rt : 4;
 }
 
-int i8_splits(int64_t rows_pad, int64_t nsuper, int64_t target_wgs, int64_t& ns_per) {
-  const int64_t wgs = rows_pad / (64 * i8_rt());
+int i8_splits(int t, int64_t rows_pad, int64_t nsuper, int64_t target_wgs, int64_t& ns_per) {
+  const bool narrow = i8_digits(t).nvt == 0;
+  const int64_t wgs = rows_pad / (64 * (narrow ? i8_rt() : kI8WRT)) * i8_groups(t);
   int64_t S = (target_wgs + wgs - 1) / wgs;
   S = std::max<int64_t>(1, std::min<int64_t>(S, nsuper));
   ns_per = (nsuper + S - 1) / S;
   return (int)((nsuper + ns_per - 1) / ns_per);
 }
 
-void launch_bp_i8(const I8Launch& l, hipStream_t st) {
-  if (l.t < 1 || 3 * l.t > 16) fail(SLEMM_ERR_ARG, "int8 narrow blocks take 1 to 5 vectors");
-  if (l.KP != 16) fail(SLEMM_ERR_ARG, "int8 narrow blocks have one 16-column tile");
-  if (l.rows_pad % 512 != 0 || l.red_pad % 512 != 0 || l.ldg * 4 < l.red_pad || l.ldg % 16 != 0)
-    fail(SLEMM_ERR_ARG, "int8 narrow block: padding mismatch");
-  if (l.red_pad > 8 * 1024 * 1024) fail(SLEMM_ERR_ARG, "int8 narrow block: reduction too long for int32 sums");
-  if (l.squared && l.t != 1) fail(SLEMM_ERR_ARG, "squared-dosage products take one vector");
-  float* sc = static_cast<float*>(l.scratch);
-  float* pmax = sc + 64;
-  i32x4_t* B = reinterpret_cast<i32x4_t*>(pmax + (size_t)l.t * kI8Blocks);
-  if (reinterpret_cast<uintptr_t>(B) % 16 != 0) fail(SLEMM_ERR_ARG, "int8 narrow block: scratch alignment");
-  const uint4* X = static_cast<const uint4*>(l.X);
-  k_i8_colmax<<<dim3(kI8Blocks, l.t), 256, 0, st>>>(X, l.KP, l.t, l.red_pad / 8, pmax);
-  const int64_t nstep = l.red_pad / 64;
-  k_i8_quant<<<dim3((unsigned)((nstep * 4 + 255) / 256), l.t), 256, 0, st>>>(X, l.KP, l.t, nstep, pmax, sc, B);
-  I8Args a;
-  a.G = l.G; a.ldg = l.ldg; a.nsuper = l.red_pad / 512; a.ns_per_split = l.ns_per_split; a.B = B; a.t = l.t;
-  a.sc = sc; a.part = l.part; a.KP = l.KP; a.rows_pad = l.rows_pad;
-  const int rt = i8_rt();
-  const dim3 grid((unsigned)(l.rows_pad / (64 * rt)), 1, (unsigned)l.nsplit);
-  if (rt == 2) {
-    if (l.squared) k_bp_i8<2, true><<<grid, 256, 0, st>>>(a);
-    else k_bp_i8<2, false><<<grid, 256, 0, st>>>(a);
-  } else if (rt == 8) {
-    if (l.squared) k_bp_i8<8, true><<<grid, 256, 0, st>>>(a);
-    else k_bp_i8<8, false><<<grid, 256, 0, st>>>(a);
+I8Digits i8_digits(int t) {
+  I8Digits d;
+  if (t <= 5) {
+    d.nvt = 0; d.r8 = t; d.ndc = 3 * t;
+  } else if (t % 16 >= 1 && t % 16 <= 5) {
+    d.nvt = t / 16; d.r8 = t % 16; d.ndc = 16 * (3 * d.nvt + 1);
   } else {
-    if (l.squared) k_bp_i8<4, true><<<grid, 256, 0, st>>>(a);
-    else k_bp_i8<4, false><<<grid, 256, 0, st>>>(a);
+    d.nvt = (t + 15) / 16; d.r8 = 0; d.ndc = 48 * d.nvt;
+  }
+  return d;
+}
+
+size_t i8_scratch_bytes(int64_t red_pad, int t) {
+  return sizeof(float) * (64 * (size_t)((t + 63) / 64) + (size_t)t * kI8Blocks) + (size_t)red_pad * i8_digits(t).ndc;
+}
+
+int i8_groups(int t) {
+  const I8Digits d = i8_digits(t);
+  return d.nvt == 0 ? 1 : (d.nvt + kI8NVG - 1) / kI8NVG;
+}
+
+void launch_bp_i8(const I8Launch& l, hipStream_t st) {
+  if (l.t < 1) fail(SLEMM_ERR_ARG, "int8 blocks take at least one vector");
+  const I8Digits dl = i8_digits(l.t);
+  const bool narrow = dl.nvt == 0;
+  if (narrow && l.KP != 16) fail(SLEMM_ERR_ARG, "int8 narrow blocks have one 16-column tile");
+  if (l.KP < l.t || l.KP % 16 != 0) fail(SLEMM_ERR_ARG, "int8 block: column padding mismatch");
+  if (l.rows_pad % 512 != 0 || l.red_pad % 512 != 0 || l.ldg * 4 < l.red_pad || l.ldg % 16 != 0)
+    fail(SLEMM_ERR_ARG, "int8 block: padding mismatch");
+  if (l.red_pad > 8 * 1024 * 1024) fail(SLEMM_ERR_ARG, "int8 block: reduction too long for int32 sums");
+  if (l.squared && l.t != 1) fail(SLEMM_ERR_ARG, "squared-dosage products take one vector");
+  if (l.tail_r > 0 && (l.tail0 + 2 * (l.t - l.tail0) > l.KP || l.tail_r != l.t - l.tail0))
+    fail(SLEMM_ERR_ARG, "int8 block: lo columns outside the block");
+  float* sc = static_cast<float*>(l.scratch);
+  float* pmax = sc + 64 * ((l.t + 63) / 64);
+  i32x4_t* B = reinterpret_cast<i32x4_t*>(pmax + (size_t)l.t * kI8Blocks);
+  if (reinterpret_cast<uintptr_t>(B) % 16 != 0) fail(SLEMM_ERR_ARG, "int8 block: scratch alignment");
+  const uint4* X = static_cast<const uint4*>(l.X);
+  OperandLayout ol{l.KP, l.t, l.lo_off16, l.tail0, l.tail_r};
+  k_i8_colmax<<<dim3(kI8Blocks, l.t), 256, 0, st>>>(X, ol, l.red_pad / 8, pmax);
+  const int64_t nstep = l.red_pad / 64;
+  k_i8_quant<<<dim3((unsigned)((nstep * 4 + 255) / 256), l.t), 256, 0, st>>>(X, ol, dl.nvt, dl.ndc, nstep, pmax,
+                                                                              sc, B);
+  if (narrow) {
+    I8Args a;
+    a.G = l.G; a.ldg = l.ldg; a.nsuper = l.red_pad / 512; a.ns_per_split = l.ns_per_split; a.B = B; a.t = l.t;
+    a.sc = sc; a.part = l.part; a.KP = l.KP; a.rows_pad = l.rows_pad;
+    const int rt = i8_rt();
+    const dim3 grid((unsigned)(l.rows_pad / (64 * rt)), 1, (unsigned)l.nsplit);
+    if (rt == 2) {
+      if (l.squared) k_bp_i8<2, true><<<grid, 256, 0, st>>>(a);
+      else k_bp_i8<2, false><<<grid, 256, 0, st>>>(a);
+    } else if (rt == 8) {
+      if (l.squared) k_bp_i8<8, true><<<grid, 256, 0, st>>>(a);
+      else k_bp_i8<8, false><<<grid, 256, 0, st>>>(a);
+    } else {
+      if (l.squared) k_bp_i8<4, true><<<grid, 256, 0, st>>>(a);
+      else k_bp_i8<4, false><<<grid, 256, 0, st>>>(a);
+    }
+  } else {
+    I8WArgs a;
+    a.G = l.G; a.ldg = l.ldg; a.nsuper = l.red_pad / 512; a.ns_per_split = l.ns_per_split; a.B = B;
+    a.ndc = dl.ndc; a.nvt = dl.nvt; a.r8 = dl.r8; a.t = l.t; a.sc = sc; a.part = l.part; a.KP = l.KP;
+    a.rows_pad = l.rows_pad; a.tail0 = l.tail0; a.tail_r = l.tail_r;
+    a.tile_nsuper = l.tile_nsuper; a.tile_rows = l.tile_rows;
+    // SLEMM_I8_XCD=1 pairs the column groups of a row block on one XCD (read per call).  At
+    // n = 100,000 x m = 500,000 it cut the K=65 block's fetched bytes by 8-14% only and its time
+    // not at all (those blocks are MFMA-bound), so the plain dispatch order is the default.
+    const char* ex = std::getenv("SLEMM_I8_XCD");
+    a.xcd_pairs = ex && ex[0] == '1';
+    // (one-vector-tile blocks measured slower with 2 row tiles per wave at four waves per SIMD:
+    // each digit load then serves half as many MFMAs)
+    const dim3 grid((unsigned)(l.rows_pad / (64 * kI8WRT)), (unsigned)i8_groups(l.t), (unsigned)l.nsplit);
+    k_bp_i8w<kI8WRT, kI8NVG><<<grid, 256, 0, st>>>(a);
   }
   HIPCHECK(hipGetLastError());
 }

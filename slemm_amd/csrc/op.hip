@@ -132,28 +132,43 @@ void tail_args(const Layout& L, int t, LdsArgs& la) {
   la.last_tile = L.hl ? 0 : (L.tail_r > 0 ? L.tail0 / 16 : (t + 15) / 16 - 1);
 }
 
-// Narrow blocks of 1 to 5 vectors run on the int8 kernels (SLEMM_I8_NARROW=0: the split-BF16
-// ones); batched-trait blocks, whose tiles carry per-column reduction limits, keep the latter.
-// SLEMM_I8_TARGET sets the workgroup target of their reduction split (read per call; by default
-// 512 for the SNP-major pass and 1024 for the individual-major one, the best of 512 to 8192 at
+// Blocks of 1 to 5 vectors (hi and lo halves as columns) run on the int8 narrow kernel
+// (SLEMM_I8_NARROW=0: the split-BF16 ones), wider blocks on the int8 wide kernel
+// (SLEMM_I8_WIDE=0: the split-BF16 ones); batched-trait blocks, whose tiles carry per-column
+// reduction limits, keep the split-BF16 kernels.  SLEMM_I8_TARGET sets the workgroup target of the
+// reduction split (read per call; by default, for narrow blocks 512 for the SNP-major pass and
+// 1024 for the individual-major one, and for wide blocks 8192, the best of 512 to 8192 at
 // n = 100,000 x m = 500,000).
-static bool use_i8(const Layout& L, int t, const TraitCols* tc) {
-  return L.hl && 3 * t <= 16 && !tc && i8_narrow_enabled();
+static bool i8_wide_enabled() {
+  const char* e = std::getenv("SLEMM_I8_WIDE");
+  return !(e && e[0] == '0');
 }
 
-static int i8_pass_splits(int pass, int64_t rows_pad, int64_t nsuper, int64_t& nsp) {
+static bool use_i8(const Layout& L, int t, const TraitCols* tc) {
+  if (t <= 5) return !tc && L.hl && i8_narrow_enabled();
+  // 6 to 8 vectors: the BF16 kernels with hi and lo halves as columns run one MFMA tile per
+  // step and stream faster than the wide kernel's three digit tiles
+  if (t <= 8 && L.hl) return false;
+  return i8_wide_enabled();
+}
+
+static int i8_pass_splits(int pass, int t, int64_t rows_pad, int64_t nsuper, int64_t& nsp) {
   const char* e = std::getenv("SLEMM_I8_TARGET");
-  const int64_t target = e ? std::max<int64_t>(1, std::atoll(e)) : (pass == 0 ? 512 : 1024);
-  return i8_splits(rows_pad, nsuper, target, nsp);
+  const int64_t dflt = t <= 5 ? (pass == 0 ? 512 : 1024) : 8192;
+  const int64_t target = e ? std::max<int64_t>(1, std::atoll(e)) : dflt;
+  return i8_splits(t, rows_pad, nsuper, target, nsp);
 }
 
 static void bp_i8(slemm_geno* h, const uint8_t* G, int64_t ldg, int64_t red_pad, int64_t rows_pad, const void* X,
-                  const Layout& L, int t, bool squared, int S, int64_t nsp, hipStream_t st) {
+                  int64_t lo_off16, const Layout& L, int t, bool squared, int S, int64_t nsp, hipStream_t st,
+                  const int32_t* tile_nsuper = nullptr, const int64_t* tile_rows = nullptr) {
   h->i8buf.reserve(i8_scratch_bytes(red_pad, t));
   I8Launch l;
   l.G = G; l.ldg = ldg; l.red_pad = red_pad; l.rows_pad = rows_pad; l.X = X; l.KP = L.KP; l.t = t;
   l.squared = squared; l.nsplit = S; l.ns_per_split = nsp; l.scratch = h->i8buf.p;
   l.part = h->partbuf.as<float>();
+  l.lo_off16 = lo_off16; l.tail0 = L.tail0; l.tail_r = L.tail_r;
+  l.tile_nsuper = tile_nsuper; l.tile_rows = tile_rows;
   launch_bp_i8(l, st);
 }
 
@@ -169,7 +184,7 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
   int64_t nsp = 0;
   const int64_t nsuper = h->n_pad / kPad;
   const bool i8 = use_i8(L, t, tc);
-  const int S = i8 ? i8_pass_splits(0, h->m_pad, nsuper, nsp) : (int)pass_splits(h, 0, L.slot, nsp);
+  const int S = i8 ? i8_pass_splits(0, t, h->m_pad, nsuper, nsp) : (int)pass_splits(h, 0, L.slot, nsp);
   h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * h->m_pad);
   LdsArgs la;
   la.G = h->gs; la.ldg = h->ld_gs; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x1.p;
@@ -178,7 +193,8 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
   tail_args(L, t, la);
   if (tc) la.tile_nsuper = tc->tile_nsuper;  // reduction over individuals stops at the subsets' end
   TimingToken tok = timing_begin(st);
-  if (i8) bp_i8(h, h->gs, h->ld_gs, h->n_pad, h->m_pad, h->x1.p, L, t, squared, S, nsp, st);
+  if (i8)
+    bp_i8(h, h->gs, h->ld_gs, h->n_pad, h->m_pad, h->x1.p, L.lo1_16, L, t, squared, S, nsp, st, la.tile_nsuper);
   else launch_bp(la, L.NT, S, bp_choice(h, 0, L.slot).variant, st);
   timing_end(tok, st, EPI_PASS1, PREC_BF16X2);
   Fin1Args f;
@@ -197,7 +213,7 @@ void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const flo
   int64_t nsp = 0;
   const int64_t nsuper = h->m_pad / kPad;
   const bool i8 = use_i8(L, t, tc);
-  const int S = i8 ? i8_pass_splits(1, h->n_pad, nsuper, nsp) : (int)pass_splits(h, 1, L.slot, nsp);
+  const int S = i8 ? i8_pass_splits(1, t, h->n_pad, nsuper, nsp) : (int)pass_splits(h, 1, L.slot, nsp);
   h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * h->n_pad);
   LdsArgs la;
   la.G = h->gi; la.ldg = h->ld_gi; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x2.p;
@@ -205,7 +221,8 @@ void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const flo
   tail_args(L, t, la);
   if (tc) la.tile_rows = tc->tile_rows;  // output rows (individuals) past the subsets' end are zero
   TimingToken tok = timing_begin(st);
-  if (i8) bp_i8(h, h->gi, h->ld_gi, h->m_pad, h->n_pad, h->x2.p, L, t, false, S, nsp, st);
+  if (i8)
+    bp_i8(h, h->gi, h->ld_gi, h->m_pad, h->n_pad, h->x2.p, L.lo2_16, L, t, false, S, nsp, st, nullptr, la.tile_rows);
   else launch_bp(la, L.NT, S, bp_choice(h, 1, L.slot).variant, st);
   timing_end(tok, st, EPI_PASS2, PREC_BF16X2);
   Fin2Args f;

@@ -95,7 +95,8 @@ def test_block_product_variants(gpu, t, monkeypatch):
     tau0 = 0.37
     ref = D.op("ldet", tau0, V)
     zref = D.rmatvec(V, weighted=True)
-    monkeypatch.setenv("SLEMM_I8_NARROW", "0")  # narrow blocks on the split-BF16 kernels
+    monkeypatch.setenv("SLEMM_I8_NARROW", "0")  # all blocks on the split-BF16 kernels
+    monkeypatch.setenv("SLEMM_I8_WIDE", "0")
     for hl in ("1", "0"):
         monkeypatch.setenv("SLEMM_HL_COLUMNS", hl)
         for v in range(BP_VARIANTS):
@@ -133,7 +134,7 @@ def test_int8_narrow_blocks(gpu, t, monkeypatch):
     monkeypatch.setenv("SLEMM_I8_NARROW", "0")
     base = g.lnr_op_select("ldet", tau0, Vd, precision="bf16x2").cpu().numpy()
     monkeypatch.setenv("SLEMM_I8_NARROW", "1")
-    for target in ("1", "3", "100000"):
+    for target in ("1", "6", "100000"):
         monkeypatch.setenv("SLEMM_I8_TARGET", target)
         out = g.lnr_op_select("ldet", tau0, Vd, precision="bf16x2").cpu().numpy()
         assert rel_err(out, ref) < TOL["bf16x2"], (target, rel_err(out, ref))
@@ -142,6 +143,46 @@ def test_int8_narrow_blocks(gpu, t, monkeypatch):
         assert rel_err(zt, zref) < TOL["bf16x2"], (target, rel_err(zt, zref))
         if t >= 3:
             assert np.all(np.abs(zt[:, 1]) <= 1e-6 * np.abs(zt).max())
+        z = g.matvec(Ud, precision="bf16x2").cpu().numpy()
+        assert rel_err(z, uref) < TOL["bf16x2"], (target, rel_err(z, uref))
+
+
+@pytest.mark.parametrize("t", [7, 9, 16, 17, 21, 33, 65, 66, 100])
+def test_int8_wide_blocks(gpu, t, monkeypatch):
+    """Blocks of more than 5 vectors on the int8 wide kernel (16-vector tiles of three digit
+    tiles, a tail tile for up to 5 trailing vectors, column groups of two vector tiles), over the
+    operand layouts the split-BF16 path uses (tail columns for t = 16 q + r with r <= 8, separate
+    hi and lo blocks otherwise; 6 to 8 vectors take the wide kernel only without hi/lo columns),
+    at one reduction split and at several, against the FP64 dense operator, the transposed product
+    and the product, and against the split-BF16 kernels."""
+    if t <= 8:
+        monkeypatch.setenv("SLEMM_HL_COLUMNS", "0")
+    p = make_problem(n=2600, m=1700, seed=31, weights=True, c=2)
+    g = _geno(gpu, p)
+    g.set_model(None, p.rvec, p.Q)
+    D = p.dense()
+    rng = np.random.default_rng(300 + t)
+    V = rng.normal(size=(p.n, t)).astype(np.float32)
+    V[:, 3] = 0.0
+    V[:, 4] *= 1e-3
+    U = rng.normal(size=(p.m, t)).astype(np.float32)
+    Vd, Ud = torch.from_numpy(V).cuda(), torch.from_numpy(U).cuda()
+    tau0 = 0.37
+    ref = D.op("ldet", tau0, V)
+    zref = D.rmatvec(V, weighted=True)
+    uref = D.matvec(U)
+    monkeypatch.setenv("SLEMM_I8_WIDE", "0")
+    base = g.lnr_op_select("ldet", tau0, Vd, precision="bf16x2").cpu().numpy()
+    monkeypatch.setenv("SLEMM_I8_WIDE", "1")
+    for target in ("1", "20", "100000"):
+        monkeypatch.setenv("SLEMM_I8_TARGET", target)
+        out = g.lnr_op_select("ldet", tau0, Vd, precision="bf16x2").cpu().numpy()
+        assert rel_err(out, ref) < TOL["bf16x2"], (target, rel_err(out, ref))
+        assert rel_err(out, base) < TOL["bf16x2"], (target, rel_err(out, base))
+        zt = g.weighted_rmatvec(Vd, precision="bf16x2").cpu().numpy()
+        assert rel_err(zt, zref) < TOL["bf16x2"], (target, rel_err(zt, zref))
+        assert np.all(zt[:, 3] == 0.0)
+        assert rel_err(zt[:, 4], zref[:, 4]) < TOL["bf16x2"]
         z = g.matvec(Ud, precision="bf16x2").cpu().numpy()
         assert rel_err(z, uref) < TOL["bf16x2"], (target, rel_err(z, uref))
 
