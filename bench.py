@@ -37,8 +37,10 @@ FP32_PEAK_TFLOPS = 157.3
 # Infinity Cache hits.
 MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 15.14e9 + 0.321e9, "pass2": 17.44e9 + 0.193e9},
                     (100_000, 500_000, 1): {"pass1": 12.64e9 + 0.064e9, "pass2": 12.71e9 + 0.039e9}}
-# The same for the int8 kernels (wide at K=65, narrow at K=1); filled in from the counter runs.
-I8_MEASURED_TRAFFIC = {}
+# The same for the int8 kernels (wide kernel at K=65: two column groups, each reading the codes;
+# narrow kernel at K=1); profiles/blockprod_int8_c2_hbm_traffic.md.
+I8_MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 27.08e9 + 0.398e9, "pass2": 30.74e9 + 0.293e9},
+                       (100_000, 500_000, 1): {"pass1": 12.57e9 + 0.004e9, "pass2": 12.59e9 + 0.005e9}}
 
 
 def issue_factor(K: int) -> float:
@@ -145,8 +147,9 @@ def small_k_passes(g, lib, L, n, m, ks=(1, 3, 5, 7, 16), reps=3):
                          "pass2_hbm_frac": round(b / p2 / 1e6 / HBM_PEAK_GBS, 3),
                          "kernels": ([["int8 narrow" if K <= 5 else "int8 wide", "default"]] * 2 if i8_on(K)
                                      else [at["pass1"][:2], at["pass2"][:2]])}
-        if (n, m, K) in MEASURED_TRAFFIC and not i8_on(K):
-            out[f"K={K}"]["traffic_bytes"] = MEASURED_TRAFFIC[(n, m, K)]
+        trf = (I8_MEASURED_TRAFFIC if i8_on(K) else MEASURED_TRAFFIC).get((n, m, K))
+        if trf:
+            out[f"K={K}"]["traffic_bytes"] = trf
     alpha = torch.randn(m, dtype=torch.float64, device="cuda")
     g.gebv(alpha)
     torch.cuda.synchronize()
