@@ -60,6 +60,35 @@ def test_sldf_against_fp64_oracle(case, sldf_fp64, prec):
     assert got["evals"] == 21
 
 
+def test_sldf_tight_search_vc_precision(case):
+    """The same probes with the golden-section search run to tol 1e-6 (30 evaluations) and the
+    Lanczos solves to 1e-6, so that neither rounds the answer, against the FP64 dense evaluation
+    of the criterion (exact solves, exact quadratic forms v'log(A)v of the same probes).
+    Measured: the default path (split-BF16 operand on the int8 kernels) 1.5e-4 / 1.2e-4 / 1.3e-4
+    relative in Vg / Ve / h2, the FP32 path 2.8e-4 / 2.1e-4 / 2.2e-4, the two paths 1.2e-4 apart;
+    what remains is common to both and outside the block product (the stochastic Lanczos
+    quadrature and the criterion's FP32 scalars, kept as in the reference).  Bounds: default
+    2.5e-4, FP32 5e-4, between them 3e-4."""
+    from oracle.dense import sldf_gss
+    import oracle
+    p, y, g = case
+    Q, _ = oreml.covariate_qr(p.X, p.rr)
+    yp = (p.rr * y).astype(np.float32)
+    yp = (yp - Q @ (Q.T @ yp)).astype(np.float32)
+    V = oracle.rademacher(0, p.n, 30)
+    vg, ve, _, h2 = sldf_gss(p.dense(), yp, Q, V, tol=1e-6)
+    ref = dict(vg=vg, ve=ve, h2=h2)
+    got = {prec: g.SLDF_REML(p.X, y, n_V=30, tol_L=1e-6, tol_VC=1e-6, precision=prec) for prec in ("fp32", "bf16x2")}
+    rel = lambda a, b: {k: abs(a[k] - b[k]) / abs(b[k]) for k in ("vg", "ve", "h2")}
+    d_prec = rel(got["bf16x2"], got["fp32"])
+    d64 = {prec: rel(got[prec], ref) for prec in got}
+    msg = f"default vs fp32 {d_prec}; vs fp64 {d64}"
+    assert all(r["evals"] == 30 for r in got.values()), msg
+    assert max(d_prec.values()) < 3e-4, msg
+    assert max(d64["bf16x2"].values()) < 2.5e-4, msg
+    assert max(d64["fp32"].values()) < 5e-4, msg
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
 def test_sldf_matches_restatement(case, prec):
     """Against the FP32 CPU restatement (same probes, same schedule).  Its FP32 Lanczos scalars
