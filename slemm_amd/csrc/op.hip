@@ -133,12 +133,12 @@ void tail_args(const Layout& L, int t, LdsArgs& la) {
 }
 
 // Blocks of 1 to 5 vectors (hi and lo halves as columns) run on the int8 narrow kernel
-// (SLEMM_I8_NARROW=0: the split-BF16 ones), wider blocks on the int8 wide kernel
-// (SLEMM_I8_WIDE=0: the split-BF16 ones); batched-trait blocks, whose tiles carry per-column
-// reduction limits, keep the split-BF16 kernels.  SLEMM_I8_TARGET sets the workgroup target of the
-// reduction split (read per call; by default, for narrow blocks 512 for the SNP-major pass and
-// 1024 for the individual-major one, and for wide blocks 8192, the best of 512 to 8192 at
-// n = 100,000 x m = 500,000).
+// (SLEMM_I8_NARROW=0: the split-BF16 ones), blocks of 9 or more vectors on the int8 wide kernel
+// (SLEMM_I8_WIDE=0: the split-BF16 ones); the batched traits' narrow blocks keep the split-BF16
+// kernels.  SLEMM_I8_TARGET sets the workgroup target of the reduction split (read per call): by
+// default 2048 for narrow blocks (at n = 1M x m = 60k the SNP-major pass needs the split: 3.06 ms
+// at a 512 target, 2.64 ms at 2048; at n = 100k x m = 500k 512 to 4096 are within 2%) and 8192
+// for wide blocks (the best of 1024 to 16384 at n = 100k x m = 500k).
 static bool i8_wide_enabled() {
   const char* e = std::getenv("SLEMM_I8_WIDE");
   return !(e && e[0] == '0');
@@ -154,7 +154,7 @@ static bool use_i8(const Layout& L, int t, const TraitCols* tc) {
 
 static int i8_pass_splits(int pass, int t, int64_t rows_pad, int64_t nsuper, int64_t& nsp) {
   const char* e = std::getenv("SLEMM_I8_TARGET");
-  const int64_t dflt = t <= 5 ? (pass == 0 ? 512 : 1024) : 8192;
+  const int64_t dflt = t <= 5 ? 2048 : 8192;
   const int64_t target = e ? std::max<int64_t>(1, std::atoll(e)) : dflt;
   return i8_splits(t, rows_pad, nsuper, target, nsp);
 }
