@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cstdlib>
 
 #include "blockprod.hpp"
@@ -53,7 +54,7 @@ __device__ __forceinline__ int digit_col(int k, int p, int nvt) {
 }
 
 // Partial maxima of |hi + lo| over 8-row blocks: block (x, k) covers blocks x, x + 64, ... of
-// vector k.
+// vector k.  A NaN or infinite element makes the partial +inf (fmaxf alone would drop NaN).
 __global__ __launch_bounds__(256) void k_i8_colmax(const uint4* __restrict__ X, OperandLayout ol, int64_t nrb,
                                                    float* __restrict__ pmax) {
   __shared__ float red[4];
@@ -64,8 +65,9 @@ __global__ __launch_bounds__(256) void k_i8_colmax(const uint4* __restrict__ X, 
     const uint32_t hw[4] = {h.x, h.y, h.z, h.w}, lw[4] = {l.x, l.y, l.z, l.w};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      mx = fmaxf(mx, fabsf(bf16_lo(hw[q]) + bf16_lo(lw[q])));
-      mx = fmaxf(mx, fabsf(bf16_hi(hw[q]) + bf16_hi(lw[q])));
+      const float a0 = fabsf(bf16_lo(hw[q]) + bf16_lo(lw[q])), a1 = fabsf(bf16_hi(hw[q]) + bf16_hi(lw[q]));
+      mx = fmaxf(mx, a0 <= FLT_MAX ? a0 : INFINITY);
+      mx = fmaxf(mx, a1 <= FLT_MAX ? a1 : INFINITY);
     }
   }
 #pragma unroll
@@ -93,14 +95,17 @@ __global__ __launch_bounds__(256) void k_i8_quant(const uint4* __restrict__ X, O
   }
   __syncthreads();
   const float mx = smax;
-  // s = 2^e >= mx (s = 1 for an all-zero column); mult = 2^22 / s scales exactly
+  // s = 2^e >= mx (s = 1 for an all-zero column); mult = 2^22 / s scales exactly.  A column with
+  // a NaN or infinite element gets zero digits and a NaN scale: all its products come out NaN,
+  // as the floating-point kernels' do.
+  const bool finite = mx <= FLT_MAX;
   int e = 0;
-  if (mx > 0.f) {
+  if (finite && mx > 0.f) {
     (void)frexpf(mx, &e);                      // mx = f 2^e with 0.5 <= f < 1, so mx < 2^e
     if (mx == ldexpf(0.5f, e)) e -= 1;         // a power of two: s = mx
   }
-  const float mult = ldexpf(1.0f, 22 - e);
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc[k] = ldexpf(1.0f, e - 22);
+  const float mult = finite ? ldexpf(1.0f, 22 - e) : 0.f;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[k] = finite ? ldexpf(1.0f, e - 22) : __builtin_nanf("");
   const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (v >= nstep * 4) return;
   const int64_t T = v >> 2;
@@ -114,8 +119,8 @@ __global__ __launch_bounds__(256) void k_i8_quant(const uint4* __restrict__ X, O
     const uint32_t hw[4] = {hv.x, hv.y, hv.z, hv.w}, lw[4] = {lv.x, lv.y, lv.z, lv.w};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      X16[h8 * 8 + 2 * q] = (int)rintf((bf16_lo(hw[q]) + bf16_lo(lw[q])) * mult);
-      X16[h8 * 8 + 2 * q + 1] = (int)rintf((bf16_hi(hw[q]) + bf16_hi(lw[q])) * mult);
+      X16[h8 * 8 + 2 * q] = finite ? (int)rintf((bf16_lo(hw[q]) + bf16_lo(lw[q])) * mult) : 0;
+      X16[h8 * 8 + 2 * q + 1] = finite ? (int)rintf((bf16_hi(hw[q]) + bf16_hi(lw[q])) * mult) : 0;
     }
   }
   uint32_t d[3][4] = {};
@@ -262,9 +267,11 @@ struct I8WArgs {
 // Codes as in the narrow kernel (whole 128-byte lines per row and super-chunk, a super-chunk
 // ahead); the group's digits of each half super-chunk (4 steps) are staged in LDS, requested a
 // half ahead and before the codes, with one barrier per half.
-template <int RT, int NVG>
-__global__ __launch_bounds__(256, 2) void k_bp_i8w(I8WArgs a) {
-  constexpr int NDT = 3 * NVG + 1;  // digit tiles of a group; the last is the tail tile
+// TAIL = false (blocks without a tail tile, one vector tile per group): 3 digit tiles, few enough
+// registers for three waves per SIMD.
+template <int RT, int NVG, bool TAIL = true>
+__global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
+  constexpr int NDT = 3 * NVG + (TAIL ? 1 : 0);  // digit tiles of a group; with TAIL the last is the tail tile
   __shared__ i32x4_t sb[2][NDT * 16 * 16];  // [half parity][tile j][4 w4 + g][column]
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = lane & 15, g = lane >> 4;
@@ -290,7 +297,7 @@ __global__ __launch_bounds__(256, 2) void k_bp_i8w(I8WArgs a) {
   const int64_t wg_row0 = bx * (64 * RT);
   const int vt0 = by * NVG;
   const int nv = min(NVG, a.nvt - vt0);                      // vector tiles of this group
-  const bool tail = a.r8 > 0 && by == (int)gridDim.y - 1;    // the group holds the tail tile
+  const bool tail = TAIL && a.r8 > 0 && by == (int)gridDim.y - 1;  // the group holds the tail tile
   const int64_t s0 = blockIdx.z * a.ns_per_split;
   int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
   const uint32_t tab = 0x02010100u;
@@ -301,8 +308,9 @@ __global__ __launch_bounds__(256, 2) void k_bp_i8w(I8WArgs a) {
   int32_t limmax = 0;
 #pragma unroll
   for (int j = 0; j < NDT; ++j) {
-    const int vtile = j < NDT - 1 ? vt0 + j / 3 : a.nvt;
-    const bool on = j < NDT - 1 ? j < 3 * nv : tail;
+    const bool tslot = TAIL && j == NDT - 1;
+    const int vtile = !tslot ? vt0 + j / 3 : a.nvt;
+    const bool on = !tslot ? j < 3 * nv : tail;
     const int32_t ln = !on ? 0 : (a.tile_nsuper ? a.tile_nsuper[vtile] : INT32_MAX);
     lim[j] = (a.tile_rows && on && row0 >= a.tile_rows[vtile]) ? 0 : ln;
     limmax = max(limmax, (a.tile_rows && on && wg_row0 >= a.tile_rows[vtile]) ? 0 : ln);
@@ -313,7 +321,7 @@ __global__ __launch_bounds__(256, 2) void k_bp_i8w(I8WArgs a) {
   for (int i = 0; i < RT; ++i)
 #pragma unroll
     for (int j = 0; j < NDT; ++j) acc[i][j] = i32x4_t{0, 0, 0, 0};
-  auto live = [&](int j) { return j < 3 * nv || (j == NDT - 1 && tail); };
+  auto live = [&](int j) { return j < 3 * nv || (TAIL && j == NDT - 1 && tail); };
   const uint8_t* lbase = a.G + (row0 + r) * a.ldg + g * 32;
   auto ld = [&](int64_t sc, uint32_t (&dst)[RT][8]) {
     sc = sc < s1 ? sc : s1 - 1;  // past the slice: re-read its last super-chunk (never used)
@@ -338,7 +346,7 @@ __global__ __launch_bounds__(256, 2) void k_bp_i8w(I8WArgs a) {
 #pragma unroll
     for (int q = 0; q < NDT; ++q) {
       if (live(q)) {
-        const int col = q < NDT - 1 ? 16 * (3 * vt0 + q) + cc16 : 48 * a.nvt + cc16 % (3 * a.r8);
+        const int col = !(TAIL && q == NDT - 1) ? 16 * (3 * vt0 + q) + cc16 : 48 * a.nvt + cc16 % (3 * a.r8);
         stg[q] = a.B[brow + col];
       }
     }
@@ -420,7 +428,7 @@ __global__ __launch_bounds__(256, 2) void k_bp_i8w(I8WArgs a) {
                                 (int64_t)acc[i][3 * jv + 2][q]) * scale);
       put(k, row, o);
     }
-    if (tail) {
+    if (TAIL && tail) {
       const int k = 16 * a.nvt + r / 3;
       const bool first = r % 3 == 0 && r < 3 * a.r8;
       const double scale = first ? (double)a.sc[k] : 0.0;
@@ -534,7 +542,9 @@ void launch_bp_i8(const I8Launch& l, hipStream_t st) {
     // (one-vector-tile blocks measured slower with 2 row tiles per wave at four waves per SIMD:
     // each digit load then serves half as many MFMAs)
     const dim3 grid((unsigned)(l.rows_pad / (64 * kI8WRT)), (unsigned)i8_groups(l.t), (unsigned)l.nsplit);
-    k_bp_i8w<kI8WRT, kI8NVG><<<grid, 256, 0, st>>>(a);
+    const char* e3 = std::getenv("SLEMM_I8_ONE_TILE");  // "0": the general kernel (read per call)
+    if (dl.nvt == 1 && dl.r8 == 0 && !(e3 && e3[0] == '0')) k_bp_i8w<kI8WRT, 1, false><<<grid, 256, 0, st>>>(a);
+    else k_bp_i8w<kI8WRT, kI8NVG><<<grid, 256, 0, st>>>(a);
   }
   HIPCHECK(hipGetLastError());
 }

@@ -187,6 +187,32 @@ def test_int8_wide_blocks(gpu, t, monkeypatch):
         assert rel_err(z, uref) < TOL["bf16x2"], (target, rel_err(z, uref))
 
 
+@pytest.mark.parametrize("t", [1, 3, 17])
+def test_int8_nonfinite_column(gpu, t, monkeypatch):
+    """A NaN or infinite element makes every product of its column NaN on the int8 kernels, as on
+    the split-BF16 ones; the other columns are unaffected."""
+    p = make_problem(n=1100, m=900, seed=37)
+    g = _geno(gpu, p)
+    g.set_model(None, None, p.Q)
+    V = np.random.default_rng(5).normal(size=(p.n, t)).astype(np.float32)
+    V[17, 0] = np.nan
+    if t > 1:
+        V[3, t - 1] = np.inf
+    Vd = torch.from_numpy(V).cuda()
+    out = {}
+    for i8 in ("1", "0"):
+        monkeypatch.setenv("SLEMM_I8_NARROW", i8)
+        monkeypatch.setenv("SLEMM_I8_WIDE", i8)
+        out[i8] = g.rmatvec(Vd, precision="bf16x2").cpu().numpy()
+    bad = [0] + ([t - 1] if t > 1 else [])
+    good = [k for k in range(t) if k not in bad]
+    for i8 in out:
+        assert np.all(np.isnan(out[i8][:, bad])), i8
+        assert np.all(np.isfinite(out[i8][:, good])), i8
+    if good:
+        assert rel_err(out["1"][:, good], out["0"][:, good]) < TOL["bf16x2"]
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
 def test_rmatvec_matvec(gpu, prec):
     p = make_problem(n=1500, m=2100, seed=13)
