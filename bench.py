@@ -63,6 +63,25 @@ def i8_issue_factor(K: int) -> float:
     return 16.0 * tiles / K
 
 
+class bf16_mfma_kernels:
+    """Context: run the split-BF16 operand on the BF16 MFMA kernels (FP32 accumulation) instead of
+    the int8 ones (SLEMM_I8_NARROW=0, SLEMM_I8_WIDE=0, read by the library per call)."""
+
+    def __enter__(self):
+        self.saved = {k: os.environ.get(k) for k in ("SLEMM_I8_NARROW", "SLEMM_I8_WIDE")}
+        os.environ["SLEMM_I8_NARROW"] = "0"
+        os.environ["SLEMM_I8_WIDE"] = "0"
+        return self
+
+    def __exit__(self, *exc):
+        for k, v in self.saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+        return False
+
+
 def i8_on(K: int) -> bool:
     """Whether a K-vector block runs on the int8 kernels (SLEMM_I8_NARROW / SLEMM_I8_WIDE); blocks
     of 6 to 8 vectors stay on the BF16 kernels with hi and lo halves as columns."""
@@ -241,6 +260,19 @@ def reml_time_to_converge(g, n, m, nv, y, realised_h2, n_causal, h2):
     out["speedup_bf16x2_over_fp32"] = round(out["fp32"]["seconds"] / out["bf16x2"]["seconds"], 2)
     out["vc_rel_diff_bf16x2_vs_fp32"] = {
         k: float(abs(res["bf16x2"][k] - res["fp32"][k]) / abs(res["fp32"][k])) for k in ("vg", "ve", "h2")}
+    # the same split-BF16 operand on the BF16 MFMA kernels (FP32 accumulation) instead of int8
+    with bf16_mfma_kernels():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = g.SLDF_REML(X, y, rng_seed=0, n_V=nv, precision="bf16x2")
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    out["bf16x2_on_bf16_mfma"] = {"seconds": round(wall, 3), "seconds_seed": round(r["seconds_seed"], 3),
+                                  "lanczos_steps": list(r["steps"]), "h2": round(float(r["h2"]), 6),
+                                  "vg": float(r["vg"]), "ve": float(r["ve"]),
+                                  "speedup_over_fp32": round(out["fp32"]["seconds"] / wall, 2),
+                                  "vc_rel_diff_vs_fp32": {k: float(abs(r[k] - res["fp32"][k]) / abs(res["fp32"][k]))
+                                                          for k in ("vg", "ve", "h2")}}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     r = g.AI_REML(X, y, rng_seed=0, n_V=nv, precision="bf16x2")
@@ -629,6 +661,28 @@ def main():
     k2, _ = kernel_ms(lib, 1, L.PREC_BF16X2)
     lib.slemm_gpu_timing_reset()
 
+    # the same operator on the BF16 MFMA kernels (split-BF16 operand, FP32 accumulation)
+    bf = None
+    if i8_on(K):
+        with bf16_mfma_kernels():
+            tune_bf = None if args.no_autotune else g.autotune(K, reps=2)
+            for _ in range(max(args.warmup, 1)):
+                step("bf16x2")
+            torch.cuda.synchronize()
+            lib.slemm_gpu_timing(1)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                step("bf16x2")
+            torch.cuda.synchronize()
+            dtb = (time.perf_counter() - t0) / args.steps
+            lib.slemm_gpu_timing(0)
+            b1, _ = kernel_ms(lib, 0, L.PREC_BF16X2)
+            b2, _ = kernel_ms(lib, 1, L.PREC_BF16X2)
+            lib.slemm_gpu_timing_reset()
+        bf = {"ms_per_step": round(dtb * 1e3, 3), "GB/s": round(2 * pass_bytes(n, m, K) / dtb / 1e9, 2),
+              "pass1_ms": round(b1, 3), "pass2_ms": round(b2, 3),
+              "kernels": None if tune_bf is None else [tune_bf["pass1"][:2], tune_bf["pass2"][:2]]}
+
     # plain FP32 path beside it
     step("fp32")
     torch.cuda.synchronize()
@@ -687,6 +741,8 @@ def main():
                       "fp32_pass1": round(f1, 3), "fp32_pass2": round(f2, 3)},
         "fp32_path": {"ms_per_step": round(dt32 * 1e3, 3), "GB/s": round(2 * pb / dt32 / 1e9, 2),
                       "speedup_bf16x2_over_fp32": round(dt32 / dt, 2)},
+        "bf16_mfma_kernels": (None if bf is None else
+                              dict(bf, speedup_over_fp32=round(dt32 * 1e3 / bf["ms_per_step"], 2))),
         "bed_generate_s": round(t_load, 3),
         "autotune": tune,
         "reml_time_to_converge": reml,
