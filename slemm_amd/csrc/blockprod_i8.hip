@@ -269,6 +269,9 @@ struct I8WArgs {
 // half ahead and before the codes, with one barrier per half.
 // TAIL = false (blocks without a tail tile, one vector tile per group): 3 digit tiles, few enough
 // registers for three waves per SIMD.
+// (An instantiation without the per-tile branches, for groups of NVG full vector tiles without
+// tile limits, was tried: the scheduler then hoisted LDS reads and decodes, the kernel spilled and
+// ran 20% slower at K=65.)
 template <int RT, int NVG, bool TAIL = true>
 __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
   constexpr int NDT = 3 * NVG + (TAIL ? 1 : 0);  // digit tiles of a group; with TAIL the last is the tail tile
@@ -344,17 +347,16 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
     hh = hh <= hlast ? hh : hlast;
     const int64_t brow = (4 * (4 * hh + (wg >> 2)) + (wg & 3)) * a.ndc;
 #pragma unroll
-    for (int q = 0; q < NDT; ++q) {
-      if (live(q)) {
-        const int col = !(TAIL && q == NDT - 1) ? 16 * (3 * vt0 + q) + cc16 : 48 * a.nvt + cc16 % (3 * a.r8);
-        stg[q] = a.B[brow + col];
-      }
-    }
+    for (int q = 0; q < 3 * NVG; ++q)
+      if (q < 3 * nv) stg[q] = a.B[brow + 16 * (3 * vt0 + q) + cc16];
+    if (TAIL && tail) stg[NDT - 1] = a.B[brow + 48 * a.nvt + cc16 % (3 * a.r8)];
   };
   auto bstore = [&](int64_t hh) {
+    i32x4_t* dst = &sb[hh & 1][threadIdx.x];
 #pragma unroll
-    for (int q = 0; q < NDT; ++q)
-      if (live(q)) sb[hh & 1][256 * q + threadIdx.x] = stg[q];
+    for (int q = 0; q < 3 * NVG; ++q)
+      if (q < 3 * nv) dst[256 * q] = stg[q];
+    if (TAIL && tail) dst[256 * (NDT - 1)] = stg[NDT - 1];
   };
   auto step = [&](const uint32_t (&cc)[RT][8], int w, int buf, int64_t s) {
     i32x4_t av[RT];
@@ -368,11 +370,18 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
     }
     const i32x4_t* bp = &sb[buf][(4 * (w & 3) + g) * 16 + r];
 #pragma unroll
-    for (int j = 0; j < NDT; ++j) {
-      if (!live(j) || s >= lim[j]) continue;
+    for (int j = 0; j < 3 * NVG; ++j) {
+      if (j >= 3 * nv || s >= lim[j]) continue;
       const i32x4_t bv = bp[256 * j];
 #pragma unroll
-      for (int i = 0; i < RT; ++i) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[i], bv, acc[i][j], 0, 0, 0);
+      for (int i = 0; i < RT; ++i)
+        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[i], bv, acc[i][j], 0, 0, 0);
+    }
+    if (TAIL && tail && s < lim[NDT - 1]) {
+      const i32x4_t bv = bp[256 * (NDT - 1)];
+#pragma unroll
+      for (int i = 0; i < RT; ++i)
+        acc[i][NDT - 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[i], bv, acc[i][NDT - 1], 0, 0, 0);
     }
   };
   uint32_t cA[RT][8], cB[RT][8];
