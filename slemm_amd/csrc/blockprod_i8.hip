@@ -369,19 +369,20 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
                       (int)__builtin_amdgcn_perm(0u, tab, (W >> 6) & 0x03030303u)};
     }
     const i32x4_t* bp = &sb[buf][(4 * (w & 3) + g) * 16 + r];
+    // each tile's digits are read one tile ahead, outside the branch around its MFMAs, so that
+    // the read overlaps the previous tile's MFMAs (reads of dead tiles stay inside the buffer and
+    // are not used)
+    i32x4_t bcur = bp[0];
 #pragma unroll
-    for (int j = 0; j < 3 * NVG; ++j) {
-      if (j >= 3 * nv || s >= lim[j]) continue;
-      const i32x4_t bv = bp[256 * j];
+    for (int j = 0; j < NDT; ++j) {
+      const i32x4_t bv = bcur;
+      if (j + 1 < NDT) bcur = bp[256 * (j + 1)];
+      const bool on = (TAIL && j == NDT - 1) ? tail : j < 3 * nv;
+      if (on && s < lim[j]) {
 #pragma unroll
-      for (int i = 0; i < RT; ++i)
-        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[i], bv, acc[i][j], 0, 0, 0);
-    }
-    if (TAIL && tail && s < lim[NDT - 1]) {
-      const i32x4_t bv = bp[256 * (NDT - 1)];
-#pragma unroll
-      for (int i = 0; i < RT; ++i)
-        acc[i][NDT - 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[i], bv, acc[i][NDT - 1], 0, 0, 0);
+        for (int i = 0; i < RT; ++i)
+          acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[i], bv, acc[i][j], 0, 0, 0);
+      }
     }
   };
   uint32_t cA[RT][8], cB[RT][8];
