@@ -39,8 +39,8 @@ MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 15.14e9 + 0.321e9, "pass2"
                     (100_000, 500_000, 1): {"pass1": 12.64e9 + 0.064e9, "pass2": 12.71e9 + 0.039e9}}
 # The same for the int8 kernels (wide kernel at K=65: two column groups, each reading the codes;
 # narrow kernel at K=1); profiles/blockprod_int8_c2_hbm_traffic.md.
-I8_MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 27.08e9 + 0.398e9, "pass2": 30.74e9 + 0.293e9},
-                       (100_000, 500_000, 1): {"pass1": 12.57e9 + 0.004e9, "pass2": 12.59e9 + 0.005e9}}
+I8_MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 27.62e9 + 0.398e9, "pass2": 32.85e9 + 0.293e9},
+                       (100_000, 500_000, 1): {"pass1": 12.59e9 + 0.012e9, "pass2": 12.59e9 + 0.009e9}}
 
 
 def issue_factor(K: int) -> float:
