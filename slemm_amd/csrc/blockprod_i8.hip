@@ -1,6 +1,8 @@
-// Narrow block products (at most 5 vectors, hi and lo halves as columns of one tile) on the int8
-// matrix cores: the Z'x / Zx passes of src/reml.cpp:6-233 for the Lanczos / CG vectors, the
-// single right-hand sides and the GWAS score vectors.
+// Block products of the split-BF16 operand on the int8 matrix cores: the Z'x / Zx passes of
+// src/reml.cpp:6-233 for the probe blocks, Lanczos / CG vectors, single right-hand sides and the
+// GWAS score vectors.  A narrow kernel (1 to 5 vectors: one MFMA tile, digits in registers, no
+// LDS) and a wide kernel (16-vector tiles of three digit tiles each plus a tail tile, digits
+// staged in LDS) share the digit preparation below.
 //
 // The operand is the split-BF16 one the pack and pass-1 epilogue kernels write (value hi + lo per
 // element; their column sums feed the exact-centring terms).  Per column it is rendered as a
@@ -77,39 +79,49 @@ __global__ __launch_bounds__(256) void k_i8_colmax(const uint4* __restrict__ X, 
   if (threadIdx.x == 0) pmax[k * kI8Blocks + blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
+// Per-vector scale from the 64 partial maxima: sc[k] = s_k 2^-22 with s_k = 2^e_k the power of
+// two at or above the column's largest magnitude (s = 1 for an all-zero column), ex[k] = e_k.  A
+// column with a NaN or infinite element (partial +inf) gets ex[k] = INT32_MIN and a NaN scale:
+// zero digits, and all its products come out NaN, as the floating-point kernels' do.
+__global__ __launch_bounds__(256) void k_i8_scale(const float* __restrict__ pmax, int t, float* __restrict__ sc,
+                                                  int* __restrict__ ex) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= t) return;
+  float mx = 0.f;
+  for (int q = 0; q < kI8Blocks; ++q) mx = fmaxf(mx, pmax[k * kI8Blocks + q]);
+  if (!(mx <= FLT_MAX)) {
+    sc[k] = __builtin_nanf("");
+    ex[k] = INT32_MIN;
+    return;
+  }
+  int e = 0;
+  if (mx > 0.f) {
+    (void)frexpf(mx, &e);               // mx = f 2^e with 0.5 <= f < 1, so mx < 2^e
+    if (mx == ldexpf(0.5f, e)) e -= 1;  // a power of two: s = mx
+  }
+  sc[k] = ldexpf(1.0f, e - 22);
+  ex[k] = e;
+}
+
 // Digits in MFMA lane order.  Step T (64 reduction rows) of lane group g covers rows
 // 512 (T / 8) + 128 g + 16 (T % 8) + j, j < 16, the rows whose codes the block-product kernels'
 // lane group g decodes at that step; byte 4 m + b of a lane's 16 bytes holds row j = 4 b + m (the
-// order in which the codes decode).  B[(4 T + g) ndc + digit_col(k, p)] holds digit p of vector k.
-// Thread (x, k): one (T, g) pair of vector k.  sc[k] = s_k 2^-22 (written by block 0).
+// order in which the codes decode).  B[(4 T + g) ndc + digit_col(k, p)] holds digit p of vector k:
+// X = rint((hi + lo) 2^(22 - e_k)) = 2^16 d0 + 2^8 d1 + d2 with balanced digits.  Thread v: vector
+// k = v mod t of the pair (T, g) = (v / t) / 4, (v / t) mod 4, so that neighbouring threads read
+// neighbouring columns of the operand's row blocks and write neighbouring digit columns.
 __global__ __launch_bounds__(256) void k_i8_quant(const uint4* __restrict__ X, OperandLayout ol, int nvt, int ndc,
-                                                  int64_t nstep, const float* __restrict__ pmax,
-                                                  float* __restrict__ sc, i32x4_t* __restrict__ B) {
-  __shared__ float smax;
-  const int k = blockIdx.y;
-  if (threadIdx.x < 64) {
-    float mx = pmax[k * kI8Blocks + threadIdx.x];
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    if (threadIdx.x == 0) smax = mx;
-  }
-  __syncthreads();
-  const float mx = smax;
-  // s = 2^e >= mx (s = 1 for an all-zero column); mult = 2^22 / s scales exactly.  A column with
-  // a NaN or infinite element gets zero digits and a NaN scale: all its products come out NaN,
-  // as the floating-point kernels' do.
-  const bool finite = mx <= FLT_MAX;
-  int e = 0;
-  if (finite && mx > 0.f) {
-    (void)frexpf(mx, &e);                      // mx = f 2^e with 0.5 <= f < 1, so mx < 2^e
-    if (mx == ldexpf(0.5f, e)) e -= 1;         // a power of two: s = mx
-  }
-  const float mult = finite ? ldexpf(1.0f, 22 - e) : 0.f;
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc[k] = finite ? ldexpf(1.0f, e - 22) : __builtin_nanf("");
+                                                  int64_t nstep, const int* __restrict__ ex,
+                                                  i32x4_t* __restrict__ B) {
   const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (v >= nstep * 4) return;
-  const int64_t T = v >> 2;
-  const int g = (int)(v & 3);
+  if (v >= nstep * 4 * ol.t) return;
+  const int k = (int)(v % ol.t);
+  const int64_t tg = v / ol.t;
+  const int64_t T = tg >> 2;
+  const int g = (int)(tg & 3);
+  const int e = ex[k];
+  const bool finite = e != INT32_MIN;
+  const float mult = finite ? ldexpf(1.0f, 22 - e) : 0.f;  // scales exactly: |X| <= 2^22
   const int64_t r0 = (T >> 3) * 512 + g * 128 + (T & 7) * 16;  // first of 16 rows
   int X16[16];
 #pragma unroll
@@ -131,10 +143,10 @@ __global__ __launch_bounds__(256) void k_i8_quant(const uint4* __restrict__ X, O
     const int x1 = (x - d2) >> 8;
     const int d1 = ((x1 + 128) & 255) - 128;
     const int d0 = (x1 - d1) >> 8;  // |x| <= 2^22: d0 in [-64, 64]
-    const int m = j & 3, b = j >> 2;  // byte 4 m + b: dword m, byte b
-    d[0][m] |= (uint32_t)(d0 & 255) << (8 * b);
-    d[1][m] |= (uint32_t)(d1 & 255) << (8 * b);
-    d[2][m] |= (uint32_t)(d2 & 255) << (8 * b);
+    const int m = j & 3, bb = j >> 2;  // byte 4 m + bb: dword m, byte bb
+    d[0][m] |= (uint32_t)(d0 & 255) << (8 * bb);
+    d[1][m] |= (uint32_t)(d1 & 255) << (8 * bb);
+    d[2][m] |= (uint32_t)(d2 & 255) << (8 * bb);
   }
 #pragma unroll
   for (int p = 0; p < 3; ++p)
@@ -491,8 +503,10 @@ I8Digits i8_digits(int t) {
   return d;
 }
 
+// scratch: sc[t] (floats) and ex[t] (ints), each rounded up to 64 entries, the 64 partial maxima
+// per vector, then the digits (16-byte aligned)
 size_t i8_scratch_bytes(int64_t red_pad, int t) {
-  return sizeof(float) * (64 * (size_t)((t + 63) / 64) + (size_t)t * kI8Blocks) + (size_t)red_pad * i8_digits(t).ndc;
+  return sizeof(float) * (2 * 64 * (size_t)((t + 63) / 64) + (size_t)t * kI8Blocks) + (size_t)red_pad * i8_digits(t).ndc;
 }
 
 int i8_groups(int t) {
@@ -512,16 +526,18 @@ void launch_bp_i8(const I8Launch& l, hipStream_t st) {
   if (l.squared && l.t != 1) fail(SLEMM_ERR_ARG, "squared-dosage products take one vector");
   if (l.tail_r > 0 && (l.tail0 + 2 * (l.t - l.tail0) > l.KP || l.tail_r != l.t - l.tail0))
     fail(SLEMM_ERR_ARG, "int8 block: lo columns outside the block");
+  const size_t t64 = 64 * (size_t)((l.t + 63) / 64);
   float* sc = static_cast<float*>(l.scratch);
-  float* pmax = sc + 64 * ((l.t + 63) / 64);
+  int* ex = reinterpret_cast<int*>(sc + t64);
+  float* pmax = sc + 2 * t64;
   i32x4_t* B = reinterpret_cast<i32x4_t*>(pmax + (size_t)l.t * kI8Blocks);
   if (reinterpret_cast<uintptr_t>(B) % 16 != 0) fail(SLEMM_ERR_ARG, "int8 block: scratch alignment");
   const uint4* X = static_cast<const uint4*>(l.X);
   OperandLayout ol{l.KP, l.t, l.lo_off16, l.tail0, l.tail_r};
   k_i8_colmax<<<dim3(kI8Blocks, l.t), 256, 0, st>>>(X, ol, l.red_pad / 8, pmax);
   const int64_t nstep = l.red_pad / 64;
-  k_i8_quant<<<dim3((unsigned)((nstep * 4 + 255) / 256), l.t), 256, 0, st>>>(X, ol, dl.nvt, dl.ndc, nstep, pmax,
-                                                                              sc, B);
+  k_i8_scale<<<(unsigned)((l.t + 255) / 256), 256, 0, st>>>(pmax, l.t, sc, ex);
+  k_i8_quant<<<(unsigned)((nstep * 4 * l.t + 255) / 256), 256, 0, st>>>(X, ol, dl.nvt, dl.ndc, nstep, ex, B);
   if (narrow) {
     I8Args a;
     a.G = l.G; a.ldg = l.ldg; a.nsuper = l.red_pad / 512; a.ns_per_split = l.ns_per_split; a.B = B; a.t = l.t;
