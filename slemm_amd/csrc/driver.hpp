@@ -87,7 +87,13 @@ void extract_dosage_cols(slemm_geno* h, const std::vector<int64_t>& snps, std::v
 void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B);
 
 // REML_criterion at h2 (src/reml.cpp:457-489): returns 2 logLL, sets vg, ve and the LRT.
-using Criterion = std::function<float(float h2, float& vg, float& ve, float& lrt)>;
+// REML criterion at h2; returns the value the golden-section search compares (rounded to float
+// as src/reml.cpp:457-489 does unless SLEMM_CRITERION_FP64=1, see criterion_value)
+using Criterion = std::function<double(float h2, float& vg, float& ve, float& lrt)>;
+// The criterion as the search compares it: rounded to float (the reference's arithmetic), or kept
+// in double with SLEMM_CRITERION_FP64=1 (read per call), which lets the search resolve the flat
+// top of the likelihood below float's resolution.
+double criterion_value(double c);
 
 // REML_GSS (src/reml.cpp:493-547): golden-section search on (s2min, s2max) to tol_VC; vg, ve
 // and lrt are those of the last evaluated point, as in the reference.

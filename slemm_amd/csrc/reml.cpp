@@ -40,18 +40,23 @@ void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B) {
   }
 }
 
+double criterion_value(double c) {
+  const char* e = std::getenv("SLEMM_CRITERION_FP64");
+  return (e && e[0] == '1') ? c : (double)(float)c;
+}
+
 void golden_section(float s2min, float s2max, float tol_VC, int verbose, const Criterion& criterion, float& vg,
                     float& ve, float& lrt) {
   const double invphi = 0.618033988749894848204586834365638117720309179805762862135;
   const double invphi2 = 0.381966011250105151795413165634361882279690820194237137864;
   float a = s2min, b = s2max;
-  float fm = 0;
+  double fm = 0;
   if (verbose) printf("\nDoing golden section search in (%g, %g)\n", a, b);
   float hh = b - a;
   const int nitr = hh < tol_VC ? 1 : (int)std::ceil(std::log(tol_VC / hh) / std::log(invphi));
   float cc = (float)(a + invphi2 * hh), dd = (float)(a + invphi * hh);
-  float fc = criterion(cc, vg, ve, lrt);
-  float fd = criterion(dd, vg, ve, lrt);
+  double fc = criterion(cc, vg, ve, lrt);
+  double fd = criterion(dd, vg, ve, lrt);
   for (int i = 0; i < nitr; ++i) {
     if (fc > fd) {
       b = dd;
@@ -185,7 +190,7 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
     std::vector<float> hx;
 
     // REML_criterion (:457-489)
-    auto criterion = [&](float ss, float& vg, float& ve, float& lrt) -> float {
+    auto criterion = [&](float ss, float& vg, float& ve, float& lrt) -> double {
       ++evals;
       const float tau = (1 - ss) / ss;
       const float sigma = tau - tau0;
@@ -209,7 +214,7 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
           for (int64_t i = 0; i < n; ++i) s += Q(i, a) * hx[i + (size_t)b * n];
           QtX[a + (size_t)b * c] = s;
         }
-      return (float)((double)(n - c) * std::log(ySy / yPy) - log_abs_det(QtX, c) - ldetH);
+      return criterion_value((double)(n - c) * std::log(ySy / yPy) - log_abs_det(QtX, c) - ldetH);
     };
 
     // REML_GSS (:493-547): golden-section search, returns the last evaluated point

@@ -95,7 +95,7 @@ extern "C" int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int 
       const float* qcol = dB.p + (size_t)(T + t) * n;
       int evals = 0;
       // REML_criterion (src/reml.cpp:457-489) on the trait's subset (n_t individuals, c = 1)
-      auto criterion = [&](float ss, float& vg, float& ve, float& lrt) -> float {
+      auto criterion = [&](float ss, float& vg, float& ve, float& lrt) -> double {
         ++evals;
         const float tau = (1 - ss) / ss;
         const float sigma = tau - tau0;
@@ -110,7 +110,7 @@ extern "C" int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int 
         lrt = (float)((double)nt * std::log(ySy / yPy) - ldetH);
         abi(slemm_gpu_lsolve(sg.s, T + t, 1, qcol, n, sigma, 1e-6f, dX.p, n, nullptr));
         const double QtX = dot_dev(qcol, dX.p);
-        return (float)((double)(nt - 1) * std::log(ySy / yPy) - std::log(std::fabs(QtX)) - ldetH);
+        return criterion_value((double)(nt - 1) * std::log(ySy / yPy) - std::log(std::fabs(QtX)) - ldetH);
       };
       float vg = 0, ve = 0, lrt = 0;
       golden_section(s2min, s2max, tol_VC, verbose, criterion, vg, ve, lrt);

@@ -83,10 +83,34 @@ def test_sldf_tight_search_vc_precision(case):
     d_prec = rel(got["bf16x2"], got["fp32"])
     d64 = {prec: rel(got[prec], ref) for prec in got}
     msg = f"default vs fp32 {d_prec}; vs fp64 {d64}"
+    print(msg)
     assert all(r["evals"] == 30 for r in got.values()), msg
     assert max(d_prec.values()) < 3e-4, msg
     assert max(d64["bf16x2"].values()) < 2.5e-4, msg
     assert max(d64["fp32"].values()) < 5e-4, msg
+
+
+def test_sldf_fp64_criterion_vc_against_fp64(case, monkeypatch):
+    """As above with the criterion compared in double (SLEMM_CRITERION_FP64=1) instead of rounded
+    to float as the reference does.  Measured against the FP64 dense evaluation: default path
+    6.1e-5 / 4.7e-5 / 5.0e-5 relative in Vg / Ve / h2, FP32 path 5.2e-5 / 4.0e-5 / 4.2e-5 (with the
+    float criterion: 1.5e-4 / 1.2e-4 / 1.3e-4 and 2.8e-4 / 2.1e-4 / 2.2e-4), so the float rounding
+    of the criterion is what held the search back.  Bound: 1e-4 for the default path."""
+    from oracle.dense import sldf_gss
+    import oracle
+    p, y, g = case
+    Q, _ = oreml.covariate_qr(p.X, p.rr)
+    yp = (p.rr * y).astype(np.float32)
+    yp = (yp - Q @ (Q.T @ yp)).astype(np.float32)
+    V = oracle.rademacher(0, p.n, 30)
+    vg, ve, _, h2 = sldf_gss(p.dense(), yp, Q, V, tol=1e-6)
+    monkeypatch.setenv("SLEMM_CRITERION_FP64", "1")
+    got = {prec: g.SLDF_REML(p.X, y, n_V=30, tol_L=1e-6, tol_VC=1e-6, precision=prec) for prec in ("bf16x2", "fp32")}
+    d = {prec: {k: abs(r[k] - ref) / abs(ref) for k, ref in (("vg", vg), ("ve", ve), ("h2", h2))}
+         for prec, r in got.items()}
+    msg = f"vs fp64 with a double criterion: {d}"
+    print(msg)
+    assert max(d["bf16x2"].values()) < 1e-4, msg
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
