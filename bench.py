@@ -273,6 +273,28 @@ def reml_time_to_converge(g, n, m, nv, y, realised_h2, n_causal, h2):
                                   "speedup_over_fp32": round(out["fp32"]["seconds"] / wall, 2),
                                   "vc_rel_diff_vs_fp32": {k: float(abs(r[k] - res["fp32"][k]) / abs(res["fp32"][k]))
                                                           for k in ("vg", "ve", "h2")}}
+    # the same fits with the criterion compared in double by the golden-section search
+    # (SLEMM_CRITERION_FP64=1; the default rounds it to float as the reference does)
+    saved = os.environ.get("SLEMM_CRITERION_FP64")
+    os.environ["SLEMM_CRITERION_FP64"] = "1"
+    try:
+        c64 = {}
+        for prec in ("bf16x2", "fp32"):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = g.SLDF_REML(X, y, rng_seed=0, n_V=nv, precision=prec)
+            torch.cuda.synchronize()
+            c64[prec] = (time.perf_counter() - t0, r)
+    finally:
+        if saved is None:
+            os.environ.pop("SLEMM_CRITERION_FP64", None)
+        else:
+            os.environ["SLEMM_CRITERION_FP64"] = saved
+    out["criterion_fp64"] = {
+        prec: {"seconds": round(sec, 3), "h2": round(float(r["h2"]), 6), "vg": float(r["vg"]), "ve": float(r["ve"]),
+               "criterion_evals": int(r["evals"])} for prec, (sec, r) in c64.items()}
+    out["criterion_fp64"]["vc_rel_diff_bf16x2_vs_fp32"] = {
+        k: float(abs(c64["bf16x2"][1][k] - c64["fp32"][1][k]) / abs(c64["fp32"][1][k])) for k in ("vg", "ve", "h2")}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     r = g.AI_REML(X, y, rng_seed=0, n_V=nv, precision="bf16x2")
