@@ -566,6 +566,7 @@ extern "C" int slemm_gpu_geno_destroy(slemm_geno* h) {
     if (h->rr) (void)hipFree(h->rr);
     if (h->Q) (void)hipFree(h->Q);
     delete h;
+    (void)hipGetLastError();  // nothing of this call may stay pending for the caller's next check
   });
 }
 

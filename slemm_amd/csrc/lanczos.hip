@@ -19,7 +19,10 @@
 #include "tridiag.hpp"
 
 struct slemm_lseed {
-  slemm_geno* h = nullptr;
+  // The device is kept by value: the seed's stored basis and coefficients outlive the genotype
+  // handle it was built from (a caller may destroy that handle first), so nothing here may
+  // reach through a slemm_geno pointer after the seed has been built.
+  int device = 0;
   int t = 0;
   int64_t n = 0;
   int keep_cols = 0;
@@ -179,7 +182,7 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
     }
 
     auto s = std::make_unique<slemm_lseed>();
-    s->h = h;
+    s->device = h->device;
     s->t = t;
     s->n = n;
     s->keep_cols = keep_cols;
@@ -431,8 +434,9 @@ extern "C" int slemm_gpu_lseed_coeffs(const slemm_lseed* s, double* beta, double
 extern "C" int slemm_gpu_lseed_destroy(slemm_lseed* s) {
   return guard([&] {
     if (!s) return;
-    if (s->h) (void)hipSetDevice(s->h->device);
+    (void)hipSetDevice(s->device);
     delete s;
+    (void)hipGetLastError();  // nothing of this call may stay pending for the caller's next check
   });
 }
 
@@ -442,7 +446,7 @@ extern "C" int slemm_gpu_lsolve(const slemm_lseed* s, int col0, int t2, const fl
     REQUIRE(s && B && X, "null argument");
     REQUIRE(t2 > 0 && col0 >= 0 && col0 + t2 <= s->keep_cols, "columns must be kept columns of the seed");
     REQUIRE(ldb >= s->n && ldx >= s->n, "leading dimension < n");
-    HIPCHECK(hipSetDevice(s->h->device));
+    HIPCHECK(hipSetDevice(s->device));
     hipStream_t st = (hipStream_t)stream;
     const int t = s->t;
     // host replay of the scalar recurrences (src/reml.cpp:404-426); ||U_{j+1}|| = 1

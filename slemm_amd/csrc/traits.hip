@@ -180,6 +180,7 @@ extern "C" int slemm_gpu_traits_create(slemm_geno* h, int T, const uint8_t* mask
     const int64_t n = h->n, m = h->m, ldn = h->n_pad, ldm = h->m_pad;
     auto tr = std::make_unique<slemm_traits>();
     tr->h = h;
+    tr->device = h->device;
     tr->T = T;
     tr->n_host.assign(T, 0);
     tr->members.assign(T, {});
@@ -250,9 +251,10 @@ extern "C" int slemm_gpu_traits_create(slemm_geno* h, int T, const uint8_t* mask
 extern "C" int slemm_gpu_traits_destroy(slemm_traits* tr) {
   return guard([&] {
     if (!tr) return;
-    (void)hipSetDevice(tr->h->device);
+    (void)hipSetDevice(tr->device);
     (void)hipDeviceSynchronize();
     delete tr;
+    (void)hipGetLastError();
   });
 }
 

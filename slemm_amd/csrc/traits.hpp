@@ -9,7 +9,8 @@
 // SNP weights over that subset (src/reml.cpp:612-622 evaluated on the subset), and an intercept
 // as its only covariate.  Vector columns of a trait are zero outside its subset.
 struct slemm_traits {
-  slemm_geno* h = nullptr;
+  slemm_geno* h = nullptr;     // used by every batched call; destroy reads only `device`
+  int device = 0;
   int T = 0;
   slemm::DevBuf mask, q;       // T x n_pad: membership (0/1) and intercept basis mask/sqrt(n_t)
   slemm::DevBuf n_t;           // T doubles

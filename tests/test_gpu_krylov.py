@@ -73,3 +73,31 @@ def test_slq_against_exact_quadrature(prob, prec):
         est = s.SLQ_LDet(tau - tau0)
         exact = D.quad_logm("ldet", tau, V) * p.n / nv
         assert abs(est - exact) / abs(exact) < (1e-6 if prec == "fp32" else 2e-5)
+
+
+def test_seed_outlives_its_genotype_handle(gpu):
+    """A seed keeps its own device and basis: L_Solve still works, and destroying it leaves no
+    pending HIP error behind, after the genotype handle it was built on is gone; a new handle is
+    built and used normally afterwards.  (The seed once read its device through the handle at
+    destroy time, which read freed memory when the handle went first.)"""
+    from slemm_amd import _lib
+    p = make_problem(n=700, m=900, seed=4)
+    g = gpu.Geno.from_bed(p.rows, p.n)
+    g.set_model()
+    tau0 = (1 - 0.7) / 0.7
+    rng = np.random.default_rng(2)
+    b = torch.from_numpy(rng.normal(size=(p.n, 1)).astype(np.float32)).cuda()
+    s = g.L_Seed("ldet", tau0, b, tol=1e-6)
+    x0 = s.L_Solve(b, 0.1).cpu().numpy()
+    s._owner = None          # drop the keep-alive reference: the handle goes first
+    g.close()
+    x1 = s.L_Solve(b, 0.1).cpu().numpy()
+    assert np.array_equal(x0, x1)
+    lib = _lib.load()
+    _lib.check(lib.slemm_gpu_lseed_destroy(s._h))
+    s._h = None
+    g2 = gpu.Geno.from_bed(p.rows, p.n)
+    g2.set_model()
+    out = g2.lnr_op_select("ldet", tau0, b, precision="bf16x2").cpu().numpy()
+    ref = p.dense().op("ldet", tau0, b.cpu().numpy())
+    assert rel_err(out, ref) < 3e-4
