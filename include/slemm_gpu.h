@@ -194,6 +194,8 @@ int slemm_gpu_lseed_col_steps(const slemm_lseed* s, int* steps);
 /* beta (host, t x (steps+1), column-major) and delta (t x steps); entries past a column's own
  * step count are zero.  Same layout as struct lseed's beta/delta (src/reml.cpp:333-336). */
 int slemm_gpu_lseed_coeffs(const slemm_lseed* s, double* beta, double* delta);
+/* A seed keeps its own copy of everything it needs: it stays valid after its genotype handle has
+ * been destroyed. */
 int slemm_gpu_lseed_destroy(slemm_lseed* s);
 
 /* L_Solve (src/reml.cpp:369-431): X (device n x t2, ldx) solving (A + sigma I) X = B by
@@ -235,6 +237,14 @@ int slemm_gpu_lmc_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const 
                        int rng_seed, int n_MC, float tol_L, float s2max, float s2min,
                        int precision, int verbose,
                        float* Py, float* snp_blup, float* blue, slemm_reml_stats* stats);
+/* The same with the secant's stop chosen by the caller (src/reml.cpp:830 stops when the update of
+ * log((1-h2)/h2) is below 0.002 and :824 after 9 evaluations, which slemm_gpu_lmc_reml keeps):
+ * secant_tol > 0, secant_max_iter >= 2 evaluations in all.  A tighter stop takes h2 to the root
+ * of REML_first_order instead of the iterate the reference reports. */
+int slemm_gpu_lmc_reml_ex(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y,
+                          int rng_seed, int n_MC, float tol_L, float s2max, float s2min,
+                          float secant_tol, int secant_max_iter, int precision, int verbose,
+                          float* Py, float* snp_blup, float* blue, slemm_reml_stats* stats);
 
 /* REML score and average-information matrix at (vg, ve) for the model of SLDF_REML (no
  * reference counterpart): score = (dl/dvg, dl/dve), ai = (AI_gg, AI_ge, AI_ee) with
@@ -265,6 +275,7 @@ int slemm_gpu_ai_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const f
  * SNP without variation in a subset gets weight 0 and is counted in `monomorphic`) and an
  * intercept as its only covariate; R = I. */
 int slemm_gpu_traits_create(slemm_geno* h, int T, const uint8_t* masks, slemm_traits** out);
+/* A traits handle uses its genotype handle in every call: destroy it before that handle. */
 int slemm_gpu_traits_destroy(slemm_traits* tr);
 /* Host outputs (T each, may be NULL): subset sizes and per-subset monomorphic SNP counts. */
 int slemm_gpu_traits_info(const slemm_traits* tr, int64_t* n_t, int64_t* monomorphic);
@@ -322,8 +333,8 @@ int slemm_gpu_iter_weighting_reml(slemm_geno* h, const float* X, int64_t ldx, in
 /* ---------------------------------------------------------------- LMM quadratic forms ------ */
 
 /* Quadratic forms of the --lmm association approximation (src/reml.cpp:884-982, real-genotype
- * branch): for each SNP snps[i] (host, k entries; the reference samples num_qf_markers SNPs with
- * MAC > 20 by std::random_shuffle, the caller chooses them here), after a fit with heritability
+ * branch): for each SNP snps[i] (host, k entries; NULL = the reference's own sample of k SNPs with
+ * MAC > 20, see slemm_gpu_lmm_sample), after a fit with heritability
  * h2 (tau1 = (1-h2)/h2) whose Py (host, n) it returns: xPy = x'Py, xPx = v'H^-1 v (Lanczos seed
  * at tau1 with tolerance tol_L, then L_Solve), xSx = v'v with v = (I-QQ') R^-1/2 (x - gm), and
  * xBx = (v'v - z'(Zw'Zw + tau1 diag(1/gw))^-1 z)/tau1 over the window_size SNPs around the SNP
@@ -333,6 +344,14 @@ int slemm_gpu_iter_weighting_reml(slemm_geno* h, const float* X, int64_t ldx, in
 int slemm_gpu_lmm_gstat(slemm_geno* h, const float* X, int64_t ldx, int c, const float* Py, float h2,
                         const int64_t* snps, int k, int window_size, float tol_L, int precision, int verbose,
                         float* xPy, float* xPx, float* xBx, float* xSx);
+
+/* The --lmm SNP sample of src/reml.cpp:920-925, host only: the SNPs i with gm_i*n > 20 and
+ * (2-gm_i)*n > 20 (float arithmetic) in the order std::random_shuffle leaves them when rand() is
+ * in its initial state (the reference never seeds it), the first min(num_qf_markers, N) of them.
+ * gm: host, m floats (the genotype means of slemm_gpu_geno_counts).  snps (host, capacity
+ * num_qf_markers; NULL to ask only for the count) receives the indices, *k their number.
+ * slemm_gpu_lmm_gstat with snps == NULL uses this sample of the handle's own gm. */
+int slemm_gpu_lmm_sample(const float* gm, int64_t m, int64_t n, int num_qf_markers, int64_t* snps, int* k);
 
 /* --fake_geno branch of the --lmm quadratic forms (src/reml.cpp:886-918), after an LMC_REML fit
  * with seed rng_seed and n_MC samples: k simulated markers continue the fit's random stream

@@ -5,7 +5,8 @@ For small n the genomic relationship operator of src/reml.cpp is formed explicit
   K  = Zc diag(gw) Zc'
   ldet(v)     = diag(rr) K diag(rr) v + tau0 v                       (H0_ldet_mv, :52-94)
   quadform(v) = P diag(rr) K diag(rr) P v + tau0 v, P = I - Q Q'     (H0_quadform_mv, :8-50)
-and solves / log-determinants are exact (Cholesky / symmetric eigendecomposition).
+and solves / log-determinants are exact: each operator kind is eigendecomposed once in FP64 and
+every shift A0 + tau I is applied through that decomposition.
 """
 from __future__ import annotations
 
@@ -35,6 +36,7 @@ class Dense:
         self.P = np.eye(self.n) - self.Q @ self.Q.T
         self.A_ldet0 = self.rr[:, None] * self.K * self.rr[None, :]
         self.A_quad0 = self.P @ self.A_ldet0 @ self.P
+        self._eig = {}
 
     def A(self, kind: str, tau0: float) -> np.ndarray:
         base = self.A_quad0 if kind == "quadform" else self.A_ldet0
@@ -51,19 +53,30 @@ class Dense:
     def matvec(self, U):
         return self.Zc @ np.asarray(U, dtype=np.float64)
 
+    def eig(self, kind: str):
+        """Eigendecomposition (w, U) of the unshifted operator A0 of `kind`, computed once: every
+        shift tau then costs only products with U, since A0 + tau I = U diag(w + tau) U'."""
+        if kind not in self._eig:
+            base = self.A_quad0 if kind == "quadform" else self.A_ldet0
+            self._eig[kind] = np.linalg.eigh(base)
+        return self._eig[kind]
+
     def solve(self, kind: str, tau: float, B) -> np.ndarray:
         """(A0 + tau I)^-1 B, the quantity L_Solve approximates with sigma = tau - tau0."""
-        return np.linalg.solve(self.A(kind, tau), np.asarray(B, dtype=np.float64))
+        w, U = self.eig(kind)
+        B = np.asarray(B, dtype=np.float64)
+        return U @ ((U.T @ B) / (w + tau).reshape((-1,) + (1,) * (B.ndim - 1)))
 
     def quad_logm(self, kind: str, tau: float, V) -> float:
         """sum_k v_k' log(A0 + tau I) v_k -- the exact value SLQ estimates for fixed probes."""
-        w, U = np.linalg.eigh(self.A(kind, tau))
+        w, U = self.eig(kind)
         C = U.T @ np.asarray(V, dtype=np.float64)
-        return float(np.sum((C ** 2) * np.log(w)[:, None]))
+        C = C.reshape(C.shape[0], -1)
+        return float(np.sum((C ** 2) * np.log(w + tau)[:, None]))
 
     def logdet(self, kind: str, tau: float) -> float:
-        sign, ld = np.linalg.slogdet(self.A(kind, tau))
-        return float(ld)
+        w, _ = self.eig(kind)
+        return float(np.sum(np.log(w + tau)))
 
 
 def reml_loglik(Z, gm, gw, X, y, h2: float, rr=None) -> float:
@@ -132,6 +145,80 @@ def sldf_gss(D: "Dense", y_proj, Q, V, s2min=1e-4, s2max=0.7, tol=1e-4):
             fd, *last = sldf_criterion(D, y_proj, Q, V, d, tau0)
     vg, ve, lrt = last
     return vg, ve, lrt, vg / (vg + ve)
+
+
+def lmc_inputs(D: "Dense", y, u, e) -> np.ndarray:
+    """The Lanczos block of LMC_REML (src/reml.cpp:788-813) in FP64 from the same float draws:
+    V = (I - QQ') [R^-1/2 y, R^-1/2 Zc (sqrt(gw) u), e], with sqrt(gw) u a float product as in
+    the reference (u: m x n_MC normals, e: n x n_MC normals)."""
+    gw32 = np.asarray(D.gw, np.float32)
+    us = (np.asarray(u, np.float32) * np.sqrt(gw32)[:, None]).astype(np.float64)
+    V = np.concatenate([np.asarray(y, np.float64)[:, None], D.Zc @ us, np.asarray(e, np.float64)], axis=1)
+    k = us.shape[1]
+    V[:, :1 + k] *= D.rr[:, None]
+    return D.P @ V
+
+
+def lmc_first_order(D: "Dense", V, h2: float, expect: bool = False):
+    """REML_first_order (src/reml.cpp:696-728) in FP64 with exact solves of the projected system
+    (A_quad0 + tau I) X = V, tau = (1 - h2)/h2:
+      Py_0 = P x_0, Py_k = P (x_k + sqrt(tau) x_{n_MC+k}),  e = Py,  u = sqrt(gw) Zc' R^-1/2 Py,
+      f = log[(|u_0|^2/|e_0|^2) / (sum_k |u_k|^2 / sum_k |e_k|^2)].
+    With expect=True the MC ratio is replaced by its expectation tr(A M)/tr(M), M the projected
+    inverse (I-QQ')(A_quad0 + tau I)^-1 (I-QQ'), for which the root of f is the REML optimum.
+    Returns (f, Py_0) with Py_0 already scaled by R^-1/2 (the Py of the fit)."""
+    tau = (1 - h2) / h2
+    k = (V.shape[1] - 1) // 2
+    X = D.solve("quadform", tau, V)
+    Py = np.concatenate([X[:, :1], X[:, 1:1 + k] + np.sqrt(tau) * X[:, 1 + k:]], axis=1)
+    Py = D.P @ Py
+    eB = Py
+    Pr = D.rr[:, None] * Py
+    uB = np.sqrt(D.gw)[:, None] * (D.Zc.T @ Pr)
+    ratio_y = np.sum(uB[:, 0] ** 2) / np.sum(eB[:, 0] ** 2)
+    if expect:
+        w, U = D.eig("quadform")
+        M = D.P @ (U * (1.0 / (w + tau))[None, :]) @ U.T @ D.P
+        ratio_mc = float(np.sum(D.A_ldet0 * M)) / float(np.trace(M))
+    else:
+        ratio_mc = np.sum(uB[:, 1:] ** 2) / np.sum(eB[:, 1:] ** 2)
+    return float(np.log(ratio_y / ratio_mc)), Pr[:, 0]
+
+
+def lmc_secant(f, s2max: float = 0.7, tol: float = 0.002, max_iter: int = 9):
+    """The secant iterations of LMC_REML on log((1-h2)/h2) (src/reml.cpp:821-853): start at
+    s2max/2, then s2max/4 or s2max by the sign of f, update
+    lt = (lt1 f2 - lt2 f1)/(f2 - f1) clamped to [-9, 9], stop when |lt - lt2| < tol or after
+    max_iter evaluations in all (the reference: tol 0.002, 9).  In double.  Returns (h2, the
+    value f returned at it, evaluations)."""
+    s2 = s2max / 2
+    lt1 = np.log((1 - s2) / s2)
+    r1 = f(s2)
+    s2 = s2max / 4 if r1[0] < 0 else s2max
+    lt2 = np.log((1 - s2) / s2)
+    r2 = f(s2)
+    evals = 2
+    for _ in range(3, max_iter + 1):
+        lt = (lt1 * r2[0] - lt2 * r1[0]) / (r2[0] - r1[0])
+        lt = min(max(lt, -9.0), 9.0)
+        if abs(lt - lt2) < tol:
+            break
+        s2 = 1 / (np.exp(lt) + 1)
+        r1, r2 = r2, f(s2)
+        evals += 1
+        lt1, lt2 = lt2, lt
+    return s2, r2, evals
+
+
+def lmc_reml(D: "Dense", y, u, e, s2max: float = 0.7, tol: float = 0.002, max_iter: int = 9, expect=False):
+    """LMC_REML's variance components (src/reml.cpp:730-862) in FP64 over the same MC draws:
+    h2 from lmc_secant, vg = y'Py/(n - c) at it (:856), ve = vg (1-h2)/h2.  Returns
+    dict(h2, vg, ve, Py, evals)."""
+    V = lmc_inputs(D, y, u, e)
+    h2, (fv, Py), evals = lmc_secant(lambda s: lmc_first_order(D, V, s, expect), s2max, tol, max_iter)
+    c = D.Q.shape[1]
+    vg = float(np.asarray(y, np.float64) @ Py) / (D.n - c)
+    return dict(h2=float(h2), vg=vg, ve=vg * (1 - h2) / h2, Py=Py, evals=evals, f=fv)
 
 
 def lmm_gstat_fake(D: "Dense", Py, tau1: float, V0):

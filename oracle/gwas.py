@@ -1,11 +1,49 @@
 """ORACLE -- test infrastructure only.  GWAS statistics of the REML path restated in numpy:
-the gstat quadratic-form statistic of src/output.cpp:98-117 and the GRAMMAR-gamma score test
-of bin/slemm_gamma.py:90-105, :296-325."""
+the gstat quadratic-form statistic of src/output.cpp:98-117, the GRAMMAR-gamma score test
+of bin/slemm_gamma.py:90-105, :296-325, and the --lmm SNP sample of src/reml.cpp:920-925
+(glibc's rand() and libstdc++'s std::random_shuffle restated in Python)."""
 from __future__ import annotations
 
 import math
 
 import numpy as np
+
+
+def glibc_rand(count: int, seed: int = 1) -> list:
+    """The first `count` outputs of glibc's rand() after srand(seed) -- seed 1 is also its state
+    when srand is never called, as in the reference.  Published definition of the TYPE_3
+    generator: r_0 = seed, r_i = 16807 r_{i-1} mod (2^31 - 1) (i < 31), r_i = r_{i-31}
+    (31 <= i < 34), r_i = r_{i-31} + r_{i-3} mod 2^32 (i >= 34), output k = r_{k+344} >> 1."""
+    r = [seed if seed else 1]
+    for i in range(1, 31):
+        r.append((16807 * r[-1]) % 2147483647)
+    for i in range(31, 34):
+        r.append(r[i - 31])
+    for i in range(34, 344 + count):
+        r.append((r[i - 31] + r[i - 3]) & 0xFFFFFFFF)
+    return [x >> 1 for x in r[344:344 + count]]
+
+
+def random_shuffle(v, rand_stream) -> list:
+    """libstdc++'s std::random_shuffle(first, last) (bits/stl_algo.h, GCC 11): for each i from 1
+    to N-1, j = rand() % (i + 1), swap v[i] and v[j] when they differ."""
+    v = list(v)
+    it = iter(rand_stream)
+    for i in range(1, len(v)):
+        j = next(it) % (i + 1)
+        if i != j:
+            v[i], v[j] = v[j], v[i]
+    return v
+
+
+def lmm_sample(gm, n: int, k: int) -> np.ndarray:
+    """src/reml.cpp:920-925: indices i with gm_i*n > 20 and (2 - gm_i)*n > 20 in float32,
+    std::random_shuffle'd with rand() in its initial state, first min(k, N)."""
+    gm = np.asarray(gm, np.float32)
+    fn = np.float32(n)
+    ok = np.flatnonzero((gm * fn > np.float32(20)) & ((np.float32(2) - gm) * fn > np.float32(20)))
+    v = random_shuffle(ok.tolist(), glibc_rand(max(len(ok) - 1, 0)))
+    return np.asarray(v[:k], np.int64)
 
 
 def one_df_chisq_pval(x: float) -> float:

@@ -86,14 +86,30 @@ void extract_dosage_cols(slemm_geno* h, const std::vector<int64_t>& snps, std::v
 // draw order and normalisation as src/reml.cpp:597-605.
 void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B);
 
-// REML_criterion at h2 (src/reml.cpp:457-489): returns 2 logLL, sets vg, ve and the LRT.
-// REML criterion at h2; returns the value the golden-section search compares (rounded to float
-// as src/reml.cpp:457-489 does unless SLEMM_CRITERION_FP64=1, see criterion_value)
+// REML_criterion at h2 (src/reml.cpp:457-489): returns the 2 logLL value the golden-section
+// search compares (see criterion_value), sets vg, ve and the LRT.
 using Criterion = std::function<double(float h2, float& vg, float& ve, float& lrt)>;
-// The criterion as the search compares it: rounded to float (the reference's arithmetic), or kept
-// in double with SLEMM_CRITERION_FP64=1 (read per call), which lets the search resolve the flat
-// top of the likelihood below float's resolution.
+// The criterion as the search compares it: in double by default, which lets the search resolve
+// the flat top of the likelihood below float's resolution (a criterion of magnitude 1e3..1e5
+// rounded to float moves the optimum by ~1e-4 relative); rounded to float as src/reml.cpp:486
+// returns it with SLEMM_CRITERION_FP32=1 (read per call), to replay the reference's search path.
 double criterion_value(double c);
+// True when SLEMM_CRITERION_FP32=1: the criterion replays the reference's arithmetic (y'Py from
+// the FP32 L_Solve solution, criterion rounded to float).
+bool criterion_replay();
+
+// y'(A + sigma I)^-1 y from the seed column of y itself: Gauss quadrature on its FP64 Lanczos
+// tridiagonal T_J, ||y||^2 e1'(T_J + sigma I)^-1 e1 = ||y||^2 sum_q W_q^2 / (D_q + sigma).  In exact
+// arithmetic this is y' x_J for the CG iterate x_J that L_Solve replays (x_J = ||y|| U_J
+// (T_J + sigma)^-1 e1 and U_J' y = ||y|| e1); unlike y' x_J formed from the FP32-rounded
+// solution it is a smooth function of sigma, so the golden-section search does not see rounding
+// jitter of ~(n - c) 1e-9 in the criterion.
+struct SeedQuadrature {
+  std::vector<double> D, W;
+  double norm2 = 0.0;
+  SeedQuadrature(const slemm_lseed* s, int col, double y_norm2);
+  double inverse(double sigma) const;
+};
 
 // REML_GSS (src/reml.cpp:493-547): golden-section search on (s2min, s2max) to tol_VC; vg, ve
 // and lrt are those of the last evaluated point, as in the reference.

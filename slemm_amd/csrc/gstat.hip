@@ -189,22 +189,95 @@ void quad_forms(slemm_geno* h, const MatD& Q, std::vector<float>& V, int k, cons
   }
 }
 
+// glibc's rand(): the TYPE_3 additive lagged-Fibonacci generator behind random(), in the state
+// srand(1) (or no srand call at all) leaves it.  Published definition: r_0 = seed,
+// r_i = 16807 r_{i-1} mod (2^31 - 1) for i = 1..30, then x_i = x_{i-31} + x_{i-3} (mod 2^32)
+// with the 31-word state read by a front index 3 ahead of the rear one, the first 310 outputs
+// discarded, and each output the 32-bit sum shifted right by one (31 bits).
+class GlibcRand {
+ public:
+  explicit GlibcRand(uint32_t seed) {
+    int64_t r = seed == 0 ? 1 : seed;
+    state_[0] = (uint32_t)r;
+    for (int i = 1; i < kDeg; ++i) {
+      r = (16807 * r) % 2147483647;
+      state_[i] = (uint32_t)r;
+    }
+    front_ = kSep;
+    rear_ = 0;
+    for (int i = 0; i < 10 * kDeg; ++i) (void)next();
+  }
+  int32_t next() {
+    state_[front_] += state_[rear_];
+    const uint32_t out = state_[front_] >> 1;
+    front_ = front_ + 1 == kDeg ? 0 : front_ + 1;
+    rear_ = rear_ + 1 == kDeg ? 0 : rear_ + 1;
+    return (int32_t)out;
+  }
+
+ private:
+  static constexpr int kDeg = 31, kSep = 3;
+  uint32_t state_[kDeg];
+  int front_, rear_;
+};
+
 }  // namespace
+
+// The --lmm SNP sample of src/reml.cpp:920-925: the indices i with gm_i * n > 20 and
+// (2 - gm_i) * n > 20 (float products, as the reference's float gm times int n), shuffled by
+// std::random_shuffle -- libstdc++'s loop `for i in 1..N-1: j = rand() % (i + 1); swap(v[i],
+// v[j]) if i != j` -- on rand() in its initial state (the reference seeds it nowhere), first
+// min(k, N) kept.  A fresh generator per call gives every call the sample of a fresh run.
+std::vector<int64_t> reference_lmm_sample(const std::vector<float>& gm, int64_t n, int k) {
+  std::vector<int64_t> v;
+  const float fn = (float)n;
+  for (size_t i = 0; i < gm.size(); ++i)
+    if (gm[i] * fn > 20.0f && (2.0f - gm[i]) * fn > 20.0f) v.push_back((int64_t)i);
+  GlibcRand rng(1);
+  for (size_t i = 1; i < v.size(); ++i) {
+    const size_t j = (size_t)((int64_t)rng.next() % (int64_t)(i + 1));
+    if (i != j) std::swap(v[i], v[j]);
+  }
+  if ((int64_t)v.size() > k) v.resize((size_t)k);
+  return v;
+}
+
 }  // namespace slemm
 
 using namespace slemm;
+
+extern "C" int slemm_gpu_lmm_sample(const float* gm, int64_t m, int64_t n, int num_qf_markers, int64_t* snps,
+                                    int* k) {
+  return guard([&] {
+    REQUIRE(gm && k, "null argument");
+    REQUIRE(m >= 0 && n >= 1, "need m >= 0 and n >= 1");
+    REQUIRE(num_qf_markers >= 1, "num_qf_markers must be positive");
+    const std::vector<int64_t> v = reference_lmm_sample(std::vector<float>(gm, gm + m), n, num_qf_markers);
+    *k = (int)v.size();
+    if (snps) std::copy(v.begin(), v.end(), snps);
+  });
+}
 
 extern "C" int slemm_gpu_lmm_gstat(slemm_geno* h, const float* X, int64_t ldx, int c, const float* Py, float h2,
                                    const int64_t* snps, int k, int window_size, float tol_L, int precision,
                                    int verbose, float* xPy, float* xPx, float* xBx, float* xSx) {
   return guard([&] {
-    REQUIRE(h && X && Py && snps && xPy && xPx && xBx && xSx, "null argument");
+    REQUIRE(h && X && Py && xPy && xPx && xBx && xSx, "null argument");
     REQUIRE(c >= 1 && ldx >= h->n, "covariates: need c >= 1 and ldx >= n");
     REQUIRE(k >= 1, "need at least one SNP");
     REQUIRE(h2 > 0 && h2 < 1, "h2 must be in (0, 1)");
     HIPCHECK(hipSetDevice(h->device));
     require_model(h);
     const int64_t n = h->n, m = h->m;
+    std::vector<int64_t> sample;
+    if (!snps) {  // the reference's own sample (src/reml.cpp:920-925)
+      std::vector<float> gm(m);
+      HIPCHECK(hipMemcpy(gm.data(), h->gm, sizeof(float) * m, hipMemcpyDeviceToHost));
+      sample = reference_lmm_sample(gm, n, k);
+      REQUIRE((int)sample.size() == k,
+              "fewer SNPs with MAC > 20 than requested; slemm_gpu_lmm_sample gives the sample size");
+      snps = sample.data();
+    }
     REQUIRE(window_size >= 2 && window_size % 2 == 0, "window_size must be even and positive");
     REQUIRE(window_size < m, "--window_size >= # of SNPs kept for --lmm.");
     for (int i = 0; i < k; ++i) REQUIRE(snps[i] >= 0 && snps[i] < m, "SNP index out of range");

@@ -40,9 +40,28 @@ void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B) {
   }
 }
 
-double criterion_value(double c) {
-  const char* e = std::getenv("SLEMM_CRITERION_FP64");
-  return (e && e[0] == '1') ? c : (double)(float)c;
+bool criterion_replay() {
+  const char* e = std::getenv("SLEMM_CRITERION_FP32");
+  return e && e[0] == '1';
+}
+
+double criterion_value(double c) { return criterion_replay() ? (double)(float)c : c; }
+
+SeedQuadrature::SeedQuadrature(const slemm_lseed* s, int col, double y_norm2) : norm2(y_norm2) {
+  std::vector<int> steps;
+  int t = 0;
+  abi(slemm_gpu_lseed_info(s, &t, nullptr, nullptr));
+  steps.resize(t);
+  abi(slemm_gpu_lseed_col_steps(s, steps.data()));
+  D.resize(steps[col]);
+  W.resize(steps[col]);
+  abi(slemm_gpu_lseed_spectrum(s, col, D.data(), W.data()));
+}
+
+double SeedQuadrature::inverse(double sigma) const {
+  double q = 0.0;
+  for (size_t i = 0; i < D.size(); ++i) q += W[i] * W[i] / (D[i] + sigma);
+  return norm2 * q;
 }
 
 void golden_section(float s2min, float s2max, float tol_VC, int verbose, const Criterion& criterion, float& vg,
@@ -188,16 +207,22 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
     const double ySy = sq_norm(yp.data(), n);
     int evals = 0;
     std::vector<float> hx;
+    const bool replay = criterion_replay();
+    const SeedQuadrature quad_y(sg.s, 0, ySy);
 
     // REML_criterion (:457-489)
     auto criterion = [&](float ss, float& vg, float& ve, float& lrt) -> double {
       ++evals;
       const float tau = (1 - ss) / ss;
       const float sigma = tau - tau0;
-      abi(slemm_gpu_lsolve(sg.s, 0, 1, dB.p, n, sigma, 1e-6f, dX.p, n, nullptr));
-      download(hx, dX.p, n);
       double yPy = 0.0;
-      for (int64_t i = 0; i < n; ++i) yPy += (double)yp[i] * hx[i];
+      if (replay) {  // y'x from the L_Solve solution, as :477
+        abi(slemm_gpu_lsolve(sg.s, 0, 1, dB.p, n, sigma, 1e-6f, dX.p, n, nullptr));
+        download(hx, dX.p, n);
+        for (int64_t i = 0; i < n; ++i) yPy += (double)yp[i] * hx[i];
+      } else {
+        yPy = quad_y.inverse(sigma);
+      }
       vg = (float)(yPy / (double)(n - c));
       ve = vg * tau;
       double acc = 0.0;
@@ -245,14 +270,16 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
   });
 }
 
-extern "C" int slemm_gpu_lmc_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y, int rng_seed,
-                                  int n_MC, float tol_L, float s2max, float s2min, int precision, int verbose,
-                                  float* Py, float* snp_blup, float* blue, slemm_reml_stats* stats) {
+extern "C" int slemm_gpu_lmc_reml_ex(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y,
+                                     int rng_seed, int n_MC, float tol_L, float s2max, float s2min, float secant_tol,
+                                     int secant_max_iter, int precision, int verbose, float* Py, float* snp_blup,
+                                     float* blue, slemm_reml_stats* stats) {
   return guard([&] {
     REQUIRE(h && X && y && Py && snp_blup && blue, "null argument");
     REQUIRE(c >= 1 && ldx >= h->n, "covariates: need c >= 1 and ldx >= n");
     REQUIRE(n_MC >= 1, "number of MC samples must be positive");
     REQUIRE(s2max > 0 && s2max < 1, "max heritability must be in (0, 1)");
+    REQUIRE(secant_tol > 0 && secant_max_iter >= 2, "secant: need tol > 0 and at least 2 evaluations");
     (void)s2min;
     HIPCHECK(hipSetDevice(h->device));
     require_model(h);
@@ -340,11 +367,11 @@ extern "C" int slemm_gpu_lmc_reml(slemm_geno* h, const float* X, int64_t ldx, in
     float log_tau2 = std::log((1 - s2) / s2);
     float f2 = first_order(s2);
     if (verbose) printf("  Iter 2: h2=%.4f log((1-h2)/h2)=%.4f funct=%.4f\n", s2, log_tau2, f2);
-    for (int si = 3; si <= 9; ++si) {
+    for (int si = 3; si <= secant_max_iter; ++si) {
       float lt = (log_tau1 * f2 - log_tau2 * f1) / (f2 - f1);
       if (lt > 9) lt = 9;
       else if (lt < -9) lt = -9;
-      if (std::fabs(lt - log_tau2) < 0.002f) break;
+      if (std::fabs(lt - log_tau2) < secant_tol) break;
       s2 = 1 / (std::exp(lt) + 1);
       f1 = f2;
       f2 = first_order(s2);
@@ -407,4 +434,12 @@ extern "C" int slemm_gpu_lmc_reml(slemm_geno* h, const float* X, int64_t ldx, in
       stats->seconds_total = seconds_since(t_start);
     }
   });
+}
+
+extern "C" int slemm_gpu_lmc_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y, int rng_seed,
+                                  int n_MC, float tol_L, float s2max, float s2min, int precision, int verbose,
+                                  float* Py, float* snp_blup, float* blue, slemm_reml_stats* stats) {
+  // the reference's secant schedule: stop at |d log tau| < 0.002, at most 9 evaluations (:824-836)
+  return slemm_gpu_lmc_reml_ex(h, X, ldx, c, y, rng_seed, n_MC, tol_L, s2max, s2min, 0.002f, 9, precision, verbose,
+                               Py, snp_blup, blue, stats);
 }

@@ -94,13 +94,20 @@ extern "C" int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int 
       const float* ycol = dB.p + (size_t)t * n;
       const float* qcol = dB.p + (size_t)(T + t) * n;
       int evals = 0;
+      const bool replay = criterion_replay();
+      const SeedQuadrature quad_y(sg.s, t, ySy);
       // REML_criterion (src/reml.cpp:457-489) on the trait's subset (n_t individuals, c = 1)
       auto criterion = [&](float ss, float& vg, float& ve, float& lrt) -> double {
         ++evals;
         const float tau = (1 - ss) / ss;
         const float sigma = tau - tau0;
-        abi(slemm_gpu_lsolve(sg.s, t, 1, ycol, n, sigma, 1e-6f, dX.p, n, nullptr));
-        const double yPy = dot_dev(ycol, dX.p);
+        double yPy = 0.0;
+        if (replay) {
+          abi(slemm_gpu_lsolve(sg.s, t, 1, ycol, n, sigma, 1e-6f, dX.p, n, nullptr));
+          yPy = dot_dev(ycol, dX.p);
+        } else {
+          yPy = quad_y.inverse(sigma);
+        }
         vg = (float)(yPy / (double)(nt - 1));
         ve = vg * tau;
         double acc = 0.0;

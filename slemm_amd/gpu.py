@@ -308,8 +308,9 @@ class Geno:
 
     def LMC_REML(self, X: np.ndarray, y: np.ndarray, rng_seed: int = 0, n_MC: Optional[int] = None,
                  tol_L: float = 5e-4, s2max: float = 0.7, s2min: float = 1e-4, precision="bf16x2",
-                 verbose: bool = False) -> dict:
-        """LMC_REML (src/reml.cpp:730-882, without --lmm); n_MC default as src/option.cpp:414."""
+                 verbose: bool = False, secant_tol: float = 0.002, secant_max_iter: int = 9) -> dict:
+        """LMC_REML (src/reml.cpp:730-882, without --lmm); n_MC default as src/option.cpp:414.
+        secant_tol / secant_max_iter: the secant's stop (the reference's 0.002 and 9 by default)."""
         if n_MC is None:
             n_MC = max(min(int(4e9 / self.n / self.n), 30), 3)
         Xf = np.asfortranarray(np.asarray(X, np.float32).reshape(self.n, -1))
@@ -319,10 +320,10 @@ class Geno:
         blup = np.zeros(self.m, np.float32)
         blue = np.zeros(c, np.float32)
         st = L.RemlStats()
-        L.check(L.load().slemm_gpu_lmc_reml(
+        L.check(L.load().slemm_gpu_lmc_reml_ex(
             self._h, _np_ptr(Xf, C.c_float), self.n, c, _np_ptr(yv, C.c_float), rng_seed, n_MC, tol_L, s2max,
-            s2min, _prec(precision), int(verbose), _np_ptr(Py, C.c_float), _np_ptr(blup, C.c_float),
-            _np_ptr(blue, C.c_float), C.byref(st)))
+            s2min, secant_tol, secant_max_iter, _prec(precision), int(verbose), _np_ptr(Py, C.c_float),
+            _np_ptr(blup, C.c_float), _np_ptr(blue, C.c_float), C.byref(st)))
         return dict(vg=st.vg, ve=st.ve, h2=st.h2, Py=Py, snp_blup=blup, blue=blue, steps=st.lanczos_steps[0],
                     evals=st.criterion_evals, n_MC=n_MC, seconds_seed=st.seconds_seed,
                     seconds_total=st.seconds_total)
@@ -387,20 +388,33 @@ class Geno:
             tol_L, _prec(precision), int(verbose), *[_np_ptr(out[key], C.c_float) for key in ("xPy", "xPx", "xSx")]))
         return out
 
-    def lmm_gstat(self, X: np.ndarray, Py: np.ndarray, h2: float, snps: np.ndarray, window_size: int = 1000,
-                  tol_L: float = 5e-4, precision="bf16x2", verbose: bool = False) -> dict:
+    def lmm_sample(self, num_qf_markers: int = 30) -> np.ndarray:
+        """The reference's --lmm SNP sample (src/reml.cpp:920-925) on this handle's genotype means."""
+        from .gwas import sample_gstat_snps
+        gm, _, _ = self.model()
+        return sample_gstat_snps(gm, self.n, num_qf_markers)
+
+    def lmm_gstat(self, X: np.ndarray, Py: np.ndarray, h2: float, snps: Optional[np.ndarray] = None,
+                  window_size: int = 1000, tol_L: float = 5e-4, precision="bf16x2", verbose: bool = False,
+                  num_qf_markers: int = 30) -> dict:
         """Quadratic forms of --lmm (src/reml.cpp:884-982): xPy, xPx, xBx, xSx for the SNPs
-        `snps` after a fit with heritability h2 that returned Py."""
+        `snps` after a fit with heritability h2 that returned Py; snps=None takes the
+        reference's own sample of num_qf_markers SNPs (returned under "snps")."""
         Xf = np.asfortranarray(np.asarray(X, np.float32).reshape(self.n, -1))
         c = Xf.shape[1]
         pv = np.ascontiguousarray(Py, np.float32)
-        idx = np.ascontiguousarray(snps, np.int64)
-        k = idx.size
+        if snps is None:
+            idx = None
+            k = int(self.lmm_sample(num_qf_markers).size)
+        else:
+            idx = np.ascontiguousarray(snps, np.int64)
+            k = idx.size
         out = {key: np.zeros(k, np.float32) for key in ("xPy", "xPx", "xBx", "xSx")}
         L.check(L.load().slemm_gpu_lmm_gstat(
             self._h, _np_ptr(Xf, C.c_float), self.n, c, _np_ptr(pv, C.c_float), h2,
-            idx.ctypes.data_as(C.POINTER(C.c_int64)), k, window_size, tol_L, _prec(precision), int(verbose),
+            _np_ptr(idx, C.c_int64), k, window_size, tol_L, _prec(precision), int(verbose),
             *[_np_ptr(out[key], C.c_float) for key in ("xPy", "xPx", "xBx", "xSx")]))
+        out["snps"] = self.lmm_sample(num_qf_markers) if snps is None else idx.copy()
         return out
 
     def reml_score(self, X: np.ndarray, y: np.ndarray, vg: float, ve: float,
@@ -462,14 +476,16 @@ class Geno:
         h = C.c_void_p()
         L.check(L.load().slemm_gpu_lseed(self._h, kind, ck, cg, tau0, _ptr(Bc), self.n, t, tol, maxit, keep,
                                          _prec(precision), int(verbose), _stream(), C.byref(h)))
-        return LSeed(h, self.n, t)
+        return LSeed(h, self.n, t, owner=self)
 
 
 class LSeed:
-    """struct lseed (src/reml.cpp:240-244) kept on the device."""
+    """struct lseed (src/reml.cpp:240-244) kept on the device.  `owner` (the Geno or Traits the
+    seed was built on) is kept alive as long as the seed, so handles are released seed first."""
 
-    def __init__(self, handle, n: int, t: int):
+    def __init__(self, handle, n: int, t: int, owner=None):
         self._h = handle
+        self._owner = owner
         self.n, self.t = n, t
         lib = L.load()
         tt, steps, conv = C.c_int(), C.c_int(), C.c_int()
@@ -590,7 +606,7 @@ class Traits:
         h = C.c_void_p()
         L.check(L.load().slemm_gpu_traits_lseed(self._h, ct, ck, cg, tau0, _ptr(Bc), self.geno.n, t, tol, maxit,
                                                 keep, _prec(precision), int(verbose), _stream(), C.byref(h)))
-        return LSeed(h, self.geno.n, t)
+        return LSeed(h, self.geno.n, t, owner=self)
 
     def SLDF_REML(self, y: np.ndarray, rng_seed: int = 0, n_V: int = 30, tol_L: float = 5e-4, s2max: float = 0.7,
                   s2min: float = 1e-4, tol_VC: float = 1e-4, precision="bf16x2", verbose: bool = False) -> dict:

@@ -98,14 +98,19 @@ def grammar_gamma(geno: "gpu.Geno", Py: np.ndarray, gst: dict, snp_var: float, m
     return out
 
 
-def sample_gstat_snps(gm: np.ndarray, n: int, k: int, seed: int = 0) -> np.ndarray:
-    """k SNPs with minor-allele count above 20 (src/reml.cpp:923-925), drawn uniformly without
-    replacement.  The reference shuffles with std::random_shuffle (rand()), whose order is not
-    reproduced; the returned indices are sorted."""
-    gm = np.asarray(gm, np.float32)
-    ok = np.nonzero((gm * n > 20) & ((2 - gm) * n > 20))[0]
-    rng = np.random.default_rng(seed)
-    return np.sort(rng.choice(ok, size=min(k, ok.size), replace=False))
+def sample_gstat_snps(gm: np.ndarray, n: int, k: int) -> np.ndarray:
+    """The --lmm SNP sample of src/reml.cpp:920-925: k SNPs with minor-allele count above 20 in
+    the order std::random_shuffle leaves them with rand() in its initial state, as the reference
+    draws them (the library's C++ restatement, slemm_gpu_lmm_sample).  Fewer than k come back
+    when fewer SNPs pass the filter."""
+    import ctypes as C
+    from . import _lib as L
+    g = np.ascontiguousarray(gm, np.float32)
+    out = np.zeros(max(int(k), 1), np.int64)
+    cnt = C.c_int()
+    L.check(L.load().slemm_gpu_lmm_sample(g.ctypes.data_as(C.POINTER(C.c_float)), g.size, int(n), int(k),
+                                          out.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(cnt)))
+    return out[:cnt.value].copy()
 
 
 def _fmt_g(v) -> str:

@@ -5,6 +5,7 @@ import math
 from pathlib import Path
 
 import numpy as np
+import pytest
 
 from oracle import gwas as ogwas
 from slemm_amd import gwas
@@ -53,8 +54,63 @@ def test_example_gamma_pvalues():
 
 def test_gstat_snp_sampler_respects_mac():
     gm = np.array([0.001, 0.5, 1.0, 1.99, 0.02, 1.5], np.float32)
-    s = gwas.sample_gstat_snps(gm, 1000, 10, seed=1)
+    s = gwas.sample_gstat_snps(gm, 1000, 10)
     assert set(s.tolist()) == {1, 2, 5} and math.isclose(gm[4] * 1000, 20.0)
+
+
+def test_oracle_glibc_rand_matches_libc():
+    """The oracle's TYPE_3 restatement against this C library's own rand() after srand(1)."""
+    import ctypes
+    from oracle import gwas as ogwas
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(1)
+    want = [libc.rand() for _ in range(5000)]
+    assert ogwas.glibc_rand(5000) == want
+
+
+@pytest.fixture(scope="module")
+def shuffle_prog(tmp_path_factory):
+    """std::random_shuffle compiled here with g++ (libstdc++ 11): no srand, as in the reference.
+    Reads N and N floats gm (one per line) and n, prints the reference's sample order: the
+    filter of src/reml.cpp:922-923 and the shuffle of :924."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    d = tmp_path_factory.mktemp("shuffle")
+    src = d / "shuffle.cpp"
+    src.write_text(
+        "#include <algorithm>\n#include <cstdio>\n#include <vector>\n"
+        "int main() { long N, n; if (scanf(\"%ld %ld\", &N, &n) != 2) return 1;\n"
+        "  std::vector<int> v; for (long i = 0; i < N; ++i) { float g; if (scanf(\"%f\", &g) != 1) return 1;\n"
+        "    if (g * n > 20 && (2 - g) * n > 20) v.push_back((int)i); }\n"
+        "  std::random_shuffle(v.begin(), v.end());\n"
+        "  for (int x : v) printf(\"%d\\n\", x); return 0; }\n")
+    exe = d / "shuffle"
+    subprocess.run(["g++", "-std=c++11", "-O1", "-w", str(src), "-o", str(exe)], check=True)
+
+    def run(gm, n):
+        inp = f"{len(gm)} {n}\n" + "\n".join("%.9g" % float(g) for g in gm) + "\n"
+        out = subprocess.run([str(exe)], input=inp, capture_output=True, text=True, check=True).stdout
+        return np.array([int(x) for x in out.split()], np.int64)
+    return run
+
+
+def test_lmm_sample_matches_compiled_random_shuffle(shuffle_prog):
+    """The reference's --lmm sample (src/reml.cpp:920-925) three ways -- the oracle's Python
+    restatement, the library's C++ (slemm_gpu_lmm_sample, host only) and libstdc++'s own
+    random_shuffle on glibc's rand() -- on genotype means with SNPs right at the MAC-20 edges."""
+    from oracle import gwas as ogwas
+    rng = np.random.default_rng(3)
+    n = 1000
+    gm = rng.uniform(0.0, 2.0, 4000).astype(np.float32)
+    gm[:6] = [0.02, 1.98, 0.0201, 1.9799, 0.0, 2.0]   # 20 and 1980 of 2000 alleles: excluded
+    want = shuffle_prog(gm, n)
+    assert want.size > 3500 and not set(want.tolist()) & {0, 1, 4, 5}
+    assert np.array_equal(ogwas.lmm_sample(gm, n, len(gm)), want)
+    assert np.array_equal(gwas.sample_gstat_snps(gm, n, len(gm)), want)
+    # the first k of the shuffled order, as V keeps min(num_qf_markers, N) columns (:927)
+    assert np.array_equal(gwas.sample_gstat_snps(gm, n, 30), want[:30])
 
 
 def _gamma_cols_of_rows(rows):

@@ -303,6 +303,152 @@ def test_sldf_restatement_near_exact_optimum():
     assert abs(r["vg"] - oreml.vg_from_py(y, r["Py"], 1)) / r["vg"] < 1e-3
 
 
+def _sldf_inputs(p, y):
+    Q, _ = oreml.covariate_qr(p.X, p.rr)
+    yp = (p.rr * np.asarray(y, np.float32)).astype(np.float32)
+    yp = (yp - Q @ (Q.T @ yp)).astype(np.float32)
+    return Q, yp
+
+
+@pytest.fixture(scope="module")
+def weighted_small():
+    """n=240 with error-variance weights R, per-SNP weights and two covariates, so that R^-1/2,
+    the (n - c) factor and the covariate log-determinant all enter the criterion."""
+    p = make_problem(n=240, m=400, seed=21, weights=True, c=2, persnp=True)
+    y = make_phenotype(p, 0.45, n_causal=80, seed=5)
+    return p, y
+
+
+def test_sldf_criterion_is_twice_reml_loglik_plus_const(weighted_small):
+    """With exact log-determinants -- the n unit vectors as probes, for which
+    n/n_V sum_k e_k' log(A + sigma I) e_k = log|A + sigma I| -- the FP64 SLDF criterion of
+    src/reml.cpp:457-489 equals 2 x the restricted log-likelihood plus a constant in h2:
+    log|H| = log|H0| + log|R|, X'H^-1 X = R_x' Q'H0^-1 Q R_x, and the projected operator's
+    y'Py is the REML y'Py.  A dropped term, a sign or an n in place of n - c breaks the
+    constancy.  The criterion is compared on a grid of h2 that spans the search range."""
+    from oracle.dense import sldf_criterion
+    p, y = weighted_small
+    Q, yp = _sldf_inputs(p, y)
+    D = p.dense()
+    I = np.eye(p.n)
+    tau0 = (1 - 0.7) / 0.7
+    hs = np.linspace(0.05, 0.69, 17)
+    diff = []
+    for h in hs:
+        crit, vg, ve, lrt = sldf_criterion(D, yp, Q, I, h, tau0)
+        ll = reml_loglik(p.Z, p.gm, p.gw, p.X, y, h, rr=p.rr)
+        diff.append(crit - 2.0 * ll)
+        # Vg = y'Py / (n - c) with the REML projection (src/reml.cpp:479), Ve = Vg tau
+        tau = (1 - h) / h
+        H = (p.rr[:, None] ** -1) * (D.A_ldet0 + tau * I) * (p.rr[None, :] ** -1)
+        Hi = np.linalg.inv(H)
+        X = p.X
+        Pm = Hi - Hi @ X @ np.linalg.solve(X.T @ Hi @ X, X.T @ Hi)
+        yd = np.asarray(y, np.float64)
+        # (y_p and Q are float32 inputs, as the reference keeps them: 1e-8 relative in y'Py)
+        assert abs(vg - yd @ Pm @ yd / (p.n - p.X.shape[1])) < 1e-6 * vg
+        assert abs(ve - vg * tau) < 1e-12 * ve
+    diff = np.asarray(diff)
+    scale = np.max(np.abs(diff))
+    print("criterion - 2 loglik over the grid: spread", np.ptp(diff), "scale", scale)
+    assert np.ptp(diff) < 1e-6 * max(scale, 1.0), diff
+    # the constant itself: (n - c) log(y_p'y_p) + log|R| + 2 log|R_x|
+    _, Rx = oreml.covariate_qr(p.X, p.rr)
+    const = ((p.n - p.X.shape[1]) * np.log(float(yp.astype(np.float64) @ yp))
+             + np.sum(np.log(1.0 / p.rr.astype(np.float64) ** 2)) + 2 * np.sum(np.log(np.abs(np.diag(Rx)))))
+    assert abs(diff[0] - const) < 1e-8 * max(abs(const), 1.0)
+
+
+def test_sldf_gss_finds_the_exact_reml_argmax(weighted_small):
+    """REML_GSS restated in FP64 (src/reml.cpp:493-547) over the exact criterion (identity
+    probes) lands on the bounded argmax of the exact REML likelihood within the bracket width,
+    and returns the variance components of its last evaluation point."""
+    from scipy.optimize import minimize_scalar
+    from oracle.dense import sldf_criterion, sldf_gss
+    p, y = weighted_small
+    Q, yp = _sldf_inputs(p, y)
+    D = p.dense()
+    I = np.eye(p.n)
+    opt = minimize_scalar(lambda h: -reml_loglik(p.Z, p.gm, p.gw, p.X, y, h, rr=p.rr),
+                          bounds=(1e-4, 0.7), method="bounded", options={"xatol": 1e-9})
+    for tol in (1e-4, 1e-6):
+        vg, ve, lrt, h2 = sldf_gss(D, yp, Q, I, tol=tol)
+        assert abs(h2 - opt.x) < 1.5 * tol, (h2, opt.x, tol)
+        _, vg_at, ve_at, lrt_at = sldf_criterion(D, yp, Q, I, h2, (1 - 0.7) / 0.7)
+        assert abs(vg - vg_at) < 1e-9 * vg and abs(ve - ve_at) < 1e-9 * ve
+    # stochastic probes move the optimum only by the probe noise of the log-determinant
+    vgs, ves, _, h2s = sldf_gss(D, yp, Q, oracle.rademacher(0, p.n, 30), tol=1e-6)
+    assert abs(h2s - opt.x) < 0.05
+
+
+def test_sldf_restatement_tracks_fp64_criterion():
+    """The FP32 restatement of SLDF_REML against the pinned FP64 criterion on the same probes
+    and schedule: its FP32 Lanczos scalars and float criterion perturb only the flat top."""
+    from oracle.dense import sldf_gss
+    p = make_problem(n=600, m=900, seed=12)
+    y = make_phenotype(p, 0.4, seed=1)
+    Q, yp = _sldf_inputs(p, y)
+    r = oreml.SLDF_REML(p.kmat, p.gm, p.gw, None, p.X, y, n_V=30)
+    vg, ve, _, h2 = sldf_gss(p.dense(), yp, Q, oracle.rademacher(0, p.n, 30))
+    assert abs(r["h2"] - h2) < 1e-3, (r["h2"], h2)
+    assert abs(r["vg"] - vg) / vg < 3e-3 and abs(r["ve"] - ve) / ve < 3e-3
+
+
+def test_lmc_fp64_expected_ratio_root_is_reml_argmax(weighted_small):
+    """With the MC ratio of REML_first_order replaced by its expectation tr(A M)/tr(M), the first
+    order condition of src/reml.cpp:696-728 is the REML stationarity condition in tau, so the
+    FP64 secant (src/reml.cpp:821-853, run to 1e-10) lands on the argmax of the exact REML
+    likelihood.  The y part (R^-1/2 scaling, projections, sqrt(gw) Zc'), the secant update and
+    vg = y'Py/(n-c) are what this checks."""
+    from scipy.optimize import minimize_scalar
+    from oracle.dense import lmc_reml
+    p, y = weighted_small
+    D = p.dense()
+    u, e = oracle.lmc_samples(0, p.n, p.m, 3)
+    r = lmc_reml(D, y, u, e, tol=1e-10, max_iter=60, expect=True)
+    opt = minimize_scalar(lambda h: -reml_loglik(p.Z, p.gm, p.gw, p.X, y, h, rr=p.rr),
+                          bounds=(1e-3, 0.99), method="bounded", options={"xatol": 1e-10})
+    assert abs(r["h2"] - opt.x) < 1e-6, (r["h2"], opt.x)
+    assert abs(r["f"]) < 1e-8
+    # vg at the optimum is the profile REML estimate y'Py/(n - c)
+    tau = (1 - r["h2"]) / r["h2"]
+    H = (D.A_ldet0 + tau * np.eye(p.n)) / (p.rr[:, None] * p.rr[None, :])
+    Hi = np.linalg.inv(H)
+    X = p.X
+    Pm = Hi - Hi @ X @ np.linalg.solve(X.T @ Hi @ X, X.T @ Hi)
+    yd = np.asarray(y, np.float64)
+    assert abs(r["vg"] - yd @ Pm @ yd / (p.n - X.shape[1])) < 1e-6 * r["vg"]
+
+
+def test_lmc_fp64_mc_ratio_estimates_its_expectation(weighted_small):
+    """The MC half of REML_first_order (src/reml.cpp:788-813, 715-727) on 400 draws from the
+    libstdc++ stream estimates tr(A M)/tr(M) (the draws have covariance R^-1/2 Zc W Zc' R^-1/2 + tau
+    I after projection): the two forms of f agree to the MC error, at two heritabilities."""
+    from oracle.dense import lmc_first_order, lmc_inputs
+    p, y = weighted_small
+    D = p.dense()
+    u, e = oracle.lmc_samples(5, p.n, p.m, 400)
+    V = lmc_inputs(D, y, u, e)
+    for h in (0.3, 0.6):
+        f_mc, _ = lmc_first_order(D, V, h)
+        f_ex, _ = lmc_first_order(D, V, h, expect=True)
+        assert abs(f_mc - f_ex) < 0.015, (h, f_mc, f_ex)
+
+
+def test_lmc_restatement_tracks_fp64_oracle():
+    """The FP32 restatement of LMC_REML against the pinned FP64 LMC on the same MC draws and the
+    reference's secant schedule (0.002, at most 9 evaluations)."""
+    from oracle.dense import lmc_reml
+    p = make_problem(n=600, m=900, seed=13)
+    y = make_phenotype(p, 0.5, seed=2)
+    r = oreml.LMC_REML(p.kmat, p.gm, p.gw, None, p.X, y)
+    u, e = oracle.lmc_samples(0, p.n, p.m, r["n_MC"])
+    ref = lmc_reml(p.dense(), y, u, e)
+    assert r["evals"] == ref["evals"]
+    assert abs(r["h2"] - ref["h2"]) < 2e-4, (r["h2"], ref["h2"])
+    assert abs(r["vg"] - ref["vg"]) / ref["vg"] < 1e-3
+
+
 def test_lmc_restatement_near_exact_optimum():
     p = make_problem(n=900, m=1200, seed=13)
     y = make_phenotype(p, 0.5, seed=2)
