@@ -139,6 +139,19 @@ int slemm_gpu_op(slemm_geno* h, int kind, const int* col_kind, float tau0,
                  const float* V, int64_t ldv, int t, float* out, int64_t ldo,
                  int precision, void* stream);
 
+/* slemm_gpu_op without device allocation.  No counterpart in the reference: it is for callers
+ * that cannot call hipMalloc (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  Same arguments, kernels and results as slemm_gpu_op; every workspace of
+ * the call (packed operands, split-reduction partials, column sums, int8 digits, kinds) comes from
+ * one static 256 MiB device array per GPU, looked up on the first call on each device.  A call
+ * whose workspace would exceed 256 MiB fails with SLEMM_ERR_ARG before any work is queued (for
+ * example n = 100k x m = 500k fits one vector, about 180 MiB, but not the 65-column block).  All
+ * calls on one GPU share its array and it keeps the last call's contents: calls that overlap on
+ * one GPU (other threads, or an earlier call still running on another stream) race. */
+int slemm_gpu_op_noalloc(slemm_geno* h, int kind, const int* col_kind, float tau0,
+                         const float* V, int64_t ldv, int t, float* out, int64_t ldo,
+                         int precision, void* stream);
+
 /* rmatvec (src/reml.cpp:159-195): out (m x t, ldo) = Zc' V.
  * weighted_rmatvec (src/reml.cpp:120-157) when weighted != 0: out = W Zc' V.
  * V device n x t (ldv).  If scale_rr != 0 the rows of V are first multiplied by r_rsqrt. */

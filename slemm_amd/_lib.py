@@ -50,6 +50,8 @@ SIGNATURES = {
     "slemm_gpu_set_model": (C.c_int, [c_vp, c_fp, c_fp, c_fp, c_i64, C.c_int]),
     "slemm_gpu_get_model": (C.c_int, [c_vp, c_fp, c_fp, c_fp]),
     "slemm_gpu_op": (C.c_int, [c_vp, C.c_int, c_intp, C.c_float, c_vp, c_i64, C.c_int, c_vp, c_i64, C.c_int, c_vp]),
+    "slemm_gpu_op_noalloc": (C.c_int, [c_vp, C.c_int, c_intp, C.c_float, c_vp, c_i64, C.c_int, c_vp, c_i64, C.c_int,
+                                       c_vp]),
     "slemm_gpu_zt": (C.c_int, [c_vp, c_vp, c_i64, C.c_int, c_vp, c_i64, C.c_int, C.c_int, C.c_int, c_vp]),
     "slemm_gpu_z": (C.c_int, [c_vp, c_vp, c_i64, C.c_int, c_vp, c_i64, C.c_int, c_vp]),
     "slemm_gpu_gebv": (C.c_int, [c_vp, c_vp, c_u8p, c_vp, c_vp]),

@@ -47,10 +47,22 @@ int guard(F&& f) {
   }
 }
 
-// Device buffer owned by a handle; grows on demand, never shrinks.
+// Bump allocator over device memory the library does not own (the no-allocation entry point's
+// static scratch array).  While one is installed on the calling thread, DevBuf::reserve carves
+// its memory from it instead of calling hipMalloc, and fails with SLEMM_ERR_ARG past `cap`.
+struct ScratchArena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* take(size_t nbytes);
+};
+ScratchArena*& thread_arena();
+
+// Device buffer owned by a handle; grows on demand, never shrinks.  `borrowed` memory comes from
+// a ScratchArena and is never freed here.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool borrowed = false;
   void reserve(size_t nbytes);
   void release();
   template <class T> T* as() const { return static_cast<T*>(p); }

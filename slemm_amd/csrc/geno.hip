@@ -29,9 +29,29 @@ void hip_check(hipError_t e, const char* what, const char* file, int line) {
   fail(e == hipErrorOutOfMemory ? SLEMM_ERR_ALLOC : SLEMM_ERR_HIP, msg);
 }
 
+ScratchArena*& thread_arena() {
+  static thread_local ScratchArena* a = nullptr;
+  return a;
+}
+
+void* ScratchArena::take(size_t nbytes) {
+  const size_t off = (used + 255) & ~(size_t)255;
+  if (off > cap || nbytes > cap - off)
+    fail(SLEMM_ERR_ARG, "workspace of this call exceeds the no-allocation scratch (" +
+                            std::to_string(cap >> 20) + " MiB)");
+  used = off + nbytes;
+  return base + off;
+}
+
 void DevBuf::reserve(size_t nbytes) {
   if (nbytes <= bytes) return;
   release();
+  if (ScratchArena* a = thread_arena()) {
+    p = a->take(nbytes);
+    bytes = nbytes;
+    borrowed = true;
+    return;
+  }
   hipError_t e = hipMalloc(&p, nbytes);
   if (e != hipSuccess) {
     p = nullptr;
@@ -42,9 +62,10 @@ void DevBuf::reserve(size_t nbytes) {
 }
 
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p && !borrowed) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
+  borrowed = false;
 }
 
 // ------------------------------------------------------------------------------ kernels ----

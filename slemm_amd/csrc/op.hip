@@ -306,6 +306,47 @@ void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const floa
   }
 }
 
+// Upper bound of the device workspace run_op reserves for a t-column call (the sum of every
+// reserve it makes, each 256-byte aligned, counting a buffer that grows between the passes at
+// both sizes).  The no-allocation entry point checks a call against its scratch with it.
+size_t op_workspace_bytes(const slemm_geno* h, int kind, const int* col_kind, int t, int precision) {
+  const Layout L = layout_for(h, t, precision);
+  bool any_quad = false;
+  for (int k = 0; k < t; ++k) {
+    const int kk = col_kind ? col_kind[k] : kind;
+    REQUIRE(kk == SLEMM_OP_LDET || kk == SLEMM_OP_QUADFORM, "Unknown linear operator in lnr_op_select()");
+    any_quad |= kk == SLEMM_OP_QUADFORM;
+  }
+  size_t total = 0;
+  auto add = [&](size_t b) { total += (b + 255) & ~(size_t)255; };
+  add(sizeof(int) * t);  // kinds
+  if (any_quad && h->c > 0) {
+    add(sizeof(double) * h->c * L.KP);                                  // qtv
+    add(sizeof(double) * ((h->n + 16383) / 16384) * h->c * t);          // redpart (Q'V, Q'out)
+  }
+  add(L.x1_bytes);
+  add(sizeof(double) * (h->n_pad / 256) * L.KP);                         // part1
+  add(sizeof(double) * 3 * L.KP);                                        // colsum
+  add(L.x2_bytes);
+  if (use_lds(precision)) {
+    add(sizeof(double) * (h->m_pad / 256) * L.KP);                       // part2
+    int64_t nsp = 0;
+    const size_t S1 = (size_t)pass_splits(h, 0, L.slot, nsp), S2 = (size_t)pass_splits(h, 1, L.slot, nsp);
+    add(sizeof(float) * L.KP * std::max(S1 * h->m_pad, S2 * h->n_pad));  // partbuf, sized up front
+    if (use_i8(L, t, nullptr)) {
+      const size_t I1 = (size_t)i8_pass_splits(0, t, h->m_pad, h->n_pad / kPad, nsp);
+      const size_t I2 = (size_t)i8_pass_splits(1, t, h->n_pad, h->m_pad / kPad, nsp);
+      add(sizeof(float) * L.KP * I1 * h->m_pad);                          // partbuf grown by the int8 splits,
+      add(sizeof(float) * L.KP * I2 * h->n_pad);                          // once per pass at most
+      add(i8_scratch_bytes(h->n_pad, t));                                 // digits, pass 1
+      add(i8_scratch_bytes(h->m_pad, t));                                 // digits, pass 2
+    }
+  } else {
+    add(sizeof(double) * (h->m_pad / blockprod_rows_per_wg(L.NT)) * L.KP);  // part2
+  }
+  return total;
+}
+
 // --------------------------------------------------------------------------- GEBV ----------
 // One thread per individual over the individual-major copy: each thread walks its own row of
 // codes SNP by SNP, so every individual's FP64 sum is accumulated in exactly the reference's

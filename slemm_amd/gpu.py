@@ -210,6 +210,24 @@ class Geno:
                                       _prec(precision), _stream()))
         return out.t()
 
+    def lnr_op_select_noalloc(self, lnr, tau0: float, v: torch.Tensor, out: torch.Tensor, precision="bf16x2",
+                              col_kind: Optional[Sequence] = None) -> torch.Tensor:
+        """lnr_op_select through slemm_gpu_op_noalloc: no device allocation inside the call, so the
+        caller passes both column-major buffers, v and out, as contiguous (t, n) tensors."""
+        kind = OPS[lnr] if isinstance(lnr, str) else int(lnr)
+        if v.dim() != 2 or v.shape[1] != self.n or not v.is_contiguous() or v.dtype != torch.float32:
+            raise ValueError("v must be a contiguous float32 (t, n) tensor")
+        if out.shape != v.shape or not out.is_contiguous() or out.dtype != torch.float32:
+            raise ValueError("out must be a contiguous float32 tensor shaped like v")
+        t = v.shape[0]
+        ck = None
+        if col_kind is not None:
+            ck = C.cast((C.c_int * t)(*[OPS[x] if isinstance(x, str) else int(x) for x in col_kind]),
+                        C.POINTER(C.c_int))
+        L.check(L.load().slemm_gpu_op_noalloc(self._h, kind, ck, tau0, _ptr(v), self.n, t, _ptr(out), self.n,
+                                              _prec(precision), _stream()))
+        return out
+
     def rmatvec(self, v: torch.Tensor, precision="bf16x2", weighted: bool = False,
                 scale_rr: bool = False) -> torch.Tensor:
         """(Z-m)'v (src/reml.cpp:159-195); W(Z-m)'v when weighted (:120-157).  (n,t) -> (m,t)."""
