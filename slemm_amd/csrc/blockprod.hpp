@@ -122,8 +122,9 @@ struct Fin1Args {
   const int32_t* popcnt = nullptr;
   const float* gm = nullptr;
   const float* gw = nullptr;       // null: unweighted
-  void* X2 = nullptr;              // BF16 pass-2 operand (null: none)
+  void* X2 = nullptr;              // pass-2 operand (null: none): split BF16, or FP32 with x2_fp32
   int64_t lo_offset16 = 0;
+  bool x2_fp32 = false;            // X2 as FP32, element (row, k) at row KP + k (int8 kernels)
   double* upart = nullptr;         // (rows_pad/256) x KP partial sums of gm * represented T
   float* out = nullptr;            // optional column-major rows x t
   int64_t ldo = 0;
@@ -198,28 +199,36 @@ struct I8Digits {
 };
 I8Digits i8_digits(int t);
 int i8_groups(int t);
+// Weights of the digit column sums: row weight w[row] (null: 1), from row col_row[k] of a
+// row-major array with leading dimension ldw when col_row is given (per-trait gm).
+struct I8SumWeights {
+  const float* w = nullptr;
+  const int* col_row = nullptr;
+  int64_t ldw = 0;
+};
+
 struct I8Launch {
   const uint8_t* G = nullptr;
   int64_t ldg = 0;
   int64_t red_pad = 0, rows_pad = 0;
-  const void* X = nullptr;
+  const void* X = nullptr;  // FP32 operand: element (row, k) at X[row KP + k], red_pad rows
   int KP = 0, t = 0;
   bool squared = false;
   int nsplit = 1;
   int64_t ns_per_split = 0;
   void* scratch = nullptr;  // i8_scratch_bytes(red_pad, t) bytes
   float* part = nullptr;    // nsplit x KP x rows_pad
-  // operand layout: hi half of vector k in column k; lo half in column k + tail_r when
-  // tail_r > 0 and k >= tail0, else in column k of the lo block at lo_off16 (16-byte units)
-  int64_t lo_off16 = 0;
-  int tail0 = 0, tail_r = 0;
+  int tail0 = 0, tail_r = 0;  // must be 0: the FP32 operand has no lo columns
   // optional per 16-column tile limits of the operand layout (wide blocks only), as in LdsArgs
   const int32_t* tile_nsuper = nullptr;
   const int64_t* tile_rows = nullptr;
+  // optional: sum_out[k] = sum over rows of sum_w(row) x (represented operand)_k, from the digits
+  double* sum_out = nullptr;
+  I8SumWeights sum_w;
 };
 bool i8_narrow_enabled();
 size_t i8_scratch_bytes(int64_t red_pad, int t);
-int i8_splits(int t, int64_t rows_pad, int64_t nsuper, int64_t target_wgs, int64_t& ns_per);
+int i8_splits(int t, int64_t rows_pad, int64_t nsuper, int64_t target_wgs, bool squared, int64_t& ns_per);
 void launch_bp_i8(const I8Launch& l, hipStream_t st);
 void launch_fin1(const Fin1Args& a, hipStream_t st);
 void launch_fin2(const Fin2Args& a, hipStream_t st);

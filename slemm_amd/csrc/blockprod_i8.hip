@@ -1,20 +1,23 @@
-// Block products of the split-BF16 operand on the int8 matrix cores: the Z'x / Zx passes of
-// src/reml.cpp:6-233 for the probe blocks, Lanczos / CG vectors, single right-hand sides and the
-// GWAS score vectors.  A narrow kernel (1 to 5 vectors: one MFMA tile, digits in registers, no
-// LDS) and a wide kernel (16-vector tiles of three digit tiles each plus a tail tile, digits
-// staged in LDS) share the digit preparation below.
+// Block products on the int8 matrix cores: the Z'x / Zx passes of src/reml.cpp:6-233 for the
+// probe blocks, Lanczos / CG vectors, single right-hand sides and the GWAS score vectors.  A
+// narrow kernel (1 to 5 vectors: one MFMA tile, digits in registers, no LDS) and a wide kernel
+// (16-vector tiles of three digit tiles each plus a tail tile, digits staged in LDS) share the
+// digit preparation below.
 //
-// The operand is the split-BF16 one the pack and pass-1 epilogue kernels write (value hi + lo per
-// element; their column sums feed the exact-centring terms).  Per column it is rendered as a
-// 23-bit fixed-point integer X = rint((hi + lo) 2^22 / s) with s the power of two at or above the
-// column's largest magnitude, split into three balanced int8 digits X = 2^16 d0 + 2^8 d1 + d2.
-// The genotype codes decode in registers to int8 dosages {0,1,1,2} ({0,1,1,4} for the squared
-// pass) with one shift, one mask and one byte permute per four codes, so no table and no LDS
-// are involved.  Each digit is one column of a 16 x 16 x 64 int8 MFMA tile, whose int32 sums
-// are exact (|sum| <= 2 * 128 * reduction length < 2^31 for reductions below 8.3M), and the
-// epilogue combines the three digit columns of a vector exactly in 64-bit integers before one
-// rounding to FP32.  Per element the rendering error is at most 2^-23 s; the split-BF16 kernels
-// instead round every product sum in FP32.
+// The operand is FP32 (element (row, k) at row KP + k), as the pack and pass-1 epilogue kernels
+// write it for these kernels.  Per column it is rendered as a 23-bit fixed-point integer
+// X = rint(x 2^22 / s) with s the power of two at or above the column's largest magnitude, split
+// into three balanced int8 digits X = 2^16 d0 + 2^8 d1 + d2; per element the rendering error is at
+// most 2^-23 s.  The column sums of the represented operand (plain for pass 1, gm-weighted for
+// pass 2) are taken from the digits themselves, so the exact-centring terms of the pass
+// epilogues match what the MFMAs multiply.  The genotype codes decode in registers to int8
+// dosages {0,1,1,2} ({0,1,1,4} for the squared pass) with one shift, one mask and one byte permute
+// per four codes, so no table and no LDS are involved.  Each digit is one column of a
+// 16 x 16 x 64 int8 MFMA tile, whose int32 sums are exact: |digit| <= 128, so each reduction split
+// is kept to at most 2^23 rows (2^22 for the squared pass) and |sum| <= 2^31.  The epilogue
+// combines the three digit columns of a vector exactly in 64-bit integers before one rounding to
+// FP32.  The split-BF16 kernels instead keep a 16-bit hi + lo operand and round every product sum
+// in FP32.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,23 +32,7 @@ namespace {
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 
-constexpr int kI8Blocks = 64;  // column-maximum partials per vector
-
-__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
-__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
-
-// Where the split-BF16 operand keeps vector k (8 rows of one column per 16 bytes, KP columns per
-// 8-row block): hi half in column k; lo half in column k + tail_r of the same block when
-// tail_r > 0 and k >= tail0 (hi and lo halves as columns), else in column k of the lo block at
-// lo_off (16-byte units).
-struct OperandLayout {
-  int KP, t;
-  int64_t lo_off;
-  int tail0, tail_r;
-  __device__ int64_t lo_index(int64_t b, int k) const {
-    return (tail_r > 0 && k >= tail0) ? b * KP + k + tail_r : lo_off + b * KP + k;
-  }
-};
+constexpr int kI8Blocks = 512;  // column-maximum partials per vector (row ranges of k_i8_colmax)
 
 // Digit columns: vectors [0, 16 nvt) in tiles of 16, digit p of vector k in column
 // 16 (3 (k / 16) + p) + k % 16 (the three digits of a vector in the same lane of three tiles);
@@ -55,42 +42,41 @@ __device__ __forceinline__ int digit_col(int k, int p, int nvt) {
   return k < 16 * nvt ? 16 * (3 * (k >> 4) + p) + (k & 15) : 48 * nvt + 3 * (k - 16 * nvt) + p;
 }
 
-// Partial maxima of |hi + lo| over 8-row blocks: block (x, k) covers blocks x, x + 64, ... of
-// vector k.  A NaN or infinite element makes the partial +inf (fmaxf alone would drop NaN).
-__global__ __launch_bounds__(256) void k_i8_colmax(const uint4* __restrict__ X, OperandLayout ol, int64_t nrb,
+// Partial maxima of |x| over the FP32 operand X[row KP + k] (rows [0, rows)): block (x, y) covers
+// columns 64 y .. 64 y + 63 (one per lane, so each wave reads 256 contiguous bytes of a row) and
+// the x-th of kI8Blocks contiguous row ranges, 4 rows at a time.  A NaN or infinite element
+// makes the partial +inf (fmaxf alone would drop NaN).  pmax[k kI8Blocks + x].
+__global__ __launch_bounds__(256) void k_i8_colmax(const float* __restrict__ X, int KP, int t, int64_t rows,
                                                    float* __restrict__ pmax) {
-  __shared__ float red[4];
-  const int k = blockIdx.y;
+  __shared__ float red[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int k = blockIdx.y * 64 + tx;
+  const int64_t per = (rows + kI8Blocks - 1) / kI8Blocks;
+  const int64_t r0 = (int64_t)blockIdx.x * per, r1 = min(rows, r0 + per);
   float mx = 0.f;
-  for (int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x; b < nrb; b += (int64_t)kI8Blocks * 256) {
-    const uint4 h = X[b * ol.KP + k], l = X[ol.lo_index(b, k)];
-    const uint32_t hw[4] = {h.x, h.y, h.z, h.w}, lw[4] = {l.x, l.y, l.z, l.w};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float a0 = fabsf(bf16_lo(hw[q]) + bf16_lo(lw[q])), a1 = fabsf(bf16_hi(hw[q]) + bf16_hi(lw[q]));
-      mx = fmaxf(mx, a0 <= FLT_MAX ? a0 : INFINITY);
-      mx = fmaxf(mx, a1 <= FLT_MAX ? a1 : INFINITY);
+  if (k < t)
+    for (int64_t r = r0 + ty; r < r1; r += 4) {
+      const float a = fabsf(X[r * KP + k]);
+      mx = fmaxf(mx, a <= FLT_MAX ? a : INFINITY);
     }
-  }
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  red[ty][tx] = mx;
   __syncthreads();
-  if (threadIdx.x == 0) pmax[k * kI8Blocks + blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  if (ty == 0 && k < t) pmax[k * kI8Blocks + blockIdx.x] = fmaxf(fmaxf(red[0][tx], red[1][tx]), fmaxf(red[2][tx], red[3][tx]));
 }
 
-// Per-vector scale from the 64 partial maxima: sc[k] = s_k 2^-22 with s_k = 2^e_k the power of
-// two at or above the column's largest magnitude (s = 1 for an all-zero column), ex[k] = e_k.  A
-// column with a NaN or infinite element (partial +inf) gets ex[k] = INT32_MIN and a NaN scale:
-// zero digits, and all its products come out NaN, as the floating-point kernels' do.
-__global__ __launch_bounds__(256) void k_i8_scale(const float* __restrict__ pmax, int t, float* __restrict__ sc,
+// Per-vector scale from the kI8Blocks partial maxima: s_k = 2^e_k, the power of two at or above the
+// column's largest magnitude (s = 1 for an all-zero column); ex[k] = e_k and sc[k] = 2^(e_k - 22)
+// in double (a float would underflow for columns below 2^-127).  A column with a NaN or infinite
+// element (partial +inf) gets ex[k] = INT32_MIN and a NaN scale: zero digits, and all its
+// products come out NaN, as the floating-point kernels' do.
+__global__ __launch_bounds__(256) void k_i8_scale(const float* __restrict__ pmax, int t, double* __restrict__ sc,
                                                   int* __restrict__ ex) {
   const int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= t) return;
   float mx = 0.f;
   for (int q = 0; q < kI8Blocks; ++q) mx = fmaxf(mx, pmax[k * kI8Blocks + q]);
   if (!(mx <= FLT_MAX)) {
-    sc[k] = __builtin_nanf("");
+    sc[k] = __builtin_nan("");
     ex[k] = INT32_MIN;
     return;
   }
@@ -99,7 +85,7 @@ __global__ __launch_bounds__(256) void k_i8_scale(const float* __restrict__ pmax
     (void)frexpf(mx, &e);               // mx = f 2^e with 0.5 <= f < 1, so mx < 2^e
     if (mx == ldexpf(0.5f, e)) e -= 1;  // a power of two: s = mx
   }
-  sc[k] = ldexpf(1.0f, e - 22);
+  sc[k] = ldexp(1.0, e - 22);
   ex[k] = e;
 }
 
@@ -107,34 +93,34 @@ __global__ __launch_bounds__(256) void k_i8_scale(const float* __restrict__ pmax
 // 512 (T / 8) + 128 g + 16 (T % 8) + j, j < 16, the rows whose codes the block-product kernels'
 // lane group g decodes at that step; byte 4 m + b of a lane's 16 bytes holds row j = 4 b + m (the
 // order in which the codes decode).  B[(4 T + g) ndc + digit_col(k, p)] holds digit p of vector k:
-// X = rint((hi + lo) 2^(22 - e_k)) = 2^16 d0 + 2^8 d1 + d2 with balanced digits.  Thread v: vector
-// k = v mod t of the pair (T, g) = (v / t) / 4, (v / t) mod 4, so that neighbouring threads read
-// neighbouring columns of the operand's row blocks and write neighbouring digit columns.
-__global__ __launch_bounds__(256) void k_i8_quant(const uint4* __restrict__ X, OperandLayout ol, int nvt, int ndc,
-                                                  int64_t nstep, const int* __restrict__ ex,
-                                                  i32x4_t* __restrict__ B) {
+// X = rint(x 2^(22 - e_k)) = 2^16 d0 + 2^8 d1 + d2 with balanced digits, from the FP32 element x
+// (the product x 2^(22 - e_k) is formed in double: exact, and no overflow for tiny columns).
+// Thread v: vector k = v mod t of the pair (T, g) = (v / t) / 4, (v / t) mod 4, so that
+// neighbouring threads read neighbouring columns of a row and write neighbouring digit columns.
+// Each thread also leaves psum[v] = sum_j w_j X_j of its 16 rows (w = 1 without weights, else
+// w[row] of the vector's weight row), for the digit column sums.
+__global__ __launch_bounds__(256) void k_i8_quant(const float* __restrict__ X, int KP, int t, int nvt, int ndc,
+                                                  int64_t nstep, const int* __restrict__ ex, I8SumWeights sw,
+                                                  i32x4_t* __restrict__ B, double* __restrict__ psum) {
   const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (v >= nstep * 4 * ol.t) return;
-  const int k = (int)(v % ol.t);
-  const int64_t tg = v / ol.t;
+  if (v >= nstep * 4 * t) return;
+  const int k = (int)(v % t);
+  const int64_t tg = v / t;
   const int64_t T = tg >> 2;
   const int g = (int)(tg & 3);
   const int e = ex[k];
   const bool finite = e != INT32_MIN;
-  const float mult = finite ? ldexpf(1.0f, 22 - e) : 0.f;  // scales exactly: |X| <= 2^22
+  const double mult = finite ? ldexp(1.0, 22 - e) : 0.0;  // |X| <= 2^22
   const int64_t r0 = (T >> 3) * 512 + g * 128 + (T & 7) * 16;  // first of 16 rows
+  const float* w = sw.w ? sw.w + (sw.col_row ? (int64_t)sw.col_row[k] * sw.ldw : 0) : nullptr;
   int X16[16];
+  double ws = 0.0;
 #pragma unroll
-  for (int h8 = 0; h8 < 2; ++h8) {
-    const int64_t b = (r0 >> 3) + h8;
-    const uint4 hv = X[b * ol.KP + k], lv = X[ol.lo_index(b, k)];
-    const uint32_t hw[4] = {hv.x, hv.y, hv.z, hv.w}, lw[4] = {lv.x, lv.y, lv.z, lv.w};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      X16[h8 * 8 + 2 * q] = finite ? (int)rintf((bf16_lo(hw[q]) + bf16_lo(lw[q])) * mult) : 0;
-      X16[h8 * 8 + 2 * q + 1] = finite ? (int)rintf((bf16_hi(hw[q]) + bf16_hi(lw[q])) * mult) : 0;
-    }
+  for (int j = 0; j < 16; ++j) {
+    X16[j] = finite ? (int)rint((double)X[(r0 + j) * KP + k] * mult) : 0;
+    ws += w ? (double)w[r0 + j] * (double)X16[j] : (double)X16[j];
   }
+  psum[v] = ws;
   uint32_t d[3][4] = {};
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
@@ -153,13 +139,44 @@ __global__ __launch_bounds__(256) void k_i8_quant(const uint4* __restrict__ X, O
     B[(4 * T + g) * ndc + digit_col(k, p, nvt)] = i32x4_t{(int)d[p][0], (int)d[p][1], (int)d[p][2], (int)d[p][3]};
 }
 
+// Column sums of the represented operand, out[k] = 2^(e_k - 22) sum_q psum[q t + k] over the nq
+// per-thread sums of the quantization (the sum over the digits themselves, so the exact-centring
+// terms of the pass epilogues match what the MFMAs multiply), NaN for a non-finite column.  Two
+// stages in fixed order: block (x, y) sums the y-th of kI8SumBlocks contiguous ranges of q for
+// columns 64 x .. 64 x + 63 (each wave reads 512 contiguous bytes), then one thread per column.
+constexpr int kI8SumBlocks = 64;
+
+__global__ __launch_bounds__(256) void k_i8_digit_sums1(const double* __restrict__ psum, int64_t nq, int t,
+                                                        double* __restrict__ part) {
+  __shared__ double red[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int k = blockIdx.x * 64 + tx;
+  const int64_t per = (nq + kI8SumBlocks - 1) / kI8SumBlocks;
+  const int64_t q0 = (int64_t)blockIdx.y * per, q1 = min(nq, q0 + per);
+  double s = 0.0;
+  if (k < t)
+    for (int64_t q = q0 + ty; q < q1; q += 4) s += psum[q * t + k];
+  red[ty][tx] = s;
+  __syncthreads();
+  if (ty == 0 && k < t) part[(int64_t)blockIdx.y * t + k] = (red[0][tx] + red[1][tx]) + (red[2][tx] + red[3][tx]);
+}
+
+__global__ __launch_bounds__(256) void k_i8_digit_sums2(const double* __restrict__ part, int t,
+                                                        const int* __restrict__ ex, double* __restrict__ out) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= t) return;
+  double s = 0.0;
+  for (int y = 0; y < kI8SumBlocks; ++y) s += part[(int64_t)y * t + k];
+  out[k] = ex[k] == INT32_MIN ? __builtin_nan("") : ldexp(s, ex[k] - 22);
+}
+
 struct I8Args {
   const uint8_t* G = nullptr;
   int64_t ldg = 0;
   int64_t nsuper = 0, ns_per_split = 0;
   const i32x4_t* B = nullptr;
   int t = 0;
-  const float* sc = nullptr;
+  const double* sc = nullptr;
   float* part = nullptr;
   int KP = 0;
   int64_t rows_pad = 0;
@@ -229,8 +246,7 @@ __global__ __launch_bounds__(256) void k_bp_i8(I8Args a) {
     }
   }
   // lane (r, g) holds rows 4 g .. 4 g + 3 of column r; column 3 k + p is digit p of vector k.
-  // Lanes of column 3 k combine the three digits; lanes of column 3 k + 1 clear the lo-half
-  // column t + k, which the pass epilogues add to column k.
+  // Lanes of column 3 k combine the three digits.
   const int k = r / 3, p = r % 3;
   const bool live = r < nc;
   const double scale = live ? (double)a.sc[k] : 0.0;
@@ -249,9 +265,6 @@ __global__ __launch_bounds__(256) void k_bp_i8(I8Args a) {
     if (live && p == 0)
       *reinterpret_cast<float4*>(a.part + (blockIdx.z * (int64_t)a.KP + k) * a.rows_pad + row) =
           make_float4(o[0], o[1], o[2], o[3]);
-    else if (live && p == 1)
-      *reinterpret_cast<float4*>(a.part + (blockIdx.z * (int64_t)a.KP + a.t + k) * a.rows_pad + row) =
-          make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -261,7 +274,7 @@ struct I8WArgs {
   int64_t nsuper = 0, ns_per_split = 0;
   const i32x4_t* B = nullptr;
   int ndc = 0, nvt = 0, r8 = 0, t = 0;
-  const float* sc = nullptr;
+  const double* sc = nullptr;
   float* part = nullptr;
   int KP = 0;
   int64_t rows_pad = 0;
@@ -482,10 +495,14 @@ static int i8_rt() {
   return rt == 2 || rt == 8 ? rt : 4;
 }
 
-int i8_splits(int t, int64_t rows_pad, int64_t nsuper, int64_t target_wgs, int64_t& ns_per) {
+int i8_splits(int t, int64_t rows_pad, int64_t nsuper, int64_t target_wgs, bool squared, int64_t& ns_per) {
   const bool narrow = i8_digits(t).nvt == 0;
   const int64_t wgs = rows_pad / (64 * (narrow ? i8_rt() : kI8WRT)) * i8_groups(t);
   int64_t S = (target_wgs + wgs - 1) / wgs;
+  // every split's int32 sums stay exact: |digit| <= 128 times dosages <= 2 (<= 4 squared) over at
+  // most 2^23 (2^22) rows, at most 2^31 in magnitude
+  const int64_t max_ns = (squared ? (int64_t)1 << 22 : (int64_t)1 << 23) / 512;
+  S = std::max<int64_t>(S, (nsuper + max_ns - 1) / max_ns);
   S = std::max<int64_t>(1, std::min<int64_t>(S, nsuper));
   ns_per = (nsuper + S - 1) / S;
   return (int)((nsuper + ns_per - 1) / ns_per);
@@ -503,10 +520,14 @@ I8Digits i8_digits(int t) {
   return d;
 }
 
-// scratch: sc[t] (floats) and ex[t] (ints), each rounded up to 64 entries, the 64 partial maxima
-// per vector, then the digits (16-byte aligned)
+// scratch: sc[t] (doubles) and ex[t] (ints), each rounded up to 64 entries, the kI8Blocks partial
+// maxima per vector (floats), the per-thread digit sums of the quantization (4 per step and
+// vector, doubles), their kI8SumBlocks stage sums per vector (doubles), then the digits
+// (16-byte aligned)
 size_t i8_scratch_bytes(int64_t red_pad, int t) {
-  return sizeof(float) * (2 * 64 * (size_t)((t + 63) / 64) + (size_t)t * kI8Blocks) + (size_t)red_pad * i8_digits(t).ndc;
+  const size_t t64 = 64 * (size_t)((t + 63) / 64);
+  return 12 * t64 + sizeof(float) * (size_t)t * kI8Blocks + sizeof(double) * 4 * (size_t)(red_pad / 64) * t +
+         sizeof(double) * (size_t)kI8SumBlocks * t + (size_t)red_pad * i8_digits(t).ndc;
 }
 
 int i8_groups(int t) {
@@ -522,22 +543,30 @@ void launch_bp_i8(const I8Launch& l, hipStream_t st) {
   if (l.KP < l.t || l.KP % 16 != 0) fail(SLEMM_ERR_ARG, "int8 block: column padding mismatch");
   if (l.rows_pad % 512 != 0 || l.red_pad % 512 != 0 || l.ldg * 4 < l.red_pad || l.ldg % 16 != 0)
     fail(SLEMM_ERR_ARG, "int8 block: padding mismatch");
-  if (l.red_pad > 8 * 1024 * 1024) fail(SLEMM_ERR_ARG, "int8 block: reduction too long for int32 sums");
+  // int32 exactness of every split's sums (i8_splits keeps to this)
+  if (l.ns_per_split * 512 > (l.squared ? (int64_t)1 << 22 : (int64_t)1 << 23))
+    fail(SLEMM_ERR_ARG, "int8 block: reduction split too long for exact int32 sums");
   if (l.squared && l.t != 1) fail(SLEMM_ERR_ARG, "squared-dosage products take one vector");
-  if (l.tail_r > 0 && (l.tail0 + 2 * (l.t - l.tail0) > l.KP || l.tail_r != l.t - l.tail0))
-    fail(SLEMM_ERR_ARG, "int8 block: lo columns outside the block");
+  if (l.tail_r != 0) fail(SLEMM_ERR_ARG, "int8 blocks take the FP32 operand layout (no lo columns)");
   const size_t t64 = 64 * (size_t)((l.t + 63) / 64);
-  float* sc = static_cast<float*>(l.scratch);
+  double* sc = static_cast<double*>(l.scratch);
   int* ex = reinterpret_cast<int*>(sc + t64);
-  float* pmax = sc + 2 * t64;
-  i32x4_t* B = reinterpret_cast<i32x4_t*>(pmax + (size_t)l.t * kI8Blocks);
-  if (reinterpret_cast<uintptr_t>(B) % 16 != 0) fail(SLEMM_ERR_ARG, "int8 block: scratch alignment");
-  const uint4* X = static_cast<const uint4*>(l.X);
-  OperandLayout ol{l.KP, l.t, l.lo_off16, l.tail0, l.tail_r};
-  k_i8_colmax<<<dim3(kI8Blocks, l.t), 256, 0, st>>>(X, ol, l.red_pad / 8, pmax);
+  float* pmax = reinterpret_cast<float*>(ex + t64);
   const int64_t nstep = l.red_pad / 64;
+  double* psum = reinterpret_cast<double*>(pmax + (size_t)l.t * kI8Blocks);
+  double* spart = psum + 4 * (size_t)nstep * l.t;
+  i32x4_t* B = reinterpret_cast<i32x4_t*>(spart + (size_t)kI8SumBlocks * l.t);
+  if (reinterpret_cast<uintptr_t>(psum) % 8 != 0 || reinterpret_cast<uintptr_t>(B) % 16 != 0)
+    fail(SLEMM_ERR_ARG, "int8 block: scratch alignment");
+  const float* X = static_cast<const float*>(l.X);
+  k_i8_colmax<<<dim3(kI8Blocks, (unsigned)((l.t + 63) / 64)), 256, 0, st>>>(X, l.KP, l.t, l.red_pad, pmax);
   k_i8_scale<<<(unsigned)((l.t + 255) / 256), 256, 0, st>>>(pmax, l.t, sc, ex);
-  k_i8_quant<<<(unsigned)((nstep * 4 * l.t + 255) / 256), 256, 0, st>>>(X, ol, dl.nvt, dl.ndc, nstep, ex, B);
+  k_i8_quant<<<(unsigned)((nstep * 4 * l.t + 255) / 256), 256, 0, st>>>(X, l.KP, l.t, dl.nvt, dl.ndc, nstep, ex,
+                                                                         l.sum_w, B, psum);
+  if (l.sum_out) {
+    k_i8_digit_sums1<<<dim3((unsigned)((l.t + 63) / 64), kI8SumBlocks), 256, 0, st>>>(psum, 4 * nstep, l.t, spart);
+    k_i8_digit_sums2<<<(unsigned)((l.t + 255) / 256), 256, 0, st>>>(spart, l.t, ex, l.sum_out);
+  }
   if (narrow) {
     I8Args a;
     a.G = l.G; a.ldg = l.ldg; a.nsuper = l.red_pad / 512; a.ns_per_split = l.ns_per_split; a.B = B; a.t = l.t;

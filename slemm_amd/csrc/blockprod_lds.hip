@@ -665,12 +665,17 @@ __global__ __launch_bounds__(256) void k_fin1(Fin1Args a) {
           const float wv = gwk ? gwk[row] : 1.0f;
           const double rem = xb * ((double)pck[row] - nk * (double)gmv);
           const float tv = a.raw ? P : wv * (float)((double)P - (double)gmv * sk + rem);
-          hb[e] = bf16_bits_rn(tv);
-          lb[e] = bf16_bits_rn(tv - bf16_to_float(hb[e]));
-          usum += (double)gmv * ((double)bf16_to_float(hb[e]) + (double)bf16_to_float(lb[e]));
+          if (a.x2_fp32) {  // FP32 operand for the int8 kernels, which sum its digits themselves
+            if (a.X2) reinterpret_cast<float*>(a.X2)[row * a.KP + k] = tv;
+            usum += (double)gmv * (double)tv;
+          } else {
+            hb[e] = bf16_bits_rn(tv);
+            lb[e] = bf16_bits_rn(tv - bf16_to_float(hb[e]));
+            usum += (double)gmv * ((double)bf16_to_float(hb[e]) + (double)bf16_to_float(lb[e]));
+          }
           if (a.out && k < a.t && row < a.rows) a.out[row + (int64_t)k * a.ldo] = tv;
         }
-        if (a.X2) {
+        if (a.X2 && !a.x2_fp32) {
           uint4* x2 = reinterpret_cast<uint4*>(a.X2);
           const int64_t idx = (i0 >> 3) * a.KP + k;
           x2[idx] = make_uint4(hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16), hb[6] | (hb[7] << 16));

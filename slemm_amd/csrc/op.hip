@@ -11,8 +11,12 @@
 
 namespace slemm {
 
-Layout layout_for(const slemm_geno* h, int t, int precision) {
+static bool i8_wide_enabled();
+
+Layout layout_for(const slemm_geno* h, int t, int precision, bool traits, bool allow_i8) {
   Layout L;
+  L.i8 = false;
+  L.op_mode = precision;
   L.NT = blockprod_nt(t);
   // Wide split-BF16 blocks (t > 80) run as 80-column groups: the pipelined kernels take at most
   // 5 column tiles (two waves per SIMD), and 6-8-tile groups would fall back to the unpipelined
@@ -29,6 +33,23 @@ Layout layout_for(const slemm_geno* h, int t, int precision) {
   L.hl = hl_ok && 2 * t <= 16;
   L.slot = L.hl ? 0 : L.NT;
   L.tail0 = L.tail_r = 0;
+  // Blocks of 1 to 5 vectors (hi and lo halves as columns) run on the int8 narrow kernel
+  // (SLEMM_I8_NARROW=0: the split-BF16 ones), blocks of 9 or more vectors on the int8 wide kernel
+  // (SLEMM_I8_WIDE=0: the split-BF16 ones); 6 to 8 vectors (hi/lo as columns: one MFMA tile per
+  // step, streaming faster than three digit tiles) and the batched traits' narrow blocks keep the
+  // split-BF16 kernels.
+  if (allow_i8 && precision == PREC_BF16X2 && use_lds(precision)) {
+    if (t <= 5) L.i8 = !traits && L.hl && i8_narrow_enabled();
+    else if (!(t <= 8 && L.hl)) L.i8 = i8_wide_enabled();
+  }
+  if (L.i8) {
+    L.hl = false;
+    L.op_mode = PREC_FP32;
+    L.x1_bytes = (size_t)h->n_pad * L.KP * 4;
+    L.x2_bytes = (size_t)h->m_pad * L.KP * 4;
+    L.lo1_16 = L.lo2_16 = 0;
+    return L;
+  }
   if (L.hl) {
     L.tail_r = t;
     L.x1_bytes = (size_t)h->n_pad * L.KP * 2;
@@ -84,7 +105,7 @@ void pack_pass1(slemm_geno* h, const Layout& L, const float* V, int64_t ldv, int
   PackArgs p;
   p.V = V; p.ldv = ldv; p.t = t; p.KP = L.KP; p.n = h->n; p.rows_pad = h->n_pad;
   p.rr = rr; p.Q = h->Q; p.ldq = h->n_pad; p.c = h->c; p.qtv = qtv; p.kinds = kinds;
-  p.mode = precision; p.X = nullptr; p.lo_offset16 = L.lo1_16; p.part = h->part1.as<double>();
+  p.mode = L.op_mode; p.X = nullptr; p.lo_offset16 = L.lo1_16; p.part = h->part1.as<double>();
   p.tail0 = L.tail0; p.tail_r = L.tail_r;
   if (tc) {
     p.tc = *tc;
@@ -132,10 +153,7 @@ void tail_args(const Layout& L, int t, LdsArgs& la) {
   la.last_tile = L.hl ? 0 : (L.tail_r > 0 ? L.tail0 / 16 : (t + 15) / 16 - 1);
 }
 
-// Blocks of 1 to 5 vectors (hi and lo halves as columns) run on the int8 narrow kernel
-// (SLEMM_I8_NARROW=0: the split-BF16 ones), blocks of 9 or more vectors on the int8 wide kernel
-// (SLEMM_I8_WIDE=0: the split-BF16 ones); the batched traits' narrow blocks keep the split-BF16
-// kernels.  SLEMM_I8_TARGET sets the workgroup target of the reduction split (read per call): by
+// SLEMM_I8_TARGET sets the workgroup target of the int8 reduction split (read per call): by
 // default 2048 for narrow blocks (at n = 1M x m = 60k the SNP-major pass needs the split: 3.06 ms
 // at a 512 target, 2.64 ms at 2048; at n = 100k x m = 500k 512 to 4096 are within 2%) and 8192
 // for wide blocks (the best of 1024 to 16384 at n = 100k x m = 500k).
@@ -144,31 +162,25 @@ static bool i8_wide_enabled() {
   return !(e && e[0] == '0');
 }
 
-static bool use_i8(const Layout& L, int t, const TraitCols* tc) {
-  if (t <= 5) return !tc && L.hl && i8_narrow_enabled();
-  // 6 to 8 vectors: the BF16 kernels with hi and lo halves as columns run one MFMA tile per
-  // step and stream faster than the wide kernel's three digit tiles
-  if (t <= 8 && L.hl) return false;
-  return i8_wide_enabled();
-}
-
-static int i8_pass_splits(int pass, int t, int64_t rows_pad, int64_t nsuper, int64_t& nsp) {
+static int i8_pass_splits(int pass, int t, int64_t rows_pad, int64_t nsuper, bool squared, int64_t& nsp) {
   const char* e = std::getenv("SLEMM_I8_TARGET");
   const int64_t dflt = t <= 5 ? 2048 : 8192;
   const int64_t target = e ? std::max<int64_t>(1, std::atoll(e)) : dflt;
-  return i8_splits(t, rows_pad, nsuper, target, nsp);
+  (void)pass;
+  return i8_splits(t, rows_pad, nsuper, target, squared, nsp);
 }
 
+// sums: where the digit column sums of the represented operand go (weighted by sw), null: none
 static void bp_i8(slemm_geno* h, const uint8_t* G, int64_t ldg, int64_t red_pad, int64_t rows_pad, const void* X,
-                  int64_t lo_off16, const Layout& L, int t, bool squared, int S, int64_t nsp, hipStream_t st,
-                  const int32_t* tile_nsuper = nullptr, const int64_t* tile_rows = nullptr) {
+                  const Layout& L, int t, bool squared, int S, int64_t nsp, hipStream_t st, double* sums,
+                  I8SumWeights sw, const int32_t* tile_nsuper = nullptr, const int64_t* tile_rows = nullptr) {
   h->i8buf.reserve(i8_scratch_bytes(red_pad, t));
   I8Launch l;
   l.G = G; l.ldg = ldg; l.red_pad = red_pad; l.rows_pad = rows_pad; l.X = X; l.KP = L.KP; l.t = t;
   l.squared = squared; l.nsplit = S; l.ns_per_split = nsp; l.scratch = h->i8buf.p;
   l.part = h->partbuf.as<float>();
-  l.lo_off16 = lo_off16; l.tail0 = L.tail0; l.tail_r = L.tail_r;
   l.tile_nsuper = tile_nsuper; l.tile_rows = tile_rows;
+  l.sum_out = sums; l.sum_w = sw;
   launch_bp_i8(l, st);
 }
 
@@ -183,8 +195,8 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
                int64_t ldo, hipStream_t st, bool raw, bool squared, const TraitCols* tc) {
   int64_t nsp = 0;
   const int64_t nsuper = h->n_pad / kPad;
-  const bool i8 = use_i8(L, t, tc);
-  const int S = i8 ? i8_pass_splits(0, t, h->m_pad, nsuper, nsp) : (int)pass_splits(h, 0, L.slot, nsp);
+  const bool i8 = L.i8;
+  const int S = i8 ? i8_pass_splits(0, t, h->m_pad, nsuper, squared, nsp) : (int)pass_splits(h, 0, L.slot, nsp);
   h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * h->m_pad);
   LdsArgs la;
   la.G = h->gs; la.ldg = h->ld_gs; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x1.p;
@@ -193,8 +205,9 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
   tail_args(L, t, la);
   if (tc) la.tile_nsuper = tc->tile_nsuper;  // reduction over individuals stops at the subsets' end
   TimingToken tok = timing_begin(st);
-  if (i8)
-    bp_i8(h, h->gs, h->ld_gs, h->n_pad, h->m_pad, h->x1.p, L.lo1_16, L, t, squared, S, nsp, st, la.tile_nsuper);
+  if (i8)  // the centring sums s_k of the epilogue become those of the digits
+    bp_i8(h, h->gs, h->ld_gs, h->n_pad, h->m_pad, h->x1.p, L, t, squared, S, nsp, st, h->colsum.as<double>(),
+          I8SumWeights{}, la.tile_nsuper);
   else launch_bp(la, L.NT, S, bp_choice(h, 0, L.slot).variant, st);
   timing_end(tok, st, EPI_PASS1, PREC_BF16X2);
   Fin1Args f;
@@ -202,18 +215,19 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
   f.colsum = h->colsum.as<double>(); f.xsum = h->colsum.as<double>() + 2 * L.KP; f.xsum_n = (double)h->n;
   f.popcnt = h->popcnt; f.gm = h->gm; f.gw = weighted ? h->gw : nullptr; f.X2 = x2; f.lo_offset16 = L.lo2_16;
   f.upart = upart; f.out = out; f.ldo = ldo; f.raw = raw; f.tail0 = L.tail0; f.tail_r = L.tail_r;
+  f.x2_fp32 = L.i8;
   if (tc) f.tc = *tc;
   launch_fin1(f, st);
 }
 
 // Z X2 pass, epilogue out = rr (P - u) (+ tau0 v for LDET columns).
-void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const float* rr, const float* V,
+void pass2_lds(slemm_geno* h, const Layout& L, int t, double* u, const float* rr, const float* V,
                int64_t ldv, float tau0, const int* kinds, float* out, int64_t ldo, hipStream_t st,
                const TraitCols* tc) {
   int64_t nsp = 0;
   const int64_t nsuper = h->m_pad / kPad;
-  const bool i8 = use_i8(L, t, tc);
-  const int S = i8 ? i8_pass_splits(1, t, h->n_pad, nsuper, nsp) : (int)pass_splits(h, 1, L.slot, nsp);
+  const bool i8 = L.i8;
+  const int S = i8 ? i8_pass_splits(1, t, h->n_pad, nsuper, false, nsp) : (int)pass_splits(h, 1, L.slot, nsp);
   h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * h->n_pad);
   LdsArgs la;
   la.G = h->gi; la.ldg = h->ld_gi; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x2.p;
@@ -221,9 +235,17 @@ void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const flo
   tail_args(L, t, la);
   if (tc) la.tile_rows = tc->tile_rows;  // output rows (individuals) past the subsets' end are zero
   TimingToken tok = timing_begin(st);
-  if (i8)
-    bp_i8(h, h->gi, h->ld_gi, h->m_pad, h->n_pad, h->x2.p, L.lo2_16, L, t, false, S, nsp, st, nullptr, la.tile_rows);
-  else launch_bp(la, L.NT, S, bp_choice(h, 1, L.slot).variant, st);
+  if (i8) {  // u_k becomes sum_j gm_j (digit operand)_jk, gm of the column's trait for batched traits
+    I8SumWeights sw;
+    sw.w = tc ? tc->gm : h->gm;
+    if (tc) {
+      sw.col_row = tc->col_trait;
+      sw.ldw = tc->ld_m;
+    }
+    bp_i8(h, h->gi, h->ld_gi, h->m_pad, h->n_pad, h->x2.p, L, t, false, S, nsp, st, u, sw, nullptr, la.tile_rows);
+  } else {
+    launch_bp(la, L.NT, S, bp_choice(h, 1, L.slot).variant, st);
+  }
   timing_end(tok, st, EPI_PASS2, PREC_BF16X2);
   Fin2Args f;
   f.part = h->partbuf.as<float>(); f.nsplit = S; f.rows_pad = h->n_pad; f.rows = h->n; f.KP = L.KP; f.t = t;
@@ -333,9 +355,9 @@ size_t op_workspace_bytes(const slemm_geno* h, int kind, const int* col_kind, in
     int64_t nsp = 0;
     const size_t S1 = (size_t)pass_splits(h, 0, L.slot, nsp), S2 = (size_t)pass_splits(h, 1, L.slot, nsp);
     add(sizeof(float) * L.KP * std::max(S1 * h->m_pad, S2 * h->n_pad));  // partbuf, sized up front
-    if (use_i8(L, t, nullptr)) {
-      const size_t I1 = (size_t)i8_pass_splits(0, t, h->m_pad, h->n_pad / kPad, nsp);
-      const size_t I2 = (size_t)i8_pass_splits(1, t, h->n_pad, h->m_pad / kPad, nsp);
+    if (L.i8) {
+      const size_t I1 = (size_t)i8_pass_splits(0, t, h->m_pad, h->n_pad / kPad, false, nsp);
+      const size_t I2 = (size_t)i8_pass_splits(1, t, h->n_pad, h->m_pad / kPad, false, nsp);
       add(sizeof(float) * L.KP * I1 * h->m_pad);                          // partbuf grown by the int8 splits,
       add(sizeof(float) * L.KP * I2 * h->n_pad);                          // once per pass at most
       add(i8_scratch_bytes(h->n_pad, t));                                 // digits, pass 1
@@ -521,7 +543,7 @@ extern "C" int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, 
     REQUIRE(h, "null handle");
     REQUIRE(t > 0 && reps > 0, "t and reps must be positive");
     HIPCHECK(hipSetDevice(h->device));
-    const Layout L = layout_for(h, t, PREC_BF16X2);
+    const Layout L = layout_for(h, t, PREC_BF16X2, false, false);  // the split-BF16 kernels only
     h->x1.reserve(L.x1_bytes);
     h->x2.reserve(L.x2_bytes);
     // operands with the value spread of real split-BF16 blocks (hi ~ U(-1,1), lo ~ 2^-9 hi): zero
@@ -721,7 +743,7 @@ extern "C" int slemm_gpu_z(slemm_geno* h, const float* U, int64_t ldu, int t, fl
     h->colsum.reserve(sizeof(double) * 3 * L.KP);
     PackArgs p;
     p.V = U; p.ldv = ldu; p.t = t; p.KP = L.KP; p.n = h->m; p.rows_pad = h->m_pad;
-    p.mode = precision; p.X = h->x2.p; p.lo_offset16 = L.lo2_16; p.part = h->part2.as<double>();
+    p.mode = L.op_mode; p.X = h->x2.p; p.lo_offset16 = L.lo2_16; p.part = h->part2.as<double>();
     p.wsum = h->gm;
     p.tail0 = L.tail0; p.tail_r = L.tail_r;
     launch_pack(p, st);

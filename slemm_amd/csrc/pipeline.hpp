@@ -19,6 +19,10 @@ struct Layout {
   // step instead of 10); hl is the case q = 0.  tail_r = 0: none.
   int tail0, tail_r;
   int slot;                // autotuner table entry: NT, or 0 for hi/lo-in-columns blocks
+  // int8 block products (blockprod_i8.hip): the pass operands are then FP32 (element (row, k) at
+  // row KP + k; no hi/lo halves, no lo columns) and the int8 digits are rendered from them
+  bool i8;
+  int op_mode;             // PREC_FP32 when i8, else the precision
 };
 
 struct BpChoice {
@@ -26,7 +30,9 @@ struct BpChoice {
   int64_t target;
 };
 
-Layout layout_for(const slemm_geno* h, int t, int precision);
+// traits: blocks of the batched traits (their narrow blocks stay on the BF16 kernels);
+// allow_i8 = false: the split-BF16 layout whatever the int8 switches say (the autotuner's).
+Layout layout_for(const slemm_geno* h, int t, int precision, bool traits = false, bool allow_i8 = true);
 void check_precision(int precision);
 void check_dense(const float* p, int64_t ld, int64_t rows, int t, const char* what);
 const int* upload_kinds(slemm_geno* h, int kind, const int* col_kind, int t, bool& any_quad, hipStream_t st);
@@ -47,7 +53,8 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
                int64_t ldo, hipStream_t st, bool raw = false, bool squared = false,
                const TraitCols* tc = nullptr);
 // Z X2 pass, epilogue out = rr (P - u) (+ tau0 v for LDET columns), masked to each column's subset.
-void pass2_lds(slemm_geno* h, const Layout& L, int t, const double* u, const float* rr, const float* V,
+// On the int8 kernels u is replaced by the gm-weighted sums of the represented (digit) operand.
+void pass2_lds(slemm_geno* h, const Layout& L, int t, double* u, const float* rr, const float* V,
                int64_t ldv, float tau0, const int* kinds, float* out, int64_t ldo, hipStream_t st,
                const TraitCols* tc = nullptr);
 // (W) Zc' (R^-1/2) V -> out (m x t), one pass over the SNP-major copy.

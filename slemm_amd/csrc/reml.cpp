@@ -161,7 +161,7 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
     HIPCHECK(hipSetDevice(h->device));
     require_model(h);
     const auto t_start = std::chrono::steady_clock::now();
-    const int64_t n = h->n, m = h->m;
+    const int64_t n = h->n;
     if (verbose) printf("\nStarted stochastic Lanczos DF-REML.\n");
 
     MatD Q;

@@ -130,7 +130,7 @@ void run_op_traits(slemm_traits* tr, const int* col_trait, const int* col_kind, 
   check_dense(V, ldv, h->n, t, "V");
   check_dense(out, ldo, h->n, t, "out");
   REQUIRE((const void*)V != (const void*)out, "out must not alias V");
-  const Layout L = layout_for(h, t, precision);
+  const Layout L = layout_for(h, t, precision, true);
   bool any_quad = false;
   const int* kinds = upload_kinds(h, SLEMM_OP_LDET, col_kind, t, any_quad, st);
   upload_col_trait(tr, col_trait, t, L, st);
@@ -306,7 +306,7 @@ extern "C" int slemm_gpu_traits_lseed(slemm_traits* tr, const int* col_trait, co
     // converged groups stop costing block-product work: their columns leave the tile limits
     const ActiveFn on_active = [&](const std::vector<int>& active, hipStream_t s) {
       tr->active = active;
-      upload_tile_limits(tr, t, layout_for(tr->h, t, precision), s);
+      upload_tile_limits(tr, t, layout_for(tr->h, t, precision, true), s);
     };
     try {
       *out = lseed_run(tr->h, op, col_group, B, ldb, t, tol, maxit, keep_cols, verbose, (hipStream_t)stream,
