@@ -222,6 +222,35 @@ def gemm_ceiling(M=16384, N=16384, Kd=8192, reps=5):
     return out
 
 
+MFMA_NAMES = ["i8_16x16x64", "i8_32x32x32", "bf16_16x16x32", "bf16_32x32x16"]
+
+
+def mfma_ceiling(lib, L, configs):
+    """Register-only MFMA issue rate of this GPU (slemm_gpu_mfma_ceiling): for each (instruction,
+    independent accumulator chains per wave, waves per SIMD) the tera-operations per second and
+    the shader clock over the loop."""
+    import ctypes as C
+    rows = []
+    for instr, chains, wps in configs:
+        tops, cr, ms = C.c_double(), C.c_double(), C.c_double()
+        iters = 20000 if instr in (0, 2) else 10000
+        L.check(lib.slemm_gpu_mfma_ceiling(instr, chains, wps, iters, C.byref(tops), C.byref(cr), C.byref(ms)))
+        rows.append({"instr": MFMA_NAMES[instr], "chains": chains, "waves_per_simd": wps, "iters": iters,
+                     "TOP/s": round(tops.value, 1), "clock_GHz": round(cr.value * 0.1, 3), "ms": round(ms.value, 3)})
+    return rows
+
+
+def mfma_ceiling_summary(lib, L):
+    """The best measured rate of the int8 and BF16 16x16 instructions over 4-8 chains and 2-4 waves
+    per SIMD (the block products' regime)."""
+    rows = mfma_ceiling(lib, L, [(i, c, w) for i in (0, 2) for c in (4, 8) for w in (2, 4)])
+    best = {}
+    for r in rows:
+        if r["instr"] not in best or r["TOP/s"] > best[r["instr"]]["TOP/s"]:
+            best[r["instr"]] = r
+    return best
+
+
 def simulate_phenotype(g, n, m, n_causal, h2, seed):
     """y = Zc beta + e + 10, beta ~ N(0,1) on n_causal random SNPs, var(e) set for h2."""
     import numpy as np
@@ -623,6 +652,10 @@ def main():
     ap.add_argument("--traits-T", type=int, default=32)
     ap.add_argument("--scale-n", type=int, default=1_000_000)
     ap.add_argument("--scale-m", type=int, default=60_000)
+    ap.add_argument("--mfma-sweep", action="store_true",
+                    help="print the register-only MFMA issue rates (4 instructions x 1-8 chains x 1-4 waves per SIMD) and exit")
+    ap.add_argument("--only-op", action="store_true",
+                    help="run only the headline operator steps (for profiling) and print their timing")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -638,6 +671,11 @@ def main():
     from slemm_amd import gpu
 
     lib = L.load()
+    if args.mfma_sweep:
+        torch.cuda.init()
+        for r in mfma_ceiling(lib, L, [(i, c, w) for i in range(4) for c in (1, 2, 4, 8) for w in (1, 2, 4)]):
+            print(json.dumps(r), flush=True)
+        return
     n, m, nv = args.n, args.m, args.probes
     K = nv + 1
     torch.cuda.set_device(0)
@@ -682,6 +720,10 @@ def main():
     k1, _ = kernel_ms(lib, 0, L.PREC_BF16X2)
     k2, _ = kernel_ms(lib, 1, L.PREC_BF16X2)
     lib.slemm_gpu_timing_reset()
+    if args.only_op:
+        print(json.dumps({"workload": f"operator K={K}, n={n}, m={m}", "ms_per_step": round(dt * 1e3, 3),
+                          "pass1_ms": round(k1, 3), "pass2_ms": round(k2, 3)}), flush=True)
+        return
 
     # the same operator on the BF16 MFMA kernels (split-BF16 operand, FP32 accumulation)
     bf = None
@@ -735,6 +777,7 @@ def main():
     smallk = None if args.no_smallk else small_k_passes(g, lib, L, n, m)
 
     ceiling = None if args.no_ceiling else gemm_ceiling()
+    measured_ceiling = mfma_ceiling_summary(lib, L)
     pb = pass_bytes(n, m, K)
     value = 2 * pb / dt / 1e9
     flops = 2.0 * n * m * K
@@ -784,7 +827,10 @@ def main():
             "hbm_GB/s_of_kernel": round(pb / (dom_ms * 1e-3) / 1e9, 1),
             "mfma_issue_factor": round(i8_issue_factor(K), 3),
             "issued_TOP/s": round(ach_tf * i8_issue_factor(K), 1),
-            "practical_mfma_ceiling": ceiling,
+            "issued_frac_of_measured_ceiling": round(ach_tf * i8_issue_factor(K) /
+                                                     measured_ceiling["i8_16x16x64"]["TOP/s"], 3),
+            "measured_mfma_ceiling": measured_ceiling,
+            "vendor_gemm": ceiling,
         } if i8_on(K) else {
             "bound": "mfma",
             "kernel": f"split-BF16 block product ({(tune or {}).get('pass2' if k2 >= k1 else 'pass1', ['pipe'])[0]} "
@@ -798,7 +844,10 @@ def main():
             "hbm_GB/s_of_kernel": round(pb / (dom_ms * 1e-3) / 1e9, 1),
             "mfma_issue_factor": round(issue_factor(K), 3),
             "issued_TFLOP/s": round(ach_tf * issue_factor(K), 1),
-            "practical_mfma_ceiling": ceiling,
+            "issued_frac_of_measured_ceiling": round(ach_tf * issue_factor(K) /
+                                                     measured_ceiling["bf16_16x16x32"]["TOP/s"], 3),
+            "measured_mfma_ceiling": measured_ceiling,
+            "vendor_gemm": ceiling,
         }),
     }
     if not args.no_cpu:

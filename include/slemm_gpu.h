@@ -410,6 +410,15 @@ int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, int* target
  * prints wall-clock stamps, src/option.cpp:21-34).  When enabled, every block-product launch
  * is bracketed by HIP events on its own stream.  pass: 0 = Z'V pass, 1 = Z U pass. */
 int slemm_gpu_timing(int enable);
+/* Matrix-core issue ceiling (no reference counterpart): a register-only loop of `chains`
+ * independent accumulators per wave (1, 2, 4 or 8), `waves_per_simd` waves on every SIMD, `iters`
+ * MFMAs per chain, of instruction instr (0 v_mfma_i32_16x16x64_i8, 1 v_mfma_i32_32x32x32_i8,
+ * 2 v_mfma_f32_16x16x32_bf16, 3 v_mfma_f32_32x32x16_bf16) on the current device.  Host outputs:
+ * tops (tera-operations per second, 2 per multiply-add), clock_ratio (shader-clock ticks per
+ * 100 MHz real-time tick over the loop of one wave: the clock in units of 100 MHz; may be NULL), ms
+ * (kernel time; may be NULL). */
+int slemm_gpu_mfma_ceiling(int instr, int chains, int waves_per_simd, int iters, double* tops,
+                           double* clock_ratio, double* ms);
 int slemm_gpu_timing_read(int pass, int precision, double* total_ms, int* launches);
 int slemm_gpu_timing_reset(void);
 

@@ -28,10 +28,16 @@ COMMON = [
 ]
 
 
+# per-file device flags: the MFMA-ceiling probe keeps its accumulators in VGPRs, as the block-product
+# kernels do by themselves (with AGPR accumulators the register allocator rotated the probe's
+# independent chains through accvgpr copies inside its loop)
+EXTRA = {"probe.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src: Path) -> tuple[Path, str]:
     obj = OBJ / (src.name + ".o")
     if src.suffix == ".hip":
-        cmd = [HIPCC, *COMMON, "-c", str(src), "-o", str(obj)]
+        cmd = [HIPCC, *COMMON, *EXTRA.get(src.name, []), "-c", str(src), "-o", str(obj)]
     else:
         # host-only translation units: C++ with the HIP runtime headers, no device pass
         cmd = [HIPCC, *[f for f in COMMON if not f.startswith("--offload-arch")], "-x", "c++",

@@ -69,6 +69,8 @@ SIGNATURES = {
                                       C.c_float, C.c_float, C.c_int, C.c_int, c_fp, c_fp, c_fp, c_fp,
                                       C.POINTER(RemlStats)]),
     "slemm_gpu_autotune": (C.c_int, [c_vp, C.c_int, C.c_int, c_intp, c_intp, c_dp]),
+    "slemm_gpu_mfma_ceiling": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "slemm_gpu_timing": (C.c_int, [C.c_int]),
     "slemm_gpu_timing_read": (C.c_int, [C.c_int, C.c_int, c_dp, c_intp]),
     "slemm_gpu_timing_reset": (C.c_int, []),
