@@ -284,6 +284,7 @@ struct I8WArgs {
   const int32_t* tile_nsuper = nullptr;
   const int64_t* tile_rows = nullptr;
   bool xcd_pairs = false;  // dispatch the column groups of a row block back to back on one XCD
+  int prio = 1;            // issue priority of a wave while it issues a step's MFMAs (0: none)
 };
 
 // Wide blocks: column group blockIdx.y holds vector tiles [NVG y, NVG y + NVG) (three digit tiles
@@ -398,6 +399,8 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
     // the read overlaps the previous tile's MFMAs (reads of dead tiles stay inside the buffer and
     // are not used)
     i32x4_t bcur = bp[0];
+    if (a.prio == 1) __builtin_amdgcn_s_setprio(1);
+    else if (a.prio == 2) __builtin_amdgcn_s_setprio(2);
 #pragma unroll
     for (int j = 0; j < NDT; ++j) {
       const i32x4_t bv = bcur;
@@ -409,6 +412,7 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
           acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[i], bv, acc[i][j], 0, 0, 0);
       }
     }
+    if (a.prio != 0) __builtin_amdgcn_s_setprio(0);
   };
   uint32_t cA[RT][8], cB[RT][8];
   if (s0 < s1) {
@@ -594,6 +598,11 @@ void launch_bp_i8(const I8Launch& l, hipStream_t st) {
     // not at all (those blocks are MFMA-bound), so the plain dispatch order is the default.
     const char* ex = std::getenv("SLEMM_I8_XCD");
     a.xcd_pairs = ex && ex[0] == '1';
+    // a wave issuing a step's MFMAs runs at issue priority 1 (SLEMM_I8_PRIO, read per call: 0 off):
+    // the MFMA stream then keeps ahead of the other wave's decode and LDS phase (K=65 operator step
+    // 20.4 -> 20.1 ms; priority 2 the same)
+    const char* ep = std::getenv("SLEMM_I8_PRIO");
+    a.prio = ep ? std::atoi(ep) : 1;
     // (one-vector-tile blocks measured slower with 2 row tiles per wave at four waves per SIMD:
     // each digit load then serves half as many MFMAs)
     const dim3 grid((unsigned)(l.rows_pad / (64 * kI8WRT)), (unsigned)i8_groups(l.t), (unsigned)l.nsplit);
