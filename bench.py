@@ -43,6 +43,54 @@ I8_MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 27.62e9 + 0.398e9, "pas
                        (100_000, 500_000, 1): {"pass1": 12.59e9 + 0.012e9, "pass2": 12.59e9 + 0.009e9}}
 
 
+def measured_traffic(n, m, K, kernel_substr, timeout=300):
+    """HBM bytes per launch of the dominant block-product kernel, measured live: two child runs
+    of this script's --only-op mode under `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`
+    (one counter per run), each scaled by what the same counter reads for a 1 GiB device copy in
+    the same run (--copy-calibration).  Returns {pass1, pass2, fetch_factor, write_factor} in bytes,
+    or {"error": ...} when the profiler is missing or fails (then the line reports traffic null)."""
+    import glob
+    import shutil
+    import sqlite3
+    import subprocess
+    import tempfile
+
+    if shutil.which("rocprofv3") is None:
+        return {"error": "rocprofv3 not found"}
+    env = dict(os.environ, TMPDIR="/tmp")
+    vals = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="slemm_traffic_", dir="/tmp")
+        cmd = ["rocprofv3", "--pmc", counter, "-d", d, "-o", "t", "--", sys.executable, os.path.abspath(__file__),
+               "--only-op", "--copy-calibration", "--steps", "1", "--warmup", "0", "--no-autotune",
+               "--n", str(n), "--m", str(m), "--probes", str(K - 1)]
+        try:
+            subprocess.run(cmd, cwd="/tmp", env=env, timeout=timeout, capture_output=True)
+            dbs = glob.glob(os.path.join(d, "**", "*.db"), recursive=True)
+            if not dbs:
+                return {"error": f"rocprofv3 --pmc {counter} wrote no results"}
+            con = sqlite3.connect(dbs[0])
+            rows = con.execute("select dispatch_id, kernel_name, sum(value) from counters_collection "
+                               "where counter_name = ? group by dispatch_id order by dispatch_id", (counter,)).fetchall()
+            con.close()
+        except Exception as e:  # reported, never fatal to the bench line
+            return {"error": f"{counter}: {e!r}"}
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        copies = [v for _, kn, v in rows if "copyBuffer" in kn]
+        kern = [v for _, kn, v in rows if kernel_substr in kn]
+        if not copies or len(kern) < 2:
+            return {"error": f"{counter}: copy or block-product dispatches missing"}
+        factor = float(1 << 30) / (max(copies) * 1024.0)  # the counter reads KiB
+        if not 0.25 < factor < 8.0:
+            return {"error": f"{counter}: implausible copy calibration factor {factor:.3f}"}
+        vals[counter] = (factor, [kern[0] * 1024.0 * factor, kern[1] * 1024.0 * factor])
+    (ff, fetch), (wf, write) = vals["FETCH_SIZE"], vals["WRITE_SIZE"]
+    return {"pass1": fetch[0] + write[0], "pass2": fetch[1] + write[1], "fetch_factor": round(ff, 4),
+            "write_factor": round(wf, 4), "fetch_bytes": [round(x) for x in fetch],
+            "write_bytes": [round(x) for x in write]}
+
+
 def issue_factor(K: int) -> float:
     """MFMA columns issued per algorithmic column of a K-vector split-BF16 block: a hi and a lo
     tile per 16 vectors, except that a last tile of at most 8 vectors holds its hi and lo halves
@@ -100,40 +148,130 @@ def kernel_ms(lib, pas, prec):
     return tot.value / max(cnt.value, 1), cnt.value
 
 
-def cpu_baseline(g, n, m, K, m_sample, reps):
-    """FP32 CPU restatement of the operator (oracle, OpenMP on all host cores) on the first
-    m_sample SNPs of the same bed, scaled linearly in m (the operator's cost is linear in m)."""
-    import numpy as np
-    import oracle
-    from oracle import bedio
+def lmc_timing(g, X, y, h2_sldf):
+    """LMC_REML (src/reml.cpp:730-882, the estimator `slemm --reml` runs by default) on the same
+    fit: n_MC as src/option.cpp:414, the reference's secant schedule, both precisions; wall time
+    from the resident bed, Lanczos steps, secant evaluations and h2 beside SLDF's."""
+    import torch
+    out = {}
+    for prec in ("bf16x2", "fp32"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = g.LMC_REML(X, y, precision=prec)
+        torch.cuda.synchronize()
+        out[prec] = {"seconds": round(time.perf_counter() - t0, 3), "n_MC": r["n_MC"], "K": 1 + 2 * r["n_MC"],
+                     "lanczos_steps": int(r["steps"]), "secant_evals": int(r["evals"]),
+                     "h2": round(float(r["h2"]), 6), "vg": float(r["vg"]), "ve": float(r["ve"])}
+    out["h2_sldf_same_phenotype"] = round(h2_sldf, 6)
+    out["speedup_bf16x2_over_fp32"] = round(out["fp32"]["seconds"] / out["bf16x2"]["seconds"], 2)
+    return out
 
-    oracle.build()
+
+def _panel_engine(rows, n):
+    from oracle.cpu_panels import PanelModel
+    return lambda gm, gw, rr, Q, ss: PanelModel(rows, n, gm, gw, rr, Q, ss)
+
+
+def cpu_op_seconds(rows, n, gm, gw, ts, reps=1):
+    """Per-call seconds of the reference-structured CPU operator (oracle.cpu_panels: per-subset
+    float panels + two MKL sgemms) over the given SNP rows, for each column count in ts."""
+    import numpy as np
+    from oracle.cpu_panels import PanelModel
+    eng = PanelModel(rows, n, gm, gw, np.ones(n, np.float32), np.full((n, 1), 1 / np.sqrt(n), np.float32))
+    rng = np.random.default_rng(0)
+    out = {}
+    for t in ts:
+        V = rng.choice([-1.0, 1.0], size=(n, t)).astype(np.float32) / np.float32(np.sqrt(n))
+        eng.op("ldet", 0.4286, V)  # page-in, thread pool
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            eng.op("ldet", 0.4286, V)
+        out[t] = (time.perf_counter() - t0) / reps
+    return out
+
+
+def c1_run(n=5000, m=50_000, nv=30, n_causal=1000, h2=0.5, cpu=True, seed=3):
+    """The reference's CPU-runnable workload: n=5,000, m=50,000, MAF~U(0.05,0.5), 1% missing, one
+    trait at h2=0.5 on 1,000 causal SNPs, 30 Rademacher probes.  SLDF-REML and LMC-REML on the GPU
+    (both precisions) and, as the reported CPU baseline, the same drivers on host cores over the
+    reference-structured panel engine (per-subset float panels, two MKL sgemms each), end to end."""
+    import numpy as np
+    import torch
+    from slemm_amd import gpu
+    from oracle import reml as oreml
+
+    g = gpu.Geno.synthesize(n, m, seed=seed, maf_lo=0.05, maf_hi=0.5, missing=0.01)
+    g.set_model()
+    y, _, _, rh2 = simulate_phenotype(g, n, m, n_causal, h2, seed + 1)
+    X = np.ones((n, 1), np.float32)
+    out = {"workload": f"n={n} m={m}, 1% missing, h2={h2} on {n_causal} causal SNPs, {nv} probes",
+           "realised_h2": round(rh2, 5)}
+    for prec in ("bf16x2", "fp32"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = g.SLDF_REML(X, y, rng_seed=0, n_V=nv, precision=prec)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        q = g.LMC_REML(X, y, precision=prec)
+        torch.cuda.synchronize()
+        out[f"gpu_{prec}"] = {"sldf_s": round(t1 - t0, 3), "sldf_h2": round(float(r["h2"]), 6),
+                              "lmc_s": round(time.perf_counter() - t1, 3), "lmc_h2": round(float(q["h2"]), 6)}
+    if cpu:
+        rows = g.download(0, m)
+        gm, gw, _ = g.model()
+        eng = _panel_engine(rows, n)
+        t0 = time.perf_counter()
+        a = oreml.SLDF_REML(None, gm, gw, None, X, y, n_V=nv, engine=eng)
+        t1 = time.perf_counter()
+        b = oreml.LMC_REML(None, gm, gw, None, X, y, engine=eng)
+        t2 = time.perf_counter()
+        out["cpu_reference_structure"] = {
+            "sldf_s": round(t1 - t0, 2), "sldf_h2": round(float(a["h2"]), 6), "lmc_s": round(t2 - t1, 2),
+            "lmc_h2": round(float(b["h2"]), 6), "threads": torch.get_num_threads(),
+            "engine": "FP32 per-subset float panels (1000 SNPs) + two MKL sgemms per subset (torch CPU), "
+                      "restated L_Seed/L_Solve/GSS/secant (oracle.cpu_panels, oracle.reml)"}
+        gpu_s = out["gpu_bf16x2"]
+        out["speedup_gpu_over_cpu"] = {"sldf": round((t1 - t0) / gpu_s["sldf_s"], 1),
+                                       "lmc": round((t2 - t1) / gpu_s["lmc_s"], 1)}
+    g.close()
+    return out
+
+
+def cpu_baseline(g, n, m, K, m_sample, reps, reml=None):
+    """The reference-structured CPU operator (oracle.cpu_panels: per call, 1000-SNP float panels of
+    centred dosages and two MKL sgemms per panel, src/reml.cpp:28-94) on the first m_sample SNPs of
+    the same bed over all n individuals, scaled linearly to m (the operator's cost is linear in m).
+    With the GPU fit's Lanczos step counts (reml), also the SLDF seed systems' CPU seconds
+    extrapolated as per-call time x steps."""
+    import torch
+
     rows = g.download(0, m_sample)
-    codes = bedio.decode_codes(rows, n)
-    kmat = bedio.kmat_of_codes(codes)
-    del codes
-    pc, n0, n1, n2, gm = oracle.counts(kmat)
-    keep = (pc > 0) & (pc < 2 * n)
-    gm = np.where(keep, gm, 0.5).astype(np.float32)
-    gw = oracle.gw(gm)
-    mdl = oracle.Model(kmat, gm, gw, np.ones(n, np.float32), np.full((n, 1), 1 / np.sqrt(n), np.float32))
-    V = oracle.rademacher(0, n, K)
-    mdl.op("ldet", 0.4286, V)  # warm-up (page-in, thread pool)
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        mdl.op("ldet", 0.4286, V)
-    dt = (time.perf_counter() - t0) / reps
-    t_full = dt * m / m_sample
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {
+    gm, gw, _ = g.model()
+    ts = sorted({K, 1, K - 1})
+    per = cpu_op_seconds(rows, n, gm[:m_sample], gw[:m_sample], ts, reps)
+    scale = m / m_sample
+    t_full = per[K] * scale
+    res = {
         "value": round(2 * pass_bytes(n, m, K) / t_full / 1e9, 3),
         "unit": "GB/s",
-        "cores": cores,
+        "cores": torch.get_num_threads(),
         "kind": "port",
-        "sample": (f"FP32 ldet operator (per-subset float panel + two products, src/reml.cpp:52-94) "
-                   f"on the first {m_sample} SNPs of the same bed, n={n}, K={K}, {reps} calls of "
-                   f"{dt:.3f} s, scaled linearly to m={m} ({t_full:.1f} s per call)"),
+        "sample": (f"FP32 ldet operator with the reference's structure (per call: 1000-SNP float panels of "
+                   f"centred dosages, two MKL sgemms each, src/reml.cpp:28-94; torch CPU) on the first "
+                   f"{m_sample} SNPs of the same bed, n={n}, K={K}, {reps} calls of {per[K]:.3f} s, scaled "
+                   f"linearly to m={m} ({t_full:.1f} s per call)"),
+        "per_call_s_extrapolated": {str(t): round(v * scale, 2) for t, v in per.items()},
     }
+    if reml is not None:
+        st = reml["bf16x2"]["lanczos_steps"]  # y, Q, probes of the GPU fit
+        ext = (st[0] * per[1] + st[1] * per[1] + st[2] * per[K - 1]) * scale
+        res["sldf_seed_s_extrapolated"] = {
+            "seconds": round(ext, 1),
+            "how": (f"per-call CPU time x the GPU fit's Lanczos steps (y {st[0]}, Q {st[1]}, probes {st[2]}), "
+                    f"three separate seed systems as the reference runs them; extrapolated, not run"),
+            "gpu_seconds_whole_fit": reml["bf16x2"]["seconds"],
+            "ratio": round(ext / reml["bf16x2"]["seconds"], 1)}
+    return res
 
 
 def small_k_passes(g, lib, L, n, m, ks=(1, 3, 5, 7, 16), reps=3):
@@ -324,6 +462,7 @@ def reml_time_to_converge(g, n, m, nv, y, realised_h2, n_causal, h2):
                "criterion_evals": int(r["evals"])} for prec, (sec, r) in c32.items()}
     out["criterion_reference_replay"]["vc_rel_diff_bf16x2_vs_fp32"] = {
         k: float(abs(c32["bf16x2"][1][k] - c32["fp32"][1][k]) / abs(c32["fp32"][1][k])) for k in ("vg", "ve", "h2")}
+    out["lmc"] = lmc_timing(g, X, y, float(res["bf16x2"]["h2"]))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     r = g.AI_REML(X, y, rng_seed=0, n_V=nv, precision="bf16x2")
@@ -489,7 +628,7 @@ def differentiation_run(g, n, m, y, vg, ve, nv=64, win_size=100):
             "euler_check_w_dot_grad": float(w.grad @ w0)}
 
 
-def scale_run(n, m, nv, n_causal, h2, seed=5):
+def scale_run(n, m, nv, n_causal, h2, seed=5, cpu=True):
     """Whole-pipeline run at biobank shape: synthetic bed with allele frequencies drawn from
     Beta(0.8, 0.8) and kept where MAF >= 0.01, phenotype y = Zc beta + e with beta ~ N(0,1) on
     n_causal SNPs and var(e) set for h2; then SLDF-REML with nv probes (src/reml.cpp:549-690),
@@ -523,6 +662,16 @@ def scale_run(n, m, nv, n_causal, h2, seed=5):
                    "lanczos_steps": list(r["steps"]), "criterion_evals": r["evals"],
                    "h2": round(float(r["h2"]), 5), "vg": float(r["vg"]), "ve": float(r["ve"])}
 
+    out["lmc"] = lmc_timing(g, X, y, float(r["h2"]))
+    if cpu:  # reference-structured CPU operator on one 1000-SNP panel, extrapolated per call x steps
+        rows = g.download(0, 1000)
+        gm, gw, _ = g.model()
+        per = cpu_op_seconds(rows, n, gm[:1000], gw[:1000], sorted({1, nv}), reps=1)
+        st = out["reml"]["lanczos_steps"]
+        out["cpu_sldf_seed_s_extrapolated"] = {
+            "seconds": round((st[0] * per[1] + st[1] * per[1] + st[2] * per[nv]) * m / 1000, 1),
+            "per_call_s": {str(t): round(v * m / 1000, 2) for t, v in per.items()},
+            "how": "one 1000-SNP panel timed on the host cores, x m/1000, x the GPU fit's Lanczos steps"}
     alpha = torch.from_numpy(r["snp_blup"].astype(np.float64)).cuda()
     g.gebv(alpha)
     torch.cuda.synchronize()
@@ -652,6 +801,10 @@ def main():
     ap.add_argument("--traits-T", type=int, default=32)
     ap.add_argument("--scale-n", type=int, default=1_000_000)
     ap.add_argument("--scale-m", type=int, default=60_000)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two child rocprofv3 --pmc runs that measure the dominant kernel's HBM bytes")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the n=5k m=50k SLDF/LMC run (GPU, and CPU end to end unless --no-cpu)")
     ap.add_argument("--mfma-sweep", action="store_true",
                     help="print the register-only MFMA issue rates (4 instructions x 1-8 chains x 1-4 waves per SIMD) and exit")
     ap.add_argument("--only-op", action="store_true",
@@ -698,9 +851,10 @@ def main():
     def step(prec):
         return g.lnr_op_select("ldet", tau0, V, precision=prec, col_kind=kinds)
 
-    if args.copy_calibration:
+    if args.copy_calibration:  # 1 GiB device-to-device copy: the byte counters' reference kernel
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
-        dst = src.clone()
+        dst = torch.empty_like(src)
+        dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
     tune = None if args.no_autotune or i8_on(K) else g.autotune(K, reps=2)  # tunes the BF16 kernels
@@ -778,6 +932,9 @@ def main():
 
     ceiling = None if args.no_ceiling else gemm_ceiling()
     measured_ceiling = mfma_ceiling_summary(lib, L)
+    traffic = None if args.no_traffic else measured_traffic(n, m, K, "k_bp_i8w" if i8_on(K) else "k_bp_")
+    dom_pass = "pass2" if k2 >= k1 else "pass1"
+    traffic_bytes = (traffic or {}).get(dom_pass)
     pb = pass_bytes(n, m, K)
     value = 2 * pb / dt / 1e9
     flops = 2.0 * n * m * K
@@ -822,7 +979,8 @@ def main():
             "peak_kind": "dense int8 MFMA, 5.0 POP/s; achieved counts the product's 2 n m K operations",
             "unit": "TFLOP/s",
             "frac": round(ach_tf / INT8_PEAK_TOPS, 4),
-            "traffic": I8_MEASURED_TRAFFIC.get((n, m, K), {}).get("pass2" if k2 >= k1 else "pass1"),
+            "traffic": None if traffic_bytes is None else round(traffic_bytes),
+            "traffic_measured": traffic,
             "algorithmic_per_launch": {"flop": flops, "bytes": pb},
             "hbm_GB/s_of_kernel": round(pb / (dom_ms * 1e-3) / 1e9, 1),
             "mfma_issue_factor": round(i8_issue_factor(K), 3),
@@ -839,7 +997,8 @@ def main():
             "peak": BF16_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(ach_tf / BF16_PEAK_TFLOPS, 4),
-            "traffic": (MEASURED_TRAFFIC.get((n, m, K), {}).get("pass2" if k2 >= k1 else "pass1")),
+            "traffic": None if traffic_bytes is None else round(traffic_bytes),
+            "traffic_measured": traffic,
             "algorithmic_per_launch": {"flop": flops, "bytes": pb},
             "hbm_GB/s_of_kernel": round(pb / (dom_ms * 1e-3) / 1e9, 1),
             "mfma_issue_factor": round(issue_factor(K), 3),
@@ -852,18 +1011,24 @@ def main():
     }
     if not args.no_cpu:
         try:
-            result["cpu_baseline"] = cpu_baseline(g, n, m, K, args.cpu_snps, reps=2)
+            result["cpu_baseline"] = cpu_baseline(g, n, m, K, args.cpu_snps, reps=2, reml=reml)
         except Exception as e:  # reported, never fatal
             result["cpu_baseline"] = {"value": None, "error": repr(e)}
     else:
         result["cpu_baseline"] = None
+    if not args.no_c1:
+        print("# C1: n=5000 m=50000 SLDF/LMC, GPU and CPU", file=sys.stderr, flush=True)
+        try:
+            result["c1"] = c1_run(cpu=not args.no_cpu)
+        except Exception as e:  # reported, never fatal to the bench line
+            result["c1"] = {"error": repr(e)}
     if not args.no_scale:
         g.close()
         del V
         torch.cuda.empty_cache()
         print(f"# scale pipeline n={args.scale_n} m={args.scale_m}", file=sys.stderr, flush=True)
         try:
-            result["scale_pipeline"] = scale_run(args.scale_n, args.scale_m, 100, 2000, 0.4)
+            result["scale_pipeline"] = scale_run(args.scale_n, args.scale_m, 100, 2000, 0.4, cpu=not args.no_cpu)
         except Exception as e:  # reported, never fatal to the bench line
             result["scale_pipeline"] = {"error": repr(e)}
             torch.cuda.empty_cache()

@@ -43,8 +43,16 @@ def slq_spectrum(seed: "oracle.Seed", k: int):
     return D, Wv[0, :]
 
 
+def _engine(kmat, engine):
+    """The operator engine of the drivers: oracle.Model over the kmat bits (the C++ restatement),
+    or engine(gm, gw, rr, Q, subset_size) -- e.g. the MKL panel engine of oracle.cpu_panels."""
+    if engine is not None:
+        return engine
+    return lambda gm, gw, rr, Q, ss: oracle.Model(kmat, gm, gw, rr, Q, ss)
+
+
 def SLDF_REML(kmat, gm, gw, rvec, X, y, rng_seed=0, n_V=30, tol_L=5e-4, s2max=0.7, s2min=1e-4,
-              tol_VC=1e-4, subset_size=1000):
+              tol_VC=1e-4, subset_size=1000, engine=None):
     """Returns dict(vg, ve, lrt, h2, Py, snp_blup, blue, var_blue, steps, evals)."""
     n = len(y)
     rvec = np.ones(n, np.float32) if rvec is None else np.asarray(rvec, np.float32)
@@ -56,7 +64,7 @@ def SLDF_REML(kmat, gm, gw, rvec, X, y, rng_seed=0, n_V=30, tol_L=5e-4, s2max=0.
     yp = (rr * np.asarray(y, np.float32)).astype(np.float32)
     yp = (yp - Q @ (Q.T @ yp)).astype(np.float32)
     V = oracle.rademacher(rng_seed, n, n_V)
-    mdl = oracle.Model(kmat, gm, gw, rr, Q, subset_size)
+    mdl = _engine(kmat, engine)(gm, gw, rr, Q, subset_size)
     tau0 = f32((1 - f32(s2max)) / f32(s2max))
     seed_y = mdl.L_Seed("quadform", float(tau0), yp, tol_L)
     seed_Q = mdl.L_Seed("ldet", float(tau0), Q, tol_L)
@@ -119,7 +127,8 @@ def SLDF_REML(kmat, gm, gw, rvec, X, y, rng_seed=0, n_V=30, tol_L=5e-4, s2max=0.
                 steps=(seed_y.steps, seed_Q.steps, seed_V.steps), evals=evals[0])
 
 
-def LMC_REML(kmat, gm, gw, rvec, X, y, rng_seed=0, n_MC=None, tol_L=5e-4, s2max=0.7, subset_size=1000):
+def LMC_REML(kmat, gm, gw, rvec, X, y, rng_seed=0, n_MC=None, tol_L=5e-4, s2max=0.7, subset_size=1000,
+             engine=None):
     """Returns dict(vg, ve, h2, Py, snp_blup, blue, steps, evals)."""
     n = len(y)
     m = len(gm)
@@ -132,7 +141,7 @@ def LMC_REML(kmat, gm, gw, rvec, X, y, rng_seed=0, n_MC=None, tol_L=5e-4, s2max=
     c = X.shape[1]
     Q, _ = covariate_qr(X, rr)
     gw = np.asarray(gw, np.float32)
-    mdl = oracle.Model(kmat, gm, gw, rr, Q, subset_size)
+    mdl = _engine(kmat, engine)(gm, gw, rr, Q, subset_size)
     u, e = oracle.lmc_samples(rng_seed, n, m, n_MC)
     u = (u * np.sqrt(gw)[:, None]).astype(np.float32)
     V = np.zeros((n, 1 + 2 * n_MC), np.float32)
