@@ -43,7 +43,7 @@ I8_MEASURED_TRAFFIC = {(100_000, 500_000, 65): {"pass1": 27.62e9 + 0.398e9, "pas
                        (100_000, 500_000, 1): {"pass1": 12.59e9 + 0.012e9, "pass2": 12.59e9 + 0.009e9}}
 
 
-def measured_traffic(n, m, K, kernel_substr, timeout=300):
+def measured_traffic(n, m, K, kernel_substr, timeout=300, env_extra=None):
     """HBM bytes per launch of the dominant block-product kernel, measured live: two child runs
     of this script's --only-op mode under `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`
     (one counter per run), each scaled by what the same counter reads for a 1 GiB device copy in
@@ -57,7 +57,7 @@ def measured_traffic(n, m, K, kernel_substr, timeout=300):
 
     if shutil.which("rocprofv3") is None:
         return {"error": "rocprofv3 not found"}
-    env = dict(os.environ, TMPDIR="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp", **(env_extra or {}))
     vals = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="slemm_traffic_", dir="/tmp")
@@ -130,12 +130,19 @@ class bf16_mfma_kernels:
         return False
 
 
-def i8_on(K: int) -> bool:
-    """Whether a K-vector block runs on the int8 kernels (SLEMM_I8_NARROW / SLEMM_I8_WIDE); blocks
-    of 6 to 8 vectors stay on the BF16 kernels with hi and lo halves as columns."""
-    if 6 <= K <= 8 and os.environ.get("SLEMM_HL_COLUMNS", "1") != "0":
-        return False
-    return os.environ.get("SLEMM_I8_NARROW" if K <= 5 else "SLEMM_I8_WIDE", "1") != "0"
+def i8_on(g, K: int) -> bool:
+    """Whether a K-vector block runs on the int8 kernels, as the library selects it now (blocks of
+    6 to 8 vectors stay on the BF16 kernels with hi and lo halves as columns; SLEMM_I8_NARROW /
+    SLEMM_I8_WIDE switch the others)."""
+    return g.block_kernel(K).startswith("int8")
+
+
+def i8_target_env(tune) -> dict:
+    """SLEMM_I8_TARGET / SLEMM_I8_RT that reproduce an int8 autotune result in a child process."""
+    if not tune or not tune.get("kernel", "").startswith("int8"):
+        return {}
+    env = {"SLEMM_I8_TARGET": f"{tune['pass1'][1]},{tune['pass2'][1]}"}
+    return env
 
 
 def pass_bytes(n, m, K):
@@ -283,7 +290,7 @@ def small_k_passes(g, lib, L, n, m, ks=(1, 3, 5, 7, 16), reps=3):
     for K in ks:
         V = torch.randn(n, K, device="cuda")
         U = torch.randn(m, K, device="cuda")
-        at = None if i8_on(K) else g.autotune(K, reps=2)
+        at = g.autotune(K, reps=2)
         g.rmatvec(V)
         g.matvec(U)
         torch.cuda.synchronize()
@@ -302,9 +309,8 @@ def small_k_passes(g, lib, L, n, m, ks=(1, 3, 5, 7, 16), reps=3):
                          "pass1_GB/s": round(b / p1 / 1e6, 1), "pass2_GB/s": round(b / p2 / 1e6, 1),
                          "pass1_hbm_frac": round(b / p1 / 1e6 / HBM_PEAK_GBS, 3),
                          "pass2_hbm_frac": round(b / p2 / 1e6 / HBM_PEAK_GBS, 3),
-                         "kernels": ([["int8 narrow" if K <= 5 else "int8 wide", "default"]] * 2 if i8_on(K)
-                                     else [at["pass1"][:2], at["pass2"][:2]])}
-        trf = (I8_MEASURED_TRAFFIC if i8_on(K) else MEASURED_TRAFFIC).get((n, m, K))
+                         "kernels": [at["pass1"][:2], at["pass2"][:2]]}
+        trf = (I8_MEASURED_TRAFFIC if i8_on(g, K) else MEASURED_TRAFFIC).get((n, m, K))
         if trf:
             out[f"K={K}"]["traffic_bytes"] = trf
     alpha = torch.randn(m, dtype=torch.float64, device="cuda")
@@ -653,6 +659,8 @@ def scale_run(n, m, nv, n_causal, h2, seed=5, cpu=True):
     y, beta, gv, rh2 = simulate_phenotype(g, n, m, n_causal, h2, seed + 1)
     X = np.ones((n, 1), np.float32)
     out["realised_h2"] = round(rh2, 5)
+    at = g.autotune(nv + 1, reps=2)  # the SLDF blocks: nv probes + 1 RHS
+    out["autotune"] = {"K": nv + 1, "kernel": at["kernel"], "pass1": at["pass1"], "pass2": at["pass2"]}
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -701,14 +709,15 @@ def scale_run(n, m, nv, n_causal, h2, seed=5, cpu=True):
     return out
 
 
-def traits_run(n, m, T, nv, seed=9, fp32=True):
+def traits_run(n, m, T, nv, seed=9, fp32=True, random_subsets=False):
     """Batching: T traits on individual subsets of one shared bed (subset sizes log-uniform in
     [n/20, n], h2 ~ U(0.1, 0.7), 1,000 causal SNPs each), SLDF-REML with nv probes + 1 RHS + the
     intercept per trait.  Batched: all T*(nv+2) columns in one block product per pass, one HIP
     graph per Lanczos step.  Baselines: the same driver one trait at a time on per-subset handles,
     split-BF16 and plain FP32.  Subsets are the first n_t individuals (individuals are
     exchangeable in the synthetic bed), which lets the batched products skip the work past each
-    column group's last member."""
+    column group's last member; random_subsets draws each trait's individuals at random instead
+    (every column group then spans nearly all individuals)."""
     import numpy as np
     import torch
     from slemm_amd import gpu
@@ -724,23 +733,43 @@ def traits_run(n, m, T, nv, seed=9, fp32=True):
     gv = g.matvec(torch.from_numpy(U).cuda(), precision="fp32").double().cpu().numpy()
     Y = np.zeros((T, n), np.float32)
     masks = np.zeros((T, n), bool)
+    members = []
     for t in range(T):
         s = int(sizes[t])
-        masks[t, :s] = True
-        gt = gv[:s, t] - gv[:s, t].mean()
+        idx = np.sort(rng.choice(n, size=s, replace=False)) if random_subsets else np.arange(s)
+        members.append(idx)
+        masks[t, idx] = True
+        gt = gv[idx, t] - gv[idx, t].mean()
         ve = gt.var() * (1 - h2s[t]) / h2s[t]
-        Y[t, :s] = (gt + rng.normal(scale=np.sqrt(ve), size=s) + 10.0).astype(np.float32)
-    out = {"workload": (f"n={n} m={m}, {T} traits on subsets of {int(sizes.min())}..{int(sizes.max())} "
-                        f"individuals (sum {int(sizes.sum())}), SLDF-REML with {nv} probes + RHS + intercept each"),
+        Y[t, idx] = (gt + rng.normal(scale=np.sqrt(ve), size=s) + 10.0).astype(np.float32)
+    out = {"workload": (f"n={n} m={m}, {T} traits on {'random' if random_subsets else 'leading'} subsets of "
+                        f"{int(sizes.min())}..{int(sizes.max())} individuals (sum {int(sizes.sum())}), "
+                        f"SLDF-REML with {nv} probes + RHS + intercept each"),
            "columns": int(T * (nv + 2))}
+    at = g.autotune(T * (nv + 2), reps=2)  # the batched block: every trait's columns
+    out["autotune_batched"] = {"K": T * (nv + 2), "kernel": at["kernel"], "pass1": at["pass1"], "pass2": at["pass2"]}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     tr = gpu.Traits(g, masks)
-    r = tr.SLDF_REML(Y, n_V=nv)
+    plan, ratio = tr.plan(nv)
+    ra = tr.SLDF_REML(Y, n_V=nv)  # the library's automatic plan
     torch.cuda.synchronize()
-    out["batched_bf16x2"] = {"seconds": round(time.perf_counter() - t0, 3),
-                             "seconds_seed": round(float(r["seconds_seed"]), 3),
-                             "lanczos_steps_max": int(r["steps"].max())}
+    out["traits_api_auto"] = {"plan": plan, "work_ratio_batched_over_one_at_a_time": round(ratio, 3),
+                              "seconds": round(time.perf_counter() - t0, 3),
+                              "seconds_seed": round(float(ra["seconds_seed"]), 3)}
+    if plan == "batched":
+        r = ra
+        out["batched_bf16x2"] = dict(out["traits_api_auto"])
+    else:
+        tr.set_plan("batched")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = tr.SLDF_REML(Y, n_V=nv)
+        torch.cuda.synchronize()
+        out["batched_bf16x2"] = {"seconds": round(time.perf_counter() - t0, 3),
+                                 "seconds_seed": round(float(r["seconds_seed"]), 3)}
+    out["batched_bf16x2"]["lanczos_steps_max"] = int(r["steps"].max())
+    out["h2_max_abs_diff_auto_vs_batched"] = float(np.max(np.abs(ra["h2"] - r["h2"])))
     tr.close()
     seq = {}
     for prec in (("bf16x2", "fp32") if fp32 else ("bf16x2",)):
@@ -748,11 +777,12 @@ def traits_run(n, m, T, nv, seed=9, fp32=True):
         tot = 0.0
         for t in range(T):
             s = int(sizes[t])
-            gk = g.keep(np.arange(s))
+            gk = g.keep(members[t])
             gk.set_model()
+            gk.autotune(nv + 1, reps=2)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            rt = gk.SLDF_REML(np.ones((s, 1), np.float32), Y[t, :s], n_V=nv, precision=prec)
+            rt = gk.SLDF_REML(np.ones((s, 1), np.float32), Y[t, members[t]], n_V=nv, precision=prec)
             torch.cuda.synchronize()
             tot += time.perf_counter() - t0
             h2[t] = rt["h2"]
@@ -761,6 +791,8 @@ def traits_run(n, m, T, nv, seed=9, fp32=True):
         out[f"sequential_{prec}"] = {"seconds": round(tot, 3)}
     out["speedup_batched_over_sequential_bf16x2"] = round(out["sequential_bf16x2"]["seconds"] /
                                                          out["batched_bf16x2"]["seconds"], 2)
+    out["speedup_auto_over_sequential_bf16x2"] = round(out["sequential_bf16x2"]["seconds"] /
+                                                      out["traits_api_auto"]["seconds"], 2)
     if fp32:
         out["speedup_batched_over_sequential_fp32"] = round(out["sequential_fp32"]["seconds"] /
                                                            out["batched_bf16x2"]["seconds"], 2)
@@ -857,7 +889,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    tune = None if args.no_autotune or i8_on(K) else g.autotune(K, reps=2)  # tunes the BF16 kernels
+    tune = None if args.no_autotune else g.autotune(K, reps=2)
+    i8 = i8_on(g, K)
 
     for _ in range(args.warmup):
         step("bf16x2")
@@ -881,7 +914,7 @@ def main():
 
     # the same operator on the BF16 MFMA kernels (split-BF16 operand, FP32 accumulation)
     bf = None
-    if i8_on(K):
+    if i8:
         with bf16_mfma_kernels():
             tune_bf = None if args.no_autotune else g.autotune(K, reps=2)
             for _ in range(max(args.warmup, 1)):
@@ -932,7 +965,8 @@ def main():
 
     ceiling = None if args.no_ceiling else gemm_ceiling()
     measured_ceiling = mfma_ceiling_summary(lib, L)
-    traffic = None if args.no_traffic else measured_traffic(n, m, K, "k_bp_i8w" if i8_on(K) else "k_bp_")
+    traffic = None if args.no_traffic else measured_traffic(n, m, K, "k_bp_i8w" if i8 else "k_bp_",
+                                                               env_extra=i8_target_env(tune))
     dom_pass = "pass2" if k2 >= k1 else "pass1"
     traffic_bytes = (traffic or {}).get(dom_pass)
     pb = pass_bytes(n, m, K)
@@ -954,7 +988,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": ("i8x3 (split-BF16 operand hi+lo rendered per vector as three int8 digits of a 23-bit fixed-point "
-                  "value, genotype dosages exact in int8, exact int32 accumulation)" if i8_on(K) else
+                  "value, genotype dosages exact in int8, exact int32 accumulation)" if i8 else
                   "bf16x2 (genotypes exact in BF16, vector block split hi+lo, FP32 accumulate)"),
         "data": "synthetic bed generated in HBM (MAF~U(0.05,0.5), HWE), Rademacher probes",
         "config": {"workload": f"Zc W Zc' operator on a K={K} block (64 probes + 1 RHS)",
@@ -989,7 +1023,7 @@ def main():
                                                      measured_ceiling["i8_16x16x64"]["TOP/s"], 3),
             "measured_mfma_ceiling": measured_ceiling,
             "vendor_gemm": ceiling,
-        } if i8_on(K) else {
+        } if i8 else {
             "bound": "mfma",
             "kernel": f"split-BF16 block product ({(tune or {}).get('pass2' if k2 >= k1 else 'pass1', ['pipe'])[0]} "
                       f"variant), {dom}",
@@ -1042,6 +1076,12 @@ def main():
             result["batched_traits"] = traits_run(args.traits_n, args.traits_m, args.traits_T, 32)
         except Exception as e:  # reported, never fatal to the bench line
             result["batched_traits"] = {"error": repr(e)}
+        torch.cuda.empty_cache()
+        try:
+            result["batched_traits_random_subsets"] = traits_run(args.traits_n, args.traits_m, args.traits_T, 32,
+                                                                 fp32=False, random_subsets=True)
+        except Exception as e:  # reported, never fatal to the bench line
+            result["batched_traits_random_subsets"] = {"error": repr(e)}
     print(json.dumps(result), flush=True)
 
 

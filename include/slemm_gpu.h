@@ -306,13 +306,28 @@ int slemm_gpu_traits_lseed(slemm_traits* tr, const int* col_trait, const int* co
                            float tau0, const float* B, int64_t ldb, int t, float tol, int maxit, int keep_cols,
                            int precision, int verbose, void* stream, slemm_lseed** out);
 
-/* SLDF_REML (src/reml.cpp:549-690) for every trait of the batch at once: one batched seed over
- * all traits' phenotype, intercept and probe columns, then the golden-section search per trait.
+/* Plans of slemm_gpu_traits_sldf_reml (no reference counterpart):
+ *  - BATCHED: one batched seed over all traits' phenotype, intercept and probe columns, then the
+ *    golden-section search per trait on replays of it;
+ *  - SEQUENTIAL: one trait at a time on a handle compacted to its individuals;
+ *  - AUTO (the default): BATCHED while the batched block products' work (each 16-column tile over
+ *    the rows up to the last member of its traits) is at most 1.25 times the one-at-a-time work
+ *    (each trait's n_V + 1 columns over its individuals), else SEQUENTIAL.  Nested subsets
+ *    (leading individuals) batch; random subsets of very different sizes go one at a time. */
+#define SLEMM_TRAITS_AUTO 0
+#define SLEMM_TRAITS_BATCHED 1
+#define SLEMM_TRAITS_SEQUENTIAL 2
+int slemm_gpu_traits_set_plan(slemm_traits* tr, int plan);
+/* The plan the next slemm_gpu_traits_sldf_reml with n_V probes runs, and the work ratio
+ * (batched over one at a time) the automatic choice reads; work_ratio may be NULL. */
+int slemm_gpu_traits_plan(const slemm_traits* tr, int n_V, int* plan, double* work_ratio);
+
+/* SLDF_REML (src/reml.cpp:549-690) for every trait of the batch, on the handle's plan (above).
  * y: T x n host (row t = trait t's phenotype; entries outside the subset are ignored).  Probes
  * of trait t are drawn from mt19937(rng_seed+54321) over its subset, as a single-trait run on
  * those individuals draws them.  Host outputs: vg, ve, h2, lrt (T each; lrt may be NULL),
  * steps (T x 3: Lanczos steps of y, intercept, probes; may be NULL), seconds[2] (seed, total;
- * may be NULL). */
+ * may be NULL; one at a time, the seed time is the sum over traits). */
 int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int rng_seed, int n_V, float tol_L,
                                float s2max, float s2min, float tol_VC, int precision, int verbose,
                                float* vg, float* ve, float* h2, float* lrt, int* steps, double* seconds);
@@ -391,16 +406,34 @@ int slemm_gpu_gwa_window(slemm_geno* h, const float* X, int64_t ldx, int c, cons
 
 /* ---------------------------------------------------------------- tuning ------------------- */
 
-/* Autotuner of the split-BF16 block products (no reference counterpart; the reference's only
- * blocking knob is --subset_size, src/option.cpp:75,297).  Times every kernel variant (B operand
- * from global memory with 4 or 8 row tiles per wave, with table-driven code unpacking, or
- * software-pipelined as an ordinary or a persistent grid or with a byte-table unpack from one
- * shared or a 16- or 32-fold replicated table; or LDS-staged in 4- or 2-step groups)
- * at three grid sizes for both passes of a t-column operator on this handle's shape, and keeps
- * the fastest per pass for later calls.  Blocks of t <= 8 vectors (hi and lo halves in one
- * 16-column tile) have only the pipelined variants; the others read -1 in ms.  Host outputs
- * (any may be NULL): variant[2], target[2] (workgroups the grid aims for),
- * ms[SLEMM_AUTOTUNE_MS] = best per pass, then 2 + pass*30 + variant*3 + grid-size index. */
+/* Kernel family that a t-vector operator block at this precision runs on (no reference
+ * counterpart), as selected from t and the SLEMM_I8_NARROW / SLEMM_I8_WIDE / SLEMM_HL_COLUMNS
+ * switches at the time of the call. */
+#define SLEMM_KERNEL_FP32 0      /* FP32 operand, FP32 kernels (precision SLEMM_PREC_FP32) */
+#define SLEMM_KERNEL_BF16X2 1    /* split-BF16 operand on BF16 MFMAs, FP32 accumulation */
+#define SLEMM_KERNEL_I8_NARROW 2 /* int8 digits, narrow kernel (1-5 vectors) */
+#define SLEMM_KERNEL_I8_TILE 3   /* int8 digits, wide kernel, one 16-vector tile without a tail */
+#define SLEMM_KERNEL_I8_WIDE 4   /* int8 digits, general wide kernel */
+int slemm_gpu_block_kernel(const slemm_geno* h, int t, int precision, int* kernel);
+
+/* Autotuner of the block products (no reference counterpart; the reference's only blocking knob
+ * is --subset_size, src/option.cpp:75,297), for a t-column operator on this handle's shape; the
+ * fastest setting per pass is kept on the handle for later calls.
+ *  - Blocks that run on the int8 kernels (1-5 and 9 or more vectors unless switched off): the
+ *    workgroup target of the reduction split (1024, 2048, 4096, 8192, 16384) and, for narrow
+ *    blocks (1-5 vectors), the row tiles per wave (4 or 8), each timed over full passes (digit
+ *    preparation + block product) on random FP32 operands; kept per pass and kernel class
+ *    (narrow, one vector tile, general wide).  Outputs: variant[2] = the narrow row tiles (0 for
+ *    wide blocks), target[2], ms[0..1] = best per pass, ms[2 + pass*30 + rt_index*5 + target_index]
+ *    the timings (rt_index 0: 4 row tiles, 1: 8).
+ *  - Split-BF16 blocks: every kernel variant (B operand from global memory with 4 or 8 row tiles
+ *    per wave, with table-driven code unpacking, or software-pipelined as an ordinary or a
+ *    persistent grid or with a byte-table unpack from one shared or a 16- or 32-fold replicated
+ *    table; or LDS-staged in 4- or 2-step groups) at three grid sizes (512, 1024, 2048) for both
+ *    passes.  Blocks of t <= 8 vectors (hi and lo halves in one 16-column tile) have only the
+ *    pipelined variants.  Outputs: variant[2], target[2], ms[0..1] = best per pass, then
+ *    ms[2 + pass*30 + variant*3 + grid-size index].
+ * Entries not timed read -1.  Host outputs, any may be NULL. */
 #define SLEMM_AUTOTUNE_MS 62
 int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, int* target, double* ms);
 

@@ -14,6 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libslemm_gpu.so"
 OK, ERR_ARG, ERR_HIP, ERR_ALLOC, ERR_STATE, ERR_RANK, ERR_DATA, ERR_NUMERIC = range(8)
 OP_LDET, OP_QUADFORM = 0, 1
 PREC_FP32, PREC_BF16X2 = 0, 1
+TRAITS_AUTO, TRAITS_BATCHED, TRAITS_SEQUENTIAL = 0, 1, 2
 
 c_i64 = C.c_int64
 c_i64p = C.POINTER(C.c_int64)
@@ -69,6 +70,7 @@ SIGNATURES = {
                                       C.c_float, C.c_float, C.c_int, C.c_int, c_fp, c_fp, c_fp, c_fp,
                                       C.POINTER(RemlStats)]),
     "slemm_gpu_autotune": (C.c_int, [c_vp, C.c_int, C.c_int, c_intp, c_intp, c_dp]),
+    "slemm_gpu_block_kernel": (C.c_int, [c_vp, C.c_int, C.c_int, c_intp]),
     "slemm_gpu_mfma_ceiling": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "slemm_gpu_timing": (C.c_int, [C.c_int]),
@@ -100,6 +102,8 @@ SIGNATURES = {
     "slemm_gpu_traits_model": (C.c_int, [c_vp, C.c_int, c_fp, c_fp, c_i32p]),
     "slemm_gpu_traits_op": (C.c_int, [c_vp, c_intp, c_intp, C.c_float, c_vp, c_i64, C.c_int, c_vp, c_i64, C.c_int,
                                       c_vp]),
+    "slemm_gpu_traits_set_plan": (C.c_int, [c_vp, C.c_int]),
+    "slemm_gpu_traits_plan": (C.c_int, [c_vp, C.c_int, c_intp, c_dp]),
     "slemm_gpu_traits_sldf_reml": (C.c_int, [c_vp, c_fp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                              C.c_float, C.c_int, C.c_int, c_fp, c_fp, c_fp, c_fp, c_intp, c_dp]),
     "slemm_gpu_traits_lseed": (C.c_int, [c_vp, c_intp, c_intp, c_intp, C.c_float, c_vp, c_i64, C.c_int, C.c_float,

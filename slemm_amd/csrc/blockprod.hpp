@@ -225,10 +225,15 @@ struct I8Launch {
   // optional: sum_out[k] = sum over rows of sum_w(row) x (represented operand)_k, from the digits
   double* sum_out = nullptr;
   I8SumWeights sum_w;
+  int rt = 0;  // narrow blocks: 16-row tiles per wave (2, 4 or 8; 0 = the default, kI8RT)
 };
 bool i8_narrow_enabled();
 size_t i8_scratch_bytes(int64_t red_pad, int t);
-int i8_splits(int t, int64_t rows_pad, int64_t nsuper, int64_t target_wgs, bool squared, int64_t& ns_per);
+// rt: narrow blocks' row tiles per wave as launched (0 = the default)
+int i8_splits(int t, int64_t rows_pad, int64_t nsuper, int64_t target_wgs, bool squared, int rt, int64_t& ns_per);
+// kernel class of a t-vector int8 block: 0 narrow (1-5 vectors), 1 one vector tile without a
+// tail (9-16), 2 the general wide kernel -- the autotuner keeps one setting per class and pass
+int i8_class(int t);
 void launch_bp_i8(const I8Launch& l, hipStream_t st);
 void launch_fin1(const Fin1Args& a, hipStream_t st);
 void launch_fin2(const Fin2Args& a, hipStream_t st);

@@ -493,15 +493,21 @@ bool i8_narrow_enabled() {
 
 // 16-row tiles per wave (2, 4 or 8; SLEMM_I8_RT, read per call).  Eight measured fastest at
 // n = 100,000 x m = 500,000 for 1 to 5 vectors: each digit load then serves 8 MFMAs.
-static int i8_rt() {
+static int i8_rt(int rt) {
   const char* e = std::getenv("SLEMM_I8_RT");
-  const int rt = e ? std::atoi(e) : kI8RT;
+  if (e) rt = std::atoi(e);
+  else if (rt <= 0) rt = kI8RT;
   return rt == 2 || rt == 8 ? rt : 4;
 }
 
-int i8_splits(int t, int64_t rows_pad, int64_t nsuper, int64_t target_wgs, bool squared, int64_t& ns_per) {
+int i8_class(int t) {
+  const I8Digits d = i8_digits(t);
+  return d.nvt == 0 ? 0 : (d.nvt == 1 && d.r8 == 0 ? 1 : 2);
+}
+
+int i8_splits(int t, int64_t rows_pad, int64_t nsuper, int64_t target_wgs, bool squared, int rt, int64_t& ns_per) {
   const bool narrow = i8_digits(t).nvt == 0;
-  const int64_t wgs = rows_pad / (64 * (narrow ? i8_rt() : kI8WRT)) * i8_groups(t);
+  const int64_t wgs = rows_pad / (64 * (narrow ? i8_rt(rt) : kI8WRT)) * i8_groups(t);
   int64_t S = (target_wgs + wgs - 1) / wgs;
   // every split's int32 sums stay exact: |digit| <= 128 times dosages <= 2 (<= 4 squared) over at
   // most 2^23 (2^22) rows, at most 2^31 in magnitude
@@ -575,7 +581,7 @@ void launch_bp_i8(const I8Launch& l, hipStream_t st) {
     I8Args a;
     a.G = l.G; a.ldg = l.ldg; a.nsuper = l.red_pad / 512; a.ns_per_split = l.ns_per_split; a.B = B; a.t = l.t;
     a.sc = sc; a.part = l.part; a.KP = l.KP; a.rows_pad = l.rows_pad;
-    const int rt = i8_rt();
+    const int rt = i8_rt(l.rt);
     const dim3 grid((unsigned)(l.rows_pad / (64 * rt)), 1, (unsigned)l.nsplit);
     if (rt == 2) {
       if (l.squared) k_bp_i8<2, true><<<grid, 256, 0, st>>>(a);

@@ -83,6 +83,7 @@ struct slemm_geno {
   int device = 0;
   int64_t n = 0, m = 0;
   int64_t n_pad = 0, m_pad = 0;
+  int64_t cap_n_pad = 0;  // individuals (padded) the allocation holds: n_pad <= cap_n_pad
   uint8_t* gs = nullptr;
   uint8_t* gi = nullptr;
   int64_t ld_gs = 0, ld_gi = 0;  // bytes per row
@@ -117,6 +118,10 @@ struct slemm_geno {
   int bp_variant[2][9] = {{9, 9, 9, 9, 9, 9, 5, 5, 5}, {9, 9, 9, 9, 9, 9, 5, 5, 5}};
   int bp_target[2][9] = {{2048, 2048, 2048, 2048, 2048, 2048, 2048, 2048, 2048},
                          {2048, 2048, 2048, 2048, 2048, 2048, 2048, 2048, 2048}};
+  // int8 block products, set by the autotuner (0: the built-in defaults): workgroup target of the
+  // reduction split per (pass, kernel class of i8_class) and the narrow kernel's row tiles per pass
+  int i8_target[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  int i8_rt[2] = {0, 0};
   std::vector<float> rr_host;    // r_rsqrt (n)
   std::vector<float> rvec_host;  // R (n)
 };
@@ -127,6 +132,14 @@ namespace slemm {
 void ensure_counts(slemm_geno* h, hipStream_t st);
 // Replaces the covariate basis Q (host n x c column-major) of the model terms.
 void set_Q(slemm_geno* h, const float* Q, int c);
+// model terms of unit error variances, no covariates and the given per-SNP weights (device, m
+// floats; zero for SNPs that drop out), as set_model leaves them for unweighted SNPs
+void set_model_weights(slemm_geno* h, const float* gw_dev);
+// a handle with room for cap_n individuals of src's SNPs, filled by geno_refill
+slemm_geno* geno_alloc_like(const slemm_geno* src, int64_t cap_n);
+// h <- the individuals keep_idx (strictly increasing) of hs, compacted on the device into h's
+// allocation (the model terms must be set again)
+void geno_refill(slemm_geno* h, const slemm_geno* hs, const int64_t* keep_idx, int64_t n_keep);
 void require_model(const slemm_geno* h);
 void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const float* V, int64_t ldv,
             int t, float* out, int64_t ldo, int precision, hipStream_t st);

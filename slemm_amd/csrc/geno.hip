@@ -221,6 +221,7 @@ static slemm_geno* alloc_geno(int64_t n, int64_t m) {
   h->n = n;
   h->m = m;
   h->n_pad = round_up(n, kPad);
+  h->cap_n_pad = h->n_pad;
   h->m_pad = round_up(m, kPad);
   h->ld_gs = h->n_pad / 4;
   h->ld_gi = h->m_pad / 4;
@@ -274,6 +275,19 @@ void set_Q(slemm_geno* h, const float* Q, int c) {
     HIPCHECK(hipMemcpy2D(h->Q, sizeof(float) * h->n_pad, Q, sizeof(float) * h->n, sizeof(float) * h->n, c,
                          hipMemcpyHostToDevice));
   }
+}
+
+void set_model_weights(slemm_geno* h, const float* gw_dev) {
+  HIPCHECK(hipSetDevice(h->device));
+  ensure_counts(h, nullptr);
+  HIPCHECK(hipMemcpy(h->gw, gw_dev, sizeof(float) * h->m, hipMemcpyDeviceToDevice));
+  std::vector<float> ones(h->n, 1.0f);
+  HIPCHECK(hipMemcpy(h->rr, ones.data(), sizeof(float) * h->n, hipMemcpyHostToDevice));
+  h->rr_host = ones;
+  h->rr_unit = true;
+  h->rvec_host = ones;
+  set_Q(h, nullptr, 0);
+  h->model_ready = true;
 }
 
 void require_model(const slemm_geno* h) {
@@ -494,34 +508,61 @@ extern "C" int slemm_gpu_geno_subset(const slemm_geno* hs, const int64_t* snp_id
   });
 }
 
+namespace slemm {
+
+slemm_geno* geno_alloc_like(const slemm_geno* src, int64_t cap_n) {
+  HIPCHECK(hipSetDevice(src->device));
+  return alloc_geno(cap_n, src->m);
+}
+
+void geno_refill(slemm_geno* h, const slemm_geno* hs, const int64_t* keep_idx, int64_t n_keep) {
+  REQUIRE(h && hs && keep_idx, "null argument");
+  std::vector<int32_t> keep32;
+  check_keep(keep_idx, n_keep, hs->n, keep32);
+  REQUIRE(h->m == hs->m && h->device == hs->device, "handles of different SNP sets or devices");
+  REQUIRE(round_up(n_keep, kPad) <= h->cap_n_pad, "more individuals than the handle holds");
+  HIPCHECK(hipSetDevice(hs->device));
+  h->n = n_keep;
+  h->n_pad = round_up(n_keep, kPad);
+  h->ld_gs = h->n_pad / 4;
+  h->counts_ready = h->model_ready = false;
+  set_Q(h, nullptr, 0);
+  // the padding rows of the SNP-major layout, the error-variance weights beyond n and the
+  // workspaces read as zero, as after allocation (a shorter layout reuses the same memory)
+  HIPCHECK(hipMemset(h->gs + h->m * h->ld_gs, 0, (size_t)(h->m_pad - h->m) * h->ld_gs));
+  HIPCHECK(hipMemset(h->rr, 0, sizeof(float) * h->cap_n_pad));
+  for (DevBuf* b : {&h->x1, &h->x2, &h->part1, &h->part2, &h->colsum, &h->qtv, &h->tmp, &h->kinds, &h->redpart,
+                    &h->partbuf, &h->i8buf})
+    if (b->p) HIPCHECK(hipMemset(b->p, 0, b->bytes));
+  h->kinds_host.clear();  // the per-column kinds on the device are gone: upload them again
+  DevBuf dkeep;
+  dkeep.reserve(sizeof(int32_t) * n_keep);
+  HIPCHECK(hipMemcpy(dkeep.p, keep32.data(), sizeof(int32_t) * n_keep, hipMemcpyHostToDevice));
+  // the same compaction as geno_create, reading the source handle's SNP-major rows in place
+  for (int64_t j0 = 0; j0 < hs->m; j0 += 65535) {
+    const int64_t rows = std::min<int64_t>(65535, hs->m - j0);
+    dim3 grid((unsigned)((h->ld_gs + 255) / 256), (unsigned)rows);
+    k_compact<<<grid, 256>>>(hs->gs + j0 * hs->ld_gs, hs->ld_gs, (int)rows, dkeep.as<int32_t>(), n_keep,
+                             h->gs + j0 * h->ld_gs, h->ld_gs);
+    HIPCHECK(hipGetLastError());
+  }
+  build_transpose(h, nullptr);
+  HIPCHECK(hipDeviceSynchronize());
+}
+
+}  // namespace slemm
+
 extern "C" int slemm_gpu_geno_keep(const slemm_geno* hs, const int64_t* keep_idx, int64_t n_keep,
                                    slemm_geno** out) {
   return guard([&] {
     REQUIRE(hs && keep_idx && out, "null argument");
     *out = nullptr;
     REQUIRE(n_keep > 0, "individual set retained for analysis is empty");
-    std::vector<int32_t> keep32(n_keep);
-    for (int64_t k = 0; k < n_keep; ++k) {
-      REQUIRE(keep_idx[k] >= 0 && keep_idx[k] < hs->n, "keep_idx out of range");
-      REQUIRE(k == 0 || keep_idx[k] > keep_idx[k - 1], "keep_idx must be strictly increasing");
-      keep32[k] = (int32_t)keep_idx[k];
-    }
-    HIPCHECK(hipSetDevice(hs->device));
-    slemm_geno* h = alloc_geno(n_keep, hs->m);
+    std::vector<int32_t> keep32;
+    check_keep(keep_idx, n_keep, hs->n, keep32);
+    slemm_geno* h = geno_alloc_like(hs, n_keep);
     try {
-      DevBuf dkeep;
-      dkeep.reserve(sizeof(int32_t) * n_keep);
-      HIPCHECK(hipMemcpy(dkeep.p, keep32.data(), sizeof(int32_t) * n_keep, hipMemcpyHostToDevice));
-      // the same compaction as geno_create, reading the source handle's SNP-major rows in place
-      for (int64_t j0 = 0; j0 < hs->m; j0 += 65535) {
-        const int64_t rows = std::min<int64_t>(65535, hs->m - j0);
-        dim3 grid((unsigned)((h->ld_gs + 255) / 256), (unsigned)rows);
-        k_compact<<<grid, 256>>>(hs->gs + j0 * hs->ld_gs, hs->ld_gs, (int)rows, dkeep.as<int32_t>(), n_keep,
-                                 h->gs + j0 * h->ld_gs, h->ld_gs);
-        HIPCHECK(hipGetLastError());
-      }
-      build_transpose(h, nullptr);
-      HIPCHECK(hipDeviceSynchronize());
+      geno_refill(h, hs, keep_idx, n_keep);
     } catch (...) {
       slemm_gpu_geno_destroy(h);
       throw;

@@ -162,17 +162,26 @@ static bool i8_wide_enabled() {
   return !(e && e[0] == '0');
 }
 
-static int i8_pass_splits(int pass, int t, int64_t rows_pad, int64_t nsuper, bool squared, int64_t& nsp) {
+// the int8 settings of a t-vector block for one pass: workgroup target (SLEMM_I8_TARGET, else the
+// autotuner's choice on this handle, else 2048 narrow / 8192 wide) and the narrow row tiles
+static int i8_pass_splits(const slemm_geno* h, int pass, int t, int64_t rows_pad, int64_t nsuper, bool squared,
+                          int64_t& nsp, int& rt) {
+  // SLEMM_I8_TARGET (read per call): one workgroup target for both passes, or "pass1,pass2"
   const char* e = std::getenv("SLEMM_I8_TARGET");
-  const int64_t dflt = t <= 5 ? 2048 : 8192;
-  const int64_t target = e ? std::max<int64_t>(1, std::atoll(e)) : dflt;
-  (void)pass;
-  return i8_splits(t, rows_pad, nsuper, target, squared, nsp);
+  const int cls = i8_class(t);
+  const int64_t dflt = h->i8_target[pass][cls] > 0 ? h->i8_target[pass][cls] : (t <= 5 ? 2048 : 8192);
+  int64_t target = dflt;
+  if (e && *e) {
+    const char* c = std::strchr(e, ',');
+    target = std::max<int64_t>(1, std::atoll(pass == 1 && c ? c + 1 : e));
+  }
+  rt = cls == 0 ? h->i8_rt[pass] : 0;
+  return i8_splits(t, rows_pad, nsuper, target, squared, rt, nsp);
 }
 
 // sums: where the digit column sums of the represented operand go (weighted by sw), null: none
 static void bp_i8(slemm_geno* h, const uint8_t* G, int64_t ldg, int64_t red_pad, int64_t rows_pad, const void* X,
-                  const Layout& L, int t, bool squared, int S, int64_t nsp, hipStream_t st, double* sums,
+                  const Layout& L, int t, bool squared, int S, int64_t nsp, int rt, hipStream_t st, double* sums,
                   I8SumWeights sw, const int32_t* tile_nsuper = nullptr, const int64_t* tile_rows = nullptr) {
   h->i8buf.reserve(i8_scratch_bytes(red_pad, t));
   I8Launch l;
@@ -180,7 +189,7 @@ static void bp_i8(slemm_geno* h, const uint8_t* G, int64_t ldg, int64_t red_pad,
   l.squared = squared; l.nsplit = S; l.ns_per_split = nsp; l.scratch = h->i8buf.p;
   l.part = h->partbuf.as<float>();
   l.tile_nsuper = tile_nsuper; l.tile_rows = tile_rows;
-  l.sum_out = sums; l.sum_w = sw;
+  l.sum_out = sums; l.sum_w = sw; l.rt = rt;
   launch_bp_i8(l, st);
 }
 
@@ -196,7 +205,8 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
   int64_t nsp = 0;
   const int64_t nsuper = h->n_pad / kPad;
   const bool i8 = L.i8;
-  const int S = i8 ? i8_pass_splits(0, t, h->m_pad, nsuper, squared, nsp) : (int)pass_splits(h, 0, L.slot, nsp);
+  int rt = 0;
+  const int S = i8 ? i8_pass_splits(h, 0, t, h->m_pad, nsuper, squared, nsp, rt) : (int)pass_splits(h, 0, L.slot, nsp);
   h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * h->m_pad);
   LdsArgs la;
   la.G = h->gs; la.ldg = h->ld_gs; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x1.p;
@@ -206,7 +216,7 @@ void pass1_lds(slemm_geno* h, const Layout& L, int t, bool weighted, void* x2, d
   if (tc) la.tile_nsuper = tc->tile_nsuper;  // reduction over individuals stops at the subsets' end
   TimingToken tok = timing_begin(st);
   if (i8)  // the centring sums s_k of the epilogue become those of the digits
-    bp_i8(h, h->gs, h->ld_gs, h->n_pad, h->m_pad, h->x1.p, L, t, squared, S, nsp, st, h->colsum.as<double>(),
+    bp_i8(h, h->gs, h->ld_gs, h->n_pad, h->m_pad, h->x1.p, L, t, squared, S, nsp, rt, st, h->colsum.as<double>(),
           I8SumWeights{}, la.tile_nsuper);
   else launch_bp(la, L.NT, S, bp_choice(h, 0, L.slot).variant, st);
   timing_end(tok, st, EPI_PASS1, PREC_BF16X2);
@@ -227,7 +237,8 @@ void pass2_lds(slemm_geno* h, const Layout& L, int t, double* u, const float* rr
   int64_t nsp = 0;
   const int64_t nsuper = h->m_pad / kPad;
   const bool i8 = L.i8;
-  const int S = i8 ? i8_pass_splits(1, t, h->n_pad, nsuper, false, nsp) : (int)pass_splits(h, 1, L.slot, nsp);
+  int rt = 0;
+  const int S = i8 ? i8_pass_splits(h, 1, t, h->n_pad, nsuper, false, nsp, rt) : (int)pass_splits(h, 1, L.slot, nsp);
   h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * h->n_pad);
   LdsArgs la;
   la.G = h->gi; la.ldg = h->ld_gi; la.nsuper = nsuper; la.ns_per_split = nsp; la.X = h->x2.p;
@@ -242,7 +253,7 @@ void pass2_lds(slemm_geno* h, const Layout& L, int t, double* u, const float* rr
       sw.col_row = tc->col_trait;
       sw.ldw = tc->ld_m;
     }
-    bp_i8(h, h->gi, h->ld_gi, h->m_pad, h->n_pad, h->x2.p, L, t, false, S, nsp, st, u, sw, nullptr, la.tile_rows);
+    bp_i8(h, h->gi, h->ld_gi, h->m_pad, h->n_pad, h->x2.p, L, t, false, S, nsp, rt, st, u, sw, nullptr, la.tile_rows);
   } else {
     launch_bp(la, L.NT, S, bp_choice(h, 1, L.slot).variant, st);
   }
@@ -356,8 +367,9 @@ size_t op_workspace_bytes(const slemm_geno* h, int kind, const int* col_kind, in
     const size_t S1 = (size_t)pass_splits(h, 0, L.slot, nsp), S2 = (size_t)pass_splits(h, 1, L.slot, nsp);
     add(sizeof(float) * L.KP * std::max(S1 * h->m_pad, S2 * h->n_pad));  // partbuf, sized up front
     if (L.i8) {
-      const size_t I1 = (size_t)i8_pass_splits(0, t, h->m_pad, h->n_pad / kPad, false, nsp);
-      const size_t I2 = (size_t)i8_pass_splits(1, t, h->n_pad, h->m_pad / kPad, false, nsp);
+      int rt = 0;
+      const size_t I1 = (size_t)i8_pass_splits(h, 0, t, h->m_pad, h->n_pad / kPad, false, nsp, rt);
+      const size_t I2 = (size_t)i8_pass_splits(h, 1, t, h->n_pad, h->m_pad / kPad, false, nsp, rt);
       add(sizeof(float) * L.KP * I1 * h->m_pad);                          // partbuf grown by the int8 splits,
       add(sizeof(float) * L.KP * I2 * h->n_pad);                          // once per pass at most
       add(i8_scratch_bytes(h->n_pad, t));                                 // digits, pass 1
@@ -538,11 +550,97 @@ static __global__ void k_fill_split(uint32_t* __restrict__ w, size_t nwords, uin
   }
 }
 
+// FP32 operand values in (-1, 1) for autotuning the int8 kernels
+static __global__ void k_fill_f32(float* __restrict__ x, size_t n, uint32_t salt) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    uint32_t v = (uint32_t)i * 0x9E3779B9u ^ (salt * 0x85EBCA6Bu);
+    v ^= v >> 15; v *= 0x2C1B3C6Du; v ^= v >> 12; v *= 0x297A2D39u; v ^= v >> 15;
+    x[i] = (float)((int32_t)v) * 4.656613e-10f;
+  }
+}
+
+// Autotuning of a t-vector block that runs on the int8 kernels: per pass, the workgroup target of
+// the reduction split (1024 ... 16384) and, for narrow blocks, the row tiles per wave (4 or 8),
+// each timed over full passes (digit preparation + block product) on random FP32 operands; the
+// fastest is kept on the handle for its kernel class (i8_class) and pass.
+static void autotune_i8(slemm_geno* h, const Layout& L, int t, int reps, int* variant, int* target, double* best_ms) {
+  h->x1.reserve(L.x1_bytes);
+  h->x2.reserve(L.x2_bytes);
+  k_fill_f32<<<4096, 256>>>(h->x1.as<float>(), L.x1_bytes / 4, 1);
+  k_fill_f32<<<4096, 256>>>(h->x2.as<float>(), L.x2_bytes / 4, 2);
+  HIPCHECK(hipGetLastError());
+  const int cls = i8_class(t);
+  const int64_t targets[] = {1024, 2048, 4096, 8192, 16384};
+  const int rts[] = {4, 8};
+  const int nrt = cls == 0 ? 2 : 1;
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  for (int pass = 0; pass < 2; ++pass) {
+    const int64_t rows_pad = pass == 0 ? h->m_pad : h->n_pad, red_pad = pass == 0 ? h->n_pad : h->m_pad;
+    double best = 1e300;
+    int brt = 0;
+    int64_t bt = 0;
+    for (int ri = 0; ri < nrt; ++ri)
+      for (int ti = 0; ti < 5; ++ti) {
+        const int rt = cls == 0 ? rts[ri] : 0;
+        int64_t nsp = 0;
+        const int S = i8_splits(t, rows_pad, red_pad / kPad, targets[ti], false, rt, nsp);
+        h->partbuf.reserve(sizeof(float) * (size_t)S * L.KP * rows_pad);
+        h->i8buf.reserve(i8_scratch_bytes(red_pad, t));
+        I8Launch l;
+        l.G = pass == 0 ? h->gs : h->gi; l.ldg = pass == 0 ? h->ld_gs : h->ld_gi;
+        l.red_pad = red_pad; l.rows_pad = rows_pad; l.X = pass == 0 ? h->x1.p : h->x2.p; l.KP = L.KP; l.t = t;
+        l.nsplit = S; l.ns_per_split = nsp; l.scratch = h->i8buf.p; l.part = h->partbuf.as<float>(); l.rt = rt;
+        launch_bp_i8(l, nullptr);  // warm-up
+        HIPCHECK(hipEventRecord(e0, nullptr));
+        for (int r = 0; r < reps; ++r) launch_bp_i8(l, nullptr);
+        HIPCHECK(hipEventRecord(e1, nullptr));
+        HIPCHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        const double avg = ms / reps;
+        if (best_ms) best_ms[2 + pass * (BP_NVARIANTS * 3) + ri * 5 + ti] = avg;
+        if (avg < best) {
+          best = avg;
+          brt = rt;
+          bt = targets[ti];
+        }
+      }
+    h->i8_target[pass][cls] = (int)bt;
+    if (cls == 0) h->i8_rt[pass] = brt;
+    if (variant) variant[pass] = brt;
+    if (target) target[pass] = (int)bt;
+    if (best_ms) best_ms[pass] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
+
+extern "C" int slemm_gpu_block_kernel(const slemm_geno* h, int t, int precision, int* kernel) {
+  return guard([&] {
+    REQUIRE(h && kernel, "null argument");
+    REQUIRE(t > 0, "t must be positive");
+    check_precision(precision);
+    const Layout L = layout_for(h, t, precision);
+    *kernel = L.i8 ? SLEMM_KERNEL_I8_NARROW + i8_class(t) : precision == PREC_BF16X2 ? SLEMM_KERNEL_BF16X2 : SLEMM_KERNEL_FP32;
+  });
+}
+
 extern "C" int slemm_gpu_autotune(slemm_geno* h, int t, int reps, int* variant, int* target, double* best_ms) {
   return guard([&] {
     REQUIRE(h, "null handle");
     REQUIRE(t > 0 && reps > 0, "t and reps must be positive");
     HIPCHECK(hipSetDevice(h->device));
+    if (best_ms)
+      for (int e = 0; e < SLEMM_AUTOTUNE_MS; ++e) best_ms[e] = -1.0;
+    {
+      const Layout Li = layout_for(h, t, PREC_BF16X2);
+      if (Li.i8) {
+        autotune_i8(h, Li, t, reps, variant, target, best_ms);
+        return;
+      }
+    }
     const Layout L = layout_for(h, t, PREC_BF16X2, false, false);  // the split-BF16 kernels only
     h->x1.reserve(L.x1_bytes);
     h->x2.reserve(L.x2_bytes);

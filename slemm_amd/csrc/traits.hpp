@@ -12,6 +12,7 @@ struct slemm_traits {
   slemm_geno* h = nullptr;     // used by every batched call; destroy reads only `device`
   int device = 0;
   int T = 0;
+  int plan = 0;                // SLEMM_TRAITS_AUTO, _BATCHED or _SEQUENTIAL (slemm_gpu_traits_set_plan)
   slemm::DevBuf mask, q;       // T x n_pad: membership (0/1) and intercept basis mask/sqrt(n_t)
   slemm::DevBuf n_t;           // T doubles
   slemm::DevBuf gm, gw, pc;    // T x m_pad: per-trait 2 x A2 frequency, weights, dosage sums

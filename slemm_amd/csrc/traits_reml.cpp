@@ -1,19 +1,120 @@
-// SLDF_REML (src/reml.cpp:549-690) for a batch of traits on individual subsets: one batched
-// Lanczos seed carries every trait's phenotype, intercept and probe columns through the shared
-// block products, and the golden-section search then runs per trait on replays of that seed.
-// Per trait the computation is the single-trait driver's on the trait's individuals: the same
-// probes (drawn from mt19937(seed+54321) over the subset), the same criterion and search.
+// SLDF_REML (src/reml.cpp:549-690) for a batch of traits on individual subsets, on one of two
+// plans:
+//  - batched: one Lanczos seed carries every trait's phenotype, intercept and probe columns
+//    through the shared block products, and the golden-section search then runs per trait on
+//    replays of that seed;
+//  - one trait at a time: each trait on a handle compacted to its individuals (device-side), with
+//    the trait's SNP weights, through the single-trait driver.
+// Per trait the computation is the single-trait driver's on the trait's individuals either way:
+// the same probes (drawn from mt19937(seed+54321) over the subset), the same criterion and search.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "driver.hpp"
 #include "traits.hpp"
 
 using namespace slemm;
+
+namespace {
+
+// A batched block product costs each 16-column tile its subsets' extent (rows up to the last
+// member of any of its traits); one trait at a time costs each trait its n_V + 1 columns over its
+// own individuals.  The batched products run at higher efficiency per column-row (one wide block
+// instead of T narrower ones, one graph per Lanczos step): on n = 200,000 individuals, m = 300,000
+// SNPs, 32 traits of 32 probes they won at a work ratio of 1.03 (nested subsets, 1.02x faster)
+// and lost at 2.3 (random subsets, 0.63x), so the automatic plan batches up to a ratio of 1.25.
+constexpr double kBatchedWorkRatio = 1.25;
+
+double batched_work_ratio(const slemm_traits* tr, int n_V) {
+  const int T = tr->T;
+  std::vector<int64_t> ext;  // per column of the batched block, in its order
+  for (int t = 0; t < T; ++t) ext.push_back(tr->extent[t]);
+  for (int t = 0; t < T; ++t) ext.push_back(tr->extent[t]);
+  for (int t = 0; t < T; ++t)
+    for (int j = 0; j < n_V; ++j) ext.push_back(tr->extent[t]);
+  double wb = 0, ws = 0;
+  for (size_t c0 = 0; c0 < ext.size(); c0 += 16)
+    wb += 16.0 * (double)*std::max_element(ext.begin() + c0, ext.begin() + std::min(ext.size(), c0 + 16));
+  for (int t = 0; t < T; ++t) ws += (double)tr->n_host[t] * (n_V + 1);
+  return wb / ws;
+}
+
+int resolve_plan(const slemm_traits* tr, int n_V, double* ratio) {
+  const double r = batched_work_ratio(tr, n_V);
+  if (ratio) *ratio = r;
+  if (tr->plan != SLEMM_TRAITS_AUTO) return tr->plan;
+  return r <= kBatchedWorkRatio ? SLEMM_TRAITS_BATCHED : SLEMM_TRAITS_SEQUENTIAL;
+}
+
+struct GenoGuard {
+  slemm_geno* g = nullptr;
+  ~GenoGuard() {
+    if (g) slemm_gpu_geno_destroy(g);
+  }
+};
+
+// one trait at a time: the trait's individuals compacted on the device into one handle sized for
+// the largest subset (allocated once: a fresh multi-GB allocation per trait costs more than the
+// trait's fit), with the trait's weights (monomorphic SNPs of the subset at weight 0), then the
+// single-trait driver with an intercept
+void traits_one_at_a_time(slemm_traits* tr, const float* y, int rng_seed, int n_V, float tol_L, float s2max,
+                          float s2min, float tol_VC, int precision, int verbose, float* vg_out, float* ve_out,
+                          float* h2_out, float* lrt_out, int* steps_out, double* seconds) {
+  slemm_geno* h = tr->h;
+  const auto t_start = std::chrono::steady_clock::now();
+  double t_seed = 0;
+  GenoGuard gk;
+  gk.g = geno_alloc_like(h, *std::max_element(tr->n_host.begin(), tr->n_host.end()));
+  for (int t = 0; t < tr->T; ++t) {
+    const std::vector<int64_t>& mem = tr->members[t];
+    const int64_t nt = (int64_t)mem.size();
+    geno_refill(gk.g, h, mem.data(), nt);
+    set_model_weights(gk.g, tr->gw.as<float>() + (size_t)t * h->m_pad);
+    std::vector<float> X(nt, 1.0f), yt(nt), Py(nt), blup(h->m), blue(1), vb(1);
+    for (int64_t i = 0; i < nt; ++i) yt[i] = y[(size_t)t * h->n + mem[i]];
+    slemm_reml_stats st;
+    std::memset(&st, 0, sizeof(st));
+    abi(slemm_gpu_sldf_reml(gk.g, X.data(), nt, 1, yt.data(), rng_seed, n_V, tol_L, s2max, s2min, tol_VC,
+                            precision, verbose, Py.data(), blup.data(), blue.data(), vb.data(), &st));
+    vg_out[t] = st.vg;
+    ve_out[t] = st.ve;
+    h2_out[t] = st.h2;
+    if (lrt_out) lrt_out[t] = st.lrt;
+    if (steps_out)
+      for (int q = 0; q < 3; ++q) steps_out[3 * t + q] = st.lanczos_steps[q];
+    t_seed += st.seconds_seed;
+  }
+  if (seconds) {
+    seconds[0] = t_seed;
+    seconds[1] = seconds_since(t_start);
+  }
+}
+
+}  // namespace
+
+extern "C" int slemm_gpu_traits_set_plan(slemm_traits* tr, int plan) {
+  return guard([&] {
+    REQUIRE(tr, "null handle");
+    REQUIRE(plan == SLEMM_TRAITS_AUTO || plan == SLEMM_TRAITS_BATCHED || plan == SLEMM_TRAITS_SEQUENTIAL,
+            "unknown plan");
+    tr->plan = plan;
+  });
+}
+
+extern "C" int slemm_gpu_traits_plan(const slemm_traits* tr, int n_V, int* plan, double* work_ratio) {
+  return guard([&] {
+    REQUIRE(tr && plan, "null argument");
+    REQUIRE(n_V >= 1, "number of probes must be positive");
+    *plan = resolve_plan(tr, n_V, work_ratio);
+  });
+}
 
 extern "C" int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int rng_seed, int n_V, float tol_L,
                                           float s2max, float s2min, float tol_VC, int precision, int verbose,
@@ -25,6 +126,11 @@ extern "C" int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int 
     REQUIRE(s2min > 0 && s2max < 1 && s2min < s2max, "heritability range must satisfy 0 < s2min < s2max < 1");
     slemm_geno* h = tr->h;
     HIPCHECK(hipSetDevice(h->device));
+    if (resolve_plan(tr, n_V, nullptr) == SLEMM_TRAITS_SEQUENTIAL) {
+      traits_one_at_a_time(tr, y, rng_seed, n_V, tol_L, s2max, s2min, tol_VC, precision, verbose, vg_out, ve_out,
+                           h2_out, lrt_out, steps_out, seconds);
+      return;
+    }
     const auto t_start = std::chrono::steady_clock::now();
     const int T = tr->T;
     const int64_t n = h->n;

@@ -290,18 +290,40 @@ class Geno:
         L.check(L.load().slemm_gpu_gwas_score(self._h, _ptr(p), skip_mac, _ptr(xy), _ptr(xx), _stream()))
         return xy, xx
 
+    BLOCK_KERNELS = ("fp32", "bf16x2", "int8 narrow", "int8 one tile", "int8 wide")
+
+    def block_kernel(self, t: int, precision="bf16x2") -> str:
+        """Kernel family a t-vector operator block at this precision runs on."""
+        k = C.c_int(-1)
+        L.check(L.load().slemm_gpu_block_kernel(self._h, t, _prec(precision), C.byref(k)))
+        return self.BLOCK_KERNELS[k.value]
+
     def autotune(self, t: int, reps: int = 2) -> dict:
-        """Times the block-product variants for a t-column operator and keeps the fastest."""
+        """Times the block-product settings for a t-column operator and keeps the fastest: the
+        int8 kernels' split target (and narrow row tiles), or the split-BF16 kernel variants."""
         v = (C.c_int * 2)()
         tg = (C.c_int * 2)()
-        names = ["reg", "lds4", "lds2", "reg8", "regl", "pipe", "persist", "pipel", "piper16", "piper32"]
-        nv = len(names)
+        kernel = self.block_kernel(t)
+        nv = 10
         ms = (C.c_double * (2 + 6 * nv))(*([float("nan")] * (2 + 6 * nv)))
         L.check(L.load().slemm_gpu_autotune(self._h, t, reps, v, tg, ms))
+        if kernel.startswith("int8"):
+            nrt = 2 if kernel == "int8 narrow" else 1
+            targets = [1024, 2048, 4096, 8192, 16384]
+            table = {}
+            for p in range(2):
+                for ri in range(nrt):
+                    for ti in range(5):
+                        key = f"pass{p + 1}/" + (f"rt{[4, 8][ri]}/" if nrt == 2 else "") + f"{targets[ti]}"
+                        table[key] = round(ms[2 + p * 3 * nv + ri * 5 + ti], 3)
+            name = (lambda p: f"{kernel} rt{v[p]}") if nrt == 2 else (lambda p: kernel)
+            return {"kernel": kernel, "pass1": (name(0), tg[0], round(ms[0], 3)),
+                    "pass2": (name(1), tg[1], round(ms[1], 3)), "table_ms": table}
+        names = ["reg", "lds4", "lds2", "reg8", "regl", "pipe", "persist", "pipel", "piper16", "piper32"]
         table = {f"pass{p + 1}/{names[vv]}/{[512, 1024, 2048][ti]}": round(ms[2 + p * 3 * nv + vv * 3 + ti], 3)
                  for p in range(2) for vv in range(nv) for ti in range(3)}
-        return {"pass1": (names[v[0]], tg[0], round(ms[0], 3)), "pass2": (names[v[1]], tg[1], round(ms[1], 3)),
-                "table_ms": table}
+        return {"kernel": kernel, "pass1": (names[v[0]], tg[0], round(ms[0], 3)),
+                "pass2": (names[v[1]], tg[1], round(ms[1], 3)), "table_ms": table}
 
     # ------------------------------------------------------------------ REML drivers
     def SLDF_REML(self, X: np.ndarray, y: np.ndarray, rng_seed: int = 0, n_V: int = 30, tol_L: float = 5e-4,
@@ -626,18 +648,35 @@ class Traits:
                                                 keep, _prec(precision), int(verbose), _stream(), C.byref(h)))
         return LSeed(h, self.geno.n, t, owner=self)
 
+    PLANS = {"auto": L.TRAITS_AUTO, "batched": L.TRAITS_BATCHED, "sequential": L.TRAITS_SEQUENTIAL}
+
+    def set_plan(self, plan: str) -> None:
+        """"auto" (default), "batched" or "sequential" (one trait at a time on compacted handles)."""
+        if plan not in self.PLANS:
+            raise ValueError(f"unknown plan {plan!r}")
+        L.check(L.load().slemm_gpu_traits_set_plan(self._h, self.PLANS[plan]))
+
+    def plan(self, n_V: int = 30) -> tuple:
+        """(plan the next SLDF_REML with n_V probes runs, batched / one-at-a-time work ratio)."""
+        pl = C.c_int(-1)
+        r = C.c_double(0)
+        L.check(L.load().slemm_gpu_traits_plan(self._h, n_V, C.byref(pl), C.byref(r)))
+        return {v: k for k, v in self.PLANS.items()}[pl.value], r.value
+
     def SLDF_REML(self, y: np.ndarray, rng_seed: int = 0, n_V: int = 30, tol_L: float = 5e-4, s2max: float = 0.7,
                   s2min: float = 1e-4, tol_VC: float = 1e-4, precision="bf16x2", verbose: bool = False) -> dict:
-        """SLDF_REML (src/reml.cpp:549-690) for all traits in one batch; y (T, n)."""
+        """SLDF_REML (src/reml.cpp:549-690) for all traits, on the handle's plan; y (T, n)."""
         yv = np.ascontiguousarray(np.asarray(y, np.float32).reshape(self.T, self.geno.n))
         vg, ve, h2, lrt = (np.zeros(self.T, np.float32) for _ in range(4))
         steps = np.zeros((self.T, 3), np.int32)
         secs = np.zeros(2)
+        plan, _ = self.plan(n_V)
         L.check(L.load().slemm_gpu_traits_sldf_reml(
             self._h, _np_ptr(yv, C.c_float), rng_seed, n_V, tol_L, s2max, s2min, tol_VC, _prec(precision),
             int(verbose), _np_ptr(vg, C.c_float), _np_ptr(ve, C.c_float), _np_ptr(h2, C.c_float),
             _np_ptr(lrt, C.c_float), _np_ptr(steps, C.c_int), _np_ptr(secs, C.c_double)))
-        return dict(vg=vg, ve=ve, h2=h2, lrt=lrt, steps=steps, seconds_seed=secs[0], seconds_total=secs[1])
+        return dict(vg=vg, ve=ve, h2=h2, lrt=lrt, steps=steps, seconds_seed=secs[0], seconds_total=secs[1],
+                    plan=plan)
 
 
 def pred_plink(estimate_file: str, bfile_prefix: str, output_file: str) -> tuple:

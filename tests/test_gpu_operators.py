@@ -196,6 +196,49 @@ def test_int8_wide_blocks(gpu, t, monkeypatch):
         assert rel_err(z, uref) < tol, (target, rel_err(z, uref))
 
 
+def test_block_kernel_selection(gpu, monkeypatch):
+    """The kernel family per block width: int8 narrow for 1-5 vectors, split-BF16 for 6-8 (hi and lo
+    halves as columns), the one-tile int8 kernel for 9-16, the general int8 wide kernel beyond, FP32
+    kernels at FP32 precision; the switches move blocks back to the split-BF16 kernels."""
+    p = make_problem(n=700, m=300, seed=3)
+    g = _geno(gpu, p)
+    want = {1: "int8 narrow", 5: "int8 narrow", 6: "bf16x2", 8: "bf16x2", 9: "int8 one tile",
+            16: "int8 one tile", 17: "int8 wide", 65: "int8 wide", 101: "int8 wide"}
+    for t, k in want.items():
+        assert g.block_kernel(t) == k, t
+        assert g.block_kernel(t, "fp32") == "fp32", t
+    monkeypatch.setenv("SLEMM_I8_NARROW", "0")
+    monkeypatch.setenv("SLEMM_I8_WIDE", "0")
+    assert {g.block_kernel(t) for t in want} == {"bf16x2"}
+
+
+@pytest.mark.parametrize("t", [3, 16, 33])
+def test_int8_autotune(gpu, t):
+    """The int8 autotuner times every split target (and the narrow kernel's row tiles) per pass,
+    keeps the fastest on the handle, and the operator on the tuned settings still matches the FP64
+    dense operator to the int8 bound."""
+    p = make_problem(n=2600, m=1700, seed=41, weights=True, c=1)
+    g = _geno(gpu, p)
+    g.set_model(None, p.rvec, p.Q)
+    D = p.dense()
+    at = g.autotune(t, reps=2)
+    assert at["kernel"] == g.block_kernel(t)
+    times = [v for v in at["table_ms"].values()]
+    assert len(times) == (20 if t <= 5 else 10) and min(times) > 0, at
+    for pas in ("pass1", "pass2"):
+        name, target, ms = at[pas]
+        assert target in (1024, 2048, 4096, 8192, 16384), at
+        key = f"{pas}/" + (name.split()[-1] + "/" if t <= 5 else "") + f"{target}"
+        assert at["table_ms"][key] == ms == min(v for k, v in at["table_ms"].items() if k.startswith(pas)), at
+    rng = np.random.default_rng(400 + t)
+    V = rng.normal(size=(p.n, t)).astype(np.float32)
+    U = rng.normal(size=(p.m, t)).astype(np.float32)
+    out = g.lnr_op_select("ldet", 0.37, torch.from_numpy(V).cuda(), precision="bf16x2").cpu().numpy()
+    assert rel_err(out, D.op("ldet", 0.37, V)) < I8_TOL
+    z = g.matvec(torch.from_numpy(U).cuda(), precision="bf16x2").cpu().numpy()
+    assert rel_err(z, D.matvec(U)) < I8_TOL
+
+
 @pytest.mark.parametrize("t", [1, 3, 17])
 def test_int8_nonfinite_column(gpu, t, monkeypatch):
     """A NaN or infinite element makes every product of its column NaN on the int8 kernels, as on

@@ -162,9 +162,33 @@ def test_batched_seed_equals_subset_seeds(batch):
         off += w
 
 
-def test_batched_sldf_equals_per_subset_runs(batch):
-    """SLDF_REML for all traits in one batch vs the single-trait driver on each subset handle
-    with the same probes (same generator over the subset), and vs the FP64 criterion.  The
+def test_plan_choice(batch, gpu):
+    """The automatic plan: random subsets (every 16-column tile spans nearly all individuals, 1.6x
+    the one-at-a-time work here) run one trait at a time; nested leading subsets batch at 30
+    probes (1.2x) but not at 20 (1.76x: tiles that mix traits span the largest one)."""
+    p, masks, g, tr, subs = batch
+    plan, ratio = tr.plan(30)
+    assert plan == "sequential" and ratio > 1.25, (plan, ratio)
+    lead = np.zeros_like(masks)
+    for t, s in enumerate(masks.sum(axis=1)):
+        lead[t, :s] = True
+    tl = gpu.Traits(g, lead)
+    plan, ratio = tl.plan(30)
+    assert plan == "batched" and ratio <= 1.25, (plan, ratio)
+    assert tl.plan(20)[0] == "sequential"  # 20 probes: the mixed 16-column tiles cost 1.76x
+    tl.set_plan("batched")
+    assert tl.plan(20)[0] == "batched"
+    with pytest.raises(ValueError):
+        tl.set_plan("parallel")
+    tl.close()
+
+
+@pytest.mark.parametrize("plan", ["batched", "sequential"])
+def test_batched_sldf_equals_per_subset_runs(batch, plan, gpu):
+    """SLDF_REML for all traits, batched and one trait at a time, vs the single-trait driver on
+    each subset handle with the same probes (same generator over the subset), and vs the FP64
+    criterion.  One trait at a time runs the single-trait driver on a device-compacted handle and
+    must match the host-compacted handle's run exactly.  The
     criterion is a float, as the reference's REML_criterion returns it (src/reml.cpp:457-489):
     at n ~ 1e3 its ulp (~6e-5 at |2logLL| ~ 1e3) over the curvature (~40) limits where the
     golden-section search can end to about 1e-3 in h2, which is the tolerance here."""
@@ -181,9 +205,20 @@ def test_batched_sldf_equals_per_subset_runs(batch):
         e = rng.normal(size=len(keep)) * np.sqrt(gv.var() * (1 - h2) / h2)
         Y[t, keep] = (gv + e + 5.0).astype(np.float32)
         Y[t, ~masks[t]] = 123.0  # ignored
+    tr.set_plan(plan)
     got = tr.SLDF_REML(Y, n_V=20)
+    tr.set_plan("auto")
+    assert got["plan"] == plan
+    if plan == "sequential":  # largest subset first: the reused handle shrinks between traits
+        rev = gpu.Traits(g, masks[::-1].copy())
+        rev.set_plan("sequential")
+        back = rev.SLDF_REML(Y[::-1].copy(), n_V=20)
+        rev.close()
+        assert np.array_equal(back["h2"][::-1], got["h2"]) and np.array_equal(back["vg"][::-1], got["vg"])
     for t, (keep, gs) in enumerate(subs):
         ref = gs.SLDF_REML(np.ones((len(keep), 1), np.float32), Y[t, keep], n_V=20)
+        if plan == "sequential":
+            assert got["h2"][t] == ref["h2"] and got["vg"][t] == ref["vg"], (t, got["h2"][t], ref["h2"])
         nt = len(keep)
         gm, gw, _ = tr.model(t)
         Q = np.full((nt, 1), 1 / np.sqrt(nt))
