@@ -92,38 +92,47 @@ __global__ void k_compact(const uint8_t* __restrict__ raw, int64_t raw_ld, int r
   gs[(int64_t)r * ld_gs + ob] = (uint8_t)out;
 }
 
-// gs (SNP-major) -> gi (individual-major).  Tile: 128 SNPs x 256 individuals; 256 threads,
-// thread t writes the 32 bytes (128 SNPs) of individual i0+t.
+// gs (SNP-major) -> gi (individual-major).  Tile: 512 SNPs x 256 individuals; 256 threads,
+// thread t writes the 128 bytes (512 SNPs, one whole cache line) of individual i0+t, so that no
+// output line is shared between workgroups (which may sit on different XCDs and L2s).  The
+// transpose of packed 2-bit codes is its own inverse: launched on gi (rows = individuals) over a
+// (m_pad / 256, n_pad / 512) grid it writes gs.
 __global__ __launch_bounds__(256) void k_transpose(const uint8_t* __restrict__ gs, int64_t ld_gs,
                                                    uint8_t* __restrict__ gi, int64_t ld_gi) {
-  __shared__ uint32_t tile[128][64 / 4 + 1];  // 128 SNP rows x 64 bytes (+pad against conflicts)
+  __shared__ uint32_t tile[512][64 / 4 + 1];  // 512 SNP rows x 64 bytes (+pad against conflicts)
   const int64_t i0 = (int64_t)blockIdx.x * 256;
-  const int64_t j0 = (int64_t)blockIdx.y * 128;
+  const int64_t j0 = (int64_t)blockIdx.y * 512;
   const int t = threadIdx.x;
-  // cooperative load: 128 rows x 16 dwords = 2048 dwords, 8 per thread
+  // cooperative load: 512 rows x 4 uint4 = 2048 uint4, 8 per thread
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int idx = q * 256 + t;
-    const int row = idx >> 4, col = idx & 15;
-    tile[row][col] = *reinterpret_cast<const uint32_t*>(gs + (j0 + row) * ld_gs + i0 / 4 + col * 4);
+    const int row = idx >> 2, c4 = idx & 3;
+    const uint4 v = *reinterpret_cast<const uint4*>(gs + (j0 + row) * ld_gs + i0 / 4 + c4 * 16);
+    tile[row][c4 * 4 + 0] = v.x;
+    tile[row][c4 * 4 + 1] = v.y;
+    tile[row][c4 * 4 + 2] = v.z;
+    tile[row][c4 * 4 + 3] = v.w;
   }
   __syncthreads();
   const int dw = t >> 4;          // dword holding individual t within a row
   const int sh = 2 * (t & 15);    // bit offset of its code in that dword
-  uint32_t outw[8];
-#pragma unroll
-  for (int w = 0; w < 8; ++w) {
-    uint32_t acc = 0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const uint32_t code = (tile[w * 16 + q][dw] >> sh) & 3u;
-      acc |= code << (2 * q);
-    }
-    outw[w] = acc;
-  }
   uint4* dst = reinterpret_cast<uint4*>(gi + (i0 + t) * ld_gi + j0 / 4);
-  dst[0] = make_uint4(outw[0], outw[1], outw[2], outw[3]);
-  dst[1] = make_uint4(outw[4], outw[5], outw[6], outw[7]);
+#pragma unroll 1
+  for (int g = 0; g < 8; ++g) {
+    uint32_t outw[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      uint32_t acc = 0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const uint32_t code = (tile[(g * 4 + w) * 16 + q][dw] >> sh) & 3u;
+        acc |= code << (2 * q);
+      }
+      outw[w] = acc;
+    }
+    dst[g] = make_uint4(outw[0], outw[1], outw[2], outw[3]);
+  }
 }
 
 // One wave per SNP row: popcount, count of 11 codes, count of nonzero codes; gm = pc/n.
@@ -247,7 +256,8 @@ static slemm_geno* alloc_geno(int64_t n, int64_t m) {
 }
 
 static void build_transpose(slemm_geno* h, hipStream_t st) {
-  dim3 grid((unsigned)(h->n_pad / 256), (unsigned)(h->m_pad / 128));
+  static_assert(kPad % 512 == 0, "the transpose tiles 512 SNPs");
+  dim3 grid((unsigned)(h->n_pad / 256), (unsigned)(h->m_pad / 512));
   k_transpose<<<grid, 256, 0, st>>>(h->gs, h->ld_gs, h->gi, h->ld_gi);
   HIPCHECK(hipGetLastError());
 }
@@ -527,26 +537,26 @@ void geno_refill(slemm_geno* h, const slemm_geno* hs, const int64_t* keep_idx, i
   h->ld_gs = h->n_pad / 4;
   h->counts_ready = h->model_ready = false;
   set_Q(h, nullptr, 0);
-  // the padding rows of the SNP-major layout, the error-variance weights beyond n and the
-  // workspaces read as zero, as after allocation (a shorter layout reuses the same memory)
-  HIPCHECK(hipMemset(h->gs + h->m * h->ld_gs, 0, (size_t)(h->m_pad - h->m) * h->ld_gs));
+  // the error-variance weights beyond n and the workspaces read as zero, as after allocation (a
+  // shorter layout reuses the same memory)
   HIPCHECK(hipMemset(h->rr, 0, sizeof(float) * h->cap_n_pad));
   for (DevBuf* b : {&h->x1, &h->x2, &h->part1, &h->part2, &h->colsum, &h->qtv, &h->tmp, &h->kinds, &h->redpart,
                     &h->partbuf, &h->i8buf})
     if (b->p) HIPCHECK(hipMemset(b->p, 0, b->bytes));
   h->kinds_host.clear();  // the per-column kinds on the device are gone: upload them again
+  // individuals are rows of the individual-major copy: gather them (16-byte copies), then
+  // transpose that into the SNP-major copy
+  REQUIRE(h->ld_gi == hs->ld_gi, "handles of different SNP layouts");
   DevBuf dkeep;
-  dkeep.reserve(sizeof(int32_t) * n_keep);
-  HIPCHECK(hipMemcpy(dkeep.p, keep32.data(), sizeof(int32_t) * n_keep, hipMemcpyHostToDevice));
-  // the same compaction as geno_create, reading the source handle's SNP-major rows in place
-  for (int64_t j0 = 0; j0 < hs->m; j0 += 65535) {
-    const int64_t rows = std::min<int64_t>(65535, hs->m - j0);
-    dim3 grid((unsigned)((h->ld_gs + 255) / 256), (unsigned)rows);
-    k_compact<<<grid, 256>>>(hs->gs + j0 * hs->ld_gs, hs->ld_gs, (int)rows, dkeep.as<int32_t>(), n_keep,
-                             h->gs + j0 * h->ld_gs, h->ld_gs);
-    HIPCHECK(hipGetLastError());
-  }
-  build_transpose(h, nullptr);
+  dkeep.reserve(sizeof(int64_t) * n_keep);
+  HIPCHECK(hipMemcpy(dkeep.p, keep_idx, sizeof(int64_t) * n_keep, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemset(h->gi + n_keep * h->ld_gi, 0, (size_t)(h->n_pad - n_keep) * h->ld_gi));
+  dim3 grid(8, (unsigned)std::min<int64_t>(n_keep, 65535), (unsigned)((n_keep + 65534) / 65535));
+  k_gather_rows<<<grid, 256>>>(hs->gi, hs->ld_gi, dkeep.as<int64_t>(), n_keep, h->gi);
+  HIPCHECK(hipGetLastError());
+  k_transpose<<<dim3((unsigned)(h->m_pad / 256), (unsigned)(h->n_pad / 512)), 256>>>(h->gi, h->ld_gi, h->gs,
+                                                                                      h->ld_gs);
+  HIPCHECK(hipGetLastError());
   HIPCHECK(hipDeviceSynchronize());
 }
 

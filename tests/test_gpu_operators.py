@@ -38,6 +38,37 @@ def test_decode_bit_exact_with_keep_and_order(gpu):
     assert np.all((full[:, -1] >> (2 * (len(keep) % 4))) == 0) if len(keep) % 4 else True
 
 
+@pytest.mark.parametrize("kind", ["all", "prefix", "half", "sparse", "mixed", "one", "last"])
+def test_device_keep_bit_exact(gpu, kind):
+    """Individuals kept on the device (handle to handle, LDS-staged chunks of 4,096 kept
+    individuals, a global gather where they are sparser than one in eight) against the oracle's
+    decode of the same bed; the transposed copy through the product with the host-compacted
+    handle (exact in FP32: integer dosages)."""
+    # the sparse case spans more than 32 KiB of source bytes per chunk: the global gather
+    p = make_problem(n=140001 if kind == "sparse" else 9001, m=64 if kind == "sparse" else 300, seed=17,
+                     missing=0.02)
+    rng = np.random.default_rng(11)
+    keep = {"all": np.arange(p.n), "prefix": np.arange(5000),
+            "half": np.sort(rng.choice(p.n, 4500, replace=False)),
+            "sparse": np.sort(rng.choice(p.n, 3000, replace=False)),
+            "mixed": np.concatenate([np.arange(0, 4200), np.arange(4200, p.n, 25)]),
+            "one": np.array([4097]), "last": np.arange(p.n - 3, p.n)}[kind]
+    g = _geno(gpu, p)
+    gk = g.keep(keep)
+    want = bedio.decode_codes(p.rows, p.n)[:, keep]
+    assert np.array_equal(bedio.decode_codes(gk.download(), len(keep)), want)
+    raw = gk.download()
+    if len(keep) % 4:
+        assert np.all((raw[:, -1] >> (2 * (len(keep) % 4))) == 0)
+    gh = _geno(gpu, p, keep)
+    assert all(np.array_equal(a, b) for a, b in zip(gk.counts(), gh.counts()))
+    if len(keep) < 100:  # too few individuals for a model without monomorphic SNPs
+        return
+    gk.set_model(), gh.set_model()
+    U = torch.from_numpy(rng.integers(-3, 4, size=(p.m, 2)).astype(np.float32)).cuda()
+    assert np.array_equal(gk.matvec(U, precision="fp32").cpu().numpy(), gh.matvec(U, precision="fp32").cpu().numpy())
+
+
 def test_counts_bit_exact(gpu):
     p = make_problem(n=2051, m=733, seed=5, missing=0.03)
     g = _geno(gpu, p)
