@@ -100,7 +100,7 @@ def test_sldf_reference_replay_criterion(case, monkeypatch):
     """SLEMM_CRITERION_FP32=1 replays the reference's arithmetic: y'Py from the L_Solve solution
     and the criterion rounded to float before each comparison (src/reml.cpp:477-489).  The
     search then stops short of the FP64 optimum by the float resolution of a criterion of
-    magnitude ~1e3-1e5 (measured 1.5e-4 / 1.2e-4 / 1.3e-4 in Vg / Ve / h2 at n=1500), which is
+    magnitude ~1e3-1e5 (measured 1.9e-4 / 1.4e-4 / 1.5e-4 in Vg / Ve / h2 at n=1500), which is
     why the default compares in double."""
     p, y, g = case
     ref = _fp64_tight(p, y)
