@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstdlib>
+#include <type_traits>
 
 #include "blockprod.hpp"
 
@@ -327,8 +328,10 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
   const int vt0 = by * NVG;
   const int nv = min(NVG, a.nvt - vt0);                      // vector tiles of this group
   const bool tail = TAIL && a.r8 > 0 && by == (int)gridDim.y - 1;  // the group holds the tail tile
-  const int64_t s0 = blockIdx.z * a.ns_per_split;
-  int64_t s1 = min(a.nsuper, s0 + a.ns_per_split);
+  // super-chunk indices in 32 bits (the host checks nsuper < 2^31): the scalar unit compares
+  // 32-bit integers, 64-bit ones would go through the VALU in every tile's liveness test
+  const int s0 = (int)blockIdx.z * (int)a.ns_per_split;
+  int s1 = min((int)a.nsuper, s0 + (int)a.ns_per_split);
   const uint32_t tab = 0x02010100u;
   // per-tile limits: digit tile j of vector tile vt0 + j / 3 (the tail tile: vector tile nvt)
   // runs MFMAs for super-chunks s < lim[j]; the staged loop bound is the largest over the
@@ -344,7 +347,7 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
     lim[j] = (a.tile_rows && on && row0 >= a.tile_rows[vtile]) ? 0 : ln;
     limmax = max(limmax, (a.tile_rows && on && wg_row0 >= a.tile_rows[vtile]) ? 0 : ln);
   }
-  s1 = min(s1, (int64_t)limmax);
+  s1 = min(s1, (int)limmax);
   i32x4_t acc[RT][NDT];
 #pragma unroll
   for (int i = 0; i < RT; ++i)
@@ -352,11 +355,11 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
     for (int j = 0; j < NDT; ++j) acc[i][j] = i32x4_t{0, 0, 0, 0};
   auto live = [&](int j) { return j < 3 * nv || (TAIL && j == NDT - 1 && tail); };
   const uint8_t* lbase = a.G + (row0 + r) * a.ldg + g * 32;
-  auto ld = [&](int64_t sc, uint32_t (&dst)[RT][8]) {
+  auto ld = [&](int sc, uint32_t (&dst)[RT][8]) {
     sc = sc < s1 ? sc : s1 - 1;  // past the slice: re-read its last super-chunk (never used)
 #pragma unroll
     for (int i = 0; i < RT; ++i) {
-      const u32x4_t* q = reinterpret_cast<const u32x4_t*>(lbase + (int64_t)i * 16 * a.ldg + sc * 128);
+      const u32x4_t* q = reinterpret_cast<const u32x4_t*>(lbase + (int64_t)i * 16 * a.ldg + (int64_t)sc * 128);
       const u32x4_t q0 = q[0], q1 = q[1];
       dst[i][0] = q0.x; dst[i][1] = q0.y; dst[i][2] = q0.z; dst[i][3] = q0.w;
       dst[i][4] = q1.x; dst[i][5] = q1.y; dst[i][6] = q1.z; dst[i][7] = q1.w;
@@ -366,25 +369,27 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
   // half's buffer, digit column c of tile q for lane group g' at step 4 hh + w4 (global step
   // numbering); tail-tile columns past 3 r8 repeat a live one.  The tile of a piece is uniform
   // across the workgroup, so dead tiles cost a scalar branch.
-  const int64_t hlast = 2 * s1 - 1;
+  const int hlast = 2 * s1 - 1;
   const int wg = threadIdx.x >> 4, cc16 = threadIdx.x & 15;
   i32x4_t stg[NDT];
-  auto bload = [&](int64_t hh) {
+  auto bload = [&](int hh) {
     hh = hh <= hlast ? hh : hlast;
-    const int64_t brow = (4 * (4 * hh + (wg >> 2)) + (wg & 3)) * a.ndc;
+    const int64_t brow = (int64_t)(4 * (4 * hh + (wg >> 2)) + (wg & 3)) * a.ndc;
 #pragma unroll
     for (int q = 0; q < 3 * NVG; ++q)
       if (q < 3 * nv) stg[q] = a.B[brow + 16 * (3 * vt0 + q) + cc16];
     if (TAIL && tail) stg[NDT - 1] = a.B[brow + 48 * a.nvt + cc16 % (3 * a.r8)];
   };
-  auto bstore = [&](int64_t hh) {
+  auto bstore = [&](int hh) {
     i32x4_t* dst = &sb[hh & 1][threadIdx.x];
 #pragma unroll
     for (int q = 0; q < 3 * NVG; ++q)
       if (q < 3 * nv) dst[256 * q] = stg[q];
     if (TAIL && tail) dst[256 * (NDT - 1)] = stg[NDT - 1];
   };
-  auto step = [&](const uint32_t (&cc)[RT][8], int w, int buf, int64_t s) {
+  // lm: bit j set when digit tile j runs its MFMAs in this super-chunk (uniform; tested as bits of
+  // a scalar integer, so that no per-tile condition is carried in vector registers)
+  auto step = [&](const uint32_t (&cc)[RT][8], int w, int buf, uint32_t lm) {
     i32x4_t av[RT];
 #pragma unroll
     for (int i = 0; i < RT; ++i) {
@@ -399,14 +404,12 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
     // the read overlaps the previous tile's MFMAs (reads of dead tiles stay inside the buffer and
     // are not used)
     i32x4_t bcur = bp[0];
-    if (a.prio == 1) __builtin_amdgcn_s_setprio(1);
-    else if (a.prio == 2) __builtin_amdgcn_s_setprio(2);
+    if (a.prio) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int j = 0; j < NDT; ++j) {
       const i32x4_t bv = bcur;
       if (j + 1 < NDT) bcur = bp[256 * (j + 1)];
-      const bool on = (TAIL && j == NDT - 1) ? tail : j < 3 * nv;
-      if (on && s < lim[j]) {
+      if ((lm >> j) & 1u) {
 #pragma unroll
         for (int i = 0; i < RT; ++i)
           acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[i], bv, acc[i][j], 0, 0, 0);
@@ -424,20 +427,25 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
   // half H of super-chunk s from the codes cc; its first half requests super-chunk s + 1 into nx.
   // The buffer of half hh + 1 is written after every wave has finished half hh - 1 (the barrier
   // that ended it), so one barrier per half suffices.
-  auto half = [&](auto hconst, const uint32_t (&cc)[RT][8], uint32_t (&nx)[RT][8], int64_t s) {
+  auto half = [&](auto hconst, const uint32_t (&cc)[RT][8], uint32_t (&nx)[RT][8], int s) {
     constexpr int H = decltype(hconst)::value;
+    uint32_t lm = 0;
+#pragma unroll
+    for (int j = 0; j < NDT; ++j)
+      if (live(j) && s < lim[j]) lm |= 1u << j;
+    lm = __builtin_amdgcn_readfirstlane(lm);
     bload(2 * s + H + 1);
     if constexpr (H == 0) ld(s + 1, nx);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int w4 = 0; w4 < 4; ++w4) step(cc, 4 * H + w4, H, s);
+    for (int w4 = 0; w4 < 4; ++w4) step(cc, 4 * H + w4, H, lm);
     __builtin_amdgcn_sched_barrier(0);
     bstore(2 * s + H + 1);
     __syncthreads();
   };
   using H0 = std::integral_constant<int, 0>;
   using H1 = std::integral_constant<int, 1>;
-  for (int64_t s = s0; s < s1; s += 2) {
+  for (int s = s0; s < s1; s += 2) {
     half(H0{}, cA, cB, s);
     half(H1{}, cA, cB, s);
     if (s + 1 < s1) {
@@ -459,6 +467,168 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
     for (int jv = 0; jv < NVG; ++jv) {
       const int k = 16 * (vt0 + jv) + r;
       if (jv >= nv || k >= a.t) continue;
+      const double scale = (double)a.sc[k];
+      float o[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        o[q] = (float)((double)((int64_t)acc[i][3 * jv][q] * 65536 + (int64_t)acc[i][3 * jv + 1][q] * 256 +
+                                (int64_t)acc[i][3 * jv + 2][q]) * scale);
+      put(k, row, o);
+    }
+    if (TAIL && tail) {
+      const int k = 16 * a.nvt + r / 3;
+      const bool first = r % 3 == 0 && r < 3 * a.r8;
+      const double scale = first ? (double)a.sc[k] : 0.0;
+      float o[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int v0 = acc[i][NDT - 1][q];
+        const int v1 = __shfl(v0, lane + 1);
+        const int v2 = __shfl(v0, lane + 2);
+        o[q] = (float)((double)((int64_t)v0 * 65536 + (int64_t)v1 * 256 + (int64_t)v2) * scale);
+      }
+      if (first) put(k, row, o);
+    }
+  }
+}
+
+// s_waitcnt immediate (gfx9 encoding) for "at most n vector-memory operations outstanding", the
+// other counters unconstrained
+constexpr int vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
+
+// Wide blocks, second form, for blocks whose column groups all hold NVG whole vector tiles and
+// no per-tile limits (the batched traits and blocks of an odd number of vector tiles keep the
+// first form): the tiling, digit layout and epilogue of k_bp_i8w, with
+//  - the digits of each half super-chunk loaded straight into LDS (global_load_lds_dwordx4: each
+//    lane's 16 bytes land at the wave's base + 16 lane, which is the staging layout), so that no
+//    registers stage them and no store pass is needed; the two half buffers are separate LDS
+//    objects, so that the reads of one never wait for the loads into the other, and each half ends
+//    with an explicit wait for its own loads before the barrier;
+//  - each step run row tile by row tile: row tile i's MFMAs on all digit tiles (their digits held
+//    in registers, each refilled with the next step's right after its last use), while row tile
+//    i + 1 (or the next step's row tile 0) is decoded between them, so that a wave issues its
+//    MFMAs back to back even when the other wave of its SIMD waits, with one decoded row tile live
+//    beside the one being decoded; only the tail tile's MFMAs sit behind a (uniform) branch;
+//  - the next super-chunk's codes requested half a super-chunk ahead (during the first half only
+//    the current codes hold registers).
+template <int RT, int NVG, bool TAIL = true>
+__global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8x(I8WArgs a) {
+  constexpr int NDT = 3 * NVG + (TAIL ? 1 : 0);  // digit tiles of a group; with TAIL the last is the tail tile
+  __shared__ i32x4_t sb0[NDT * 256];  // [tile j][4 w4 + g][column] of even halves
+  __shared__ i32x4_t sb1[NDT * 256];  // the same for odd halves
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t bx = blockIdx.x;
+  const int by = blockIdx.y;
+  const int64_t row0 = bx * (64 * RT) + wave * (16 * RT);
+  const int vt0 = by * NVG;
+  const bool tail = TAIL && a.r8 > 0 && by == (int)gridDim.y - 1;  // the group holds the tail tile
+  const int s0 = (int)blockIdx.z * (int)a.ns_per_split;
+  const int s1 = min((int)a.nsuper, s0 + (int)a.ns_per_split);
+  i32x4_t acc[RT][NDT];
+#pragma unroll
+  for (int i = 0; i < RT; ++i)
+#pragma unroll
+    for (int j = 0; j < NDT; ++j) acc[i][j] = i32x4_t{0, 0, 0, 0};
+  const uint8_t* lbase = a.G + (row0 + r) * a.ldg + g * 32;
+  auto ld = [&](int sc, uint32_t (&dst)[RT][8]) __attribute__((always_inline)) {
+    sc = sc < s1 ? sc : s1 - 1;  // past the slice: re-read its last super-chunk (never used)
+#pragma unroll
+    for (int i = 0; i < RT; ++i) {
+      const u32x4_t* q = reinterpret_cast<const u32x4_t*>(lbase + (int64_t)i * 16 * a.ldg + (int64_t)sc * 128);
+      const u32x4_t q0 = q[0], q1 = q[1];
+      dst[i][0] = q0.x; dst[i][1] = q0.y; dst[i][2] = q0.z; dst[i][3] = q0.w;
+      dst[i][4] = q1.x; dst[i][5] = q1.y; dst[i][6] = q1.z; dst[i][7] = q1.w;
+    }
+  };
+  const int hlast = 2 * s1 - 1;
+  const int wg = threadIdx.x >> 4, cc16 = threadIdx.x & 15;
+  // digits of half hh into the buffer of its parity: thread piece q is entry 256 q + threadIdx.x
+  // (tail-tile columns past 3 r8 repeat a live one)
+  auto bload = [&](auto bufc, int hh) __attribute__((always_inline)) {
+    constexpr int BUF = decltype(bufc)::value;
+    i32x4_t* dst = (BUF == 0 ? sb0 : sb1) + 64 * wave;
+    hh = hh <= hlast ? hh : hlast;
+    const i32x4_t* src = a.B + (int64_t)(4 * (4 * hh + (wg >> 2)) + (wg & 3)) * a.ndc + cc16;
+#pragma unroll
+    for (int q = 0; q < 3 * NVG; ++q) __builtin_amdgcn_global_load_lds(src + 16 * (3 * vt0 + q), dst + 256 * q, 16, 0, 0);
+    if (TAIL && tail)
+      __builtin_amdgcn_global_load_lds(src - cc16 + 48 * a.nvt + cc16 % (3 * a.r8), dst + 256 * (NDT - 1), 16, 0, 0);
+  };
+  const uint32_t tab = 0x02010100u;
+  auto decode1 = [&](uint32_t W) __attribute__((always_inline)) {
+    return i32x4_t{(int)__builtin_amdgcn_perm(0u, tab, W & 0x03030303u),
+                   (int)__builtin_amdgcn_perm(0u, tab, (W >> 2) & 0x03030303u),
+                   (int)__builtin_amdgcn_perm(0u, tab, (W >> 4) & 0x03030303u),
+                   (int)__builtin_amdgcn_perm(0u, tab, (W >> 6) & 0x03030303u)};
+  };
+  uint32_t cA[RT][8], cB[RT][8];
+  i32x4_t avc;  // the next row tile to multiply, decoded
+  if (s0 < s1) {
+    bload(std::integral_constant<int, 0>{}, 2 * s0);
+    ld(s0, cA);
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+    __syncthreads();
+    avc = decode1(cA[0][0]);
+  }
+  // half H of super-chunk s from the codes cc; its second half requests super-chunk s + 1 into nx.
+  // The buffer of half hh + 1 is written after every wave has finished half hh - 1 (the barrier
+  // that ended it), so one barrier per half suffices.
+  auto half = [&](auto hconst, const uint32_t (&cc)[RT][8], uint32_t (&nx)[RT][8], int s)
+      __attribute__((always_inline)) {
+    constexpr int H = decltype(hconst)::value;
+    bload(std::integral_constant<int, 1 - H>{}, 2 * s + H + 1);
+    if constexpr (H == 1) ld(s + 1, nx);
+    const i32x4_t* bp = (H == 0 ? sb0 : sb1) + g * 16 + r;  // step w4 at + 64 w4, tile j at + 256 j
+    i32x4_t B[NDT];
+#pragma unroll
+    for (int j = 0; j < NDT; ++j) B[j] = bp[256 * j];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int w4 = 0; w4 < 4; ++w4) {
+      const int w = 4 * H + w4;
+      if (a.prio) __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+      for (int i = 0; i < RT; ++i) {
+        const uint32_t nwd = i + 1 < RT ? cc[i + 1][w] : (w + 1 < 8 ? cc[0][w + 1] : nx[0][0]);
+        const i32x4_t avn = decode1(nwd);
+#pragma unroll
+        for (int j = 0; j < NDT; ++j) {
+          if (!(TAIL && j == NDT - 1) || tail)
+            acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(avc, B[j], acc[i][j], 0, 0, 0);
+          if (i == RT - 1 && w4 < 3) B[j] = bp[256 * j + 64 * (w4 + 1)];
+        }
+        avc = avn;
+        __builtin_amdgcn_sched_barrier(0);  // one row tile per scheduling region
+      }
+      if (a.prio) __builtin_amdgcn_s_setprio(0);
+    }
+    // this half's digit loads done (the second half's code loads, issued after them, may still be
+    // in flight), then the barrier that hands the buffer to every wave
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(H == 1 ? 2 * RT : 0));
+    __syncthreads();
+  };
+  using H0 = std::integral_constant<int, 0>;
+  using H1 = std::integral_constant<int, 1>;
+  for (int s = s0; s < s1; s += 2) {
+    half(H0{}, cA, cB, s);
+    half(H1{}, cA, cB, s);
+    if (s + 1 < s1) {
+      half(H0{}, cB, cA, s + 1);
+      half(H1{}, cB, cA, s + 1);
+    }
+  }
+  auto put = [&](int k, int64_t row, const float (&o)[4]) {
+    float* base = a.part + (blockIdx.z * (int64_t)a.KP + k) * a.rows_pad + row;
+    *reinterpret_cast<float4*>(base) = make_float4(o[0], o[1], o[2], o[3]);
+  };
+#pragma unroll
+  for (int i = 0; i < RT; ++i) {
+    const int64_t row = row0 + 16 * i + 4 * g;
+#pragma unroll
+    for (int jv = 0; jv < NVG; ++jv) {
+      const int k = 16 * (vt0 + jv) + r;
+      if (k >= a.t) continue;
       const double scale = (double)a.sc[k];
       float o[4];
 #pragma unroll
@@ -606,15 +776,27 @@ void launch_bp_i8(const I8Launch& l, hipStream_t st) {
     a.xcd_pairs = ex && ex[0] == '1';
     // a wave issuing a step's MFMAs runs at issue priority 1 (SLEMM_I8_PRIO, read per call: 0 off):
     // the MFMA stream then keeps ahead of the other wave's decode and LDS phase (K=65 operator step
-    // 20.4 -> 20.1 ms; priority 2 the same)
+    // 20.4 -> 20.1 ms)
     const char* ep = std::getenv("SLEMM_I8_PRIO");
-    a.prio = ep ? std::atoi(ep) : 1;
+    a.prio = ep && ep[0] == '0' ? 0 : 1;
+    if (a.nsuper >= INT32_MAX / 2) fail(SLEMM_ERR_ARG, "int8 block: reduction too long");
     // (one-vector-tile blocks measured slower with 2 row tiles per wave at four waves per SIMD:
     // each digit load then serves half as many MFMAs)
     const dim3 grid((unsigned)(l.rows_pad / (64 * kI8WRT)), (unsigned)i8_groups(l.t), (unsigned)l.nsplit);
     const char* e3 = std::getenv("SLEMM_I8_ONE_TILE");  // "0": the general kernel (read per call)
-    if (dl.nvt == 1 && dl.r8 == 0 && !(e3 && e3[0] == '0')) k_bp_i8w<kI8WRT, 1, false><<<grid, 256, 0, st>>>(a);
-    else k_bp_i8w<kI8WRT, kI8NVG><<<grid, 256, 0, st>>>(a);
+    // the second form takes groups of whole vector tiles without tile limits and without lo columns
+    // (SLEMM_I8_WIDE_FORM=1: the first form for every block, read per call)
+    const char* e4 = std::getenv("SLEMM_I8_WIDE_FORM");
+    const bool second_ok = !a.tile_nsuper && !a.tile_rows && !a.xcd_pairs && a.tail_r == 0 &&
+                           (dl.nvt % (dl.nvt == 1 && dl.r8 == 0 ? 1 : kI8NVG)) == 0;
+    const bool first_form = (e4 && e4[0] == '1') || !second_ok;
+    if (dl.nvt == 1 && dl.r8 == 0 && !(e3 && e3[0] == '0')) {
+      if (first_form) k_bp_i8w<kI8WRT, 1, false><<<grid, 256, 0, st>>>(a);
+      else k_bp_i8x<kI8WRT, 1, false><<<grid, 256, 0, st>>>(a);
+    } else {
+      if (first_form) k_bp_i8w<kI8WRT, kI8NVG><<<grid, 256, 0, st>>>(a);
+      else k_bp_i8x<kI8WRT, kI8NVG><<<grid, 256, 0, st>>>(a);
+    }
   }
   HIPCHECK(hipGetLastError());
 }
