@@ -509,8 +509,8 @@ constexpr int vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) |
 //    i + 1 (or the next step's row tile 0) is decoded between them, so that a wave issues its
 //    MFMAs back to back even when the other wave of its SIMD waits, with one decoded row tile live
 //    beside the one being decoded; only the tail tile's MFMAs sit behind a (uniform) branch;
-//  - the next super-chunk's codes requested half a super-chunk ahead (during the first half only
-//    the current codes hold registers).
+//  - codes loaded per half super-chunk into a ring of three, each requested two halves ahead of
+//    its use (the registers of one whole super-chunk ahead, twice the lead).
 template <int RT, int NVG, bool TAIL = true>
 __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8x(I8WArgs a) {
   constexpr int NDT = 3 * NVG + (TAIL ? 1 : 0);  // digit tiles of a group; with TAIL the last is the tail tile
@@ -518,8 +518,22 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8x(I8WArgs a) {
   __shared__ i32x4_t sb1[NDT * 256];  // the same for odd halves
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = lane & 15, g = lane >> 4;
-  const int64_t bx = blockIdx.x;
-  const int by = blockIdx.y;
+  // (row block bx, column group by), remapped with xcd_pairs as in k_bp_i8w
+  int64_t bx = blockIdx.x;
+  int by = blockIdx.y;
+  {
+    const int64_t gx = gridDim.x, gy = gridDim.y, gx8 = gx - gx % 8;
+    const int64_t L = blockIdx.x + gx * blockIdx.y;
+    if (a.xcd_pairs && gy > 1 && L < gx8 * gy) {
+      const int64_t slot = L / 8;
+      bx = (slot / gy) * 8 + L % 8;
+      by = (int)(slot % gy);
+    } else if (a.xcd_pairs && gy > 1) {
+      const int64_t L2 = L - gx8 * gy, rem = gx - gx8;
+      bx = gx8 + L2 % rem;
+      by = (int)(L2 / rem);
+    }
+  }
   const int64_t row0 = bx * (64 * RT) + wave * (16 * RT);
   const int vt0 = by * NVG;
   const bool tail = TAIL && a.r8 > 0 && by == (int)gridDim.y - 1;  // the group holds the tail tile
@@ -531,17 +545,18 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8x(I8WArgs a) {
 #pragma unroll
     for (int j = 0; j < NDT; ++j) acc[i][j] = i32x4_t{0, 0, 0, 0};
   const uint8_t* lbase = a.G + (row0 + r) * a.ldg + g * 32;
-  auto ld = [&](int sc, uint32_t (&dst)[RT][8]) __attribute__((always_inline)) {
-    sc = sc < s1 ? sc : s1 - 1;  // past the slice: re-read its last super-chunk (never used)
+  const int hlast = 2 * s1 - 1;
+  // codes of half super-chunk hh (one 16-byte load per row tile): dst[i][w4] is the code word of
+  // step w4 of that half; past the slice the last half is re-read (never used)
+  auto ldh = [&](int hh, uint32_t (&dst)[RT][4]) __attribute__((always_inline)) {
+    hh = hh <= hlast ? hh : hlast;
 #pragma unroll
     for (int i = 0; i < RT; ++i) {
-      const u32x4_t* q = reinterpret_cast<const u32x4_t*>(lbase + (int64_t)i * 16 * a.ldg + (int64_t)sc * 128);
-      const u32x4_t q0 = q[0], q1 = q[1];
-      dst[i][0] = q0.x; dst[i][1] = q0.y; dst[i][2] = q0.z; dst[i][3] = q0.w;
-      dst[i][4] = q1.x; dst[i][5] = q1.y; dst[i][6] = q1.z; dst[i][7] = q1.w;
+      const u32x4_t q = *reinterpret_cast<const u32x4_t*>(lbase + (int64_t)i * 16 * a.ldg + (int64_t)(hh >> 1) * 128 +
+                                                          (hh & 1) * 16);
+      dst[i][0] = q.x; dst[i][1] = q.y; dst[i][2] = q.z; dst[i][3] = q.w;
     }
   };
-  const int hlast = 2 * s1 - 1;
   const int wg = threadIdx.x >> 4, cc16 = threadIdx.x & 15;
   // digits of half hh into the buffer of its parity: thread piece q is entry 256 q + threadIdx.x
   // (tail-tile columns past 3 r8 repeat a live one)
@@ -562,23 +577,26 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8x(I8WArgs a) {
                    (int)__builtin_amdgcn_perm(0u, tab, (W >> 4) & 0x03030303u),
                    (int)__builtin_amdgcn_perm(0u, tab, (W >> 6) & 0x03030303u)};
   };
-  uint32_t cA[RT][8], cB[RT][8];
+  // codes in a ring of three half super-chunks, each requested two halves ahead of its use
+  uint32_t c0[RT][4], c1[RT][4], c2[RT][4];
   i32x4_t avc;  // the next row tile to multiply, decoded
   if (s0 < s1) {
     bload(std::integral_constant<int, 0>{}, 2 * s0);
-    ld(s0, cA);
+    ldh(2 * s0, c0);
+    ldh(2 * s0 + 1, c1);
     __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
     __syncthreads();
-    avc = decode1(cA[0][0]);
+    avc = decode1(c0[0][0]);
   }
-  // half H of super-chunk s from the codes cc; its second half requests super-chunk s + 1 into nx.
-  // The buffer of half hh + 1 is written after every wave has finished half hh - 1 (the barrier
-  // that ended it), so one barrier per half suffices.
-  auto half = [&](auto hconst, const uint32_t (&cc)[RT][8], uint32_t (&nx)[RT][8], int s)
-      __attribute__((always_inline)) {
+  // half hh (of parity H) from its codes cc, the next half's codes cn (whose first step the last
+  // step here decodes); it requests the codes of half hh + 2 into cl and the digits of half hh + 1
+  // into the other buffer.  That buffer is written after every wave has finished half hh - 1 (the
+  // barrier that ended it), so one barrier per half suffices.
+  auto half = [&](auto hconst, int hh, const uint32_t (&cc)[RT][4], const uint32_t (&cn)[RT][4],
+                  uint32_t (&cl)[RT][4]) __attribute__((always_inline)) {
     constexpr int H = decltype(hconst)::value;
-    bload(std::integral_constant<int, 1 - H>{}, 2 * s + H + 1);
-    if constexpr (H == 1) ld(s + 1, nx);
+    bload(std::integral_constant<int, 1 - H>{}, hh + 1);
+    ldh(hh + 2, cl);
     const i32x4_t* bp = (H == 0 ? sb0 : sb1) + g * 16 + r;  // step w4 at + 64 w4, tile j at + 256 j
     i32x4_t B[NDT];
 #pragma unroll
@@ -586,36 +604,52 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8x(I8WArgs a) {
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int w4 = 0; w4 < 4; ++w4) {
-      const int w = 4 * H + w4;
       if (a.prio) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int i = 0; i < RT; ++i) {
-        const uint32_t nwd = i + 1 < RT ? cc[i + 1][w] : (w + 1 < 8 ? cc[0][w + 1] : nx[0][0]);
+        const uint32_t nwd = i + 1 < RT ? cc[i + 1][w4] : (w4 + 1 < 4 ? cc[0][w4 + 1] : cn[0][0]);
         const i32x4_t avn = decode1(nwd);
+        // the tail tile first (behind its branch), then the other tiles in one block, where the
+        // decode's 11 VALU are interleaved with the MFMAs: a wave issues in order, so VALU placed
+        // after all the MFMAs would wait behind each of them
+        if (TAIL && tail) acc[i][NDT - 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(avc, B[NDT - 1], acc[i][NDT - 1], 0, 0, 0);
+        if (TAIL && i == RT - 1 && w4 < 3) B[NDT - 1] = bp[256 * (NDT - 1) + 64 * (w4 + 1)];
+        constexpr int NF = 3 * NVG;  // full digit tiles
 #pragma unroll
-        for (int j = 0; j < NDT; ++j) {
-          if (!(TAIL && j == NDT - 1) || tail)
-            acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(avc, B[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < NF; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(avc, B[j], acc[i][j], 0, 0, 0);
           if (i == RT - 1 && w4 < 3) B[j] = bp[256 * j + 64 * (w4 + 1)];
         }
         avc = avn;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                  // MFMA
+          if (i == RT - 1 && w4 < 3) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);       // next step's digits
+          if (j < 11 - NF + 1) __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);             // decode VALU
+          else __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);
+        }
         __builtin_amdgcn_sched_barrier(0);  // one row tile per scheduling region
       }
       if (a.prio) __builtin_amdgcn_s_setprio(0);
     }
-    // this half's digit loads done (the second half's code loads, issued after them, may still be
+    // this half's digit loads done (the code loads of half hh + 2, issued after them, may still be
     // in flight), then the barrier that hands the buffer to every wave
-    __builtin_amdgcn_s_waitcnt(vmcnt_imm(H == 1 ? 2 * RT : 0));
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(RT));
     __syncthreads();
   };
   using H0 = std::integral_constant<int, 0>;
   using H1 = std::integral_constant<int, 1>;
-  for (int s = s0; s < s1; s += 2) {
-    half(H0{}, cA, cB, s);
-    half(H1{}, cA, cB, s);
-    if (s + 1 < s1) {
-      half(H0{}, cB, cA, s + 1);
-      half(H1{}, cB, cA, s + 1);
+  const int hend = 2 * s1;  // halves [2 s0, hend), an even number
+  for (int hh = 2 * s0; hh < hend; hh += 6) {
+    half(H0{}, hh, c0, c1, c2);
+    half(H1{}, hh + 1, c1, c2, c0);
+    if (hh + 2 < hend) {
+      half(H0{}, hh + 2, c2, c0, c1);
+      half(H1{}, hh + 3, c0, c1, c2);
+    }
+    if (hh + 4 < hend) {
+      half(H0{}, hh + 4, c1, c2, c0);
+      half(H1{}, hh + 5, c2, c0, c1);
     }
   }
   auto put = [&](int k, int64_t row, const float (&o)[4]) {
@@ -769,27 +803,28 @@ void launch_bp_i8(const I8Launch& l, hipStream_t st) {
     a.ndc = dl.ndc; a.nvt = dl.nvt; a.r8 = dl.r8; a.t = l.t; a.sc = sc; a.part = l.part; a.KP = l.KP;
     a.rows_pad = l.rows_pad; a.tail0 = l.tail0; a.tail_r = l.tail_r;
     a.tile_nsuper = l.tile_nsuper; a.tile_rows = l.tile_rows;
-    // SLEMM_I8_XCD=1 pairs the column groups of a row block on one XCD (read per call).  At
-    // n = 100,000 x m = 500,000 it cut the K=65 block's fetched bytes by 8-14% only and its time
-    // not at all (those blocks are MFMA-bound), so the plain dispatch order is the default.
+    // the second form takes groups of whole vector tiles without tile limits and without lo columns
+    // (SLEMM_I8_WIDE_FORM=1: the first form for every block, read per call)
+    const char* e4 = std::getenv("SLEMM_I8_WIDE_FORM");
+    const bool second_ok = !a.tile_nsuper && !a.tile_rows && a.tail_r == 0 &&
+                           (dl.nvt % (dl.nvt == 1 && dl.r8 == 0 ? 1 : kI8NVG)) == 0;
+    const bool first_form = (e4 && e4[0] == '1') || !second_ok;
+    // SLEMM_I8_XCD (0/1, read per call): the column groups of a row block dispatched back to back on
+    // one XCD, so that all but the first read the codes from that XCD's L2.  SLEMM_I8_PRIO (0/1):
+    // issue priority 1 while a wave issues a step's MFMAs.  Defaults as measured at n = 100,000 x
+    // m = 500,000, K = 65: the second form is fastest with the pairing and without the priority
+    // (operator 16.94 -> 15.77 ms; pairing alone 16.61, no priority alone 16.93); the first form
+    // with the priority and without the pairing (20.4 -> 20.1 ms from the priority; the pairing cut
+    // 8-14% of its fetched bytes and no time).
     const char* ex = std::getenv("SLEMM_I8_XCD");
-    a.xcd_pairs = ex && ex[0] == '1';
-    // a wave issuing a step's MFMAs runs at issue priority 1 (SLEMM_I8_PRIO, read per call: 0 off):
-    // the MFMA stream then keeps ahead of the other wave's decode and LDS phase (K=65 operator step
-    // 20.4 -> 20.1 ms)
+    a.xcd_pairs = ex ? ex[0] == '1' : !first_form;
     const char* ep = std::getenv("SLEMM_I8_PRIO");
-    a.prio = ep && ep[0] == '0' ? 0 : 1;
+    a.prio = ep ? (ep[0] == '1' ? 1 : 0) : (first_form ? 1 : 0);
     if (a.nsuper >= INT32_MAX / 2) fail(SLEMM_ERR_ARG, "int8 block: reduction too long");
     // (one-vector-tile blocks measured slower with 2 row tiles per wave at four waves per SIMD:
     // each digit load then serves half as many MFMAs)
     const dim3 grid((unsigned)(l.rows_pad / (64 * kI8WRT)), (unsigned)i8_groups(l.t), (unsigned)l.nsplit);
     const char* e3 = std::getenv("SLEMM_I8_ONE_TILE");  // "0": the general kernel (read per call)
-    // the second form takes groups of whole vector tiles without tile limits and without lo columns
-    // (SLEMM_I8_WIDE_FORM=1: the first form for every block, read per call)
-    const char* e4 = std::getenv("SLEMM_I8_WIDE_FORM");
-    const bool second_ok = !a.tile_nsuper && !a.tile_rows && !a.xcd_pairs && a.tail_r == 0 &&
-                           (dl.nvt % (dl.nvt == 1 && dl.r8 == 0 ? 1 : kI8NVG)) == 0;
-    const bool first_form = (e4 && e4[0] == '1') || !second_ok;
     if (dl.nvt == 1 && dl.r8 == 0 && !(e3 && e3[0] == '0')) {
       if (first_form) k_bp_i8w<kI8WRT, 1, false><<<grid, 256, 0, st>>>(a);
       else k_bp_i8x<kI8WRT, 1, false><<<grid, 256, 0, st>>>(a);
