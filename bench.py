@@ -47,7 +47,7 @@ def measured_traffic(n, m, K, kernel_substr, timeout=300, env_extra=None):
     """HBM bytes per launch of the dominant block-product kernel, measured live: two child runs
     of this script's --only-op mode under `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`
     (one counter per run), each scaled by what the same counter reads for a 1 GiB device copy in
-    the same run (--copy-calibration).  Returns {pass1, pass2, fetch_factor, write_factor} in bytes,
+    the same run (--copy-calibration); kernel_substr: "|"-separated names of the kernel.  Returns {pass1, pass2, fetch_factor, write_factor} in bytes,
     or {"error": ...} when the profiler is missing or fails (then the line reports traffic null)."""
     import glob
     import shutil
@@ -78,7 +78,7 @@ def measured_traffic(n, m, K, kernel_substr, timeout=300, env_extra=None):
         finally:
             shutil.rmtree(d, ignore_errors=True)
         copies = [v for _, kn, v in rows if "copyBuffer" in kn]
-        kern = [v for _, kn, v in rows if kernel_substr in kn]
+        kern = [v for _, kn, v in rows if any(k in kn for k in kernel_substr.split("|"))]
         if not copies or len(kern) < 2:
             return {"error": f"{counter}: copy or block-product dispatches missing"}
         factor = float(1 << 30) / (max(copies) * 1024.0)  # the counter reads KiB
@@ -751,25 +751,33 @@ def traits_run(n, m, T, nv, seed=9, fp32=True, random_subsets=False):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     tr = gpu.Traits(g, masks)
+    torch.cuda.synchronize()
+    out["traits_create_s"] = round(time.perf_counter() - t0, 3)
     plan, ratio = tr.plan(nv)
+    t0 = time.perf_counter()
     ra = tr.SLDF_REML(Y, n_V=nv)  # the library's automatic plan
     torch.cuda.synchronize()
     out["traits_api_auto"] = {"plan": plan, "work_ratio_batched_over_one_at_a_time": round(ratio, 3),
                               "seconds": round(time.perf_counter() - t0, 3),
                               "seconds_seed": round(float(ra["seconds_seed"]), 3)}
-    if plan == "batched":
-        r = ra
-        out["batched_bf16x2"] = dict(out["traits_api_auto"])
-    else:
-        tr.set_plan("batched")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        r = tr.SLDF_REML(Y, n_V=nv)
-        torch.cuda.synchronize()
-        out["batched_bf16x2"] = {"seconds": round(time.perf_counter() - t0, 3),
-                                 "seconds_seed": round(float(r["seconds_seed"]), 3)}
-    out["batched_bf16x2"]["lanczos_steps_max"] = int(r["steps"].max())
+    # both plans of the library, timed alike (the other one run explicitly)
+    plans = {plan: (out["traits_api_auto"]["seconds"], ra)}
+    other = "sequential" if plan == "batched" else "batched"
+    tr.set_plan(other)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ro = tr.SLDF_REML(Y, n_V=nv)
+    torch.cuda.synchronize()
+    plans[other] = (round(time.perf_counter() - t0, 3), ro)
+    tr.set_plan("auto")
+    r = plans["batched"][1]
+    out["batched_bf16x2"] = {"seconds": plans["batched"][0], "seconds_seed": round(float(r["seconds_seed"]), 3)}
+    out["one_at_a_time_bf16x2"] = {"seconds": plans["sequential"][0],
+                                   "seconds_seed": round(float(plans["sequential"][1]["seconds_seed"]), 3),
+                                   "note": "the library's plan: each trait compacted on the device, then SLDF"}
     out["h2_max_abs_diff_auto_vs_batched"] = float(np.max(np.abs(ra["h2"] - r["h2"])))
+    out["h2_max_abs_diff_plans"] = float(np.max(np.abs(plans["batched"][1]["h2"] - plans["sequential"][1]["h2"])))
+    out["batched_bf16x2"]["lanczos_steps_max"] = int(r["steps"].max())
     tr.close()
     seq = {}
     for prec in (("bf16x2", "fp32") if fp32 else ("bf16x2",)):
@@ -788,11 +796,15 @@ def traits_run(n, m, T, nv, seed=9, fp32=True, random_subsets=False):
             h2[t] = rt["h2"]
             gk.close()
         seq[prec] = h2
-        out[f"sequential_{prec}"] = {"seconds": round(tot, 3)}
+        out[f"sequential_{prec}"] = {"seconds": round(tot, 3),
+                                     "note": "per-subset handles built outside the timing, one SLDF each"}
     out["speedup_batched_over_sequential_bf16x2"] = round(out["sequential_bf16x2"]["seconds"] /
                                                          out["batched_bf16x2"]["seconds"], 2)
     out["speedup_auto_over_sequential_bf16x2"] = round(out["sequential_bf16x2"]["seconds"] /
                                                       out["traits_api_auto"]["seconds"], 2)
+    out["speedup_auto_over_other_plan"] = round(max(out["batched_bf16x2"]["seconds"],
+                                                    out["one_at_a_time_bf16x2"]["seconds"]) /
+                                                out["traits_api_auto"]["seconds"], 2)
     if fp32:
         out["speedup_batched_over_sequential_fp32"] = round(out["sequential_fp32"]["seconds"] /
                                                            out["batched_bf16x2"]["seconds"], 2)
@@ -965,7 +977,7 @@ def main():
 
     ceiling = None if args.no_ceiling else gemm_ceiling()
     measured_ceiling = mfma_ceiling_summary(lib, L)
-    traffic = None if args.no_traffic else measured_traffic(n, m, K, "k_bp_i8w" if i8 else "k_bp_",
+    traffic = None if args.no_traffic else measured_traffic(n, m, K, "k_bp_i8w|k_bp_i8x" if i8 else "k_bp_",
                                                                env_extra=i8_target_env(tune))
     dom_pass = "pass2" if k2 >= k1 else "pass1"
     traffic_bytes = (traffic or {}).get(dom_pass)
@@ -1007,7 +1019,7 @@ def main():
         "small_k_passes": smallk,
         "roofline": ({
             "bound": "mfma",
-            "kernel": f"int8 wide block product (k_bp_i8w, three fixed-point digits per vector), {dom}",
+            "kernel": f"int8 wide block product (k_bp_i8x, three fixed-point digits per vector), {dom}",
             "achieved": round(ach_tf, 2),
             "peak": INT8_PEAK_TOPS,
             "peak_kind": "dense int8 MFMA, 5.0 POP/s; achieved counts the product's 2 n m K operations",

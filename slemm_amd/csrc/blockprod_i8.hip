@@ -496,9 +496,9 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8w(I8WArgs a) {
 // other counters unconstrained
 constexpr int vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
 
-// Wide blocks, second form, for blocks whose column groups all hold NVG whole vector tiles and
-// no per-tile limits (the batched traits and blocks of an odd number of vector tiles keep the
-// first form): the tiling, digit layout and epilogue of k_bp_i8w, with
+// Wide blocks, second form, for blocks whose column groups all hold NVG whole vector tiles (blocks
+// of an odd number of vector tiles keep the first form): the tiling, digit layout and epilogue of
+// k_bp_i8w, with per-group rather than per-tile limits for the batched traits, and with
 //  - the digits of each half super-chunk loaded straight into LDS (global_load_lds_dwordx4: each
 //    lane's 16 bytes land at the wave's base + 16 lane, which is the staging layout), so that no
 //    registers stage them and no store pass is needed; the two half buffers are separate LDS
@@ -538,7 +538,23 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8x(I8WArgs a) {
   const int vt0 = by * NVG;
   const bool tail = TAIL && a.r8 > 0 && by == (int)gridDim.y - 1;  // the group holds the tail tile
   const int s0 = (int)blockIdx.z * (int)a.ns_per_split;
-  const int s1 = min((int)a.nsuper, s0 + (int)a.ns_per_split);
+  int s1 = min((int)a.nsuper, s0 + (int)a.ns_per_split);
+  if (a.tile_nsuper || a.tile_rows) {
+    // batched traits: the group's limits, the largest over its tiles.  The operand is zero past
+    // each tile's own reduction limit and the epilogues mask the rows outside every subset, so the
+    // limits only skip work, and a group's tiles (one trait's probes, as a rule) share them.
+    const int64_t wg_row0 = bx * (64 * RT);
+    int32_t gl = 0;
+#pragma unroll
+    for (int jv = 0; jv <= NVG; ++jv) {
+      if (jv == NVG && !tail) continue;
+      const int vtile = jv < NVG ? vt0 + jv : a.nvt;
+      int32_t ln = a.tile_nsuper ? a.tile_nsuper[vtile] : INT32_MAX;
+      if (a.tile_rows && wg_row0 >= a.tile_rows[vtile]) ln = 0;
+      gl = max(gl, ln);
+    }
+    s1 = min(s1, (int)gl);
+  }
   i32x4_t acc[RT][NDT];
 #pragma unroll
   for (int i = 0; i < RT; ++i)
@@ -803,10 +819,10 @@ void launch_bp_i8(const I8Launch& l, hipStream_t st) {
     a.ndc = dl.ndc; a.nvt = dl.nvt; a.r8 = dl.r8; a.t = l.t; a.sc = sc; a.part = l.part; a.KP = l.KP;
     a.rows_pad = l.rows_pad; a.tail0 = l.tail0; a.tail_r = l.tail_r;
     a.tile_nsuper = l.tile_nsuper; a.tile_rows = l.tile_rows;
-    // the second form takes groups of whole vector tiles without tile limits and without lo columns
-    // (SLEMM_I8_WIDE_FORM=1: the first form for every block, read per call)
+    // the second form takes groups of whole vector tiles without lo columns (SLEMM_I8_WIDE_FORM=1:
+    // the first form for every block, read per call)
     const char* e4 = std::getenv("SLEMM_I8_WIDE_FORM");
-    const bool second_ok = !a.tile_nsuper && !a.tile_rows && a.tail_r == 0 &&
+    const bool second_ok = a.tail_r == 0 &&
                            (dl.nvt % (dl.nvt == 1 && dl.r8 == 0 ? 1 : kI8NVG)) == 0;
     const bool first_form = (e4 && e4[0] == '1') || !second_ok;
     // SLEMM_I8_XCD (0/1, read per call): the column groups of a row block dispatched back to back on
