@@ -501,9 +501,10 @@ constexpr int vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) |
 // k_bp_i8w, with per-group rather than per-tile limits for the batched traits, and with
 //  - the digits of each half super-chunk loaded straight into LDS (global_load_lds_dwordx4: each
 //    lane's 16 bytes land at the wave's base + 16 lane, which is the staging layout), so that no
-//    registers stage them and no store pass is needed; the two half buffers are separate LDS
-//    objects, so that the reads of one never wait for the loads into the other, and each half ends
-//    with an explicit wait for its own loads before the barrier;
+//    registers stage them and no store pass is needed; the full tiles two halves ahead in a ring
+//    of three buffers, the tail tile one half ahead in two, all separate LDS objects, so that the
+//    reads of one never wait for the loads into another, and each half ends with an explicit
+//    wait for the next half's digits before the barrier;
 //  - each step run row tile by row tile: row tile i's MFMAs on all digit tiles (their digits held
 //    in registers, each refilled with the next step's right after its last use), while row tile
 //    i + 1 (or the next step's row tile 0) is decoded between them, so that a wave issues its
@@ -514,8 +515,13 @@ constexpr int vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) |
 template <int RT, int NVG, bool TAIL = true>
 __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8x(I8WArgs a) {
   constexpr int NDT = 3 * NVG + (TAIL ? 1 : 0);  // digit tiles of a group; with TAIL the last is the tail tile
-  __shared__ i32x4_t sb0[NDT * 256];  // [tile j][4 w4 + g][column] of even halves
-  __shared__ i32x4_t sb1[NDT * 256];  // the same for odd halves
+  constexpr int NF = 3 * NVG;                    // full digit tiles
+  // digit staging, [tile j][4 w4 + g][column] per half super-chunk: the full tiles in a ring of
+  // three buffers (loaded two halves ahead), the tail tile in two (one half ahead); five separate
+  // LDS objects, so that reads of one never wait for loads into another.  80 KB with the tail tile,
+  // which two workgroups per CU still fit.
+  __shared__ i32x4_t fb0[NF * 256], fb1[NF * 256], fb2[NF * 256];
+  __shared__ i32x4_t tb0[TAIL ? 256 : 1], tb1[TAIL ? 256 : 1];
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = lane & 15, g = lane >> 4;
   // (row block bx, column group by), remapped with xcd_pairs as in k_bp_i8w
@@ -574,17 +580,27 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8x(I8WArgs a) {
     }
   };
   const int wg = threadIdx.x >> 4, cc16 = threadIdx.x & 15;
-  // digits of half hh into the buffer of its parity: thread piece q is entry 256 q + threadIdx.x
-  // (tail-tile columns past 3 r8 repeat a live one)
-  auto bload = [&](auto bufc, int hh) __attribute__((always_inline)) {
-    constexpr int BUF = decltype(bufc)::value;
-    i32x4_t* dst = (BUF == 0 ? sb0 : sb1) + 64 * wave;
+  // digits of half hh: thread piece q is entry 256 q + threadIdx.x of the buffer (tail-tile
+  // columns past 3 r8 repeat a live one)
+  auto brow = [&](int hh) __attribute__((always_inline)) {
     hh = hh <= hlast ? hh : hlast;
-    const i32x4_t* src = a.B + (int64_t)(4 * (4 * hh + (wg >> 2)) + (wg & 3)) * a.ndc + cc16;
+    return a.B + (int64_t)(4 * (4 * hh + (wg >> 2)) + (wg & 3)) * a.ndc;
+  };
+  auto fbuf = [&](auto fc) __attribute__((always_inline)) {
+    constexpr int F = decltype(fc)::value;
+    return F == 0 ? fb0 : (F == 1 ? fb1 : fb2);
+  };
+  auto load_full = [&](auto fc, int hh) __attribute__((always_inline)) {
+    i32x4_t* dst = fbuf(fc) + 64 * wave;
+    const i32x4_t* src = brow(hh) + cc16;
 #pragma unroll
-    for (int q = 0; q < 3 * NVG; ++q) __builtin_amdgcn_global_load_lds(src + 16 * (3 * vt0 + q), dst + 256 * q, 16, 0, 0);
-    if (TAIL && tail)
-      __builtin_amdgcn_global_load_lds(src - cc16 + 48 * a.nvt + cc16 % (3 * a.r8), dst + 256 * (NDT - 1), 16, 0, 0);
+    for (int q = 0; q < NF; ++q) __builtin_amdgcn_global_load_lds(src + 16 * (3 * vt0 + q), dst + 256 * q, 16, 0, 0);
+  };
+  auto tbuf = [&](auto tc) __attribute__((always_inline)) { return decltype(tc)::value == 0 ? tb0 : tb1; };
+  auto load_tail = [&](auto tc, int hh) __attribute__((always_inline)) {
+    i32x4_t* dst = tbuf(tc) + 64 * wave;
+    const i32x4_t* src = brow(hh) + 48 * a.nvt + cc16 % (3 * a.r8);
+    if (TAIL && tail) __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
   };
   const uint32_t tab = 0x02010100u;
   auto decode1 = [&](uint32_t W) __attribute__((always_inline)) {
@@ -597,26 +613,32 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8x(I8WArgs a) {
   uint32_t c0[RT][4], c1[RT][4], c2[RT][4];
   i32x4_t avc;  // the next row tile to multiply, decoded
   if (s0 < s1) {
-    bload(std::integral_constant<int, 0>{}, 2 * s0);
+    load_tail(std::integral_constant<int, 0>{}, 2 * s0);
+    load_full(std::integral_constant<int, 0>{}, 2 * s0);
+    load_full(std::integral_constant<int, 1>{}, 2 * s0 + 1);
     ldh(2 * s0, c0);
     ldh(2 * s0 + 1, c1);
     __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
     __syncthreads();
     avc = decode1(c0[0][0]);
   }
-  // half hh (of parity H) from its codes cc, the next half's codes cn (whose first step the last
-  // step here decodes); it requests the codes of half hh + 2 into cl and the digits of half hh + 1
-  // into the other buffer.  That buffer is written after every wave has finished half hh - 1 (the
-  // barrier that ended it), so one barrier per half suffices.
-  auto half = [&](auto hconst, int hh, const uint32_t (&cc)[RT][4], const uint32_t (&cn)[RT][4],
+  // half hh (full-tile buffer F = hh mod 3, tail buffer T = hh mod 2, counting from 2 s0) from
+  // its codes cc and the next half's codes cn (whose first step the last step here decodes).  It
+  // requests, in this order, the tail digits of half hh + 1, the full-tile digits of half hh + 2
+  // and the codes of half hh + 2 into cl; each buffer written was last read in half hh - 1, which
+  // every wave finished at the barrier that ended it, so one barrier per half suffices.
+  auto half = [&](auto fc, auto tc, int hh, const uint32_t (&cc)[RT][4], const uint32_t (&cn)[RT][4],
                   uint32_t (&cl)[RT][4]) __attribute__((always_inline)) {
-    constexpr int H = decltype(hconst)::value;
-    bload(std::integral_constant<int, 1 - H>{}, hh + 1);
+    constexpr int F = decltype(fc)::value, T = decltype(tc)::value;
+    load_tail(std::integral_constant<int, 1 - T>{}, hh + 1);
+    load_full(std::integral_constant<int, (F + 2) % 3>{}, hh + 2);
     ldh(hh + 2, cl);
-    const i32x4_t* bp = (H == 0 ? sb0 : sb1) + g * 16 + r;  // step w4 at + 64 w4, tile j at + 256 j
+    const i32x4_t* bp = fbuf(fc) + g * 16 + r;                   // step w4 at + 64 w4, tile j at + 256 j
+    const i32x4_t* tp = tbuf(tc) + g * 16 + r;                   // the tail tile's buffer
     i32x4_t B[NDT];
 #pragma unroll
-    for (int j = 0; j < NDT; ++j) B[j] = bp[256 * j];
+    for (int j = 0; j < NF; ++j) B[j] = bp[256 * j];
+    if (TAIL) B[NDT - 1] = tp[0];
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int w4 = 0; w4 < 4; ++w4) {
@@ -629,8 +651,7 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8x(I8WArgs a) {
         // decode's 11 VALU are interleaved with the MFMAs: a wave issues in order, so VALU placed
         // after all the MFMAs would wait behind each of them
         if (TAIL && tail) acc[i][NDT - 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(avc, B[NDT - 1], acc[i][NDT - 1], 0, 0, 0);
-        if (TAIL && i == RT - 1 && w4 < 3) B[NDT - 1] = bp[256 * (NDT - 1) + 64 * (w4 + 1)];
-        constexpr int NF = 3 * NVG;  // full digit tiles
+        if (TAIL && i == RT - 1 && w4 < 3) B[NDT - 1] = tp[64 * (w4 + 1)];
 #pragma unroll
         for (int j = 0; j < NF; ++j) {
           acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(avc, B[j], acc[i][j], 0, 0, 0);
@@ -648,24 +669,26 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8x(I8WArgs a) {
       }
       if (a.prio) __builtin_amdgcn_s_setprio(0);
     }
-    // this half's digit loads done (the code loads of half hh + 2, issued after them, may still be
-    // in flight), then the barrier that hands the buffer to every wave
-    __builtin_amdgcn_s_waitcnt(vmcnt_imm(RT));
+    // the digits of half hh + 1 are in (its full tiles were requested a half earlier, its tail
+    // tile first thing here; only this half's full-tile and code requests, issued after, may still
+    // be in flight), then the barrier that hands them to every wave
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(NF + RT));
     __syncthreads();
   };
-  using H0 = std::integral_constant<int, 0>;
-  using H1 = std::integral_constant<int, 1>;
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
   const int hend = 2 * s1;  // halves [2 s0, hend), an even number
   for (int hh = 2 * s0; hh < hend; hh += 6) {
-    half(H0{}, hh, c0, c1, c2);
-    half(H1{}, hh + 1, c1, c2, c0);
+    half(I0{}, I0{}, hh, c0, c1, c2);
+    half(I1{}, I1{}, hh + 1, c1, c2, c0);
     if (hh + 2 < hend) {
-      half(H0{}, hh + 2, c2, c0, c1);
-      half(H1{}, hh + 3, c0, c1, c2);
+      half(I2{}, I0{}, hh + 2, c2, c0, c1);
+      half(I0{}, I1{}, hh + 3, c0, c1, c2);
     }
     if (hh + 4 < hend) {
-      half(H0{}, hh + 4, c1, c2, c0);
-      half(H1{}, hh + 5, c2, c0, c1);
+      half(I1{}, I0{}, hh + 4, c1, c2, c0);
+      half(I2{}, I1{}, hh + 5, c2, c0, c1);
     }
   }
   auto put = [&](int k, int64_t row, const float (&o)[4]) {
