@@ -324,6 +324,18 @@ def small_k_passes(g, lib, L, n, m, ks=(1, 3, 5, 7, 16), reps=3):
     gb = m * ((n + 3) // 4) + m * 8 + n * 8
     out["gebv_fp64"] = {"ms": round(dt * 1e3, 3), "GB/s": round(gb / dt / 1e9, 1),
                         "hbm_frac": round(gb / dt / 1e9 / HBM_PEAK_GBS, 3)}
+    exact = g.gebv(alpha)
+    g.gebv_fast(alpha)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fast = g.gebv_fast(alpha)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    out["gebv_fast"] = {"ms": round(dt * 1e3, 3), "GB/s": round(gb / dt / 1e9, 1),
+                        "hbm_frac": round(gb / dt / 1e9 / HBM_PEAK_GBS, 3),
+                        "max_rel_err_vs_exact": float((fast - exact).abs().max() / exact.abs().max()),
+                        "how": "one pass of the 1-vector block product (int8 narrow kernel) + FP64 constant"}
     return out
 
 
@@ -690,6 +702,15 @@ def scale_run(n, m, nv, n_causal, h2, seed=5, cpu=True):
     out["gebv"] = {"ms": round(dt * 1e3, 3),
                    "GB/s": round((m * ((n + 3) // 4) + 8 * (m + n)) / dt / 1e9, 1),
                    "corr_with_true_genetic_value": round(float(np.corrcoef(gebv.cpu().numpy(), gv)[0, 1]), 4)}
+    g.gebv_fast(alpha)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fast = g.gebv_fast(alpha)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out["gebv_fast"] = {"ms": round(dt * 1e3, 3),
+                        "GB/s": round((m * ((n + 3) // 4) + 8 * (m + n)) / dt / 1e9, 1),
+                        "max_rel_err_vs_exact": float((fast - gebv).abs().max() / gebv.abs().max())}
 
     Py = torch.from_numpy(r["Py"]).cuda()
     g.gwas_score(Py)

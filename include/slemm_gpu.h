@@ -177,6 +177,13 @@ int slemm_gpu_z(slemm_geno* h, const float* U, int64_t ldu, int t, float* out, i
  * alpha device (m doubles), flip host or NULL (m bytes).  FP64 accumulation. */
 int slemm_gpu_gebv(slemm_geno* h, const double* alpha, const uint8_t* flip, double* gebv,
                    void* stream);
+/* The same GEBV at FP32 accuracy in one pass over the individual-major codes (no reference
+ * counterpart: a caller that needs the scores to ~1e-6 relative rather than the reference's
+ * summation order bit for bit): Z u with u_j = +-alpha_j on the block-product kernels (the int8
+ * narrow kernel on the default path), plus 2 sum of alpha over the flipped SNPs in FP64.  Same
+ * arguments as slemm_gpu_gebv; uses the handle's workspaces. */
+int slemm_gpu_gebv_fast(slemm_geno* h, const double* alpha, const uint8_t* flip, double* gebv,
+                        void* stream);
 
 /* GRAMMAR-gamma score terms of bin/slemm_gamma.py:306-319 for every SNP j (device outputs,
  * m floats each): xy_j = x_c' Py, xx_j = || (I-QQ') R^-1/2 x_c ||^2, with x_c the mean-

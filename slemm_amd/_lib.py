@@ -56,6 +56,7 @@ SIGNATURES = {
     "slemm_gpu_zt": (C.c_int, [c_vp, c_vp, c_i64, C.c_int, c_vp, c_i64, C.c_int, C.c_int, C.c_int, c_vp]),
     "slemm_gpu_z": (C.c_int, [c_vp, c_vp, c_i64, C.c_int, c_vp, c_i64, C.c_int, c_vp]),
     "slemm_gpu_gebv": (C.c_int, [c_vp, c_vp, c_u8p, c_vp, c_vp]),
+    "slemm_gpu_gebv_fast": (C.c_int, [c_vp, c_vp, c_u8p, c_vp, c_vp]),
     "slemm_gpu_gwas_score": (C.c_int, [c_vp, c_vp, C.c_float, c_vp, c_vp, c_vp]),
     "slemm_gpu_lseed": (C.c_int, [c_vp, C.c_int, c_intp, c_intp, C.c_float, c_vp, c_i64, C.c_int, C.c_float,
                                   C.c_int, C.c_int, C.c_int, C.c_int, c_vp, C.POINTER(c_vp)]),

@@ -858,6 +858,77 @@ extern "C" int slemm_gpu_z(slemm_geno* h, const float* U, int64_t ldu, int t, fl
   });
 }
 
+// Operand of the fast GEBV: u_j = -alpha_j for a flipped SNP, alpha_j otherwise (FP32; zero past
+// m), and per block the FP64 sum of 2 alpha_j over the flipped SNPs (2 - z = 2 + (-1) z).
+static __global__ void k_gebv_operand(const double* __restrict__ alpha, const uint8_t* __restrict__ flip,
+                                      int64_t m, int64_t m_pad, float* __restrict__ u, double* __restrict__ part) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < m_pad; j += (int64_t)gridDim.x * 256) {
+    const double aj = j < m ? alpha[j] : 0.0;
+    const bool f = j < m && flip && flip[j];
+    u[j] = (float)(f ? -aj : aj);
+    if (f) acc += 2.0 * aj;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// gebv_i = (Zc u)_i + sum_j gm_j (represented u)_j + sum of the flipped 2 alpha_j, in FP64
+static __global__ void k_gebv_finish(const float* __restrict__ zc, int64_t n, const double* __restrict__ gm_u,
+                                     const double* __restrict__ part, int nblk, double* __restrict__ gebv) {
+  __shared__ double c;
+  if (threadIdx.x == 0) {
+    double s = gm_u[0];
+    for (int b = 0; b < nblk; ++b) s += part[b];
+    c = s;
+  }
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    gebv[i] = (double)zc[i] + c;
+}
+
+// Fast GEBV (no reference counterpart; src/pred.cpp:112-130 is what slemm_gpu_gebv reproduces bit
+// for bit): with s_j = -1 for a flipped SNP, z -> 2 - z is 2 + s_j z, so GEBV = Z u + 2 sum_flipped
+// alpha with u_j = s_j alpha_j.  Z u is one pass-2 product of the 1-vector block on the block
+// kernels (FP32 operand; on the default path int8 digits of its 23-bit rendering with exact
+// sums), returned centred, Zc u = Z u - sum_j gm_j u_j: adding back the centring sum the kernels
+// used (of the operand as represented) gives Z u of that operand.  Accuracy: FP32 rendering of
+// alpha and one FP32 rounding of Zc u, against one HBM pass instead of the reference order's
+// dependent FP64 chain per individual.
+extern "C" int slemm_gpu_gebv_fast(slemm_geno* h, const double* alpha, const uint8_t* flip, double* gebv,
+                                   void* stream) {
+  return guard([&] {
+    REQUIRE(h && alpha && gebv, "null argument");
+    HIPCHECK(hipSetDevice(h->device));
+    hipStream_t st = (hipStream_t)stream;
+    DevBuf du, dflip, dpart, dzc;
+    constexpr int nblk = 256;
+    du.reserve(sizeof(float) * h->m_pad);
+    dpart.reserve(sizeof(double) * nblk);
+    dzc.reserve(sizeof(float) * h->n);
+    if (flip) {
+      dflip.reserve(h->m);
+      HIPCHECK(hipMemcpyAsync(dflip.p, flip, h->m, hipMemcpyHostToDevice, st));
+    }
+    k_gebv_operand<<<nblk, 256, 0, st>>>(alpha, dflip.as<uint8_t>(), h->m, h->m_pad, du.as<float>(),
+                                         dpart.as<double>());
+    HIPCHECK(hipGetLastError());
+    const int rc = slemm_gpu_z(h, du.as<float>(), h->m, 1, dzc.as<float>(), h->n, PREC_BF16X2, stream);
+    if (rc != SLEMM_OK) fail(rc, slemm_gpu_last_error());
+    const Layout L = layout_for(h, 1, PREC_BF16X2);
+    k_gebv_finish<<<(unsigned)std::min<int64_t>(4096, (h->n + 255) / 256), 256, 0, st>>>(
+        dzc.as<float>(), h->n, h->colsum.as<double>() + L.KP, dpart.as<double>(), nblk, gebv);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+  });
+}
+
 extern "C" int slemm_gpu_gebv(slemm_geno* h, const double* alpha, const uint8_t* flip, double* gebv,
                               void* stream) {
   return guard([&] {

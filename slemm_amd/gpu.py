@@ -260,6 +260,16 @@ class Geno:
         L.check(L.load().slemm_gpu_gebv(self._h, _ptr(a), _np_ptr(fl, C.c_uint8), _ptr(out), _stream()))
         return out
 
+    def gebv_fast(self, alpha: torch.Tensor, flip: Optional[np.ndarray] = None) -> torch.Tensor:
+        """The same GEBV at FP32 accuracy in one pass on the block-product kernels."""
+        if alpha.dtype != torch.float64 or not alpha.is_cuda or alpha.numel() != self.m:
+            raise TypeError("alpha must be a float64 CUDA tensor of length m")
+        a = alpha.contiguous()
+        out = torch.empty(self.n, dtype=torch.float64, device=a.device)
+        fl = None if flip is None else np.ascontiguousarray(flip, dtype=np.uint8)
+        L.check(L.load().slemm_gpu_gebv_fast(self._h, _ptr(a), _np_ptr(fl, C.c_uint8), _ptr(out), _stream()))
+        return out
+
     def adjoint(self, A: torch.Tensor, B: torch.Tensor, coef, scale_rr: bool = True,
                 precision="bf16x2") -> torch.Tensor:
         """out_j = sum_k coef_k (Zc'A)_jk (Zc'B)_jk (rows of A, B scaled by r_rsqrt first when

@@ -60,6 +60,14 @@ def test_c2_gebv_block_bit_exact_and_checksum(c2):
     got = g.gebv(torch.from_numpy(alpha).cuda()).cpu().numpy()
     want = float(np.dot(alpha, pc.astype(np.float64)))
     assert abs(got.sum() - want) <= 1e-9 * np.abs(alpha).dot(pc)
+    # the one-pass GEBV over all 500,000 SNPs, with flips, against the exact FP64 scores
+    flip = (np.random.default_rng(3).random(m) < 0.3).astype(np.uint8)
+    a = torch.from_numpy(alpha).cuda()
+    exact = g.gebv(a, flip).cpu().numpy()
+    fast = g.gebv_fast(a, flip).cpu().numpy()
+    err = float(np.max(np.abs(fast - exact)) / np.max(np.abs(exact)))
+    print(f"C2 fast GEBV: max |fast - exact| / max |exact| = {err:.2e}")
+    assert err < 2e-6
 
 
 def test_c2_operator_k65_symmetry_and_fp32_agreement(c2):
@@ -144,6 +152,13 @@ def test_c5_gebv_block_bit_exact(gpu):
     g = gpu.Geno.synthesize(n, m, seed=7, maf_lo=0.01, maf_hi=0.5, missing=0.0)
     try:
         _gebv_block_check(g, n, m, 40_001, 300, seed=5)
+        alpha = np.random.default_rng(6).normal(size=m)
+        a = torch.from_numpy(alpha).cuda()
+        exact = g.gebv(a).cpu().numpy()
+        fast = g.gebv_fast(a).cpu().numpy()
+        err = float(np.max(np.abs(fast - exact)) / np.max(np.abs(exact)))
+        print(f"C5 fast GEBV: max |fast - exact| / max |exact| = {err:.2e}")
+        assert err < 2e-6
     finally:
         g.close()
 
