@@ -453,7 +453,10 @@ int slemm_gpu_timing(int enable);
 /* Matrix-core issue ceiling (no reference counterpart): a register-only loop of `chains`
  * independent accumulators per wave (1, 2, 4 or 8), `waves_per_simd` waves on every SIMD, `iters`
  * MFMAs per chain, of instruction instr (0 v_mfma_i32_16x16x64_i8, 1 v_mfma_i32_32x32x32_i8,
- * 2 v_mfma_f32_16x16x32_bf16, 3 v_mfma_f32_32x32x16_bf16) on the current device.  Host outputs:
+ * 2 v_mfma_f32_16x16x32_bf16, 3 v_mfma_f32_32x32x16_bf16; instr + 4: the same instruction with
+ * per-chain random operands -- int8 dosages 0/1/2 against full-range digits, BF16 values with
+ * random mantissas -- whose bit switching draws the power of real data) on the current device.
+ * Host outputs:
  * tops (tera-operations per second, 2 per multiply-add), clock_ratio (shader-clock ticks per
  * 100 MHz real-time tick over the loop of one wave: the clock in units of 100 MHz; may be NULL), ms
  * (kernel time; may be NULL). */

@@ -3,7 +3,12 @@
 // reference counterpart (instrumentation, like the kernel timing of blockprod.hip).
 //
 // Every lane holds operands derived from its own index (so nothing folds) and `chains`
-// independent accumulators; each loop iteration issues one MFMA per chain.  A grid of
+// independent accumulators; each loop iteration issues one MFMA per chain.  Two operand sets: the
+// same small operands for every chain, or (instruction ids + 4) the block products' kind of data
+// -- per chain its own full-range random B operand (int8 digits; BF16 values with random
+// mantissas), the A operand (dosages 0/1/2 for int8) shared by four chains -- so that, as in the
+// block products, consecutive MFMAs see different operand bits; switching bits costs power, and
+// under a power limit the clock, so this set gives the ceiling for real data.  A grid of
 // 256 x waves_per_simd workgroups of four waves puts waves_per_simd waves on every SIMD of the
 // 256 CUs.  Wave 0 of workgroup 0 reads the shader clock counter (s_memtime) and the 100 MHz
 // real-time counter (s_memrealtime) before and after its loop, which gives the clock the matrix
@@ -45,7 +50,25 @@ __device__ __forceinline__ bf16x8_t bf16_operand(uint32_t seed) {
   return v;
 }
 
-template <int I, int NACC>
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// four random bytes, each 0, 1 or 2 (genotype dosages)
+__device__ __forceinline__ int dosage_word(uint32_t seed) {
+  const uint32_t h = mix32(seed);
+  uint32_t w = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) w |= (((h >> (8 * b)) & 0xffu) % 3u) << (8 * b);
+  return (int)w;
+}
+
+template <int I, int NACC, bool VARY>
 __global__ __launch_bounds__(256) void k_mfma_probe(int iters, long long* clocks, float* sink) {
   using T = typename Acc<I>::T;
   T acc[NACC];
@@ -56,9 +79,28 @@ __global__ __launch_bounds__(256) void k_mfma_probe(int iters, long long* clocks
     acc[j][0] = j + 1;
     acc[j][1] = (int)(tid & 7);
   }
-  const i32x4_t ia = {(int)((tid & 3) + 1), (int)((tid >> 2) & 3), 1, (int)(tid & 1)};
-  const i32x4_t ib = {1, (int)((tid >> 1) & 3), (int)((tid & 1) + 1), 2};
-  const bf16x8_t fa = bf16_operand(tid), fb = bf16_operand(tid ^ 0x5bd1u);
+  constexpr int NA = VARY ? (NACC + 3) / 4 : 1, NB = VARY ? NACC : 1;
+  i32x4_t ia[NA], ib[NB];
+  bf16x8_t fa[NA], fb[NB];
+  if constexpr (VARY) {
+#pragma unroll
+    for (int j = 0; j < NA; ++j) {
+      ia[j] = i32x4_t{dosage_word(tid * 64 + j * 4), dosage_word(tid * 64 + j * 4 + 1),
+                      dosage_word(tid * 64 + j * 4 + 2), dosage_word(tid * 64 + j * 4 + 3)};
+      fa[j] = bf16_operand(mix32(tid * 64 + j));
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      ib[j] = i32x4_t{(int)mix32(tid * 97 + j * 4 + 11), (int)mix32(tid * 97 + j * 4 + 12),
+                      (int)mix32(tid * 97 + j * 4 + 13), (int)mix32(tid * 97 + j * 4 + 14)};
+      fb[j] = bf16_operand(mix32(tid * 97 + j + 0x5bd1u));
+    }
+  } else {
+    ia[0] = i32x4_t{(int)((tid & 3) + 1), (int)((tid >> 2) & 3), 1, (int)(tid & 1)};
+    ib[0] = i32x4_t{1, (int)((tid >> 1) & 3), (int)((tid & 1) + 1), 2};
+    fa[0] = bf16_operand(tid);
+    fb[0] = bf16_operand(tid ^ 0x5bd1u);
+  }
   const bool timer = blockIdx.x == 0 && threadIdx.x == 0;
   long long t0 = 0, r0 = 0;
   if (timer) {
@@ -68,10 +110,11 @@ __global__ __launch_bounds__(256) void k_mfma_probe(int iters, long long* clocks
   for (int it = 0; it < iters; ++it) {
 #pragma unroll
     for (int j = 0; j < NACC; ++j) {
-      if constexpr (I == I8_16x16x64) acc[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ia, ib, acc[j], 0, 0, 0);
-      else if constexpr (I == I8_32x32x32) acc[j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ia, ib, acc[j], 0, 0, 0);
-      else if constexpr (I == BF16_16x16x32) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc[j], 0, 0, 0);
-      else acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, acc[j], 0, 0, 0);
+      const int ja = VARY ? j / 4 : 0, jb = VARY ? j : 0;
+      if constexpr (I == I8_16x16x64) acc[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ia[ja], ib[jb], acc[j], 0, 0, 0);
+      else if constexpr (I == I8_32x32x32) acc[j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ia[ja], ib[jb], acc[j], 0, 0, 0);
+      else if constexpr (I == BF16_16x16x32) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ja], fb[jb], acc[j], 0, 0, 0);
+      else acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[ja], fb[jb], acc[j], 0, 0, 0);
     }
   }
   if (timer) {
@@ -87,13 +130,23 @@ __global__ __launch_bounds__(256) void k_mfma_probe(int iters, long long* clocks
   sink[tid] = s;
 }
 
-template <int I>
+template <int I, bool VARY>
 void launch_probe(int chains, dim3 grid, int iters, long long* clocks, float* sink, hipStream_t st) {
   switch (chains) {
-    case 1: k_mfma_probe<I, 1><<<grid, 256, 0, st>>>(iters, clocks, sink); break;
-    case 2: k_mfma_probe<I, 2><<<grid, 256, 0, st>>>(iters, clocks, sink); break;
-    case 4: k_mfma_probe<I, 4><<<grid, 256, 0, st>>>(iters, clocks, sink); break;
-    default: k_mfma_probe<I, 8><<<grid, 256, 0, st>>>(iters, clocks, sink); break;
+    case 1: k_mfma_probe<I, 1, VARY><<<grid, 256, 0, st>>>(iters, clocks, sink); break;
+    case 2: k_mfma_probe<I, 2, VARY><<<grid, 256, 0, st>>>(iters, clocks, sink); break;
+    case 4: k_mfma_probe<I, 4, VARY><<<grid, 256, 0, st>>>(iters, clocks, sink); break;
+    default: k_mfma_probe<I, 8, VARY><<<grid, 256, 0, st>>>(iters, clocks, sink); break;
+  }
+}
+
+template <bool VARY>
+void launch_probe_instr(int instr, int chains, dim3 grid, int iters, long long* clocks, float* sink) {
+  switch (instr) {
+    case 0: launch_probe<I8_16x16x64, VARY>(chains, grid, iters, clocks, sink, nullptr); break;
+    case 1: launch_probe<I8_32x32x32, VARY>(chains, grid, iters, clocks, sink, nullptr); break;
+    case 2: launch_probe<BF16_16x16x32, VARY>(chains, grid, iters, clocks, sink, nullptr); break;
+    default: launch_probe<BF16_32x32x16, VARY>(chains, grid, iters, clocks, sink, nullptr); break;
   }
 }
 
@@ -105,7 +158,8 @@ using namespace slemm;
 extern "C" int slemm_gpu_mfma_ceiling(int instr, int chains, int waves_per_simd, int iters, double* tops,
                                       double* clock_ratio, double* ms) {
   return guard([&] {
-    REQUIRE(instr >= 0 && instr <= 3, "instr: 0 i8 16x16x64, 1 i8 32x32x32, 2 bf16 16x16x32, 3 bf16 32x32x16");
+    REQUIRE(instr >= 0 && instr <= 7,
+            "instr: 0 i8 16x16x64, 1 i8 32x32x32, 2 bf16 16x16x32, 3 bf16 32x32x16; + 4 with random operands");
     REQUIRE(chains == 1 || chains == 2 || chains == 4 || chains == 8, "chains must be 1, 2, 4 or 8");
     REQUIRE(waves_per_simd >= 1 && waves_per_simd <= 8, "waves_per_simd must be in [1, 8]");
     REQUIRE(iters >= 1 && iters <= (1 << 20), "iters must be in [1, 2^20]");
@@ -121,12 +175,8 @@ extern "C" int slemm_gpu_mfma_ceiling(int instr, int chains, int waves_per_simd,
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
     auto run = [&](int it) {
-      switch (instr) {
-        case 0: launch_probe<I8_16x16x64>(chains, grid, it, clk.as<long long>(), sink.as<float>(), nullptr); break;
-        case 1: launch_probe<I8_32x32x32>(chains, grid, it, clk.as<long long>(), sink.as<float>(), nullptr); break;
-        case 2: launch_probe<BF16_16x16x32>(chains, grid, it, clk.as<long long>(), sink.as<float>(), nullptr); break;
-        default: launch_probe<BF16_32x32x16>(chains, grid, it, clk.as<long long>(), sink.as<float>(), nullptr); break;
-      }
+      if (instr < 4) launch_probe_instr<false>(instr, chains, grid, it, clk.as<long long>(), sink.as<float>());
+      else launch_probe_instr<true>(instr - 4, chains, grid, it, clk.as<long long>(), sink.as<float>());
       HIPCHECK(hipGetLastError());
     };
     run(std::max(1, iters / 10));  // warm up (clocks ramp, code object loaded)
@@ -140,7 +190,7 @@ extern "C" int slemm_gpu_mfma_ceiling(int instr, int chains, int waves_per_simd,
     HIPCHECK(hipMemcpy(c, clk.p, sizeof(c), hipMemcpyDeviceToHost));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    const double ops = kOps[instr] * chains * (double)iters * 4.0 * grid.x;
+    const double ops = kOps[instr % 4] * chains * (double)iters * 4.0 * grid.x;
     *tops = ops / (t * 1e-3) / 1e12;
     if (ms) *ms = t;
     // shader-clock ticks per 100 MHz tick: the core clock in units of 100 MHz
