@@ -221,6 +221,55 @@ def lmc_reml(D: "Dense", y, u, e, s2max: float = 0.7, tol: float = 0.002, max_it
     return dict(h2=float(h2), vg=vg, ve=vg * (1 - h2) / h2, Py=Py, evals=evals, f=fv)
 
 
+def lmc_fit(Z, gm, gw, rvec, X, y, rng_seed=0, n_MC=None, s2max: float = 0.7, tol: float = 0.002,
+            max_iter: int = 9):
+    """Every output of LMC_REML (src/reml.cpp:730-882) in FP64 with exact solves, over the float MC
+    draws of the reference's stream (mt19937(seed + 54321), src/reml.cpp:788-801): h2, vg, ve and
+    Py from lmc_reml; snp_blup = gw Zc' Py (:866); blue = (X'X)^-1 X' (y - Zc snp_blup - tau1 R Py)
+    (:870-877).  Q is the FP64 orthonormal basis of R^-1/2 X.  n_MC defaults to the reference's
+    rule max(min(4e9/n^2, 30), 3) (src/option.cpp:414)."""
+    import oracle
+    y = np.asarray(y, np.float64)
+    n = len(y)
+    m = np.asarray(gm).shape[0]
+    if n_MC is None:
+        n_MC = max(min(int(4e9 / n / n), 30), 3)
+    rv = np.ones(n) if rvec is None else np.asarray(rvec, np.float64)
+    rr = 1.0 / np.sqrt(rv)
+    X = np.asarray(X, np.float64).reshape(n, -1)
+    Q, R = np.linalg.qr(rr[:, None] * X)
+    if np.min(np.abs(np.diag(R))) <= 1e-6 * np.max(np.abs(np.diag(R))):
+        raise ValueError("Fixed-effects design matrix is not of full rank.")
+    D = Dense(Z, gm, gw, rr, Q)
+    u, e = oracle.lmc_samples(rng_seed, n, m, n_MC)
+    r = lmc_reml(D, y, u, e, s2max=s2max, tol=tol, max_iter=max_iter)
+    Py = r["Py"]
+    snp_blup = D.rmatvec(Py[:, None], weighted=True)[:, 0]
+    tau1 = (1 - r["h2"]) / r["h2"]
+    resid = y - (D.matvec(snp_blup) + tau1 * rv * Py)
+    blue = np.linalg.solve(X.T @ X, X.T @ resid)
+    return dict(r, snp_blup=snp_blup, blue=blue, n_MC=n_MC)
+
+
+def iter_weighting_reml(kmat, gm, rvec, X, y, maf, persnp=None, window_size=20, sig_chisq=1e6, indep_r2=0.5,
+                        rng_seed=0, n_MC=None, s2max: float = 0.7, tol: float = 0.002, max_iter: int = 9):
+    """--iter_weighting (src/option.cpp:417-528) with LMC fits in FP64 (lmc_fit): the restated
+    driver of oracle/reml.py (float per-SNP variances, clumping, window-mean logistic weights as
+    the reference computes them) over exact fits on the same MC draws."""
+    import oracle
+    from oracle import reml as oreml
+    Z = dosage_of_kmat(kmat)
+
+    def fit(sldf, pw, Xf):
+        if sldf:
+            raise ValueError("the FP64 driver restates the LMC fits (lrt = False)")
+        return lmc_fit(Z, gm, oracle.gw(gm, pw), rvec, Xf, y, rng_seed, n_MC, s2max, tol, max_iter)
+
+    return oreml.iter_weighting_reml(kmat, gm, rvec, X, y, maf, persnp=persnp, window_size=window_size,
+                                     sig_chisq=sig_chisq, indep_r2=indep_r2, lrt=False, rng_seed=rng_seed,
+                                     n_MC=n_MC, s2max=s2max, fit=fit)
+
+
 def lmm_gstat_fake(D: "Dense", Py, tau1: float, V0):
     """Quadratic forms of the --fake_geno branch (src/reml.cpp:900-917) in FP64 with exact solves,
     for uncentred fake dosages V0 (n x k): the columns centred by their float mean (one float

@@ -198,21 +198,24 @@ def LMC_REML(kmat, gm, gw, rvec, X, y, rng_seed=0, n_MC=None, tol_L=5e-4, s2max=
 
 
 def iter_weighting_reml(kmat, gm, rvec, X, y, maf, persnp=None, window_size=20, sig_chisq=1e6, indep_r2=0.5,
-                        lrt=False, rng_seed=0, n_V=30, n_MC=None, tol_L=5e-4, s2max=0.7):
+                        lrt=False, rng_seed=0, n_V=30, n_MC=None, tol_L=5e-4, s2max=0.7, fit=None):
     """--iter_weighting (src/option.cpp:417-528) over the restated REML drivers: equal-weight fit,
     2 maf (1-maf) b^2 normalised, clumping by the largest variance above sig_chisq with the
     indep_r2 residual rule, window-mean logistic weights, refit with the clumped SNPs as
-    covariates.  Returns the final fit's dict plus persnp and sig_snps."""
+    covariates.  Returns the final fit's dict plus persnp and sig_snps.  fit(sldf, persnp, X)
+    replaces the FP32 restated drivers (dense.iter_weighting_reml passes FP64 fits)."""
     n = len(y)
     m = len(gm)
     Z = (np.asarray(kmat)[:, 0::2].astype(np.float32) + np.asarray(kmat)[:, 1::2]).T  # n x m dosages
     base = np.ones(m, np.float32) if persnp is None else np.asarray(persnp, np.float32)
 
-    def fit(sldf, pw, Xf):
+    def fit32(sldf, pw, Xf):
         gw = oracle.gw(gm, pw)
         if sldf:
             return SLDF_REML(kmat, gm, gw, rvec, Xf, y, rng_seed=rng_seed, n_V=n_V, tol_L=tol_L, s2max=s2max)
         return LMC_REML(kmat, gm, gw, rvec, Xf, y, rng_seed=rng_seed, n_MC=n_MC, tol_L=tol_L, s2max=s2max)
+
+    fit = fit or fit32
 
     if m < window_size + 1:
         pw = base / np.float32(base.mean())

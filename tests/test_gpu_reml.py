@@ -301,6 +301,45 @@ def test_iter_weighting_against_restatement(gpu):
     assert rel_err(got["snp_blup"][sig], ref["snp_blup"][sig]) < 1e-2
 
 
+def test_iter_weighting_against_fp64(gpu):
+    """--iter_weighting against the same driver over exact FP64 LMC fits on the same MC draws
+    (dense.iter_weighting_reml), with error-variance weights and a second covariate: the same
+    clumped SNPs, and the weights, h2, Vg, the BLUPs and the clumped SNPs' estimates within 1e-4
+    (the bar for variance components).  The Lanczos seed runs to 1e-6 (tol_L = 1e-7: LMC seeds at
+    10 tol_L, src/reml.cpp:810), so that f(h2) is the exact solves' value to FP32 rounding and the
+    secant, with the reference's stop, takes the same evaluations as the FP64 driver; at the
+    reference's tol_L the seed's 5e-3 residuals move the first fit's secant path (5 evaluations
+    instead of 7 here) and the weights by 6e-3."""
+    import oracle
+    from oracle import bedio
+    from oracle import dense as odense
+    from oracle.dense import dosage_of_kmat
+    p = make_problem(n=1500, m=1200, seed=31, weights=True, c=2)
+    codes = p.codes.copy()
+    codes[41] = codes[40]
+    rows = bedio.encode_codes(codes)
+    kmat = bedio.kmat_of_codes(codes)
+    _, _, _, _, gm = oracle.counts(kmat)
+    Z = dosage_of_kmat(kmat)
+    rng = np.random.default_rng(8)
+    y = (Z[:, 40] * 3.0 + Z[:, 700] * 2.5 + Z @ rng.normal(scale=0.05, size=p.m) + rng.normal(size=p.n))
+    y = y.astype(np.float32)
+    maf = oracle.maf(kmat)
+    X = np.asarray(p.X, np.float32)
+    ref = odense.iter_weighting_reml(kmat, gm, p.rvec, X, y, maf, window_size=20, sig_chisq=8.0, n_MC=8)
+    g = gpu.Geno.from_bed(rows, p.n)
+    got = g.iter_weighting_REML(X, y, maf, rvec=p.rvec, window_size=20, sig_chisq=8.0, n_MC=8, tol_L=1e-7)
+    sig = got["sig_snps"]
+    errs = dict(persnp=rel_err(got["persnp"], ref["persnp"]), h2=abs(got["h2"] - ref["h2"]) / ref["h2"],
+                vg=abs(got["vg"] - ref["vg"]) / ref["vg"], blup=rel_err(got["snp_blup"], ref["snp_blup"]),
+                sig_blup=rel_err(got["snp_blup"][sig], ref["snp_blup"][sig]), blue=rel_err(got["blue"], ref["blue"]))
+    msg = " ".join(f"{k} {v:.2e}" for k, v in errs.items())
+    print("iter_weighting vs FP64:", msg)
+    assert list(sig) == list(ref["sig_snps"]) and 700 in sig and (40 in sig) != (41 in sig), msg
+    assert errs["persnp"] < 1e-4 and errs["h2"] < 1e-4 and errs["vg"] < 1e-4, msg
+    assert errs["blup"] < 1e-4 and errs["sig_blup"] < 1e-4 and errs["blue"] < 1e-4, msg
+
+
 @pytest.mark.parametrize("n,m", [(900, 1300), (5001, 700)])
 def test_gwa_window_scan_against_fp64(gpu, n, m):
     """Windowed score tests of slemm_gwa step 2 (bin/slemm_gwa.py:322-407) vs the FP64

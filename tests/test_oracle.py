@@ -560,6 +560,49 @@ def test_iter_weighting_restatement_rules():
     assert r2["h2"] == r3["h2"] and len(r2["sig_snps"]) == 0
 
 
+def test_lmc_fit_blup_and_blue_are_the_mixed_model_solutions():
+    """dense.lmc_fit's BLUP and BLUE (src/reml.cpp:866-877) at its h2 against the mixed-model
+    solutions formed directly: with H = Zc W Zc' + tau R, b = (X'H^-1 X)^-1 X'H^-1 y (GLS) and
+    a = W Zc' H^-1 (y - X b); Py = H^-1 (y - X b) and vg = y'Py/(n - c).  A tau in place of 1/tau,
+    a dropped R or an unweighted Zc' fails these."""
+    from oracle import dense as odense
+    p = make_problem(n=300, m=250, seed=44, weights=True, c=2, persnp=True)
+    y = make_phenotype(p, 0.5, n_causal=60, seed=9)
+    r = odense.lmc_fit(p.Z, p.gm, p.gw, p.rvec, p.X, y, rng_seed=3, n_MC=6)
+    tau = (1 - r["h2"]) / r["h2"]
+    Zc = p.Z - p.gm[None, :].astype(np.float64)
+    H = (Zc * np.asarray(p.gw, np.float64)[None, :]) @ Zc.T + tau * np.diag(np.asarray(p.rvec, np.float64))
+    X = np.asarray(p.X, np.float64)
+    Hi_X = np.linalg.solve(H, X)
+    b = np.linalg.solve(X.T @ Hi_X, Hi_X.T @ y.astype(np.float64))
+    Py = np.linalg.solve(H, y.astype(np.float64) - X @ b)
+    a = np.asarray(p.gw, np.float64) * (Zc.T @ Py)
+    assert rel_err(r["Py"], Py) < 1e-9
+    assert rel_err(r["snp_blup"], a) < 1e-9
+    assert rel_err(r["blue"], b) < 1e-9
+    assert abs(r["vg"] - float(y.astype(np.float64) @ Py) / (p.n - X.shape[1])) < 1e-9 * r["vg"]
+    assert r["n_MC"] == 6 and r["evals"] <= 9
+
+
+def test_iter_weighting_fp64_driver_tracks_restatement():
+    """The FP64 --iter_weighting driver (exact LMC fits on the same draws) against the FP32
+    restatement: same clumped SNPs, weights at FP32-fit scale, and with the weighting disabled
+    exactly one FP64 fit with persnp / mean(persnp)."""
+    from oracle import dense as odense
+    p, codes, rows, kmat, gm, y = _iterw_problem()
+    maf = oracle.maf(kmat)
+    X = np.ones((p.n, 1))
+    r32 = oreml.iter_weighting_reml(kmat, gm, None, X, y, maf, window_size=20, sig_chisq=8.0, n_MC=8)
+    r64 = odense.iter_weighting_reml(kmat, gm, None, X, y, maf, window_size=20, sig_chisq=8.0, n_MC=8)
+    assert list(r64["sig_snps"]) == list(r32["sig_snps"])
+    assert rel_err(r64["persnp"], r32["persnp"]) < 2e-2
+    assert abs(r64["h2"] - r32["h2"]) < 5e-3
+    ps = np.random.default_rng(2).uniform(0.5, 2.0, p.m).astype(np.float32)
+    r2 = odense.iter_weighting_reml(kmat, gm, None, X, y, maf, persnp=ps, window_size=p.m, n_MC=8)
+    r3 = odense.lmc_fit(dosage_of_kmat(kmat), gm, oracle.gw(gm, ps / np.float32(ps.mean())), None, X, y, n_MC=8)
+    assert r2["h2"] == r3["h2"] and len(r2["sig_snps"]) == 0
+
+
 # ------------------------------------------------------------------------ windowed GWA scan ----
 def test_gwa_blocks_known_layout():
     """Blocks of 8 over 20 model SNPs at odd positions: step 4, bound at block position 6."""
