@@ -266,6 +266,12 @@ int slemm_gpu_lmc_reml_ex(slemm_geno* h, const float* X, int64_t ldx, int c, con
                           float secant_tol, int secant_max_iter, int precision, int verbose,
                           float* Py, float* snp_blup, float* blue, slemm_reml_stats* stats);
 
+/* The Monte Carlo draws of LMC_REML (src/reml.cpp:788-801), host only: u (m x n_MC, column-major)
+ * then e (n x n_MC) from one std::mt19937(rng_seed + 54321) stream through
+ * std::normal_distribution<float>(0, 1), value for value as libstdc++ draws them (the polar
+ * method's logarithms taken in parallel host threads).  u, e: host. */
+int slemm_gpu_lmc_draws(int rng_seed, int64_t m, int64_t n, int n_MC, float* u, float* e);
+
 /* REML score and average-information matrix at (vg, ve) for the model of SLDF_REML (no
  * reference counterpart): score = (dl/dvg, dl/dve), ai = (AI_gg, AI_ge, AI_ee) with
  *   dl/dvg = -1/2 [tr(PA) - y'PAPy], dl/dve = -1/2 [tr(P) - y'PPy], AI_kl = 1/2 y'P V_k P V_l P y

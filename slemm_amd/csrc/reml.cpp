@@ -1,9 +1,9 @@
-// REML drivers over the device engine: SLDF_REML (src/reml.cpp:457-690) and LMC_REML
-// (src/reml.cpp:692-882, without the --lmm association part).
+// REML driver over the device engine: SLDF_REML (src/reml.cpp:457-690).  LMC_REML
+// (src/reml.cpp:692-882) is in lmc.hip.
 //
-// The control flow -- covariate QR, probe/MC draws from the same libstdc++ generators, the
-// three seed systems, golden-section search / secant iterations and the final Py, SNP BLUP and
-// BLUE -- follows the reference step by step.  Differences, all deliberate:
+// The control flow -- covariate QR, probe draws from the same libstdc++ generators, the three
+// seed systems, golden-section search and the final Py, SNP BLUP and BLUE -- follows the
+// reference step by step.  Differences, all deliberate:
 //  * the seed systems run as one batched device Lanczos (y, Q and probes share every pass over
 //    the genotypes; each group still stops on its own tolerance);
 //  * Lanczos/CG scalars, dot products, the tridiagonal eigensolver and the SLQ sum are FP64;
@@ -270,176 +270,3 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
   });
 }
 
-extern "C" int slemm_gpu_lmc_reml_ex(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y,
-                                     int rng_seed, int n_MC, float tol_L, float s2max, float s2min, float secant_tol,
-                                     int secant_max_iter, int precision, int verbose, float* Py, float* snp_blup,
-                                     float* blue, slemm_reml_stats* stats) {
-  return guard([&] {
-    REQUIRE(h && X && y && Py && snp_blup && blue, "null argument");
-    REQUIRE(c >= 1 && ldx >= h->n, "covariates: need c >= 1 and ldx >= n");
-    REQUIRE(n_MC >= 1, "number of MC samples must be positive");
-    REQUIRE(s2max > 0 && s2max < 1, "max heritability must be in (0, 1)");
-    REQUIRE(secant_tol > 0 && secant_max_iter >= 2, "secant: need tol > 0 and at least 2 evaluations");
-    (void)s2min;
-    HIPCHECK(hipSetDevice(h->device));
-    require_model(h);
-    const auto t_start = std::chrono::steady_clock::now();
-    const int64_t n = h->n, m = h->m;
-    if (verbose) printf("\nStarted Lanczos MC-REML.\n");
-    MatD Q;
-    covariate_basis(h, X, ldx, c, Q);  // src/reml.cpp:765-769
-    std::vector<float> gw(m);
-    HIPCHECK(hipMemcpy(gw.data(), h->gw, sizeof(float) * m, hipMemcpyDeviceToHost));
-
-    // V = [y, Zc u_MC, e_MC] (:788-813)
-    const int T = 1 + 2 * n_MC;
-    std::vector<float> V((size_t)n * T);
-    std::copy(y, y + n, V.begin());
-    std::mt19937 rng(rng_seed + 54321);
-    std::normal_distribution<float> rnorm(0.0, 1.0);
-    {
-      std::vector<float> u((size_t)m * n_MC);
-      for (int j = 0; j < n_MC; ++j)
-        for (int64_t i = 0; i < m; ++i) u[(size_t)j * m + i] = rnorm(rng);
-      for (int j = 0; j < n_MC; ++j)
-        for (int64_t i = 0; i < m; ++i) u[(size_t)j * m + i] *= std::sqrt(gw[i]);
-      DevF du(u.size()), dzu((size_t)n * n_MC);
-      upload(du.p, u);
-      abi(slemm_gpu_z(h, du.p, m, n_MC, dzu.p, n, precision, nullptr));
-      HIPCHECK(hipDeviceSynchronize());
-      HIPCHECK(hipMemcpy(&V[(size_t)n], dzu.p, sizeof(float) * n * n_MC, hipMemcpyDeviceToHost));
-    }
-    for (int j = 1 + n_MC; j < T; ++j)
-      for (int64_t i = 0; i < n; ++i) V[(size_t)j * n + i] = rnorm(rng);
-    for (int j = 0; j < 1 + n_MC; ++j)
-      for (int64_t i = 0; i < n; ++i) V[(size_t)j * n + i] *= h->rr_host[i];
-    project_out(Q, V, n, T);
-
-    // one seed system at 10x the tolerance, as BOLT-LMM relaxes CG (:817-819)
-    const float tau0 = (1 - s2max) / s2max;
-    DevF dV((size_t)n * T), dPV((size_t)n * T), dU((size_t)m * (1 + n_MC));
-    upload(dV.p, V);
-    SeedGuard sg;
-    abi(slemm_gpu_lseed(h, SLEMM_OP_QUADFORM, nullptr, nullptr, tau0, dV.p, n, T, 10 * tol_L, 600, T, precision,
-                        verbose, nullptr, &sg.s));
-    int steps = 0;
-    abi(slemm_gpu_lseed_info(sg.s, nullptr, &steps, nullptr));
-    const double t_seed = seconds_since(t_start);
-
-    int evals = 0;
-    std::vector<float> PyMC((size_t)n * (1 + n_MC)), pv, ub;
-    // REML_first_order (:696-728)
-    auto first_order = [&](float ss) -> float {
-      ++evals;
-      const float tau = (1 - ss) / ss;
-      const float sigma = tau - tau0;
-      abi(slemm_gpu_lsolve(sg.s, 0, T, dV.p, n, sigma, 1e-6f, dPV.p, n, nullptr));
-      download(pv, dPV.p, (size_t)n * T);
-      const float st = std::sqrt(tau);
-      for (int64_t i = 0; i < n; ++i) PyMC[i] = pv[i];
-      for (int k = 1; k <= n_MC; ++k)
-        for (int64_t i = 0; i < n; ++i)
-          PyMC[(size_t)k * n + i] = pv[(size_t)k * n + i] + st * pv[(size_t)(n_MC + k) * n + i];
-      project_out(Q, PyMC, n, 1 + n_MC);
-      double e0 = sq_norm(PyMC.data(), n), eMC = 0.0;
-      for (int k = 1; k <= n_MC; ++k) eMC += sq_norm(&PyMC[(size_t)k * n], n);
-      for (int k = 0; k <= n_MC; ++k)
-        for (int64_t i = 0; i < n; ++i) PyMC[(size_t)k * n + i] *= h->rr_host[i];
-      upload(dPV.p, PyMC);
-      abi(slemm_gpu_zt(h, dPV.p, n, 1 + n_MC, dU.p, m, 0, 0, precision, nullptr));
-      HIPCHECK(hipDeviceSynchronize());
-      download(ub, dU.p, (size_t)m * (1 + n_MC));
-      double u0 = 0.0, uMC = 0.0;
-      for (int64_t i = 0; i < m; ++i) {
-        const double g = gw[i];
-        u0 += (double)ub[i] * ub[i] * g;
-        for (int k = 1; k <= n_MC; ++k) uMC += (double)ub[(size_t)k * m + i] * ub[(size_t)k * m + i] * g;
-      }
-      return (float)std::log((u0 / e0) / (uMC / eMC));
-    };
-
-    // secant iterations on log((1-h2)/h2) (:821-853)
-    float s2 = s2max / 2;
-    float log_tau1 = std::log((1 - s2) / s2);
-    float f1 = first_order(s2);
-    if (verbose) printf("  Iter 1: h2=%.4f log((1-h2)/h2)=%.4f funct=%.4f\n", s2, log_tau1, f1);
-    s2 = f1 < 0 ? s2max / 4 : s2max;
-    float log_tau2 = std::log((1 - s2) / s2);
-    float f2 = first_order(s2);
-    if (verbose) printf("  Iter 2: h2=%.4f log((1-h2)/h2)=%.4f funct=%.4f\n", s2, log_tau2, f2);
-    for (int si = 3; si <= secant_max_iter; ++si) {
-      float lt = (log_tau1 * f2 - log_tau2 * f1) / (f2 - f1);
-      if (lt > 9) lt = 9;
-      else if (lt < -9) lt = -9;
-      if (std::fabs(lt - log_tau2) < secant_tol) break;
-      s2 = 1 / (std::exp(lt) + 1);
-      f1 = f2;
-      f2 = first_order(s2);
-      log_tau1 = log_tau2;
-      log_tau2 = lt;
-      if (verbose) printf("  Iter %d: h2=%.4f log((1-h2)/h2)=%.4f funct=%.4f\n", si, s2, lt, f2);
-    }
-    if (std::fabs(f2) > 0.005f && verbose)
-      printf("\nWarning: secant iterations not converged. h2 close to 0 or small sample size?\n");
-
-    // Py, variance components, SNP BLUP (:855-866)
-    std::copy(PyMC.begin(), PyMC.begin() + n, Py);
-    double ypy = 0.0;
-    for (int64_t i = 0; i < n; ++i) ypy += (double)y[i] * Py[i];
-    const float vg = (float)(ypy / (double)(n - c));
-    const float tau1 = (1 - s2) / s2;
-    const float ve = tau1 * vg;
-    std::vector<float> py(Py, Py + n);
-    upload(dPV.p, py);
-    abi(slemm_gpu_zt(h, dPV.p, n, 1, dU.p, m, 1, 0, precision, nullptr));
-    HIPCHECK(hipDeviceSynchronize());
-    HIPCHECK(hipMemcpy(snp_blup, dU.p, sizeof(float) * m, hipMemcpyDeviceToHost));
-    if (verbose) printf("\nVg,%g\nVe,%g\nh2,%g\n", vg, ve, s2);
-
-    // BLUE: residual y - Zc snp_blup - tau1 R Py regressed on X (:871-875)
-    abi(slemm_gpu_z(h, dU.p, m, 1, dPV.p, n, precision, nullptr));
-    HIPCHECK(hipDeviceSynchronize());
-    std::vector<float> zb;
-    download(zb, dPV.p, n);
-    std::vector<double> XtX((size_t)c * c), rhs(c);
-    for (int a = 0; a < c; ++a) {
-      double s = 0.0;
-      for (int64_t i = 0; i < n; ++i) {
-        const double r = (double)y[i] - ((double)zb[i] + (double)tau1 * h->rvec_host[i] * Py[i]);
-        s += (double)X[i + (size_t)a * ldx] * r;
-      }
-      rhs[a] = s;
-      for (int b = 0; b < c; ++b) {
-        double t = 0.0;
-        for (int64_t i = 0; i < n; ++i) t += (double)X[i + (size_t)a * ldx] * X[i + (size_t)b * ldx];
-        XtX[a + (size_t)b * c] = t;
-      }
-    }
-    if (!chol_solve(XtX, c, rhs)) fail(SLEMM_ERR_RANK, "X'X is not positive definite");
-    for (int q = 0; q < c; ++q) blue[q] = (float)rhs[q];
-    if (verbose) printf("\nCompleted Lanczos MC-REML.\n");
-    if (s2 > s2max && verbose)
-      printf("\nWarning: the h2 estimate is larger than the max you specified.\n"
-             "Can rerun with a higher --max_heritability to gain accuracy.\n");
-    if (stats) {
-      stats->vg = vg;
-      stats->ve = ve;
-      stats->h2 = s2;
-      stats->lrt = 0;
-      stats->lanczos_steps[0] = steps;
-      stats->lanczos_steps[1] = 0;
-      stats->lanczos_steps[2] = 0;
-      stats->criterion_evals = evals;
-      stats->seconds_seed = t_seed;
-      stats->seconds_total = seconds_since(t_start);
-    }
-  });
-}
-
-extern "C" int slemm_gpu_lmc_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y, int rng_seed,
-                                  int n_MC, float tol_L, float s2max, float s2min, int precision, int verbose,
-                                  float* Py, float* snp_blup, float* blue, slemm_reml_stats* stats) {
-  // the reference's secant schedule: stop at |d log tau| < 0.002, at most 9 evaluations (:824-836)
-  return slemm_gpu_lmc_reml_ex(h, X, ldx, c, y, rng_seed, n_MC, tol_L, s2max, s2min, 0.002f, 9, precision, verbose,
-                               Py, snp_blup, blue, stats);
-}

@@ -689,6 +689,15 @@ class Traits:
                     plan=plan)
 
 
+def lmc_draws(rng_seed: int, m: int, n: int, n_MC: int) -> tuple:
+    """The Monte Carlo draws of LMC_REML (src/reml.cpp:788-801): u (m x n_MC) then e (n x n_MC),
+    column-major float32, from mt19937(rng_seed + 54321) through normal_distribution<float>."""
+    u = np.zeros((m, n_MC), np.float32, order="F")
+    e = np.zeros((n, n_MC), np.float32, order="F")
+    L.check(L.load().slemm_gpu_lmc_draws(rng_seed, m, n, n_MC, _np_ptr(u, C.c_float), _np_ptr(e, C.c_float)))
+    return u, e
+
+
 def pred_plink(estimate_file: str, bfile_prefix: str, output_file: str) -> tuple:
     """--pred (src/pred.cpp:4-151): writes IID,GEBV for the fileset; returns (n, SNPs used)."""
     n = C.c_int64()

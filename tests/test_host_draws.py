@@ -1,0 +1,33 @@
+"""The library's Monte Carlo draws of LMC_REML (host only, no GPU) against libstdc++ itself: the
+oracle's orc_lmc_samples runs std::mt19937 + std::normal_distribution<float> compiled by g++."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from slemm_amd import gpu
+
+
+@pytest.mark.parametrize("seed,m,n,nmc", [(0, 1, 1, 1), (0, 7, 5, 3), (3, 1001, 333, 4), (11, 40_000, 9_999, 3),
+                                          (-54321, 100, 100, 2)])
+def test_lmc_draws_equal_libstdcxx_normal_distribution(seed, m, n, nmc):
+    u, e = gpu.lmc_draws(seed, m, n, nmc)
+    ru, re = oracle.lmc_samples(seed, n, m, nmc)
+    assert np.array_equal(u.view(np.uint32), ru.view(np.uint32))
+    assert np.array_equal(e.view(np.uint32), re.view(np.uint32))
+
+
+def test_lmc_draws_at_full_size_and_empty():
+    """n = 100k, m = 500k, 3 samples (the reference's n_MC there): 1.8M draws, bit for bit; and
+    zero-sized blocks."""
+    t0 = time.perf_counter()
+    u, e = gpu.lmc_draws(0, 500_000, 100_000, 3)
+    t1 = time.perf_counter()
+    ru, re = oracle.lmc_samples(0, 100_000, 500_000, 3)
+    t2 = time.perf_counter()
+    print(f"library {t1 - t0:.3f} s, libstdc++ {t2 - t1:.3f} s")
+    assert np.array_equal(u.view(np.uint32), ru.view(np.uint32))
+    assert np.array_equal(e.view(np.uint32), re.view(np.uint32))
+    u0, e0 = gpu.lmc_draws(0, 0, 5, 2)
+    assert u0.shape == (0, 2) and np.array_equal(e0, oracle.lmc_samples(0, 5, 0, 2)[1])
