@@ -266,6 +266,12 @@ int slemm_gpu_lmc_reml_ex(slemm_geno* h, const float* X, int64_t ldx, int c, con
                           float secant_tol, int secant_max_iter, int precision, int verbose,
                           float* Py, float* snp_blup, float* blue, slemm_reml_stats* stats);
 
+/* The Rademacher probes of SLDF_REML (src/reml.cpp:597-605), host only: n_V unit-norm columns of
+ * n entries -1/sqrt(n) or +1/sqrt(n) (B, host, n x n_V column-major) from
+ * std::mt19937(rng_seed + 54321) through uniform_real_distribution<float>(0, 1) < 0.5, value for
+ * value as the reference draws and normalises them.  slemm_gpu_sldf_reml draws the same probes. */
+int slemm_gpu_rademacher_probes(int rng_seed, int64_t n, int n_V, float* B);
+
 /* The Monte Carlo draws of LMC_REML (src/reml.cpp:788-801), host only: u (m x n_MC, column-major)
  * then e (n x n_MC) from one std::mt19937(rng_seed + 54321) stream through
  * std::normal_distribution<float>(0, 1), value for value as libstdc++ draws them (the polar

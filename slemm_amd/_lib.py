@@ -83,6 +83,7 @@ SIGNATURES = {
                                         C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, c_fp, c_fp, c_fp,
                                         C.POINTER(RemlStats)]),
     "slemm_gpu_lmc_draws": (C.c_int, [C.c_int, c_i64, c_i64, C.c_int, c_fp, c_fp]),
+    "slemm_gpu_rademacher_probes": (C.c_int, [C.c_int, c_i64, C.c_int, c_fp]),
     "slemm_gpu_set_gw": (C.c_int, [c_vp, c_fp]),
     "slemm_gpu_geno_load_bed": (C.c_int, [C.c_char_p, c_i64, c_i64, c_i64p, c_i64, c_i64p, c_i64, C.POINTER(c_vp)]),
     "slemm_gpu_pred_plink": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, c_i64p, c_i64p]),

@@ -83,8 +83,10 @@ inline double seconds_since(std::chrono::steady_clock::time_point t0) {
 void extract_dosage_cols(slemm_geno* h, const std::vector<int64_t>& snps, std::vector<float>& out);
 
 // Rademacher probes of unit norm from mt19937(seed+54321), written to columns of B (ld n),
-// draw order and normalisation as src/reml.cpp:597-605.
+// draw order and normalisation as src/reml.cpp:597-605 (probes.hip).
 void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B);
+// The same probes written to the device block dB (columns of length n, leading dimension ldb).
+void rademacher_probes_device(int rng_seed, int64_t n, int n_V, float* dB, int64_t ldb);
 
 // REML_criterion at h2 (src/reml.cpp:457-489): returns the 2 logLL value the golden-section
 // search compares (see criterion_value), sets vg, ve and the LRT.

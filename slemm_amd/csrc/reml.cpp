@@ -24,22 +24,6 @@ using namespace slemm;
 
 namespace slemm {
 
-void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B) {
-  std::mt19937 rng(rng_seed + 54321);
-  std::uniform_real_distribution<float> runif(0.0, 1.0);
-  for (int j = 0; j < n_V; ++j) {
-    float* v = B + (size_t)j * n;
-    for (int64_t i = 0; i < n; ++i) v[i] = runif(rng) < 0.5 ? -1.0f : 1.0f;
-  }
-  for (int j = 0; j < n_V; ++j) {
-    float* v = B + (size_t)j * n;
-    float ss = 0.f;
-    for (int64_t i = 0; i < n; ++i) ss += v[i] * v[i];
-    const float nr = std::sqrt(ss);
-    for (int64_t i = 0; i < n; ++i) v[i] /= nr;
-  }
-}
-
 bool criterion_replay() {
   const char* e = std::getenv("SLEMM_CRITERION_FP32");
   return e && e[0] == '1';
@@ -172,16 +156,17 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
     for (int64_t i = 0; i < n; ++i) yp[i] = h->rr_host[i] * y[i];
     project_out(Q, yp, n, 1);
 
-    // Rademacher probes from mt19937(seed+54321) (:597-605)
+    // [y~, Q, Rademacher probes from mt19937(seed+54321) (:597-605)], the probes written on the
+    // device from their signs
     const int T = 1 + c + n_V;
-    std::vector<float> B((size_t)n * T);
+    std::vector<float> B((size_t)n * (1 + c));
     std::copy(yp.begin(), yp.end(), B.begin());
     for (int q = 0; q < c; ++q)
       for (int64_t i = 0; i < n; ++i) B[(size_t)(1 + q) * n + i] = (float)Q(i, q);
-    rademacher_probes(rng_seed, n, n_V, &B[(size_t)(1 + c) * n]);
-    if (verbose) printf("Generated %d Rademacher probing vectors.\n", n_V);
     DevF dB((size_t)n * T), dX((size_t)n * std::max(c, 1));
     upload(dB.p, B);
+    rademacher_probes_device(rng_seed, n, n_V, dB.p + (size_t)(1 + c) * n, n);
+    if (verbose) printf("Generated %d Rademacher probing vectors.\n", n_V);
 
     // seed systems y (QUADFORM), Q (LDET), probes (LDET) in one batch (:625-631)
     const float tau0 = (1 - s2max) / s2max;

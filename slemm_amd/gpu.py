@@ -689,6 +689,13 @@ class Traits:
                     plan=plan)
 
 
+def rademacher_probes(rng_seed: int, n: int, n_V: int) -> np.ndarray:
+    """The Rademacher probes of SLDF_REML (src/reml.cpp:597-605): n x n_V float32, column-major."""
+    B = np.zeros((n, n_V), np.float32, order="F")
+    L.check(L.load().slemm_gpu_rademacher_probes(rng_seed, n, n_V, _np_ptr(B, C.c_float)))
+    return B
+
+
 def lmc_draws(rng_seed: int, m: int, n: int, n_MC: int) -> tuple:
     """The Monte Carlo draws of LMC_REML (src/reml.cpp:788-801): u (m x n_MC) then e (n x n_MC),
     column-major float32, from mt19937(rng_seed + 54321) through normal_distribution<float>."""

@@ -1,5 +1,6 @@
-"""The library's Monte Carlo draws of LMC_REML (host only, no GPU) against libstdc++ itself: the
-oracle's orc_lmc_samples runs std::mt19937 + std::normal_distribution<float> compiled by g++."""
+"""The library's random streams (host only, no GPU) against libstdc++ itself: the oracle's
+orc_lmc_samples and orc_rademacher run std::mt19937 with std::normal_distribution<float> and
+std::uniform_real_distribution<float> compiled by g++."""
 import time
 
 import numpy as np
@@ -31,3 +32,11 @@ def test_lmc_draws_at_full_size_and_empty():
     assert np.array_equal(e.view(np.uint32), re.view(np.uint32))
     u0, e0 = gpu.lmc_draws(0, 0, 5, 2)
     assert u0.shape == (0, 2) and np.array_equal(e0, oracle.lmc_samples(0, 5, 0, 2)[1])
+
+
+@pytest.mark.parametrize("seed,n,nv", [(0, 1, 1), (0, 5, 3), (7, 1000, 4), (0, 100_000, 7), (2, 623, 2), (2, 625, 1)])
+def test_rademacher_probes_equal_libstdcxx(seed, n, nv):
+    """Probe entries bit for bit (signs and the normalisation) across MT19937 block edges."""
+    got = gpu.rademacher_probes(seed, n, nv)
+    ref = oracle.rademacher(seed, n, nv)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
