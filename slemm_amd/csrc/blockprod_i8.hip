@@ -284,6 +284,7 @@ struct I8WArgs {
   // nonzero, and rows from which the tile's products are zero (both skipped)
   const int32_t* tile_nsuper = nullptr;
   const int64_t* tile_rows = nullptr;
+  int vt_base = 0;         // first vector tile of group 0 (second form: a block split over two launches)
   bool xcd_pairs = false;  // dispatch the column groups of a row block back to back on one XCD
   int prio = 1;            // issue priority of a wave while it issues a step's MFMAs (0: none)
 };
@@ -541,7 +542,7 @@ __global__ __launch_bounds__(256, TAIL ? 2 : 3) void k_bp_i8x(I8WArgs a) {
     }
   }
   const int64_t row0 = bx * (64 * RT) + wave * (16 * RT);
-  const int vt0 = by * NVG;
+  const int vt0 = a.vt_base + by * NVG;
   const bool tail = TAIL && a.r8 > 0 && by == (int)gridDim.y - 1;  // the group holds the tail tile
   const int s0 = (int)blockIdx.z * (int)a.ns_per_split;
   int s1 = min((int)a.nsuper, s0 + (int)a.ns_per_split);
@@ -844,10 +845,12 @@ void launch_bp_i8(const I8Launch& l, hipStream_t st) {
     a.tile_nsuper = l.tile_nsuper; a.tile_rows = l.tile_rows;
     // the second form takes groups of whole vector tiles without lo columns (SLEMM_I8_WIDE_FORM=1:
     // the first form for every block, read per call)
+    // Blocks whose vector tiles do not pair up (an odd number, the tail tile aside) run as two
+    // launches of the second form: the pairs, then the last vector tile with the tail tile as one
+    // group of its own (each group reads the codes once either way, and the two write disjoint
+    // columns of the partials).
     const char* e4 = std::getenv("SLEMM_I8_WIDE_FORM");
-    const bool second_ok = a.tail_r == 0 &&
-                           (dl.nvt % (dl.nvt == 1 && dl.r8 == 0 ? 1 : kI8NVG)) == 0;
-    const bool first_form = (e4 && e4[0] == '1') || !second_ok;
+    const bool first_form = (e4 && e4[0] == '1') || a.tail_r != 0;
     // SLEMM_I8_XCD (0/1, read per call): the column groups of a row block dispatched back to back on
     // one XCD, so that all but the first read the codes from that XCD's L2.  SLEMM_I8_PRIO (0/1):
     // issue priority 1 while a wave issues a step's MFMAs.  Defaults as measured at n = 100,000 x
@@ -862,14 +865,30 @@ void launch_bp_i8(const I8Launch& l, hipStream_t st) {
     if (a.nsuper >= INT32_MAX / 2) fail(SLEMM_ERR_ARG, "int8 block: reduction too long");
     // (one-vector-tile blocks measured slower with 2 row tiles per wave at four waves per SIMD:
     // each digit load then serves half as many MFMAs)
-    const dim3 grid((unsigned)(l.rows_pad / (64 * kI8WRT)), (unsigned)i8_groups(l.t), (unsigned)l.nsplit);
+    const unsigned nrb = (unsigned)(l.rows_pad / (64 * kI8WRT));
+    const dim3 grid(nrb, (unsigned)i8_groups(l.t), (unsigned)l.nsplit);
     const char* e3 = std::getenv("SLEMM_I8_ONE_TILE");  // "0": the general kernel (read per call)
     if (dl.nvt == 1 && dl.r8 == 0 && !(e3 && e3[0] == '0')) {
       if (first_form) k_bp_i8w<kI8WRT, 1, false><<<grid, 256, 0, st>>>(a);
       else k_bp_i8x<kI8WRT, 1, false><<<grid, 256, 0, st>>>(a);
+    } else if (first_form) {
+      k_bp_i8w<kI8WRT, kI8NVG><<<grid, 256, 0, st>>>(a);
+    } else if (dl.nvt % kI8NVG == 0) {
+      k_bp_i8x<kI8WRT, kI8NVG><<<grid, 256, 0, st>>>(a);
     } else {
-      if (first_form) k_bp_i8w<kI8WRT, kI8NVG><<<grid, 256, 0, st>>>(a);
-      else k_bp_i8x<kI8WRT, kI8NVG><<<grid, 256, 0, st>>>(a);
+      const int pairs = dl.nvt / kI8NVG;
+      if (pairs > 0) {
+        I8WArgs a2 = a;
+        a2.r8 = 0;  // the tail tile goes with the last vector tile
+        k_bp_i8x<kI8WRT, kI8NVG><<<dim3(nrb, (unsigned)pairs, (unsigned)l.nsplit), 256, 0, st>>>(a2);
+        HIPCHECK(hipGetLastError());
+      }
+      I8WArgs a1 = a;
+      a1.vt_base = pairs * kI8NVG;
+      a1.xcd_pairs = false;
+      const dim3 g1(nrb, 1, (unsigned)l.nsplit);
+      if (dl.r8 > 0) k_bp_i8x<kI8WRT, 1, true><<<g1, 256, 0, st>>>(a1);
+      else k_bp_i8x<kI8WRT, 1, false><<<g1, 256, 0, st>>>(a1);
     }
   }
   HIPCHECK(hipGetLastError());

@@ -183,14 +183,16 @@ def test_int8_narrow_blocks(gpu, t, monkeypatch):
         assert rel_err(z, uref) < tol, (target, rel_err(z, uref))
 
 
-@pytest.mark.parametrize("t", [7, 9, 16, 17, 21, 33, 65, 66, 100])
+@pytest.mark.parametrize("t", [7, 9, 16, 17, 21, 33, 40, 51, 65, 66, 100, 102])
 def test_int8_wide_blocks(gpu, t, monkeypatch):
     """Blocks of more than 5 vectors on the int8 wide kernel (16-vector tiles of three digit
-    tiles, a tail tile for up to 5 trailing vectors, column groups of two vector tiles), over the
-    operand layouts the split-BF16 path uses (tail columns for t = 16 q + r with r <= 8, separate
-    hi and lo blocks otherwise; 6 to 8 vectors take the wide kernel only without hi/lo columns),
-    at one reduction split and at several, against the FP64 dense operator, the transposed product
-    and the product, and against the split-BF16 kernels."""
+    tiles, a tail tile for up to 5 trailing vectors, column groups of two vector tiles; an odd
+    number of vector tiles -- t = 17, 21, 40, 51, 102 -- runs the pairs and the last tile with the
+    tail as two launches), over the operand layouts the split-BF16 path uses (tail columns for
+    t = 16 q + r with r <= 8, separate hi and lo blocks otherwise; 6 to 8 vectors take the wide
+    kernel only without hi/lo columns), at one reduction split and at several, against the FP64
+    dense operator, the transposed product and the product, against the split-BF16 kernels, and
+    bit for bit against the first form of the wide kernel (exact integer sums either way)."""
     if t <= 8:
         monkeypatch.setenv("SLEMM_HL_COLUMNS", "0")
     p = make_problem(n=2600, m=1700, seed=31, weights=True, c=2)
@@ -214,6 +216,11 @@ def test_int8_wide_blocks(gpu, t, monkeypatch):
     for target in ("1", "20", "100000"):
         monkeypatch.setenv("SLEMM_I8_TARGET", target)
         out = g.lnr_op_select("ldet", tau0, Vd, precision="bf16x2").cpu().numpy()
+        monkeypatch.setenv("SLEMM_I8_WIDE_FORM", "1")
+        first = g.lnr_op_select("ldet", tau0, Vd, precision="bf16x2").cpu().numpy()
+        monkeypatch.delenv("SLEMM_I8_WIDE_FORM")
+        if t > 8:
+            assert np.array_equal(out, first), (target, rel_err(out, first))
         tol = I8_TOL
         for k in range(t):  # every column against its own scale (column 5 is 1e-33 of the others)
             assert rel_err(out[:, k], ref[:, k]) < tol, (target, k, rel_err(out[:, k], ref[:, k]))
