@@ -111,6 +111,9 @@ struct LdsArgs {
   const int64_t* tile_rows = nullptr;
 };
 
+// rows of the pass-1 epilogue's workgroups, the granularity of its partial sums (upart)
+constexpr int64_t kFin1Rows = 64;
+
 struct Fin1Args {
   const float* part = nullptr;
   int nsplit = 1;
@@ -125,7 +128,7 @@ struct Fin1Args {
   void* X2 = nullptr;              // pass-2 operand (null: none): split BF16, or FP32 with x2_fp32
   int64_t lo_offset16 = 0;
   bool x2_fp32 = false;            // X2 as FP32, element (row, k) at row KP + k (int8 kernels)
-  double* upart = nullptr;         // (rows_pad/256) x KP partial sums of gm * represented T
+  double* upart = nullptr;         // (rows_pad/kFin1Rows) x KP partial sums of gm * represented T
   float* out = nullptr;            // optional column-major rows x t
   int64_t ldo = 0;
   bool raw = false;                // out = summed partial products, no centring or weighting

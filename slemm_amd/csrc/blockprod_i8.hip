@@ -45,8 +45,9 @@ __device__ __forceinline__ int digit_col(int k, int p, int nvt) {
 
 // Partial maxima of |x| over the FP32 operand X[row KP + k] (rows [0, rows)): block (x, y) covers
 // columns 64 y .. 64 y + 63 (one per lane, so each wave reads 256 contiguous bytes of a row) and
-// the x-th of kI8Blocks contiguous row ranges, 4 rows at a time.  A NaN or infinite element
-// makes the partial +inf (fmaxf alone would drop NaN).  pmax[k kI8Blocks + x].
+// the x-th of kI8Blocks contiguous row ranges, 4 rows at a time, eight of them in flight per
+// thread (independent maxima: the loop is bound by memory latency, not by the compares).  A NaN or
+// infinite element makes the partial +inf (fmaxf alone would drop NaN).  pmax[k kI8Blocks + x].
 __global__ __launch_bounds__(256) void k_i8_colmax(const float* __restrict__ X, int KP, int t, int64_t rows,
                                                    float* __restrict__ pmax) {
   __shared__ float red[4][64];
@@ -54,12 +55,22 @@ __global__ __launch_bounds__(256) void k_i8_colmax(const float* __restrict__ X, 
   const int k = blockIdx.y * 64 + tx;
   const int64_t per = (rows + kI8Blocks - 1) / kI8Blocks;
   const int64_t r0 = (int64_t)blockIdx.x * per, r1 = min(rows, r0 + per);
+  auto absx = [](float v) { const float a = fabsf(v); return a <= FLT_MAX ? a : INFINITY; };
   float mx = 0.f;
-  if (k < t)
-    for (int64_t r = r0 + ty; r < r1; r += 4) {
-      const float a = fabsf(X[r * KP + k]);
-      mx = fmaxf(mx, a <= FLT_MAX ? a : INFINITY);
+  if (k < t) {
+    float m8[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int64_t r = r0 + ty;
+    for (; r + 28 < r1; r += 32) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = X[(r + 4 * u) * KP + k];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) m8[u] = fmaxf(m8[u], absx(v[u]));
     }
+    for (; r < r1; r += 4) m8[0] = fmaxf(m8[0], absx(X[r * KP + k]));
+#pragma unroll
+    for (int u = 0; u < 8; ++u) mx = fmaxf(mx, m8[u]);
+  }
   red[ty][tx] = mx;
   __syncthreads();
   if (ty == 0 && k < t) pmax[k * kI8Blocks + blockIdx.x] = fmaxf(fmaxf(red[0][tx], red[1][tx]), fmaxf(red[2][tx], red[3][tx]));

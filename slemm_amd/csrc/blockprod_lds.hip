@@ -603,20 +603,21 @@ __global__ __launch_bounds__(256, 2) void k_bp_persist(LdsArgs a, int64_t nbx, i
 }
 
 // Pass-1 epilogue from the split partials: T_jk = w_j (P_jk - gm_j s_k + xbar_k (pc_j - n gm_j));
-// writes the BF16 hi/lo pass-2 operand, optionally the column-major result, and per-block
-// partial sums of gm_j * represented(T_jk).  A workgroup covers 64 columns x 256 rows in four
-// tiles of 64 rows: the partials of a tile are summed over the splits with row-contiguous reads
-// (16 threads per column, 256 contiguous bytes) into LDS, then thread (tx = column, ty) finishes
-// rows 16 ty .. 16 ty + 15 of the tile, so that the operand's writes stay contiguous across the
-// columns.  Tail hi/lo layouts: P_jk of a tail vector is the sum of the partials of columns k and
-// k + tail_r, and threads of the columns holding lo halves only write their (zero) workgroup
-// sums; pad columns past them write their hi halves (zero) only.
+// writes the BF16 hi/lo pass-2 operand, optionally the column-major result, and per-workgroup
+// partial sums of gm_j * represented(T_jk).  A workgroup covers 64 columns x kFin1Rows (64) rows:
+// the partials are summed over the splits with row-contiguous reads (16 threads per column, 256
+// contiguous bytes) into LDS, then thread (tx = column, ty) finishes rows 16 ty .. 16 ty + 15, so
+// that the operand's writes stay contiguous across the columns.  Every global load of a thread
+// (its partials and the 16 rows' gm, w and popcounts) is issued before the one barrier, so a
+// workgroup waits on one memory round trip; small workgroups keep enough of them resident.  Tail
+// hi/lo layouts: P_jk of a tail vector is the sum of the partials of columns k and k + tail_r,
+// and threads of the columns holding lo halves only write their (zero) workgroup sums; pad columns
+// past them write their hi halves (zero) only.
 __global__ __launch_bounds__(256) void k_fin1(Fin1Args a) {
   __shared__ double red[4][64];
   __shared__ float pt[64][65];  // [column][row of the tile], padded against bank conflicts
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int k = blockIdx.x * 64 + tx;
-  double usum = 0.0;
   // batched traits: this column's per-SNP model terms and subset size
   const int trait = (a.tc.col_trait && k < a.t) ? a.tc.col_trait[k] : -1;
   const float* gmk = trait >= 0 ? a.tc.gm + (int64_t)trait * a.tc.ld_m : a.gm;
@@ -626,67 +627,77 @@ __global__ __launch_bounds__(256) void k_fin1(Fin1Args a) {
   const bool tail = a.tail_r > 0 && k >= a.tail0 && k < a.t;
   const bool lo_col = a.tail_r > 0 && k >= a.t && k < a.tail0 + 2 * a.tail_r;
   const bool active = k < a.KP && !lo_col;
+  const int64_t trow0 = (int64_t)blockIdx.y * kFin1Rows;
+  const int rl0 = ty * 16;
+  // the rows' model terms, requested with the partials below
+  float gmv[16], wv[16];
+  int32_t pcv[16];
+  if (active) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int64_t row = trow0 + rl0 + e;
+      gmv[e] = gmk[row];
+      wv[e] = gwk ? gwk[row] : 1.0f;
+      pcv[e] = pck[row];
+    }
+  }
   const double sk = active ? a.colsum[k] : 0.0;
   const double xb = (active && a.xsum && k < a.t) ? (double)(float)(a.xsum[k] / nk) : 0.0;
-  for (int tt = 0; tt < 4; ++tt) {
-    const int64_t trow0 = (int64_t)blockIdx.y * 256 + tt * 64;
-    // sums of the partials: piece q = column q / 16, rows 4 (q % 16) .. + 3 of the tile.  Padding
-    // columns (k >= t) are zero; their partials are not read (the block products leave them
-    // undefined).
+  // sums of the partials: piece q = column q / 16, rows 4 (q % 16) .. + 3 of the tile.  Padding
+  // columns (k >= t) are zero; their partials are not read (the block products leave them
+  // undefined).
 #pragma unroll
-    for (int it = 0; it < 4; ++it) {
-      const int q = threadIdx.x + 256 * it;
-      const int c = q >> 4, r4 = (q & 15) * 4;
-      const int kk = blockIdx.x * 64 + c;
-      float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (kk < a.t) {
-        const bool tl = a.tail_r > 0 && kk >= a.tail0;
-        for (int h = 0; h < (tl ? 2 : 1); ++h)
-          for (int z = 0; z < a.nsplit; ++z) {
-            const float4 v = *reinterpret_cast<const float4*>(
-                a.part + ((int64_t)z * a.KP + kk + h * a.tail_r) * a.rows_pad + trow0 + r4);
-            sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
-          }
-      }
-      pt[c][r4] = sum.x; pt[c][r4 + 1] = sum.y; pt[c][r4 + 2] = sum.z; pt[c][r4 + 3] = sum.w;
+  for (int it = 0; it < 4; ++it) {
+    const int q = threadIdx.x + 256 * it;
+    const int c = q >> 4, r4 = (q & 15) * 4;
+    const int kk = blockIdx.x * 64 + c;
+    float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (kk < a.t) {
+      const bool tl = a.tail_r > 0 && kk >= a.tail0;
+      for (int h = 0; h < (tl ? 2 : 1); ++h)
+        for (int z = 0; z < a.nsplit; ++z) {
+          const float4 v = *reinterpret_cast<const float4*>(
+              a.part + ((int64_t)z * a.KP + kk + h * a.tail_r) * a.rows_pad + trow0 + r4);
+          sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
+        }
     }
-    __syncthreads();
-    if (active) {
+    pt[c][r4] = sum.x; pt[c][r4 + 1] = sum.y; pt[c][r4 + 2] = sum.z; pt[c][r4 + 3] = sum.w;
+  }
+  __syncthreads();
+  double usum = 0.0;
+  if (active) {
 #pragma unroll
-      for (int g8 = 0; g8 < 2; ++g8) {
-        const int rl = ty * 16 + g8 * 8;
-        const int64_t i0 = trow0 + rl;
-        uint32_t hb[8], lb[8];
+    for (int g8 = 0; g8 < 2; ++g8) {
+      const int rl = rl0 + g8 * 8;
+      const int64_t i0 = trow0 + rl;
+      uint32_t hb[8], lb[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int64_t row = i0 + e;
-          const float P = pt[tx][rl + e];
-          const float gmv = gmk[row];
-          const float wv = gwk ? gwk[row] : 1.0f;
-          const double rem = xb * ((double)pck[row] - nk * (double)gmv);
-          const float tv = a.raw ? P : wv * (float)((double)P - (double)gmv * sk + rem);
-          if (a.x2_fp32) {  // FP32 operand for the int8 kernels, which sum its digits themselves
-            if (a.X2) reinterpret_cast<float*>(a.X2)[row * a.KP + k] = tv;
-            usum += (double)gmv * (double)tv;
-          } else {
-            hb[e] = bf16_bits_rn(tv);
-            lb[e] = bf16_bits_rn(tv - bf16_to_float(hb[e]));
-            usum += (double)gmv * ((double)bf16_to_float(hb[e]) + (double)bf16_to_float(lb[e]));
-          }
-          if (a.out && k < a.t && row < a.rows) a.out[row + (int64_t)k * a.ldo] = tv;
+      for (int e = 0; e < 8; ++e) {
+        const int64_t row = i0 + e;
+        const float P = pt[tx][rl + e];
+        const float gm1 = gmv[g8 * 8 + e];
+        const double rem = xb * ((double)pcv[g8 * 8 + e] - nk * (double)gm1);
+        const float tv = a.raw ? P : wv[g8 * 8 + e] * (float)((double)P - (double)gm1 * sk + rem);
+        if (a.x2_fp32) {  // FP32 operand for the int8 kernels, which sum its digits themselves
+          if (a.X2) reinterpret_cast<float*>(a.X2)[row * a.KP + k] = tv;
+          usum += (double)gm1 * (double)tv;
+        } else {
+          hb[e] = bf16_bits_rn(tv);
+          lb[e] = bf16_bits_rn(tv - bf16_to_float(hb[e]));
+          usum += (double)gm1 * ((double)bf16_to_float(hb[e]) + (double)bf16_to_float(lb[e]));
         }
-        if (a.X2 && !a.x2_fp32) {
-          uint4* x2 = reinterpret_cast<uint4*>(a.X2);
-          const int64_t idx = (i0 >> 3) * a.KP + k;
-          x2[idx] = make_uint4(hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16), hb[6] | (hb[7] << 16));
-          const uint4 lo4 =
-              make_uint4(lb[0] | (lb[1] << 16), lb[2] | (lb[3] << 16), lb[4] | (lb[5] << 16), lb[6] | (lb[7] << 16));
-          if (tail) x2[idx + a.tail_r] = lo4;
-          else if (a.tail_r == 0 || k < a.tail0) x2[idx + a.lo_offset16] = lo4;
-        }
+        if (a.out && k < a.t && row < a.rows) a.out[row + (int64_t)k * a.ldo] = tv;
+      }
+      if (a.X2 && !a.x2_fp32) {
+        uint4* x2 = reinterpret_cast<uint4*>(a.X2);
+        const int64_t idx = (i0 >> 3) * a.KP + k;
+        x2[idx] = make_uint4(hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16), hb[6] | (hb[7] << 16));
+        const uint4 lo4 =
+            make_uint4(lb[0] | (lb[1] << 16), lb[2] | (lb[3] << 16), lb[4] | (lb[5] << 16), lb[6] | (lb[7] << 16));
+        if (tail) x2[idx + a.tail_r] = lo4;
+        else if (a.tail_r == 0 || k < a.tail0) x2[idx + a.lo_offset16] = lo4;
       }
     }
-    __syncthreads();  // the next tile overwrites pt
   }
   red[ty][tx] = usum;
   __syncthreads();
@@ -867,7 +878,7 @@ void launch_bp(const LdsArgs& a, int NT, int nsplit, int variant, hipStream_t st
 }
 
 void launch_fin1(const Fin1Args& a, hipStream_t st) {
-  dim3 grid((unsigned)((a.KP + 63) / 64), (unsigned)(a.rows_pad / 256));
+  dim3 grid((unsigned)((a.KP + 63) / 64), (unsigned)(a.rows_pad / kFin1Rows));
   k_fin1<<<grid, 256, 0, st>>>(a);
   HIPCHECK(hipGetLastError());
 }

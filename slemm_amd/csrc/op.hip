@@ -294,7 +294,7 @@ void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const floa
   // pass 1: T = W (Z' X1 - gm s'), written as the pass-2 operand
   h->x2.reserve(L.x2_bytes);
   if (use_lds(precision)) {
-    const int64_t nparts = h->m_pad / 256;
+    const int64_t nparts = h->m_pad / kFin1Rows;
     h->part2.reserve(sizeof(double) * nparts * L.KP);
     {  // size the split-partial buffer for both passes before anything is in flight
       int64_t nsp = 0;
@@ -362,7 +362,7 @@ size_t op_workspace_bytes(const slemm_geno* h, int kind, const int* col_kind, in
   add(sizeof(double) * 3 * L.KP);                                        // colsum
   add(L.x2_bytes);
   if (use_lds(precision)) {
-    add(sizeof(double) * (h->m_pad / 256) * L.KP);                       // part2
+    add(sizeof(double) * (h->m_pad / kFin1Rows) * L.KP);                 // part2
     int64_t nsp = 0;
     const size_t S1 = (size_t)pass_splits(h, 0, L.slot, nsp), S2 = (size_t)pass_splits(h, 1, L.slot, nsp);
     add(sizeof(float) * L.KP * std::max(S1 * h->m_pad, S2 * h->n_pad));  // partbuf, sized up front

@@ -144,7 +144,7 @@ void run_op_traits(slemm_traits* tr, const int* col_trait, const int* col_kind, 
   }
   pack_pass1(h, L, V, ldv, t, nullptr, qtv, kinds, precision, st, true, &tc);
   h->x2.reserve(L.x2_bytes);
-  const int64_t nparts = h->m_pad / 256;
+  const int64_t nparts = h->m_pad / kFin1Rows;
   h->part2.reserve(sizeof(double) * nparts * L.KP);
   {
     int64_t nsp = 0;
