@@ -351,13 +351,17 @@ namespace slemm {
 __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
   __shared__ double red[4][64];
   __shared__ float vt[64][257];  // [column][row]; the pad makes the column-wise reads conflict-free
+  __shared__ float rrs[256];     // the rows' r_rsqrt
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int k = blockIdx.x * 64 + tx;
   const int64_t rblk = (int64_t)blockIdx.y * 256;
   const int64_t rbase = rblk + ty * 64;
   {
+    // 16 loads in flight per thread (the tile is latency-bound otherwise)
     const int64_t i = rblk + threadIdx.x;
     const int kmax = min(64, a.t - (int)blockIdx.x * 64);
+    if (a.rr) rrs[threadIdx.x] = i < a.n ? a.rr[i] : 0.f;
+#pragma unroll 16
     for (int kk = 0; kk < 64; ++kk)
       vt[kk][threadIdx.x] = (kk < kmax && i < a.n) ? a.V[i + (int64_t)(blockIdx.x * 64 + kk) * a.ldv] : 0.f;
   }
@@ -388,7 +392,7 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
             for (int q = 0; q < a.c; ++q) acc += (double)Qk[i + (int64_t)q * a.ldq] * a.qtv[q * a.KP + k];
             v = (float)((double)v - acc);
           }
-          val = a.rr ? a.rr[i] * v : v;
+          val = a.rr ? rrs[i - rblk] * v : v;
           val -= xbar;
         }
         x[e] = val;
