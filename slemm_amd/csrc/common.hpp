@@ -95,6 +95,7 @@ struct slemm_geno {
   float* gw = nullptr;
   float* rr = nullptr;
   float* Q = nullptr;  // n_pad x c, column-major, padding rows zero
+  size_t q_bytes = 0;  // allocated bytes of Q (kept when c drops, so that resetting Q frees nothing)
   int c = 0;
   bool counts_ready = false;
   bool model_ready = false;

@@ -273,15 +273,20 @@ void ensure_counts(slemm_geno* h, hipStream_t st) {
   h->counts_ready = true;
 }
 
+// The allocation is kept while it is large enough (a handle reused for many fits, as the
+// one-trait-at-a-time plan does, then frees and allocates nothing here).
 void set_Q(slemm_geno* h, const float* Q, int c) {
-  if (h->Q) {
-    (void)hipFree(h->Q);
+  const size_t bytes = sizeof(float) * h->n_pad * (size_t)c;
+  if (bytes > h->q_bytes) {
+    if (h->Q) (void)hipFree(h->Q);
     h->Q = nullptr;
+    h->q_bytes = 0;
+    HIPCHECK(hipMalloc(&h->Q, bytes));
+    h->q_bytes = bytes;
   }
   h->c = c;
   if (c > 0) {
-    HIPCHECK(hipMalloc(&h->Q, sizeof(float) * h->n_pad * c));
-    HIPCHECK(hipMemset(h->Q, 0, sizeof(float) * h->n_pad * c));
+    HIPCHECK(hipMemset(h->Q, 0, bytes));
     HIPCHECK(hipMemcpy2D(h->Q, sizeof(float) * h->n_pad, Q, sizeof(float) * h->n, sizeof(float) * h->n, c,
                          hipMemcpyHostToDevice));
   }

@@ -57,19 +57,23 @@ void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B) {
   });
 }
 
+// Stream-ordered on the null stream (allocation, copy, kernel, release), waiting only for that
+// stream.
 void rademacher_probes_device(int rng_seed, int64_t n, int n_V, float* dB, int64_t ldb) {
   if (n <= 0 || n_V <= 0) return;
   const std::vector<uint8_t> neg = probe_signs(rng_seed, n, n_V);
   uint8_t* dneg = nullptr;
-  HIPCHECK(hipMalloc(&dneg, neg.size()));
-  const hipError_t cp = hipMemcpy(dneg, neg.data(), neg.size(), hipMemcpyHostToDevice);
-  if (cp == hipSuccess) {
+  HIPCHECK(hipMallocAsync(reinterpret_cast<void**>(&dneg), neg.size(), nullptr));
+  hipError_t e = hipMemcpyAsync(dneg, neg.data(), neg.size(), hipMemcpyHostToDevice, nullptr);
+  if (e == hipSuccess) {
     const int64_t total = n * n_V;
     k_probe_cols<<<(unsigned)((total + 255) / 256), 256>>>(dneg, n, n_V, probe_scale(n), dB, ldb);
+    e = hipGetLastError();
   }
-  const hipError_t ke = cp == hipSuccess ? hipGetLastError() : cp;
-  const hipError_t se = ke == hipSuccess ? hipDeviceSynchronize() : ke;
-  (void)hipFree(dneg);
+  const hipError_t fe = hipFreeAsync(dneg, nullptr);
+  const hipError_t se = hipStreamSynchronize(nullptr);  // the signs' host buffer is released on return
+  HIPCHECK(e);
+  HIPCHECK(fe);
   HIPCHECK(se);
 }
 
