@@ -911,9 +911,10 @@ def main():
     t_load = time.perf_counter() - t0
     g.set_model(None, None, np.full((n, 1), 1.0 / np.sqrt(n), np.float32))
 
-    # block: column 0 = phenotype-like RHS (projected operator), 1..nv = Rademacher probes
+    # block: column 0 = phenotype-like RHS (projected operator), 1..nv = Rademacher probes; stored
+    # column-major (n x K, leading dimension n) as the C ABI and the reference's Eigen blocks hold it
     gen = torch.Generator(device=dev).manual_seed(7)
-    V = torch.empty((n, K), dtype=torch.float32, device=dev)
+    V = torch.empty((K, n), dtype=torch.float32, device=dev).t()
     V[:, 0] = torch.randn(n, generator=gen, device=dev)
     V[:, 1:] = torch.where(torch.rand((n, nv), generator=gen, device=dev) < 0.5, -1.0, 1.0) / np.sqrt(n)
     kinds = ["quadform"] + ["ldet"] * nv
