@@ -19,10 +19,12 @@ namespace {
 // 1 where the draw is -1: uniform_real_distribution<float>(0, 1) is canonical24(u) * (1 - 0) + 0
 uint8_t negative(uint32_t u) { return canonical24(u) * 1.0f + 0.0f < 0.5f ? 1 : 0; }
 
-// the signs of all n * n_V draws in draw order (column j, row i at j n + i)
-std::vector<uint8_t> probe_signs(int rng_seed, int64_t n, int n_V) {
-  const int64_t total = n * n_V;
-  std::vector<uint8_t> neg((size_t)total);
+}  // namespace
+
+// the signs of the first `total` draws in draw order (probe column j of an n-row block is draws
+// j n .. j n + n - 1)
+std::vector<uint8_t> rademacher_signs(int rng_seed, int64_t total) {
+  std::vector<uint8_t> neg((size_t)std::max<int64_t>(total, 0));
   Mt19937Blocks mt((uint32_t)(rng_seed + 54321));
   uint32_t w[Mt19937Blocks::kN];
   for (int64_t e = 0; e < total; e += Mt19937Blocks::kN) {
@@ -34,10 +36,12 @@ std::vector<uint8_t> probe_signs(int rng_seed, int64_t n, int n_V) {
 }
 
 // 1 / sqrt of the float sum of n unit squares
-float probe_scale(int64_t n) {
+float rademacher_scale(int64_t n) {
   const float ss = n < (1 << 24) ? (float)n : 16777216.0f;
   return 1.0f / std::sqrt(ss);
 }
+
+namespace {
 
 __global__ __launch_bounds__(256) void k_probe_cols(const uint8_t* __restrict__ neg, int64_t n, int n_V, float q,
                                                     float* __restrict__ B, int64_t ldb) {
@@ -50,8 +54,8 @@ __global__ __launch_bounds__(256) void k_probe_cols(const uint8_t* __restrict__ 
 }  // namespace
 
 void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B) {
-  const std::vector<uint8_t> neg = probe_signs(rng_seed, n, n_V);
-  const float q = probe_scale(n);
+  const std::vector<uint8_t> neg = rademacher_signs(rng_seed, n * n_V);
+  const float q = rademacher_scale(n);
   parallel_parts(n_V, host_threads(), [&](int64_t j) {
     for (int64_t i = 0; i < n; ++i) B[(size_t)j * n + i] = neg[(size_t)j * n + i] ? -q : q;
   });
@@ -61,13 +65,13 @@ void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B) {
 // stream.
 void rademacher_probes_device(int rng_seed, int64_t n, int n_V, float* dB, int64_t ldb) {
   if (n <= 0 || n_V <= 0) return;
-  const std::vector<uint8_t> neg = probe_signs(rng_seed, n, n_V);
+  const std::vector<uint8_t> neg = rademacher_signs(rng_seed, n * n_V);
   uint8_t* dneg = nullptr;
   HIPCHECK(hipMallocAsync(reinterpret_cast<void**>(&dneg), neg.size(), nullptr));
   hipError_t e = hipMemcpyAsync(dneg, neg.data(), neg.size(), hipMemcpyHostToDevice, nullptr);
   if (e == hipSuccess) {
     const int64_t total = n * n_V;
-    k_probe_cols<<<(unsigned)((total + 255) / 256), 256>>>(dneg, n, n_V, probe_scale(n), dB, ldb);
+    k_probe_cols<<<(unsigned)((total + 255) / 256), 256>>>(dneg, n, n_V, rademacher_scale(n), dB, ldb);
     e = hipGetLastError();
   }
   const hipError_t fe = hipFreeAsync(dneg, nullptr);

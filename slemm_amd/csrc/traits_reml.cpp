@@ -135,11 +135,15 @@ extern "C" int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int 
     const int T = tr->T;
     const int64_t n = h->n;
     const int ncol = T * (2 + n_V);
-    // columns: [y~_0 .. y~_{T-1}, q_0 .. q_{T-1}, probes of trait 0, probes of trait 1, ...]
-    std::vector<float> B((size_t)n * ncol, 0.f);
+    // columns: [y~_0 .. y~_{T-1}, q_0 .. q_{T-1}, probes of trait 0, probes of trait 1, ...],
+    // written on the device from each trait's y~ and q values and one stream of probe signs
+    // (trait t's probes, drawn from mt19937(seed+54321) over its n_t individuals, are the first
+    // n_t n_V draws of that stream)
+    std::vector<float> yq;
     std::vector<int> col_trait(ncol), kinds(ncol, SLEMM_OP_LDET), groups(ncol);
     std::vector<std::vector<float>> yp(T);
     std::vector<MatD> Qt(T);
+    int64_t nmax = 0;
     for (int t = 0; t < T; ++t) {
       const std::vector<int64_t>& mem = tr->members[t];
       const int64_t nt = (int64_t)mem.size();
@@ -151,13 +155,9 @@ extern "C" int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int 
       yp[t].resize(nt);
       for (int64_t i = 0; i < nt; ++i) yp[t][i] = y[(size_t)t * n + mem[i]];
       project_out(Qt[t], yp[t], nt, 1);
-      std::vector<float> probes((size_t)nt * n_V);
-      rademacher_probes(rng_seed, nt, n_V, probes.data());
-      for (int64_t i = 0; i < nt; ++i) {
-        B[(size_t)t * n + mem[i]] = yp[t][i];
-        B[(size_t)(T + t) * n + mem[i]] = (float)Qt[t](i, 0);
-        for (int j = 0; j < n_V; ++j) B[(size_t)(2 * T + t * n_V + j) * n + mem[i]] = probes[(size_t)j * nt + i];
-      }
+      yq.insert(yq.end(), yp[t].begin(), yp[t].end());
+      for (int64_t i = 0; i < nt; ++i) yq.push_back((float)Qt[t](i, 0));
+      nmax = std::max(nmax, nt);
       col_trait[t] = t;
       kinds[t] = SLEMM_OP_QUADFORM;
       groups[t] = t;
@@ -168,9 +168,9 @@ extern "C" int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int 
         groups[2 * T + t * n_V + j] = 2 * T + t;
       }
     }
-    if (verbose) printf("Generated %d Rademacher probing vectors for each of %d traits.\n", n_V, T);
     DevF dB((size_t)n * ncol), dX(n);
-    upload(dB.p, B);
+    traits_fill_seed_block(tr, n_V, yq, rademacher_signs(rng_seed, nmax * n_V), dB.p);
+    if (verbose) printf("Generated %d Rademacher probing vectors for each of %d traits.\n", n_V, T);
     const float tau0 = (1 - s2max) / s2max;
     SeedGuard sg;
     abi(slemm_gpu_traits_lseed(tr, col_trait.data(), kinds.data(), groups.data(), tau0, dB.p, n, ncol, tol_L, 600,
