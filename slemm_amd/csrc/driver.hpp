@@ -87,10 +87,11 @@ void extract_dosage_cols(slemm_geno* h, const std::vector<int64_t>& snps, std::v
 void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B);
 // The same probes written to the device block dB (columns of length n, leading dimension ldb).
 void rademacher_probes_device(int rng_seed, int64_t n, int n_V, float* dB, int64_t ldb);
-// The signs (1: -1) of the first `total` Rademacher draws of that stream, in draw order: probe
-// column j of an n-row block holds draws j n .. j n + n - 1, so blocks of different n share the
-// prefix of one stream; every entry of such a column is -rademacher_scale(n) or +rademacher_scale(n).
-std::vector<uint8_t> rademacher_signs(int rng_seed, int64_t total);
+// The signs of the first `total` Rademacher draws of that stream, one bit per draw in draw order
+// (bit e % 8 of byte e / 8 set where draw e is -1; ceil(total / 8) bytes): probe column j of an
+// n-row block holds draws j n .. j n + n - 1, so blocks of different n share the prefix of one
+// stream; every entry of such a column is -rademacher_scale(n) or +rademacher_scale(n) (signs.cpp).
+std::vector<uint8_t> rademacher_sign_bits(int rng_seed, int64_t total);
 float rademacher_scale(int64_t n);
 
 // REML_criterion at h2 (src/reml.cpp:457-489): returns the 2 logLL value the golden-section

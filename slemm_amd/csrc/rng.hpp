@@ -30,12 +30,13 @@ inline float polar_coord(uint32_t u) { return (float)((double)(2.0f * canonical2
 class Mt19937Blocks {
  public:
   static constexpr int kN = 624, kM = 397;
-  explicit Mt19937Blocks(uint32_t seed) {
+  // always inlined, so that a caller compiled for wider vectors (target_clones) vectorises them too
+  __attribute__((always_inline)) explicit Mt19937Blocks(uint32_t seed) {
     x_[0] = seed;
     for (int i = 1; i < kN; ++i) x_[i] = 1812433253u * (x_[i - 1] ^ (x_[i - 1] >> 30)) + (uint32_t)i;
   }
   // the next kN outputs into dst
-  void block(uint32_t* dst) {
+  __attribute__((always_inline)) void block(uint32_t* dst) {
     int i = 0;
     for (; i < kN - kM; ++i) x_[i] = step(x_[i], x_[i + 1], x_[i + kM]);
     for (; i < kN - 1; ++i) x_[i] = step(x_[i], x_[i + 1], x_[i + kM - kN]);
@@ -51,7 +52,7 @@ class Mt19937Blocks {
   }
 
  private:
-  static uint32_t step(uint32_t a, uint32_t b, uint32_t c) {
+  __attribute__((always_inline)) static uint32_t step(uint32_t a, uint32_t b, uint32_t c) {
     const uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
     return c ^ (y >> 1) ^ ((0u - (y & 1u)) & 0x9908b0dfu);
   }

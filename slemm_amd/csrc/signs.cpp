@@ -1,0 +1,80 @@
+// Signs of the Rademacher probes of SLDF_REML (src/reml.cpp:597-605) and of the batched traits'
+// seed blocks: draw e is -1 when uniform_real_distribution<float>(0, 1) applied to word e of
+// mt19937(seed + 54321) is below 0.5.  That distribution evaluates generate_canonical<float, 24>
+// (float(u) / 2^32) times 1 plus 0, which is below 0.5 exactly when float(u) < 2^31, that is for
+// u < 2^31 - 64: words in [2^31 - 128, 2^31) lie between the neighbouring floats 2^31 - 128 and
+// 2^31, and from the midpoint 2^31 - 64 on they round up (the tie to the even 2^31).  So each
+// draw is one integer compare of the tempered word, kept as one bit.  The engine's output is the
+// only sequential part; with AVX2 the twist and tempering run eight words to an instruction and
+// each compare of eight words gives one byte of bits.
+#include <immintrin.h>
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "driver.hpp"
+#include "rng.hpp"
+
+namespace slemm {
+namespace {
+
+constexpr uint32_t kNegBelow = 0x7fffffc0u;  // 2^31 - 64
+
+// Bits of draws [e, e + 624) of one block w into o (bytes ceil(k1 / 8), k1 live draws): bit k of
+// byte b is draw 8 b + k; bits past k1 are zero.
+inline __attribute__((always_inline)) void tail_bits(const uint32_t* w, int64_t k1, uint8_t* o) {
+  for (int64_t b = 0; 8 * b < k1; ++b) {
+    uint32_t byte = 0;
+    for (int k = 0; k < 8 && 8 * b + k < k1; ++k) byte |= (uint32_t)(w[8 * b + k] < kNegBelow) << k;
+    o[b] = (uint8_t)byte;
+  }
+}
+
+// The engine is passed in (constructed by the caller): its inlined twist and tempering are then
+// compiled with this function's instruction set.
+__attribute__((target("avx2"))) void sign_bits_avx2(Mt19937Blocks& mt, int64_t total, uint8_t* bits) {
+  alignas(32) uint32_t w[Mt19937Blocks::kN];
+  const __m256i flip = _mm256_set1_epi32(INT32_MIN);
+  const __m256i lim = _mm256_set1_epi32((int)(kNegBelow ^ 0x80000000u));  // unsigned order as signed
+  for (int64_t e = 0; e < total; e += Mt19937Blocks::kN) {
+    mt.block(w);
+    uint8_t* o = bits + e / 8;  // kN = 624 is a multiple of 8
+    const int64_t k1 = std::min<int64_t>(Mt19937Blocks::kN, total - e);
+    if (k1 < Mt19937Blocks::kN) {
+      tail_bits(w, k1, o);
+      break;
+    }
+    for (int b = 0; b < Mt19937Blocks::kN / 8; ++b) {
+      const __m256i v = _mm256_xor_si256(_mm256_load_si256(reinterpret_cast<const __m256i*>(w + 8 * b)), flip);
+      o[b] = (uint8_t)_mm256_movemask_ps(_mm256_castsi256_ps(_mm256_cmpgt_epi32(lim, v)));
+    }
+  }
+}
+
+void sign_bits_plain(Mt19937Blocks& mt, int64_t total, uint8_t* bits) {
+  uint32_t w[Mt19937Blocks::kN];
+  for (int64_t e = 0; e < total; e += Mt19937Blocks::kN) {
+    mt.block(w);
+    tail_bits(w, std::min<int64_t>(Mt19937Blocks::kN, total - e), bits + e / 8);
+  }
+}
+
+}  // namespace
+
+std::vector<uint8_t> rademacher_sign_bits(int rng_seed, int64_t total) {
+  std::vector<uint8_t> bits((size_t)((std::max<int64_t>(total, 0) + 7) / 8));
+  if (total <= 0) return bits;
+  Mt19937Blocks mt((uint32_t)(rng_seed + 54321));
+  if (__builtin_cpu_supports("avx2")) sign_bits_avx2(mt, total, bits.data());
+  else sign_bits_plain(mt, total, bits.data());
+  return bits;
+}
+
+// 1 / sqrt of the float sum of n unit squares
+float rademacher_scale(int64_t n) {
+  const float ss = n < (1 << 24) ? (float)n : 16777216.0f;
+  return 1.0f / std::sqrt(ss);
+}
+
+}  // namespace slemm

@@ -128,7 +128,7 @@ namespace {
 
 // Thread per member i of trait t (blockIdx.y): its row of the y~, q and probe columns.
 __global__ __launch_bounds__(256) void k_traits_seed_block(const int64_t* __restrict__ mem, const int64_t* __restrict__ off,
-                                                           const float* __restrict__ yq, const uint8_t* __restrict__ neg,
+                                                           const float* __restrict__ yq, const uint8_t* __restrict__ bits,
                                                            const float* __restrict__ qscale, int T, int n_V, int64_t n,
                                                            float* __restrict__ B) {
   const int t = blockIdx.y;
@@ -139,14 +139,16 @@ __global__ __launch_bounds__(256) void k_traits_seed_block(const int64_t* __rest
   B[(int64_t)t * n + row] = yq[2 * o + i];
   B[(int64_t)(T + t) * n + row] = yq[2 * o + nt + i];
   const float q = qscale[t];
-  for (int j = 0; j < n_V; ++j)
-    B[(int64_t)(2 * T + t * n_V + j) * n + row] = neg[(int64_t)j * nt + i] ? -q : q;
+  for (int j = 0; j < n_V; ++j) {
+    const int64_t e = (int64_t)j * nt + i;  // draw index
+    B[(int64_t)(2 * T + t * n_V + j) * n + row] = (bits[e >> 3] >> (e & 7)) & 1 ? -q : q;
+  }
 }
 
 }  // namespace
 
 void traits_fill_seed_block(const slemm_traits* tr, int n_V, const std::vector<float>& yq,
-                            const std::vector<uint8_t>& neg, float* B) {
+                            const std::vector<uint8_t>& bits, float* B) {
   const int T = tr->T;
   const int64_t n = tr->h->n;
   std::vector<int64_t> off(T + 1, 0), mem;
@@ -159,23 +161,23 @@ void traits_fill_seed_block(const slemm_traits* tr, int n_V, const std::vector<f
     nmax = std::max(nmax, nt);
   }
   REQUIRE((int64_t)yq.size() == 2 * off[T], "seed block: phenotype/basis values do not match the subsets");
-  REQUIRE((int64_t)neg.size() >= nmax * n_V, "seed block: too few probe signs");
+  REQUIRE((int64_t)bits.size() * 8 >= nmax * n_V, "seed block: too few probe signs");
   mem.reserve(off[T]);
   for (int t = 0; t < T; ++t) mem.insert(mem.end(), tr->members[t].begin(), tr->members[t].end());
-  DevBuf dmem, doff, dyq, dneg, dqs;
+  DevBuf dmem, doff, dyq, dbits, dqs;
   dmem.reserve(sizeof(int64_t) * mem.size());
   doff.reserve(sizeof(int64_t) * off.size());
   dyq.reserve(sizeof(float) * yq.size());
-  dneg.reserve(std::max<size_t>(neg.size(), 1));
+  dbits.reserve(std::max<size_t>(bits.size(), 1));
   dqs.reserve(sizeof(float) * qs.size());
   HIPCHECK(hipMemcpy(dmem.p, mem.data(), sizeof(int64_t) * mem.size(), hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(doff.p, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(dyq.p, yq.data(), sizeof(float) * yq.size(), hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(dneg.p, neg.data(), neg.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(dbits.p, bits.data(), bits.size(), hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(dqs.p, qs.data(), sizeof(float) * qs.size(), hipMemcpyHostToDevice));
   HIPCHECK(hipMemset(B, 0, sizeof(float) * (size_t)n * T * (2 + n_V)));
   k_traits_seed_block<<<dim3((unsigned)((nmax + 255) / 256), (unsigned)T), 256>>>(
-      dmem.as<int64_t>(), doff.as<int64_t>(), dyq.as<float>(), dneg.as<uint8_t>(), dqs.as<float>(), T, n_V, n, B);
+      dmem.as<int64_t>(), doff.as<int64_t>(), dyq.as<float>(), dbits.as<uint8_t>(), dqs.as<float>(), T, n_V, n, B);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(nullptr));
 }

@@ -50,8 +50,8 @@ void run_op_traits(slemm_traits* tr, const int* col_trait, const int* col_kind, 
 // The batched SLDF seed block B (n x T (2 + n_V), leading dimension n, zero outside each subset):
 // columns [y~_0 .. y~_{T-1}, q_0 .. q_{T-1}, the n_V probes of trait 0, of trait 1, ...], written on
 // the device.  yq (host): per trait t its n_t projected phenotype values then its n_t intercept
-// basis values, traits one after another; neg (host): the signs of the first max_t n_t n_V
-// Rademacher draws (rademacher_signs), trait t's probe j entry i being draw j n_t + i.
+// basis values, traits one after another; bits (host): the sign bits of the first max_t n_t n_V
+// Rademacher draws (rademacher_sign_bits), trait t's probe j entry i being draw j n_t + i.
 void traits_fill_seed_block(const slemm_traits* tr, int n_V, const std::vector<float>& yq,
-                            const std::vector<uint8_t>& neg, float* B);
+                            const std::vector<uint8_t>& bits, float* B);
 }  // namespace slemm

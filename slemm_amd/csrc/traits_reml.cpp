@@ -171,7 +171,7 @@ extern "C" int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int 
       }
     }
     DevF dB((size_t)n * ncol), dX(n);
-    traits_fill_seed_block(tr, n_V, yq, rademacher_signs(rng_seed, nmax * n_V), dB.p);
+    traits_fill_seed_block(tr, n_V, yq, rademacher_sign_bits(rng_seed, nmax * n_V), dB.p);
     if (verbose) printf("Generated %d Rademacher probing vectors for each of %d traits.\n", n_V, T);
     const float tau0 = (1 - s2max) / s2max;
     SeedGuard sg;

@@ -40,3 +40,16 @@ def test_rademacher_probes_equal_libstdcxx(seed, n, nv):
     got = gpu.rademacher_probes(seed, n, nv)
     ref = oracle.rademacher(seed, n, nv)
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_rademacher_sign_is_an_integer_compare():
+    """The library takes a draw as -1 when its engine word u < 2^31 - 64, in place of
+    generate_canonical<float, 24>(u) = float(u) / 2^32 < 0.5: float(u) rounds to nearest even, so
+    from the midpoint 2^31 - 64 of the floats 2^31 - 128 and 2^31 on it reaches 2^31.  Checked on
+    every word within 2^16 of 2^31 and on 2^22 random words."""
+    near = np.arange(2**31 - 2**16, 2**31 + 2**16, dtype=np.uint64).astype(np.uint32)
+    rnd = np.random.default_rng(1).integers(0, 2**32, size=2**22, dtype=np.uint64).astype(np.uint32)
+    for u in (near, rnd):
+        canon = u.astype(np.float32) * np.float32(2.0**-32)
+        assert np.array_equal(canon < np.float32(0.5), u < np.uint32(2**31 - 64))
+    assert (np.float32(np.uint32(2**31 - 65)) < 2**31) and not (np.float32(np.uint32(2**31 - 64)) < 2**31)
