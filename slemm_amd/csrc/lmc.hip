@@ -53,7 +53,7 @@ void normal_draws(uint32_t seed, int64_t count, float* out) {
     if (words > have) {
       const int64_t nblk = (words - have + Mt19937Blocks::kN - 1) / Mt19937Blocks::kN;
       raw.resize((size_t)(have + nblk * Mt19937Blocks::kN));
-      for (int64_t b = 0; b < nblk; ++b) mt.block(&raw[(size_t)(have + b * Mt19937Blocks::kN)]);
+      mt_fill_blocks(mt, nblk, &raw[(size_t)have]);
     }
     const int64_t attempts = (int64_t)raw.size() / 2;
     const int64_t nchunk = (attempts + kChunk - 1) / kChunk;

@@ -59,6 +59,10 @@ class Mt19937Blocks {
   uint32_t x_[kN];
 };
 
+// The next nblk blocks of mt into dst (nblk * 624 words), with AVX2 where the CPU has it
+// (signs.cpp).
+void mt_fill_blocks(Mt19937Blocks& mt, int64_t nblk, uint32_t* dst);
+
 // Runs f(k) for k < parts on up to `threads` host threads (contiguous ranges).
 template <typename F>
 void parallel_parts(int64_t parts, int threads, const F& f) {

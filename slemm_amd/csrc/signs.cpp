@@ -6,7 +6,8 @@
 // 2^31, and from the midpoint 2^31 - 64 on they round up (the tie to the even 2^31).  So each
 // draw is one integer compare of the tempered word, kept as one bit.  The engine's output is the
 // only sequential part; with AVX2 the twist and tempering run eight words to an instruction and
-// each compare of eight words gives one byte of bits.
+// each compare of eight words gives one byte of bits.  mt_fill_blocks gives the same choice of
+// instruction set to callers that take the engine's words themselves.
 #include <immintrin.h>
 
 #include <cmath>
@@ -60,7 +61,20 @@ void sign_bits_plain(Mt19937Blocks& mt, int64_t total, uint8_t* bits) {
   }
 }
 
+__attribute__((target("avx2"))) void fill_blocks_avx2(Mt19937Blocks& mt, int64_t nblk, uint32_t* dst) {
+  for (int64_t b = 0; b < nblk; ++b) mt.block(dst + b * Mt19937Blocks::kN);
+}
+
+void fill_blocks_plain(Mt19937Blocks& mt, int64_t nblk, uint32_t* dst) {
+  for (int64_t b = 0; b < nblk; ++b) mt.block(dst + b * Mt19937Blocks::kN);
+}
+
 }  // namespace
+
+void mt_fill_blocks(Mt19937Blocks& mt, int64_t nblk, uint32_t* dst) {
+  if (__builtin_cpu_supports("avx2")) fill_blocks_avx2(mt, nblk, dst);
+  else fill_blocks_plain(mt, nblk, dst);
+}
 
 std::vector<uint8_t> rademacher_sign_bits(int rng_seed, int64_t total) {
   std::vector<uint8_t> bits((size_t)((std::max<int64_t>(total, 0) + 7) / 8));
