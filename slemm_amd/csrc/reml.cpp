@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "blockprod.hpp"
 #include "driver.hpp"
 
 using namespace slemm;
@@ -195,6 +196,10 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
     const bool replay = criterion_replay();
     const SeedQuadrature quad_y(sg.s, 0, ySy);
 
+    DevBuf dqtx;
+    dqtx.reserve(sizeof(double) * c * c);
+    std::vector<double> qtx_t((size_t)c * c);
+
     // REML_criterion (:457-489)
     auto criterion = [&](float ss, float& vg, float& ve, float& lrt) -> double {
       ++evals;
@@ -216,14 +221,13 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
       const double ldetH = acc * (double)n / (double)n_V;
       lrt = (float)((double)n * std::log(ySy / yPy) - ldetH);
       abi(slemm_gpu_lsolve(sg.s, 1, c, dB.p + n, n, sigma, 1e-6f, dX.p, n, nullptr));
-      download(hx, dX.p, (size_t)n * c);
-      std::vector<double> QtX((size_t)c * c, 0.0);
+      // Q' H^-1 Q from the solution in FP64 on the device (the basis as the handle holds it), c x c
+      // doubles back instead of the n x c solution
+      tn_product(h->Q, h->n_pad, c, dX.p, n, c, n, dqtx.as<double>(), c, h->redpart, nullptr);
+      HIPCHECK(hipMemcpy(qtx_t.data(), dqtx.p, sizeof(double) * c * c, hipMemcpyDeviceToHost));
+      std::vector<double> QtX((size_t)c * c);
       for (int a = 0; a < c; ++a)
-        for (int b = 0; b < c; ++b) {
-          double s = 0.0;
-          for (int64_t i = 0; i < n; ++i) s += Q(i, a) * hx[i + (size_t)b * n];
-          QtX[a + (size_t)b * c] = s;
-        }
+        for (int b = 0; b < c; ++b) QtX[a + (size_t)b * c] = qtx_t[(size_t)a * c + b];  // Q_a' x_b
       return criterion_value((double)(n - c) * std::log(ySy / yPy) - log_abs_det(QtX, c) - ldetH);
     };
 
