@@ -16,6 +16,7 @@
 
 #include "blockprod.hpp"
 #include "lanczos.hpp"
+#include "pipeline.hpp"
 #include "tridiag.hpp"
 
 struct slemm_lseed {
@@ -182,6 +183,7 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
     }
 
     auto s = std::make_unique<slemm_lseed>();
+    bool eager_next = false;  // the operator's launches changed: run one step eagerly
     s->device = h->device;
     s->t = t;
     s->n = n;
@@ -297,7 +299,7 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
       // beta_j of each column to device slot [t, 2t)
       for (int k = 0; k < t; ++k) hb[k] = beta[(size_t)j * t + k];
       HIPCHECK(hipMemcpyAsync(d_dvec + t, hb.data(), sizeof(double) * t, hipMemcpyHostToDevice, st));
-      if (graphs.enabled && j >= 1) {
+      if (graphs.enabled && j >= 1 && !eager_next) {
         hipGraphExec_t& ge = graphs.exec[(j - 1) % 3];
         if (!ge) {
           hipGraph_t gr = nullptr;
@@ -317,6 +319,7 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
         HIPCHECK(hipGraphLaunch(ge, st));
       } else {
         step_work(st);
+        eager_next = false;
       }
       keep_basis(Unext);
       HIPCHECK(hipMemcpyAsync(hd.data(), d_dvec, sizeof(double) * 2 * t, hipMemcpyDeviceToHost, st));
@@ -354,7 +357,15 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
       if (changed) {
         for (int k = 0; k < t; ++k) active[k] = gdone[group[k]] ? 0 : 1;
         HIPCHECK(hipMemcpyAsync(dact.p, active.data(), sizeof(int) * t, hipMemcpyHostToDevice, st));
-        if (on_active && !all_done) on_active(active, st);
+        if (on_active && !all_done && on_active(active, st)) {
+          for (auto& e : graphs.exec)
+            if (e) {
+              HIPCHECK(hipStreamSynchronize(st));
+              (void)hipGraphExecDestroy(e);
+              e = nullptr;
+            }
+          eager_next = true;
+        }
       }
       if (all_done) break;
       // rotate buffers: prev <- cur <- next
@@ -393,11 +404,66 @@ extern "C" int slemm_gpu_lseed(slemm_geno* h, int kind, const int* col_kind, con
     *out = nullptr;
     HIPCHECK(hipSetDevice(h->device));
     require_model(h);
+    // Columns of converged groups leave the operator's block once that cuts the MFMA column
+    // tiles the block product issues (e.g. 102 -> 101 columns: 21 -> 19 int8 digit tiles, or the
+    // last unconverged column alone): the active columns are gathered into a compact block, the
+    // operator runs on it and the results go back to their columns.  Each column's products do
+    // not depend on the other columns of the block, so this changes no column's arithmetic
+    // beyond the reduction split the smaller block may choose.
+    struct Compact {
+      bool on = false;
+      int k = 0;
+      std::vector<std::pair<int, int>> runs;  // (first column, count) of the active columns
+      std::vector<int> kinds;
+      DevBuf in, out;
+    } cp;
     const LanczosOp op = [&](const float* U, int64_t ldu, float* o, int64_t ldo, hipStream_t s) {
-      run_op(h, kind, col_kind, tau0, U, ldu, t, o, ldo, precision, s);
+      if (!cp.on) {
+        run_op(h, kind, col_kind, tau0, U, ldu, t, o, ldo, precision, s);
+        return;
+      }
+      const int64_t ld = h->n_pad, n = h->n;
+      float* ci = cp.in.as<float>();
+      float* co = cp.out.as<float>();
+      int64_t off = 0;
+      for (const auto& r : cp.runs) {
+        HIPCHECK(hipMemcpy2DAsync(ci + off * ld, sizeof(float) * ld, U + r.first * ldu, sizeof(float) * ldu,
+                                  sizeof(float) * n, r.second, hipMemcpyDeviceToDevice, s));
+        off += r.second;
+      }
+      run_op(h, kind, cp.kinds.data(), tau0, ci, ld, cp.k, co, ld, precision, s);
+      off = 0;
+      for (const auto& r : cp.runs) {
+        HIPCHECK(hipMemcpy2DAsync(o + r.first * ldo, sizeof(float) * ldo, co + off * ld, sizeof(float) * ld,
+                                  sizeof(float) * n, r.second, hipMemcpyDeviceToDevice, s));
+        off += r.second;
+      }
+    };
+    // SLEMM_LSEED_COMPACT=0 (read per call) keeps every column in the block
+    const char* ecp = std::getenv("SLEMM_LSEED_COMPACT");
+    const bool compact = !(ecp && ecp[0] == '0');
+    const ActiveFn on_active = [&](const std::vector<int>& active, hipStream_t) -> bool {
+      if (!compact) return false;
+      int k = 0;
+      for (int c = 0; c < t; ++c) k += active[c] ? 1 : 0;
+      if (k == 0 || (cp.on && k == cp.k) || op_column_tiles(h, k, precision) >= op_column_tiles(h, t, precision))
+        return false;
+      cp.on = true;
+      cp.k = k;
+      cp.runs.clear();
+      cp.kinds.clear();
+      for (int c = 0; c < t; ++c) {
+        if (!active[c]) continue;
+        if (!cp.runs.empty() && cp.runs.back().first + cp.runs.back().second == c) ++cp.runs.back().second;
+        else cp.runs.emplace_back(c, 1);
+        cp.kinds.push_back(col_kind ? col_kind[c] : kind);
+      }
+      cp.in.reserve(sizeof(float) * h->n_pad * k);
+      cp.out.reserve(sizeof(float) * h->n_pad * k);
+      return true;
     };
     *out = lseed_run(h, op, col_group, B, ldb, t, tol, maxit, keep_cols, verbose, (hipStream_t)stream,
-                     lanczos_graphs());
+                     lanczos_graphs(), on_active);
   });
 }
 

@@ -12,8 +12,10 @@ namespace slemm {
 // out (ldo) = A U (ldu) for the t columns of the seed system, on the given stream.
 using LanczosOp = std::function<void(const float* U, int64_t ldu, float* out, int64_t ldo, hipStream_t st)>;
 // Called between steps when columns stop (active[k] = 0 for converged groups), so that an
-// operator can skip their work; never called while a step is being captured.
-using ActiveFn = std::function<void(const std::vector<int>& active, hipStream_t st)>;
+// operator can skip their work; never called while a step is being captured.  Returns true when
+// the operator's launches change (another column count): the captured steps are then dropped,
+// the next step runs eagerly (so that its buffers exist) and later ones are captured again.
+using ActiveFn = std::function<bool(const std::vector<int>& active, hipStream_t st)>;
 
 // Runs the seed systems for the columns of B (n x t, ldb) and returns the seed (owned by the
 // caller, released with slemm_gpu_lseed_destroy).  With use_graph, steps after the first replay

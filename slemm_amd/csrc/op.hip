@@ -617,6 +617,18 @@ static void autotune_i8(slemm_geno* h, const Layout& L, int t, int reps, int* va
   (void)hipEventDestroy(e1);
 }
 
+// MFMA instructions a t-column block's product issues per 16 x 64 tile of the output rows and
+// reduction (int8 16x16x64: one per digit tile; split-BF16 16x16x32: two per 16-column tile with
+// hi and lo as columns, four with separate hi and lo tiles; FP32: one unit per 16-column tile,
+// compared only with FP32 blocks): what dropping columns from a block can save.
+int slemm::op_column_tiles(const slemm_geno* h, int t, int precision) {
+  const Layout L = layout_for(h, t, precision);
+  if (L.i8) return (i8_digits(t).ndc + 15) / 16;
+  const int tiles = (int)(L.KP / 16);
+  if (precision != PREC_BF16X2) return tiles;
+  return (L.hl ? 2 : 4) * tiles;
+}
+
 extern "C" int slemm_gpu_block_kernel(const slemm_geno* h, int t, int precision, int* kernel) {
   return guard([&] {
     REQUIRE(h && kernel, "null argument");

@@ -33,6 +33,8 @@ struct BpChoice {
 // traits: blocks of the batched traits (their narrow blocks stay on the BF16 kernels);
 // allow_i8 = false: the split-BF16 layout whatever the int8 switches say (the autotuner's).
 Layout layout_for(const slemm_geno* h, int t, int precision, bool traits = false, bool allow_i8 = true);
+// MFMA instructions a t-column block's product issues per 16 x 64 output-by-reduction tile
+int op_column_tiles(const slemm_geno* h, int t, int precision);
 void check_precision(int precision);
 void check_dense(const float* p, int64_t ld, int64_t rows, int t, const char* what);
 const int* upload_kinds(slemm_geno* h, int kind, const int* col_kind, int t, bool& any_quad, hipStream_t st);

@@ -366,6 +366,7 @@ extern "C" int slemm_gpu_traits_lseed(slemm_traits* tr, const int* col_trait, co
     const ActiveFn on_active = [&](const std::vector<int>& active, hipStream_t s) {
       tr->active = active;
       upload_tile_limits(tr, t, layout_for(tr->h, t, precision, true), s);
+      return false;  // the same launches, with new tile limits in device memory
     };
     try {
       *out = lseed_run(tr->h, op, col_group, B, ldb, t, tol, maxit, keep_cols, verbose, (hipStream_t)stream,

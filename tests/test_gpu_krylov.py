@@ -41,6 +41,55 @@ def test_lseed_lsolve_against_dense_solve(prob, prec):
         assert rel_err(XQ, refQ) < (1e-4 if prec == "fp32" else 5e-4)
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
+def test_converged_groups_leave_the_block(prob, prec, monkeypatch):
+    """[y, E, 30 probes] in three groups that stop at different steps (E: two columns in the span
+    of three eigenvectors of the operator, done in a few steps): once a group has converged the
+    seed runs the operator on the remaining columns only (33 -> 31 columns cuts the int8 digit
+    tiles from 7 to 6 and the FP32 operand from 48 to 32 columns).  Step counts, the tridiagonal
+    coefficients and the solves agree with the seed that keeps every column in the block
+    (SLEMM_LSEED_COMPACT=0), and the solves with the dense FP64 ones."""
+    p, g = prob
+    D = p.dense()
+    tau0 = (1 - 0.7) / 0.7
+    rng = np.random.default_rng(4)
+    y = rng.normal(size=p.n).astype(np.float32)
+    yp = (y - p.Q @ (p.Q.T @ y)).astype(np.float32)
+    w, U = D.eig("ldet")
+    c = 2
+    E = U[:, -3:] @ rng.normal(size=(3, c))
+    E /= np.linalg.norm(E, axis=0)
+    nv = 30
+    V = oracle.rademacher(3, p.n, nv)
+    B = np.ascontiguousarray(np.concatenate([yp[:, None], E, V], axis=1).astype(np.float32))
+    kinds = ["quadform"] + ["ldet"] * (c + nv)
+    groups = [0] + [1] * c + [2] * nv
+    seeds = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("SLEMM_LSEED_COMPACT", mode)
+        seeds[mode] = g.L_Seed("ldet", tau0, torch.from_numpy(B).cuda(), tol=1e-6, maxit=400,
+                               keep_cols=1 + c, precision=prec, col_kind=kinds, col_group=groups)
+    a, b = seeds["1"], seeds["0"]
+    cs = a.col_steps
+    assert int(cs[1]) < min(int(cs[0]), int(cs[1 + c])), cs  # E stops first
+    assert np.abs(a.col_steps.astype(int) - b.col_steps.astype(int)).max() <= 1
+    ba, da = a.coeffs()
+    bb, db = b.coeffs()
+    k = min(5, int(cs.min()))
+    assert rel_err(da[:, :k], db[:, :k]) < 1e-5
+    assert rel_err(ba[:, :k + 1], bb[:, :k + 1]) < 1e-5
+    tol = 1e-4 if prec == "fp32" else 5e-4
+    for h2 in (0.2, 0.5):
+        tau = (1 - h2) / h2
+        ref = D.solve("quadform", tau, B[:, :1])
+        for s in (a, b):
+            X = s.L_Solve(torch.from_numpy(B[:, :1]).cuda(), tau - tau0, tol=1e-6, col0=0).cpu().numpy()
+            assert rel_err(X, ref) < tol
+        la = a.SLQ_LDet(tau - tau0, col0=1 + c, nv=nv)
+        lb = b.SLQ_LDet(tau - tau0, col0=1 + c, nv=nv)
+        assert abs(la - lb) <= 5e-5 * abs(lb), (la, lb)
+
+
 def test_lseed_matches_cpu_restatement(prob):
     """Same inputs, same stopping rule: step counts agree and the tridiagonal coefficients match
     the FP32 CPU restatement to FP32 rounding (the first steps, before rounding compounds)."""
