@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <memory>
 #include <random>
 #include <string>
 #include <vector>
@@ -199,6 +200,11 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
     DevBuf dqtx;
     dqtx.reserve(sizeof(double) * c * c);
     std::vector<double> qtx_t((size_t)c * c);
+    // One covariate: Q'H^-1Q from the quadrature of Q's own seed column, as y'Py (smooth in sigma,
+    // no device work per evaluation); several: the cross terms need the solutions.  The
+    // reference's arithmetic (replay) forms both from L_Solve solutions.
+    std::unique_ptr<SeedQuadrature> quad_q;
+    if (c == 1 && !replay) quad_q = std::make_unique<SeedQuadrature>(sg.s, 1, sq_norm(B.data() + n, n));
 
     // REML_criterion (:457-489)
     auto criterion = [&](float ss, float& vg, float& ve, float& lrt) -> double {
@@ -220,14 +226,18 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
         for (size_t q = 0; q < Dk[j].size(); ++q) acc += Wk[j][q] * Wk[j][q] * std::log(Dk[j][q] + sigma);
       const double ldetH = acc * (double)n / (double)n_V;
       lrt = (float)((double)n * std::log(ySy / yPy) - ldetH);
-      abi(slemm_gpu_lsolve(sg.s, 1, c, dB.p + n, n, sigma, 1e-6f, dX.p, n, nullptr));
-      // Q' H^-1 Q from the solution in FP64 on the device (the basis as the handle holds it), c x c
-      // doubles back instead of the n x c solution
-      tn_product(h->Q, h->n_pad, c, dX.p, n, c, n, dqtx.as<double>(), c, h->redpart, nullptr);
-      HIPCHECK(hipMemcpy(qtx_t.data(), dqtx.p, sizeof(double) * c * c, hipMemcpyDeviceToHost));
       std::vector<double> QtX((size_t)c * c);
-      for (int a = 0; a < c; ++a)
-        for (int b = 0; b < c; ++b) QtX[a + (size_t)b * c] = qtx_t[(size_t)a * c + b];  // Q_a' x_b
+      if (quad_q) {
+        QtX[0] = quad_q->inverse(sigma);
+      } else {
+        abi(slemm_gpu_lsolve(sg.s, 1, c, dB.p + n, n, sigma, 1e-6f, dX.p, n, nullptr));
+        // Q' H^-1 Q from the solution in FP64 on the device (the basis as the handle holds it),
+        // c x c doubles back instead of the n x c solution
+        tn_product(h->Q, h->n_pad, c, dX.p, n, c, n, dqtx.as<double>(), c, h->redpart, nullptr);
+        HIPCHECK(hipMemcpy(qtx_t.data(), dqtx.p, sizeof(double) * c * c, hipMemcpyDeviceToHost));
+        for (int a = 0; a < c; ++a)
+          for (int b = 0; b < c; ++b) QtX[a + (size_t)b * c] = qtx_t[(size_t)a * c + b];  // Q_a' x_b
+      }
       return criterion_value((double)(n - c) * std::log(ySy / yPy) - log_abs_det(QtX, c) - ldetH);
     };
 

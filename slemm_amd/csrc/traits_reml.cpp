@@ -204,6 +204,11 @@ extern "C" int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int 
       int evals = 0;
       const bool replay = criterion_replay();
       const SeedQuadrature quad_y(sg.s, t, ySy);
+      // q'H^-1q of the trait's intercept from the quadrature of its own seed column, as y'Py (the
+      // reference's arithmetic, replay: from the L_Solve solution)
+      double qSq = 0.0;
+      for (int64_t i = 0; i < nt; ++i) qSq += (double)(float)Qt[t](i, 0) * (double)(float)Qt[t](i, 0);
+      const SeedQuadrature quad_q(sg.s, T + t, qSq);
       // REML_criterion (src/reml.cpp:457-489) on the trait's subset (n_t individuals, c = 1)
       auto criterion = [&](float ss, float& vg, float& ve, float& lrt) -> double {
         ++evals;
@@ -223,8 +228,13 @@ extern "C" int slemm_gpu_traits_sldf_reml(slemm_traits* tr, const float* y, int 
           for (size_t q = 0; q < Dk[j].size(); ++q) acc += Wk[j][q] * Wk[j][q] * std::log(Dk[j][q] + sigma);
         const double ldetH = acc * (double)nt / (double)n_V;
         lrt = (float)((double)nt * std::log(ySy / yPy) - ldetH);
-        abi(slemm_gpu_lsolve(sg.s, T + t, 1, qcol, n, sigma, 1e-6f, dX.p, n, nullptr));
-        const double QtX = dot_dev(qcol, dX.p);
+        double QtX = 0.0;
+        if (replay) {
+          abi(slemm_gpu_lsolve(sg.s, T + t, 1, qcol, n, sigma, 1e-6f, dX.p, n, nullptr));
+          QtX = dot_dev(qcol, dX.p);
+        } else {
+          QtX = quad_q.inverse(sigma);
+        }
         return criterion_value((double)(nt - 1) * std::log(ySy / yPy) - std::log(std::fabs(QtX)) - ldetH);
       };
       float vg = 0, ve = 0, lrt = 0;
