@@ -345,25 +345,28 @@ namespace slemm {
 
 // ------------------------------------------------------------------ pack (pass-1 operand) ----
 // x_ik = rr_i * (v_ik - sum_q Q_iq a_qk)   (projection only for QUADFORM columns, a = Q'V)
-// A block covers 256 rows x 64 columns.  The block's V tile is first staged in LDS with
-// row-contiguous reads (thread = row, one column per load), then thread (tx = column within 64,
-// ty) packs 8 blocks of 8 rows of its column, so the packed writes stay column-adjacent too.
+// A block covers kPackRows = 64 rows x 64 columns (17 KB of LDS, so that eight blocks fit a CU
+// and keep enough loads in flight).  The block's V tile is first staged in LDS with
+// row-contiguous reads (lane = row, wave ty loading columns ty, ty + 4, ...), then thread
+// (tx = column within 64, ty) packs 2 blocks of 8 rows of its column, so the packed writes stay
+// column-adjacent too.
 __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
   __shared__ double red[4][64];
-  __shared__ float vt[64][257];  // [column][row]; the pad makes the column-wise reads conflict-free
-  __shared__ float rrs[256];     // the rows' r_rsqrt
+  __shared__ float vt[64][kPackRows + 1];  // [column][row]; the pad makes the column-wise reads conflict-free
+  __shared__ float rrs[kPackRows];         // the rows' r_rsqrt
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int k = blockIdx.x * 64 + tx;
-  const int64_t rblk = (int64_t)blockIdx.y * 256;
-  const int64_t rbase = rblk + ty * 64;
+  const int64_t rblk = (int64_t)blockIdx.y * kPackRows;
+  const int64_t rbase = rblk + ty * (kPackRows / 4);
   {
-    // 16 loads in flight per thread (the tile is latency-bound otherwise)
-    const int64_t i = rblk + threadIdx.x;
+    const int64_t i = rblk + tx;
     const int kmax = min(64, a.t - (int)blockIdx.x * 64);
-    if (a.rr) rrs[threadIdx.x] = i < a.n ? a.rr[i] : 0.f;
-#pragma unroll 16
-    for (int kk = 0; kk < 64; ++kk)
-      vt[kk][threadIdx.x] = (kk < kmax && i < a.n) ? a.V[i + (int64_t)(blockIdx.x * 64 + kk) * a.ldv] : 0.f;
+    if (a.rr && ty == 0) rrs[tx] = i < a.n ? a.rr[i] : 0.f;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {  // 16 loads in flight per thread
+      const int kk = ty + 4 * q;
+      vt[kk][tx] = (kk < kmax && i < a.n) ? a.V[i + (int64_t)(blockIdx.x * 64 + kk) * a.ldv] : 0.f;
+    }
   }
   __syncthreads();
   double csum = 0.0;
@@ -378,7 +381,7 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
   // epilogue), so the MFMA accumulates a centred sum and a constant column costs no accuracy
   const float xbar = (live && a.xsum) ? (float)(a.xsum[k] / nk) : 0.f;
   if (k < a.KP) {
-    for (int b8 = 0; b8 < 8; ++b8) {
+    for (int b8 = 0; b8 < kPackRows / 32; ++b8) {
       const int64_t i0 = rbase + b8 * 8;
       float x[8];
 #pragma unroll
@@ -438,7 +441,7 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
 }
 
 void launch_pack(const PackArgs& a, hipStream_t st) {
-  dim3 grid((unsigned)((a.KP + 63) / 64), (unsigned)(a.rows_pad / 256));
+  dim3 grid((unsigned)((a.KP + 63) / 64), (unsigned)(a.rows_pad / kPackRows));
   k_pack<<<grid, 256, 0, st>>>(a);
   HIPCHECK(hipGetLastError());
 }

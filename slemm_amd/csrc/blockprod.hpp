@@ -58,6 +58,8 @@ struct TraitCols {
   const int64_t* tile_rows = nullptr;
 };
 
+constexpr int64_t kPackRows = 64;  // rows per workgroup of the pack kernel (one partial sum each)
+
 struct PackArgs {
   const float* V = nullptr;
   int64_t ldv = 0;
@@ -78,7 +80,7 @@ struct PackArgs {
   // block (columns [t, tail0 + 2 tail_r) hold those halves); pad columns beyond get their hi only
   int tail0 = 0, tail_r = 0;
   const double* xsum = nullptr; // if set, x_ik -= float(xsum_k / n) before packing
-  double* part = nullptr;       // (rows_pad/256) x KP partial column sums
+  double* part = nullptr;       // (rows_pad / kPackRows) x KP partial column sums
   const float* wsum = nullptr;  // optional row weights of the column sums (sum_i wsum_i x_ik)
   TraitCols tc;                 // batched traits: per-column subset, intercept and de-meaning
 };

@@ -98,7 +98,7 @@ void pack_pass1(slemm_geno* h, const Layout& L, const float* V, int64_t ldv, int
                 const double* qtv, const int* kinds, int precision, hipStream_t st, bool center,
                 const TraitCols* tc) {
   h->x1.reserve(L.x1_bytes);
-  const int64_t nparts = h->n_pad / 256;
+  const int64_t nparts = h->n_pad / kPackRows;
   h->part1.reserve(sizeof(double) * nparts * L.KP);
   h->colsum.reserve(sizeof(double) * 3 * L.KP);
   double* xsum = h->colsum.as<double>() + 2 * L.KP;
@@ -358,7 +358,7 @@ size_t op_workspace_bytes(const slemm_geno* h, int kind, const int* col_kind, in
     add(sizeof(double) * ((h->n + 16383) / 16384) * h->c * t);          // redpart (Q'V, Q'out)
   }
   add(L.x1_bytes);
-  add(sizeof(double) * (h->n_pad / 256) * L.KP);                         // part1
+  add(sizeof(double) * (h->n_pad / kPackRows) * L.KP);                   // part1
   add(sizeof(double) * 3 * L.KP);                                        // colsum
   add(L.x2_bytes);
   if (use_lds(precision)) {
@@ -848,7 +848,7 @@ extern "C" int slemm_gpu_z(slemm_geno* h, const float* U, int64_t ldu, int t, fl
     const Layout L = layout_for(h, t, precision);
     // pack U as the pass-2 operand; gm-weighted column sums give the centring term
     h->x2.reserve(L.x2_bytes);
-    const int64_t nparts = h->m_pad / 256;
+    const int64_t nparts = h->m_pad / kPackRows;
     h->part2.reserve(sizeof(double) * nparts * L.KP);
     h->colsum.reserve(sizeof(double) * 3 * L.KP);
     PackArgs p;
