@@ -625,24 +625,30 @@ def differentiation_run(g, n, m, y, vg, ve, nv=64, win_size=100):
     gen = torch.Generator(device=dev).manual_seed(17)
     V = torch.where(torch.rand((n, nv), generator=gen, device=dev) < 0.5, -1.0, 1.0) / np.sqrt(n)
     prob = RemlProblem(g, X, y, probes=V, tol=1e-5)
-    w = w0.clone().requires_grad_(True)
-    a = torch.tensor(vg, dtype=torch.float64, device=dev, requires_grad=True)
-    b = torch.tensor(ve, dtype=torch.float64, device=dev, requires_grad=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ll = loglik(prob, w, a, b, win)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ll.backward()
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
+    # the first forward/backward pays for the workspaces of the 2(nv+1)-column adjoint block and
+    # the first launches (timed apart as first_call_s); the timed pair is the second
+    times = []
+    for _ in range(2):
+        w = w0.clone().requires_grad_(True)
+        a = torch.tensor(vg, dtype=torch.float64, device=dev, requires_grad=True)
+        b = torch.tensor(ve, dtype=torch.float64, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ll = loglik(prob, w, a, b, win)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ll.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        times.append((t1 - t0, t2 - t1))
     # the device score on the same probes, with the same window weights as persnp
     g.set_model(w0[win].float().cpu().numpy(), None, None)
     score, _ = g.reml_score(X, y, vg, ve, probes=V, tol_L=1e-5)
     got = np.array([float(a.grad), float(b.grad)])
     g.set_model()
     return {"workload": f"n={n} m={m}, {nwin} windows of {win_size} SNPs, {nv} probes + y",
-            "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
+            "forward_s": round(times[1][0], 3), "backward_s": round(times[1][1], 3),
+            "first_call_s": {"forward": round(times[0][0], 3), "backward": round(times[0][1], 3)},
             "loglik": float(ll.detach()), "dl_dsigma_g2": got[0], "dl_dsigma_e2": got[1],
             "dl_dw_norm": float(torch.linalg.vector_norm(w.grad)),
             "device_score": [float(score[0]), float(score[1])],
