@@ -196,3 +196,35 @@ def test_squared_pass_splits_past_4m_individuals(gpu, monkeypatch):
         assert np.max(np.abs(xx - wxx) / wxx) < 1e-4, np.max(np.abs(xx - wxx) / wxx)
     finally:
         g.close()
+
+
+def test_c5_sldf_compacted_seed_equals_full_block(gpu, monkeypatch):
+    """SLDF-REML at n = 1,000,000 x m = 60,000 with 100 probes: the block holds y, the intercept
+    and the probes (102 columns); the intercept's system stops after a couple of steps, after
+    which the seed runs the operator on the 101 columns still running (19 int8 digit tiles in
+    one launch instead of 21 in two), and on y alone once the probes stop.  The fit equals the
+    one that keeps all 102 columns in the block (SLEMM_LSEED_COMPACT=0) to the rounding of the
+    reduction splits."""
+    n, m = 1_000_000, 60_000
+    g = gpu.Geno.synthesize(n, m, seed=5, maf_lo=0.01, maf_hi=0.5, missing=0.0)
+    try:
+        g.set_model()
+        rng = np.random.default_rng(8)
+        beta = np.zeros(m, np.float32)
+        beta[rng.choice(m, size=2000, replace=False)] = rng.normal(size=2000)
+        gv = g.matvec(torch.from_numpy(beta[:, None]).cuda(), precision="fp32")[:, 0].double().cpu().numpy()
+        ve = gv.var() * (1 - 0.4) / 0.4
+        y = (gv + rng.normal(scale=np.sqrt(ve), size=n) + 5.0).astype(np.float32)
+        X = np.ones((n, 1), np.float32)
+        fits = {}
+        for mode in ("1", "0"):
+            monkeypatch.setenv("SLEMM_LSEED_COMPACT", mode)
+            fits[mode] = g.SLDF_REML(X, y, rng_seed=0, n_V=100)
+        a, b = fits["1"], fits["0"]
+        print("steps", a["steps"], "h2", a["h2"], b["h2"])
+        assert a["steps"] == b["steps"] and a["steps"][1] < min(a["steps"][0], a["steps"][2])
+        for k in ("vg", "ve", "h2"):
+            assert abs(a[k] - b[k]) <= 1e-5 * abs(b[k]), (k, a[k], b[k])
+        assert np.max(np.abs(a["snp_blup"] - b["snp_blup"])) <= 1e-4 * np.max(np.abs(b["snp_blup"]))
+    finally:
+        g.close()
