@@ -34,6 +34,7 @@ typedef int i32x4_t __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kI8Blocks = 512;  // column-maximum partials per vector (row ranges of k_i8_colmax)
+static_assert(kI8Blocks % 64 == 0, "k_i8_scale gives each lane kI8Blocks / 64 partials");
 
 // Digit columns: vectors [0, 16 nvt) in tiles of 16, digit p of vector k in column
 // 16 (3 (k / 16) + p) + k % 16 (the three digits of a vector in the same lane of three tiles);
@@ -81,12 +82,16 @@ __global__ __launch_bounds__(256) void k_i8_colmax(const float* __restrict__ X, 
 // in double (a float would underflow for columns below 2^-127).  A column with a NaN or infinite
 // element (partial +inf) gets ex[k] = INT32_MIN and a NaN scale: zero digits, and all its
 // products come out NaN, as the floating-point kernels' do.
-__global__ __launch_bounds__(256) void k_i8_scale(const float* __restrict__ pmax, int t, double* __restrict__ sc,
-                                                  int* __restrict__ ex) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= t) return;
+// One wave per column (the maximum is exact in any order).
+__global__ __launch_bounds__(64) void k_i8_scale(const float* __restrict__ pmax, int t, double* __restrict__ sc,
+                                                 int* __restrict__ ex) {
+  const int k = blockIdx.x, l = threadIdx.x;
   float mx = 0.f;
-  for (int q = 0; q < kI8Blocks; ++q) mx = fmaxf(mx, pmax[k * kI8Blocks + q]);
+#pragma unroll
+  for (int u = 0; u < kI8Blocks / 64; ++u) mx = fmaxf(mx, pmax[(int64_t)k * kI8Blocks + l + 64 * u]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+  if (l != 0) return;
   if (!(mx <= FLT_MAX)) {
     sc[k] = __builtin_nan("");
     ex[k] = INT32_MIN;
@@ -155,8 +160,9 @@ __global__ __launch_bounds__(256) void k_i8_quant(const float* __restrict__ X, i
 // per-thread sums of the quantization (the sum over the digits themselves, so the exact-centring
 // terms of the pass epilogues match what the MFMAs multiply), NaN for a non-finite column.  Two
 // stages in fixed order: block (x, y) sums the y-th of kI8SumBlocks contiguous ranges of q for
-// columns 64 x .. 64 x + 63 (each wave reads 512 contiguous bytes), then one thread per column.
-constexpr int kI8SumBlocks = 64;
+// columns 64 x .. 64 x + 63 (each wave reads 512 contiguous bytes), then one wave per column.
+constexpr int kI8SumBlocks = 256;
+static_assert(kI8SumBlocks % 64 == 0, "the second stage gives each lane kI8SumBlocks / 64 stage sums");
 
 __global__ __launch_bounds__(256) void k_i8_digit_sums1(const double* __restrict__ psum, int64_t nq, int t,
                                                         double* __restrict__ part) {
@@ -166,20 +172,32 @@ __global__ __launch_bounds__(256) void k_i8_digit_sums1(const double* __restrict
   const int64_t per = (nq + kI8SumBlocks - 1) / kI8SumBlocks;
   const int64_t q0 = (int64_t)blockIdx.y * per, q1 = min(nq, q0 + per);
   double s = 0.0;
-  if (k < t)
-    for (int64_t q = q0 + ty; q < q1; q += 4) s += psum[q * t + k];
+  if (k < t) {
+    // four independent sums, so that four loads are in flight per thread
+    double s4[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t q = q0 + ty;
+    for (; q + 12 < q1; q += 16) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s4[u] += psum[(q + 4 * u) * t + k];
+    }
+    for (; q < q1; q += 4) s4[0] += psum[q * t + k];
+    s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+  }
   red[ty][tx] = s;
   __syncthreads();
   if (ty == 0 && k < t) part[(int64_t)blockIdx.y * t + k] = (red[0][tx] + red[1][tx]) + (red[2][tx] + red[3][tx]);
 }
 
-__global__ __launch_bounds__(256) void k_i8_digit_sums2(const double* __restrict__ part, int t,
-                                                        const int* __restrict__ ex, double* __restrict__ out) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= t) return;
+// one wave per column: lane l sums stage sums l, l + 64, ..., then a fixed butterfly
+__global__ __launch_bounds__(64) void k_i8_digit_sums2(const double* __restrict__ part, int t,
+                                                       const int* __restrict__ ex, double* __restrict__ out) {
+  const int k = blockIdx.x, l = threadIdx.x;
   double s = 0.0;
-  for (int y = 0; y < kI8SumBlocks; ++y) s += part[(int64_t)y * t + k];
-  out[k] = ex[k] == INT32_MIN ? __builtin_nan("") : ldexp(s, ex[k] - 22);
+#pragma unroll
+  for (int u = 0; u < kI8SumBlocks / 64; ++u) s += part[(int64_t)(l + 64 * u) * t + k];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (l == 0) out[k] = ex[k] == INT32_MIN ? __builtin_nan("") : ldexp(s, ex[k] - 22);
 }
 
 struct I8Args {
@@ -825,12 +843,12 @@ void launch_bp_i8(const I8Launch& l, hipStream_t st) {
     fail(SLEMM_ERR_ARG, "int8 block: scratch alignment");
   const float* X = static_cast<const float*>(l.X);
   k_i8_colmax<<<dim3(kI8Blocks, (unsigned)((l.t + 63) / 64)), 256, 0, st>>>(X, l.KP, l.t, l.red_pad, pmax);
-  k_i8_scale<<<(unsigned)((l.t + 255) / 256), 256, 0, st>>>(pmax, l.t, sc, ex);
+  k_i8_scale<<<(unsigned)l.t, 64, 0, st>>>(pmax, l.t, sc, ex);
   k_i8_quant<<<(unsigned)((nstep * 4 * l.t + 255) / 256), 256, 0, st>>>(X, l.KP, l.t, dl.nvt, dl.ndc, nstep, ex,
                                                                          l.sum_w, B, psum);
   if (l.sum_out) {
     k_i8_digit_sums1<<<dim3((unsigned)((l.t + 63) / 64), kI8SumBlocks), 256, 0, st>>>(psum, 4 * nstep, l.t, spart);
-    k_i8_digit_sums2<<<(unsigned)((l.t + 255) / 256), 256, 0, st>>>(spart, l.t, ex, l.sum_out);
+    k_i8_digit_sums2<<<(unsigned)l.t, 64, 0, st>>>(spart, l.t, ex, l.sum_out);
   }
   if (narrow) {
     I8Args a;
