@@ -92,11 +92,13 @@ __global__ void k_compact(const uint8_t* __restrict__ raw, int64_t raw_ld, int r
   gs[(int64_t)r * ld_gs + ob] = (uint8_t)out;
 }
 
-// gs (SNP-major) -> gi (individual-major).  Tile: 512 SNPs x 256 individuals; 256 threads,
-// thread t writes the 128 bytes (512 SNPs, one whole cache line) of individual i0+t, so that no
-// output line is shared between workgroups (which may sit on different XCDs and L2s).  The
-// transpose of packed 2-bit codes is its own inverse: launched on gi (rows = individuals) over a
-// (m_pad / 256, n_pad / 512) grid it writes gs.
+// gs (SNP-major) -> gi (individual-major).  Tile: 512 SNPs x 256 individuals; 256 threads.
+// Thread t assembles the 128 bytes (512 SNPs, one whole cache line) of individual i0 + t from the
+// staged tile, so that no output line is shared between workgroups (which may sit on different
+// XCDs and L2s); the lines go back through the same LDS so that eight neighbouring lanes store
+// each line's eight 16-byte pieces (whole lines per store instead of 64 lines touched 16 bytes
+// at a time).  The transpose of packed 2-bit codes is its own inverse: launched on gi (rows =
+// individuals) over a (m_pad / 256, n_pad / 512) grid it writes gs.
 __global__ __launch_bounds__(256) void k_transpose(const uint8_t* __restrict__ gs, int64_t ld_gs,
                                                    uint8_t* __restrict__ gi, int64_t ld_gi) {
   __shared__ uint32_t tile[512][64 / 4 + 1];  // 512 SNP rows x 64 bytes (+pad against conflicts)
@@ -117,21 +119,29 @@ __global__ __launch_bounds__(256) void k_transpose(const uint8_t* __restrict__ g
   __syncthreads();
   const int dw = t >> 4;          // dword holding individual t within a row
   const int sh = 2 * (t & 15);    // bit offset of its code in that dword
-  uint4* dst = reinterpret_cast<uint4*>(gi + (i0 + t) * ld_gi + j0 / 4);
-#pragma unroll 1
-  for (int g = 0; g < 8; ++g) {
-    uint32_t outw[4];
+  uint32_t outw[32];              // the individual's 512 codes, 16 to a word
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      uint32_t acc = 0;
+  for (int w = 0; w < 32; ++w) {
+    uint32_t acc = 0;
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const uint32_t code = (tile[(g * 4 + w) * 16 + q][dw] >> sh) & 3u;
-        acc |= code << (2 * q);
-      }
-      outw[w] = acc;
-    }
-    dst[g] = make_uint4(outw[0], outw[1], outw[2], outw[3]);
+    for (int q = 0; q < 16; ++q) acc |= ((tile[w * 16 + q][dw] >> sh) & 3u) << (2 * q);
+    outw[w] = acc;
+  }
+  __syncthreads();  // the tile is read; its memory now holds the output lines
+  // output staging: line t's word w at obuf[t * 33 + w] (the odd stride keeps both the row-wise
+  // writes and the column-wise reads free of bank conflicts); 256 x 33 words fit the tile
+  uint32_t* obuf = &tile[0][0];
+  static_assert(256 * 33 <= 512 * 17, "output staging fits the tile");
+#pragma unroll
+  for (int w = 0; w < 32; ++w) obuf[t * 33 + w] = outw[w];
+  __syncthreads();
+  // stores: piece idx = 256 q + t is 16-byte piece idx & 7 of line idx >> 3
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int idx = q * 256 + t;
+    const int line = idx >> 3, pc = idx & 7;
+    const uint32_t* src = obuf + line * 33 + pc * 4;
+    *reinterpret_cast<uint4*>(gi + (i0 + line) * ld_gi + j0 / 4 + pc * 16) = make_uint4(src[0], src[1], src[2], src[3]);
   }
 }
 
