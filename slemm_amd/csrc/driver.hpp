@@ -124,6 +124,12 @@ struct SeedQuadrature {
 void golden_section(float s2min, float s2max, float tol_VC, int verbose, const Criterion& criterion, float& vg,
                     float& ve, float& lrt);
 
+// SLDF_REML (src/reml.cpp:549-690) as slemm_gpu_sldf_reml runs it; with estimates = false the fit
+// stops at the variance components (no Py, SNP BLUP or BLUE; those outputs may be null).
+void sldf_reml_run(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y, int rng_seed, int n_V,
+                   float tol_L, float s2max, float s2min, float tol_VC, int precision, int verbose, float* Py,
+                   float* snp_blup, float* blue, float* var_blue, slemm_reml_stats* stats, bool estimates);
+
 // Final estimates at tau1 = ve/vg from a seed whose column 0 is the projected phenotype and
 // columns 1..c the covariate basis (src/reml.cpp:655-677): Py = R^-1/2 (I-QQ') H^-1 y~,
 // snp_blup = W Zc' Py, BLUE and (optionally) its covariance scaled by vg.

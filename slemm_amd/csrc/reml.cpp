@@ -135,12 +135,12 @@ void finish_estimates(slemm_geno* h, slemm_lseed* seed, const float* dB, float t
 
 }  // namespace slemm
 
-extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y,
-                                   int rng_seed, int n_V, float tol_L, float s2max, float s2min, float tol_VC,
-                                   int precision, int verbose, float* Py, float* snp_blup, float* blue,
-                                   float* var_blue, slemm_reml_stats* stats) {
-  return guard([&] {
-    REQUIRE(h && X && y && Py && snp_blup && blue, "null argument");
+void slemm::sldf_reml_run(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y, int rng_seed, int n_V,
+                          float tol_L, float s2max, float s2min, float tol_VC, int precision, int verbose, float* Py,
+                          float* snp_blup, float* blue, float* var_blue, slemm_reml_stats* stats, bool estimates) {
+  {
+    REQUIRE(h && X && y, "null argument");
+    REQUIRE(!estimates || (Py && snp_blup && blue), "null argument");
     REQUIRE(c >= 1 && ldx >= h->n, "covariates: need c >= 1 and ldx >= n");
     REQUIRE(n_V >= 1, "number of probes must be positive");
     REQUIRE(s2min > 0 && s2max < 1 && s2min < s2max, "heritability range must satisfy 0 < s2min < s2max < 1");
@@ -247,8 +247,9 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
 
     // Py, SNP BLUP, BLUE (:655-677)
     const float tau1 = ve / vg;
-    finish_estimates(h, sg.s, dB.p, tau0, tau1, Q, qrR, X, ldx, c, y, vg, precision, Py, snp_blup, blue,
-                     var_blue);
+    if (estimates)
+      finish_estimates(h, sg.s, dB.p, tau0, tau1, Q, qrR, X, ldx, c, y, vg, precision, Py, snp_blup, blue,
+                       var_blue);
     if (verbose) printf("\nVg,%g\nVe,%g\nh2,%g\nLRT,%g\n", vg, ve, vg / (vg + ve), lrt);
     if (verbose) printf("\nCompleted stochastic Lanczos DF-REML.\n");
     if (s2max - vg / (vg + ve) < tol_VC && verbose)
@@ -266,6 +267,16 @@ extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, i
       stats->seconds_seed = t_seed;
       stats->seconds_total = seconds_since(t_start);
     }
+  }
+}
+
+extern "C" int slemm_gpu_sldf_reml(slemm_geno* h, const float* X, int64_t ldx, int c, const float* y,
+                                   int rng_seed, int n_V, float tol_L, float s2max, float s2min, float tol_VC,
+                                   int precision, int verbose, float* Py, float* snp_blup, float* blue,
+                                   float* var_blue, slemm_reml_stats* stats) {
+  return guard([&] {
+    sldf_reml_run(h, X, ldx, c, y, rng_seed, n_V, tol_L, s2max, s2min, tol_VC, precision, verbose, Py, snp_blup,
+                  blue, var_blue, stats, true);
   });
 }
 

@@ -79,12 +79,13 @@ void traits_one_at_a_time(slemm_traits* tr, const float* y, int rng_seed, int n_
     const int64_t nt = (int64_t)mem.size();
     geno_refill(gk.g, h, mem.data(), nt);
     set_model_weights(gk.g, tr->gw.as<float>() + (size_t)t * h->m_pad);
-    std::vector<float> X(nt, 1.0f), yt(nt), Py(nt), blup(h->m), blue(1), vb(1);
+    std::vector<float> X(nt, 1.0f), yt(nt);
     for (int64_t i = 0; i < nt; ++i) yt[i] = y[(size_t)t * h->n + mem[i]];
     slemm_reml_stats st;
     std::memset(&st, 0, sizeof(st));
-    abi(slemm_gpu_sldf_reml(gk.g, X.data(), nt, 1, yt.data(), rng_seed, n_V, tol_L, s2max, s2min, tol_VC,
-                            precision, verbose, Py.data(), blup.data(), blue.data(), vb.data(), &st));
+    // the variance components only, as the batched plan returns (no Py, SNP BLUP or BLUE)
+    sldf_reml_run(gk.g, X.data(), nt, 1, yt.data(), rng_seed, n_V, tol_L, s2max, s2min, tol_VC, precision,
+                  verbose, nullptr, nullptr, nullptr, nullptr, &st, false);
     vg_out[t] = st.vg;
     ve_out[t] = st.ve;
     h2_out[t] = st.h2;
