@@ -330,7 +330,7 @@ int slemm_gpu_traits_lseed(slemm_traits* tr, const int* col_trait, const int* co
  *    golden-section search per trait on replays of it;
  *  - SEQUENTIAL: one trait at a time on a handle compacted to its individuals;
  *  - AUTO (the default): BATCHED while the batched block products' work (each 16-column tile over
- *    the rows up to the last member of its traits) is at most 1.4 times the one-at-a-time work
+ *    the rows up to the last member of its traits) is at most 1.25 times the one-at-a-time work
  *    (each trait's n_V + 1 columns over its individuals), else SEQUENTIAL.  Nested subsets
  *    (leading individuals) batch; random subsets of very different sizes go one at a time. */
 #define SLEMM_TRAITS_AUTO 0

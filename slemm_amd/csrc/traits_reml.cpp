@@ -28,11 +28,11 @@ namespace {
 // member of any of its traits); one trait at a time costs each trait its n_V + 1 columns over its
 // own individuals.  The batched products run at higher efficiency per column-row (one wide block
 // instead of T narrower ones, one graph per Lanczos step, one seed block built on the device): on
-// n = 200,000 individuals, m = 300,000 SNPs, 32 traits of 32 probes they took 1.76 s against 2.12 s
-// at a work ratio of 1.065 (nested subsets) and 3.12 s against 2.12 s at 2.35 (random subsets).
-// The line through the two batched times crosses the one-at-a-time time at a ratio of 1.4, so
-// the automatic plan batches up to that ratio.
-constexpr double kBatchedWorkRatio = 1.4;
+// n = 200,000 individuals, m = 300,000 SNPs, 32 traits of 32 probes they took 1.61 s against 1.81 s
+// at a work ratio of 1.065 (nested subsets) and 2.87 s against 1.80 s at 2.35 (random subsets).
+// The line through the two batched times crosses the one-at-a-time time at a ratio of 1.27, so
+// the automatic plan batches up to a ratio of 1.25.
+constexpr double kBatchedWorkRatio = 1.25;
 
 double batched_work_ratio(const slemm_traits* tr, int n_V) {
   const int T = tr->T;
