@@ -98,9 +98,14 @@ __global__ void k_compact(const uint8_t* __restrict__ raw, int64_t raw_ld, int r
 // XCDs and L2s); the lines go back through the same LDS so that eight neighbouring lanes store
 // each line's eight 16-byte pieces (whole lines per store instead of 64 lines touched 16 bytes
 // at a time).  The transpose of packed 2-bit codes is its own inverse: launched on gi (rows =
-// individuals) over a (m_pad / 256, n_pad / 512) grid it writes gs.
+// individuals) over a (m_pad / 256, n_pad / 512) grid it writes gs.  With a row list (rows
+// n_rows and beyond read as zero) input row j is row rows[j] of gs, and the rows as read are also
+// written to copy (leading dimension ld_gs): one pass then gathers a subset of individuals into
+// both layouts.
 __global__ __launch_bounds__(256) void k_transpose(const uint8_t* __restrict__ gs, int64_t ld_gs,
-                                                   uint8_t* __restrict__ gi, int64_t ld_gi) {
+                                                   uint8_t* __restrict__ gi, int64_t ld_gi,
+                                                   const int64_t* __restrict__ rows = nullptr, int64_t n_rows = 0,
+                                                   uint8_t* __restrict__ copy = nullptr) {
   __shared__ uint32_t tile[512][64 / 4 + 1];  // 512 SNP rows x 64 bytes (+pad against conflicts)
   const int64_t i0 = (int64_t)blockIdx.x * 256;
   const int64_t j0 = (int64_t)blockIdx.y * 512;
@@ -110,7 +115,15 @@ __global__ __launch_bounds__(256) void k_transpose(const uint8_t* __restrict__ g
   for (int q = 0; q < 8; ++q) {
     const int idx = q * 256 + t;
     const int row = idx >> 2, c4 = idx & 3;
-    const uint4 v = *reinterpret_cast<const uint4*>(gs + (j0 + row) * ld_gs + i0 / 4 + c4 * 16);
+    uint4 v;
+    if (rows) {
+      const int64_t j = j0 + row;
+      v = j < n_rows ? *reinterpret_cast<const uint4*>(gs + rows[j] * ld_gs + i0 / 4 + c4 * 16)
+                     : make_uint4(0u, 0u, 0u, 0u);
+      *reinterpret_cast<uint4*>(copy + j * ld_gs + i0 / 4 + c4 * 16) = v;
+    } else {
+      v = *reinterpret_cast<const uint4*>(gs + (j0 + row) * ld_gs + i0 / 4 + c4 * 16);
+    }
     tile[row][c4 * 4 + 0] = v.x;
     tile[row][c4 * 4 + 1] = v.y;
     tile[row][c4 * 4 + 2] = v.z;
@@ -559,18 +572,15 @@ void geno_refill(slemm_geno* h, const slemm_geno* hs, const int64_t* keep_idx, i
                     &h->partbuf, &h->i8buf})
     if (b->p) HIPCHECK(hipMemset(b->p, 0, b->bytes));
   h->kinds_host.clear();  // the per-column kinds on the device are gone: upload them again
-  // individuals are rows of the individual-major copy: gather them (16-byte copies), then
-  // transpose that into the SNP-major copy
+  // individuals are rows of the individual-major copy: gathered and transposed in one pass
   REQUIRE(h->ld_gi == hs->ld_gi, "handles of different SNP layouts");
   DevBuf dkeep;
   dkeep.reserve(sizeof(int64_t) * n_keep);
   HIPCHECK(hipMemcpy(dkeep.p, keep_idx, sizeof(int64_t) * n_keep, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemset(h->gi + n_keep * h->ld_gi, 0, (size_t)(h->n_pad - n_keep) * h->ld_gi));
-  dim3 grid(8, (unsigned)std::min<int64_t>(n_keep, 65535), (unsigned)((n_keep + 65534) / 65535));
-  k_gather_rows<<<grid, 256>>>(hs->gi, hs->ld_gi, dkeep.as<int64_t>(), n_keep, h->gi);
-  HIPCHECK(hipGetLastError());
-  k_transpose<<<dim3((unsigned)(h->m_pad / 256), (unsigned)(h->n_pad / 512)), 256>>>(h->gi, h->ld_gi, h->gs,
-                                                                                      h->ld_gs);
+  // one pass: the kept rows of the source's individual-major copy (zero rows past n_keep) into
+  // this handle's individual-major copy and, transposed, its SNP-major copy
+  k_transpose<<<dim3((unsigned)(h->m_pad / 256), (unsigned)(h->n_pad / 512)), 256>>>(
+      hs->gi, h->ld_gi, h->gs, h->ld_gs, dkeep.as<int64_t>(), n_keep, h->gi);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipDeviceSynchronize());
 }
