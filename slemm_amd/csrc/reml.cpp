@@ -97,16 +97,13 @@ void finish_estimates(slemm_geno* h, slemm_lseed* seed, const float* dB, float t
   for (int64_t i = 0; i < n; ++i) py[i] *= h->rr_host[i];
   std::copy(py.begin(), py.end(), Py);
   upload(dv.p, py);
-  {
-    DevF dbl(m);
-    abi(slemm_gpu_zt(h, dv.p, n, 1, dbl.p, m, 1, 0, precision, nullptr));
-    HIPCHECK(hipDeviceSynchronize());
-    HIPCHECK(hipMemcpy(snp_blup, dbl.p, sizeof(float) * m, hipMemcpyDeviceToHost));
-  }
-  // BLUE of the fixed effects (:674-677)
+  // BLUE of the fixed effects (:674-677): the solves first, then the SNP BLUP pass is queued and
+  // runs while the host forms X'H^-1X
   std::vector<float> hx;
   abi(slemm_gpu_lsolve(seed, 1, c, dB + n, n, tau1 - tau0, 1e-6f, dX.p, n, nullptr));
   download(hx, dX.p, (size_t)n * c);
+  DevF dbl(m);
+  abi(slemm_gpu_zt(h, dv.p, n, 1, dbl.p, m, 1, 0, precision, nullptr));
   MatD HinvX(n, c);
   for (int j = 0; j < c; ++j)
     for (int64_t i = 0; i < n; ++i) {
@@ -131,6 +128,8 @@ void finish_estimates(slemm_geno* h, slemm_lseed* seed, const float* dB, float t
   for (int q = 0; q < c; ++q) blue[q] = (float)bl[q];
   if (var_blue)
     for (int e = 0; e < c * c; ++e) var_blue[e] = (float)(inv[e] * vg);
+  HIPCHECK(hipDeviceSynchronize());
+  HIPCHECK(hipMemcpy(snp_blup, dbl.p, sizeof(float) * m, hipMemcpyDeviceToHost));
 }
 
 }  // namespace slemm
