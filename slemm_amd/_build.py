@@ -34,14 +34,29 @@ COMMON = [
 EXTRA = {"probe.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
+def _generate() -> None:
+    """Generated sources into OBJ: the MT19937 jump polynomials (csrc/gen_mt_jump.py, a few
+    seconds), remade when the generator is newer than its output."""
+    gen, inc = CSRC / "gen_mt_jump.py", OBJ / "mt_jump.inc"
+    if inc.exists() and inc.stat().st_mtime >= gen.stat().st_mtime:
+        return
+    p = subprocess.run([sys.executable, str(gen)], capture_output=True, text=True, cwd=str(CSRC))
+    if p.returncode != 0:
+        raise RuntimeError("gen_mt_jump.py failed:\n" + p.stderr)
+    tmp = inc.with_suffix(".tmp")
+    tmp.write_text(p.stdout)
+    os.replace(tmp, inc)
+
+
 def _compile(src: Path) -> tuple[Path, str]:
     obj = OBJ / (src.name + ".o")
     if src.suffix == ".hip":
         cmd = [HIPCC, *COMMON, *EXTRA.get(src.name, []), "-c", str(src), "-o", str(obj)]
     else:
-        # host-only translation units: C++ with the HIP runtime headers, no device pass
+        # host-only translation units: C++ with the HIP runtime headers, no device pass; the
+        # generated sources in OBJ on the include path
         cmd = [HIPCC, *[f for f in COMMON if not f.startswith("--offload-arch")], "-x", "c++",
-               "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", str(src), "-o", str(obj)]
+               "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", f"-I{OBJ}", "-c", str(src), "-o", str(obj)]
     p = subprocess.run(cmd, capture_output=True, text=True)
     # hipcc's host pass does not know the packed-fp32 feature and says so; that is harmless
     log = "\n".join(l for l in (p.stdout + p.stderr).splitlines() if "packed-fp32-ops" not in l)
@@ -53,6 +68,7 @@ def _compile(src: Path) -> tuple[Path, str]:
 def build(verbose: bool = False, jobs: int | None = None) -> Path:
     """Compile all sources and link ``libslemm_gpu.so``; returns the library path."""
     OBJ.mkdir(exist_ok=True)
+    _generate()
     sources = sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
     jobs = jobs or min(8, os.cpu_count() or 4)
     objs = []

@@ -35,6 +35,12 @@ class Mt19937Blocks {
     x_[0] = seed;
     for (int i = 1; i < kN; ++i) x_[i] = 1812433253u * (x_[i - 1] ^ (x_[i - 1] >> 30)) + (uint32_t)i;
   }
+  // The state as the window of the last kN untempered words, oldest first: the next block's
+  // outputs are those that follow them.
+  const uint32_t* window() const { return x_; }
+  void set_window(const uint32_t* w) {
+    for (int i = 0; i < kN; ++i) x_[i] = w[i];
+  }
   // the next kN outputs into dst
   __attribute__((always_inline)) void block(uint32_t* dst) {
     int i = 0;

@@ -4,10 +4,12 @@
 // (float(u) / 2^32) times 1 plus 0, which is below 0.5 exactly when float(u) < 2^31, that is for
 // u < 2^31 - 64: words in [2^31 - 128, 2^31) lie between the neighbouring floats 2^31 - 128 and
 // 2^31, and from the midpoint 2^31 - 64 on they round up (the tie to the even 2^31).  So each
-// draw is one integer compare of the tempered word, kept as one bit.  The engine's output is the
-// only sequential part; with AVX2 the twist and tempering run eight words to an instruction and
-// each compare of eight words gives one byte of bits.  mt_fill_blocks gives the same choice of
-// instruction set to callers that take the engine's words themselves.
+// draw is one integer compare of the tempered word, kept as one bit.  With AVX2 the twist and
+// tempering run eight words to an instruction and each compare of eight words gives one byte of
+// bits.  Streams of at least two chunks (kMtJumpChunk words) are cut into up to 16 chunks that
+// host threads produce at once, each starting from the engine state moved ahead by its offset
+// (jump polynomials of gen_mt_jump.py, generated at build time).  mt_fill_blocks gives the same
+// choice of instruction set to callers that take the engine's words themselves.
 #include <immintrin.h>
 
 #include <cmath>
@@ -15,10 +17,38 @@
 #include <vector>
 
 #include "driver.hpp"
+#include "mt_jump.inc"  // kMtJumpChunk, kMtJumpPolys, kMtJumpWords, kMtJumpPoly (build time)
 #include "rng.hpp"
 
 namespace slemm {
 namespace {
+
+static_assert(kMtJumpChunk % Mt19937Blocks::kN == 0 && kMtJumpChunk % 8 == 0,
+              "chunks start on engine blocks and on whole bytes of sign bits");
+
+// out = P(A) w0: the window after e more words, for P = x^e mod phi (phi the minimal polynomial of
+// the one-word step A).  Horner's rule on a linear buffer: each step appends the next word of the
+// accumulator window; where P has a term the start window is added.  The result can differ from
+// the true window only in the 31 low bits of its oldest word, which no later word depends on.
+__attribute__((target("avx2"))) void mt_jump_avx2(const uint32_t* w0, const uint64_t* P, uint32_t* out) {
+  constexpr int N = Mt19937Blocks::kN, M = Mt19937Blocks::kM;
+  int deg = kMtJumpWords * 64 - 1;
+  while (deg > 0 && !((P[deg >> 6] >> (deg & 63)) & 1)) --deg;
+  std::vector<uint32_t> a((size_t)N + deg + 1, 0u);
+  size_t base = 0;
+  for (int i = deg; i >= 0; --i) {
+    if (i != deg) {
+      const uint32_t y = (a[base] & 0x80000000u) | (a[base + 1] & 0x7fffffffu);
+      a[base + N] = a[base + M] ^ (y >> 1) ^ ((0u - (y & 1u)) & 0x9908b0dfu);
+      ++base;
+    }
+    if ((P[i >> 6] >> (i & 63)) & 1) {
+      uint32_t* d = a.data() + base;
+      for (int k = 0; k < N; ++k) d[k] ^= w0[k];
+    }
+  }
+  std::copy(a.begin() + (int64_t)base, a.begin() + (int64_t)base + N, out);
+}
 
 constexpr uint32_t kNegBelow = 0x7fffffc0u;  // 2^31 - 64
 
@@ -79,9 +109,31 @@ void mt_fill_blocks(Mt19937Blocks& mt, int64_t nblk, uint32_t* dst) {
 std::vector<uint8_t> rademacher_sign_bits(int rng_seed, int64_t total) {
   std::vector<uint8_t> bits((size_t)((std::max<int64_t>(total, 0) + 7) / 8));
   if (total <= 0) return bits;
-  Mt19937Blocks mt((uint32_t)(rng_seed + 54321));
-  if (__builtin_cpu_supports("avx2")) sign_bits_avx2(mt, total, bits.data());
-  else sign_bits_plain(mt, total, bits.data());
+  const Mt19937Blocks mt0((uint32_t)(rng_seed + 54321));
+  const bool avx2 = __builtin_cpu_supports("avx2");
+  auto run = [&](Mt19937Blocks& mt, int64_t count, uint8_t* out) {
+    if (avx2) sign_bits_avx2(mt, count, out);
+    else sign_bits_plain(mt, count, out);
+  };
+  const int threads = host_threads();
+  const int64_t L = kMtJumpChunk;
+  if (!avx2 || threads < 2 || total < 2 * L) {
+    Mt19937Blocks mt = mt0;
+    run(mt, total, bits.data());
+    return bits;
+  }
+  // chunk c: draws [c L, (c + 1) L), the last one to the end of the stream
+  const int nch = (int)std::min<int64_t>(kMtJumpPolys + 1, total / L);
+  parallel_parts(nch, threads, [&](int64_t c) {
+    Mt19937Blocks mt = mt0;
+    if (c > 0) {
+      uint32_t w[Mt19937Blocks::kN];
+      mt_jump_avx2(mt0.window(), kMtJumpPoly[c - 1], w);
+      mt.set_window(w);
+    }
+    const int64_t e0 = c * L, e1 = c + 1 == nch ? total : e0 + L;
+    run(mt, e1 - e0, bits.data() + e0 / 8);
+  });
   return bits;
 }
 

@@ -53,3 +53,12 @@ def test_rademacher_sign_is_an_integer_compare():
         canon = u.astype(np.float32) * np.float32(2.0**-32)
         assert np.array_equal(canon < np.float32(0.5), u < np.uint32(2**31 - 64))
     assert (np.float32(np.uint32(2**31 - 65)) < 2**31) and not (np.float32(np.uint32(2**31 - 64)) < 2**31)
+
+
+def test_rademacher_probes_in_parallel_chunks_equal_libstdcxx():
+    """Streams of at least two chunks of 624 * 2^14 draws are produced by host threads from jumped
+    engine states (21,000,030 draws: two chunks, the second to the stream's end, a last partial
+    byte of sign bits); bit for bit the single libstdc++ stream."""
+    got = gpu.rademacher_probes(0, 700_001, 30)
+    ref = oracle.rademacher(0, 700_001, 30)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
