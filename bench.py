@@ -882,6 +882,8 @@ def main():
                     help="skip the n=5k m=50k SLDF/LMC run (GPU, and CPU end to end unless --no-cpu)")
     ap.add_argument("--mfma-sweep", action="store_true",
                     help="print the register-only MFMA issue rates (4 instructions x 1-8 chains x 1-4 waves per SIMD) and exit")
+    ap.add_argument("--only-sldf", action="store_true",
+                    help="time SLDF-REML alone (one untimed fit, then --steps fits) and print a line per fit")
     ap.add_argument("--only-op", action="store_true",
                     help="run only the headline operator steps (for profiling) and print their timing")
     args = ap.parse_args()
@@ -916,6 +918,23 @@ def main():
     torch.cuda.synchronize()
     t_load = time.perf_counter() - t0
     g.set_model(None, None, np.full((n, 1), 1.0 / np.sqrt(n), np.float32))
+
+    if args.only_sldf:
+        # SLDF-REML alone, as reml_time_to_converge runs it: one untimed fit, then --steps timed
+        # fits (for a kernel trace of the time to converge)
+        y, _, _, rh2 = simulate_phenotype(g, n, m, args.causal, args.h2, 11)
+        X = np.ones((n, 1), np.float32)
+        g.autotune(nv + 2, reps=2)
+        g.SLDF_REML(X, y, rng_seed=0, n_V=nv)
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = g.SLDF_REML(X, y, rng_seed=0, n_V=nv)
+            torch.cuda.synchronize()
+            print(json.dumps({"workload": f"SLDF-REML n={n} m={m}, {nv} probes", "seconds": round(time.perf_counter() - t0, 4),
+                              "seconds_seed": round(r["seconds_seed"], 4), "lanczos_steps": list(r["steps"]),
+                              "h2": round(float(r["h2"]), 6)}), flush=True)
+        return
 
     # block: column 0 = phenotype-like RHS (projected operator), 1..nv = Rademacher probes; stored
     # column-major (n x K, leading dimension n) as the C ABI and the reference's Eigen blocks hold it
