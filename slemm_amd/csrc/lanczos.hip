@@ -254,7 +254,16 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
     float* Ucur = bufA.as<float>();
     float* Unext = bufB.as<float>();
     float* spare = bufC.as<float>();
-    std::vector<double> hd(2 * t);
+    // per-step scalars through pinned host memory (direct DMA, no staging copy on either side)
+    struct Pinned {
+      double* p = nullptr;
+      ~Pinned() {
+        if (p) (void)hipHostFree(p);
+      }
+    } pin;
+    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&pin.p), sizeof(double) * 3 * t, hipHostMallocDefault));
+    double* hd = pin.p;      // [0, 2t): delta_j, beta_{j+1} from the device
+    double* hb = pin.p + 2 * t;  // [0, t): beta_j to the device
     // graph replay needs a capturable (non-null) stream: use a private one when none is given
     struct Graphs {
       bool enabled = false;
@@ -294,11 +303,10 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
                                                                            d_dvec + t, dact.as<int>());
       HIPCHECK(hipGetLastError());
     };
-    std::vector<double> hb(t);
     while (j < maxit - 1) {
       // beta_j of each column to device slot [t, 2t)
       for (int k = 0; k < t; ++k) hb[k] = beta[(size_t)j * t + k];
-      HIPCHECK(hipMemcpyAsync(d_dvec + t, hb.data(), sizeof(double) * t, hipMemcpyHostToDevice, st));
+      HIPCHECK(hipMemcpyAsync(d_dvec + t, hb, sizeof(double) * t, hipMemcpyHostToDevice, st));
       if (graphs.enabled && j >= 1 && !eager_next) {
         hipGraphExec_t& ge = graphs.exec[(j - 1) % 3];
         if (!ge) {
@@ -322,7 +330,7 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
         eager_next = false;
       }
       keep_basis(Unext);
-      HIPCHECK(hipMemcpyAsync(hd.data(), d_dvec, sizeof(double) * 2 * t, hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipMemcpyAsync(hd, d_dvec, sizeof(double) * 2 * t, hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
 
       // CG coefficients (src/reml.cpp:304-307) and per-group stopping (:314-321)
