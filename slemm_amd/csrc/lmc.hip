@@ -150,8 +150,10 @@ __global__ void k_wsq2(const double* __restrict__ part, int64_t nblk, int t, dou
 
 unsigned blocks_of(int64_t rows) { return (unsigned)((rows + 255) / 256); }
 
-// The device state of one LMC fit: V (n x T), the L_Solve solutions X and the combined Py block
-// P (n x (1 + n_MC)), Zc' P (m x (1 + n_MC)) and the FP64 reduction buffers.
+// The device state of one LMC fit: V (n x T), the L_Solve solutions X and, for up to kMaxPts
+// secant points at once, the combined Py blocks P (n x (1 + n_MC) each), Zc' P and the FP64
+// reduction buffers.
+constexpr int kMaxPts = 3;
 struct LmcSystem {
   slemm_geno* h;
   int64_t n, m;
@@ -162,9 +164,9 @@ struct LmcSystem {
       : h(hh), n(hh->n), m(hh->m), n_mc(nmc), T(1 + 2 * nmc), c(cc) {
     dV.reserve(sizeof(float) * n * T);
     dX.reserve(sizeof(float) * n * T);
-    dP.reserve(sizeof(float) * n * (1 + n_mc));
-    dU.reserve(sizeof(float) * m * (1 + n_mc));
-    dots.reserve(sizeof(double) * 2 * (1 + n_mc));
+    dP.reserve(sizeof(float) * n * (1 + n_mc) * kMaxPts);
+    dU.reserve(sizeof(float) * m * (1 + n_mc) * kMaxPts);
+    dots.reserve(sizeof(double) * 2 * (1 + n_mc) * kMaxPts);
   }
 
   // A <- (I - QQ') A on an n x t device block (FP64 Q'A)
@@ -261,38 +263,61 @@ extern "C" int slemm_gpu_lmc_reml_ex(slemm_geno* h, const float* X, int64_t ldx,
     float* P = L.dP.as<float>();
     float* U = L.dU.as<float>();
     double* dots = L.dots.as<double>();
-    std::vector<double> nr(2 * (1 + (size_t)n_MC));
-    // REML_first_order (:696-728)
+    const int w = 1 + n_MC;
+    std::vector<double> nr(2 * (size_t)w * kMaxPts);
+    // REML_first_order (:696-728) at npt heritabilities in one batch: each point's solves and
+    // combined block, then one Zc' pass over all the points' columns (each column's products are
+    // independent of the others in the block)
+    auto first_order_pts = [&](const float* ss, int npt, float* f) {
+      for (int p = 0; p < npt; ++p) {
+        const float tau = (1 - ss[p]) / ss[p];
+        const float sigma = tau - tau0;
+        abi(slemm_gpu_lsolve(sg.s, 0, T, V, n, sigma, 1e-6f, Xs, n, nullptr));
+        k_lmc_combine<<<blocks_of(n), 256>>>(Xs, n, n_MC, std::sqrt(tau), P + (size_t)p * w * n);
+        HIPCHECK(hipGetLastError());
+      }
+      const int t = npt * w;
+      L.project(P, t);
+      col_dots(h, P, n, P, n, n, t, dots, nullptr);  // |e_k|^2
+      L.scale_rr(P, t);
+      abi(slemm_gpu_zt(h, P, n, t, U, m, 0, 0, precision, nullptr));
+      L.wsq(U, m, h->gw, m, t, dots + t);  // |sqrt(gw) Zc' R^-1/2 Py_k|^2
+      HIPCHECK(hipMemcpy(nr.data(), dots, sizeof(double) * 2 * t, hipMemcpyDeviceToHost));
+      for (int p = 0; p < npt; ++p) {
+        const double* e = nr.data() + (size_t)p * w;
+        const double* u = nr.data() + t + (size_t)p * w;
+        double eMC = 0.0, uMC = 0.0;
+        for (int k = 1; k <= n_MC; ++k) {
+          eMC += e[k];
+          uMC += u[k];
+        }
+        f[p] = (float)std::log((u[0] / e[0]) / (uMC / eMC));
+      }
+    };
+    float* Plast = P;  // the last evaluated point's block (its column 0 is R^-1/2 P y)
     auto first_order = [&](float ss) -> float {
       ++evals;
-      const float tau = (1 - ss) / ss;
-      const float sigma = tau - tau0;
-      abi(slemm_gpu_lsolve(sg.s, 0, T, V, n, sigma, 1e-6f, Xs, n, nullptr));
-      k_lmc_combine<<<blocks_of(n), 256>>>(Xs, n, n_MC, std::sqrt(tau), P);
-      HIPCHECK(hipGetLastError());
-      L.project(P, 1 + n_MC);
-      col_dots(h, P, n, P, n, n, 1 + n_MC, dots, nullptr);  // |e_k|^2
-      L.scale_rr(P, 1 + n_MC);
-      abi(slemm_gpu_zt(h, P, n, 1 + n_MC, U, m, 0, 0, precision, nullptr));
-      L.wsq(U, m, h->gw, m, 1 + n_MC, dots + 1 + n_MC);  // |sqrt(gw) Zc' R^-1/2 Py_k|^2
-      HIPCHECK(hipMemcpy(nr.data(), dots, sizeof(double) * nr.size(), hipMemcpyDeviceToHost));
-      const double e0 = nr[0], u0 = nr[1 + n_MC];
-      double eMC = 0.0, uMC = 0.0;
-      for (int k = 1; k <= n_MC; ++k) {
-        eMC += nr[k];
-        uMC += nr[1 + n_MC + k];
-      }
-      return (float)std::log((u0 / e0) / (uMC / eMC));
+      float f = 0.f;
+      first_order_pts(&ss, 1, &f);
+      Plast = P;
+      return f;
     };
 
-    // secant iterations on log((1-h2)/h2) (:821-853)
+    // secant iterations on log((1-h2)/h2) (:821-853).  The second point depends on the sign at
+    // the first (s2max / 4 or s2max), so the first batch evaluates the first point and both
+    // candidates for the second: one Zc' pass instead of two
     float s2 = s2max / 2;
     float log_tau1 = std::log((1 - s2) / s2);
-    float f1 = first_order(s2);
+    const float first3[3] = {s2, s2max / 4, s2max};
+    float fb[3];
+    first_order_pts(first3, 3, fb);
+    evals += 2;
+    float f1 = fb[0];
     if (verbose) printf("  Iter 1: h2=%.4f log((1-h2)/h2)=%.4f funct=%.4f\n", s2, log_tau1, f1);
     s2 = f1 < 0 ? s2max / 4 : s2max;
     float log_tau2 = std::log((1 - s2) / s2);
-    float f2 = first_order(s2);
+    float f2 = f1 < 0 ? fb[1] : fb[2];
+    Plast = P + (size_t)(f1 < 0 ? 1 : 2) * w * n;
     if (verbose) printf("  Iter 2: h2=%.4f log((1-h2)/h2)=%.4f funct=%.4f\n", s2, log_tau2, f2);
     for (int si = 3; si <= secant_max_iter; ++si) {
       float lt = (log_tau1 * f2 - log_tau2 * f1) / (f2 - f1);
@@ -309,14 +334,15 @@ extern "C" int slemm_gpu_lmc_reml_ex(slemm_geno* h, const float* X, int64_t ldx,
     if (std::fabs(f2) > 0.005f && verbose)
       printf("\nWarning: secant iterations not converged. h2 close to 0 or small sample size?\n");
 
-    // Py (the last point's R^-1/2 P y, column 0 of P), variance components, SNP BLUP (:855-866)
-    HIPCHECK(hipMemcpy(Py, P, sizeof(float) * n, hipMemcpyDeviceToHost));
+    // Py (the last point's R^-1/2 P y, column 0 of its block), variance components, SNP BLUP
+    // (:855-866)
+    HIPCHECK(hipMemcpy(Py, Plast, sizeof(float) * n, hipMemcpyDeviceToHost));
     double ypy = 0.0;
     for (int64_t i = 0; i < n; ++i) ypy += (double)y[i] * Py[i];
     const float vg = (float)(ypy / (double)(n - c));
     const float tau1 = (1 - s2) / s2;
     const float ve = tau1 * vg;
-    abi(slemm_gpu_zt(h, P, n, 1, U, m, 1, 0, precision, nullptr));
+    abi(slemm_gpu_zt(h, Plast, n, 1, U, m, 1, 0, precision, nullptr));
     HIPCHECK(hipMemcpy(snp_blup, U, sizeof(float) * m, hipMemcpyDeviceToHost));
     if (verbose) printf("\nVg,%g\nVe,%g\nh2,%g\n", vg, ve, s2);
 
