@@ -92,6 +92,12 @@ void rademacher_probes_device(int rng_seed, int64_t n, int n_V, float* dB, int64
 // n-row block holds draws j n .. j n + n - 1, so blocks of different n share the prefix of one
 // stream; every entry of such a column is -rademacher_scale(n) or +rademacher_scale(n) (signs.cpp).
 std::vector<uint8_t> rademacher_sign_bits(int rng_seed, int64_t total);
+// The same bits into bits (ceil(total / 8) bytes).
+void rademacher_sign_bits_into(int rng_seed, int64_t total, uint8_t* bits);
+
+// Page-locked host memory of at least `bytes`, per host thread and slot (0 Lanczos scalars, 1 LMC
+// draws, 2 probe sign bits), kept for the life of the process (pinned.cpp).
+void* pinned_scratch(int slot, size_t bytes);
 float rademacher_scale(int64_t n);
 
 // REML_criterion at h2 (src/reml.cpp:457-489): returns the 2 logLL value the golden-section

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "blockprod.hpp"
+#include "driver.hpp"
 #include "lanczos.hpp"
 #include "pipeline.hpp"
 #include "tridiag.hpp"
@@ -154,22 +155,6 @@ using namespace slemm;
 
 namespace slemm {
 
-// Pinned host memory for the per-step scalars of the seeds run on this host thread, grown on
-// demand and kept for the life of the process (pinning costs more than a seed's scalars are
-// worth; a seed runs on one thread from start to end, so the thread's buffer is its own).
-static double* pinned_scalars(size_t count) {
-  thread_local double* buf = nullptr;
-  thread_local size_t cap = 0;
-  if (count > cap) {
-    double* p = nullptr;
-    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(double) * count, hipHostMallocDefault));
-    if (buf) (void)hipHostFree(buf);
-    buf = p;
-    cap = count;
-  }
-  return buf;
-}
-
 bool lanczos_graphs() {
   const char* e = std::getenv("SLEMM_LANCZOS_GRAPH");
   return !(e && e[0] == '0');
@@ -271,7 +256,7 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
     float* Unext = bufB.as<float>();
     float* spare = bufC.as<float>();
     // per-step scalars through pinned host memory (direct DMA, no staging copy on either side)
-    double* pin = pinned_scalars(3 * (size_t)t);
+    double* pin = static_cast<double*>(pinned_scratch(0, sizeof(double) * 3 * (size_t)t));
     double* hd = pin;          // [0, 2t): delta_j, beta_{j+1} from the device
     double* hb = pin + 2 * t;  // [0, t): beta_j to the device
     // graph replay needs a capturable (non-null) stream: use a private one when none is given

@@ -236,15 +236,15 @@ extern "C" int slemm_gpu_lmc_reml_ex(slemm_geno* h, const float* X, int64_t ldx,
     // V = (I - QQ') [R^-1/2 y, R^-1/2 Zc (sqrt(gw) u), e] (:788-813); u drawn first, then e
     {
       const int64_t nu = m * n_MC, ne = n * n_MC;
-      std::vector<float> draws((size_t)(nu + ne));
-      normal_draws((uint32_t)(rng_seed + 54321), nu + ne, draws.data());
+      float* draws = static_cast<float*>(pinned_scratch(1, sizeof(float) * (size_t)(nu + ne)));
+      normal_draws((uint32_t)(rng_seed + 54321), nu + ne, draws);
       DevF du((size_t)nu);
-      HIPCHECK(hipMemcpy(du.p, draws.data(), sizeof(float) * nu, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(du.p, draws, sizeof(float) * nu, hipMemcpyHostToDevice));
       k_scale_rows<true><<<blocks_of(m), 256>>>(du.p, m, m, n_MC, h->gw);
       HIPCHECK(hipGetLastError());
       abi(slemm_gpu_z(h, du.p, m, n_MC, V + n, n, precision, nullptr));
       HIPCHECK(hipMemcpy(V, y, sizeof(float) * n, hipMemcpyHostToDevice));
-      HIPCHECK(hipMemcpy(V + (size_t)(1 + n_MC) * n, draws.data() + nu, sizeof(float) * ne, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(V + (size_t)(1 + n_MC) * n, draws + nu, sizeof(float) * ne, hipMemcpyHostToDevice));
     }
     L.scale_rr(V, 1 + n_MC);
     L.project(V, T);

@@ -42,10 +42,12 @@ void rademacher_probes(int rng_seed, int64_t n, int n_V, float* B) {
 // stream.
 void rademacher_probes_device(int rng_seed, int64_t n, int n_V, float* dB, int64_t ldb) {
   if (n <= 0 || n_V <= 0) return;
-  const std::vector<uint8_t> bits = rademacher_sign_bits(rng_seed, n * n_V);
+  const size_t nbytes = (size_t)((n * n_V + 7) / 8);
+  uint8_t* bits = static_cast<uint8_t*>(pinned_scratch(2, nbytes));
+  rademacher_sign_bits_into(rng_seed, n * n_V, bits);
   uint8_t* dbits = nullptr;
-  HIPCHECK(hipMallocAsync(reinterpret_cast<void**>(&dbits), bits.size(), nullptr));
-  hipError_t e = hipMemcpyAsync(dbits, bits.data(), bits.size(), hipMemcpyHostToDevice, nullptr);
+  HIPCHECK(hipMallocAsync(reinterpret_cast<void**>(&dbits), nbytes, nullptr));
+  hipError_t e = hipMemcpyAsync(dbits, bits, nbytes, hipMemcpyHostToDevice, nullptr);
   if (e == hipSuccess) {
     const int64_t total = n * n_V;
     k_probe_cols<<<(unsigned)((total + 255) / 256), 256>>>(dbits, n, n_V, rademacher_scale(n), dB, ldb);

@@ -108,7 +108,12 @@ void mt_fill_blocks(Mt19937Blocks& mt, int64_t nblk, uint32_t* dst) {
 
 std::vector<uint8_t> rademacher_sign_bits(int rng_seed, int64_t total) {
   std::vector<uint8_t> bits((size_t)((std::max<int64_t>(total, 0) + 7) / 8));
-  if (total <= 0) return bits;
+  rademacher_sign_bits_into(rng_seed, total, bits.data());
+  return bits;
+}
+
+void rademacher_sign_bits_into(int rng_seed, int64_t total, uint8_t* bits) {
+  if (total <= 0) return;
   const Mt19937Blocks mt0((uint32_t)(rng_seed + 54321));
   const bool avx2 = __builtin_cpu_supports("avx2");
   auto run = [&](Mt19937Blocks& mt, int64_t count, uint8_t* out) {
@@ -119,8 +124,8 @@ std::vector<uint8_t> rademacher_sign_bits(int rng_seed, int64_t total) {
   const int64_t L = kMtJumpChunk;
   if (!avx2 || threads < 2 || total < 2 * L) {
     Mt19937Blocks mt = mt0;
-    run(mt, total, bits.data());
-    return bits;
+    run(mt, total, bits);
+    return;
   }
   // chunk c: draws [c L, (c + 1) L), the last one to the end of the stream
   const int nch = (int)std::min<int64_t>(kMtJumpPolys + 1, total / L);
@@ -132,9 +137,8 @@ std::vector<uint8_t> rademacher_sign_bits(int rng_seed, int64_t total) {
       mt.set_window(w);
     }
     const int64_t e0 = c * L, e1 = c + 1 == nch ? total : e0 + L;
-    run(mt, e1 - e0, bits.data() + e0 / 8);
+    run(mt, e1 - e0, bits + e0 / 8);
   });
-  return bits;
 }
 
 // 1 / sqrt of the float sum of n unit squares
