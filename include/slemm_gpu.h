@@ -200,9 +200,11 @@ int slemm_gpu_gwas_score(slemm_geno* h, const float* Py, float skip_mac, float* 
  * col_group (host, t entries, or NULL = one group): several independent seed systems in one
  * batch -- each group stops on its own, exactly as a separate L_Seed call would, when
  * max over its columns of |rho|/||b|| <= tol, or after maxit-1 steps; the batch ends when
- * every group has stopped.  keep_cols: the leading keep_cols columns keep every basis
- * vector U_j in HBM (needed by slemm_gpu_lsolve); the other columns keep only their
- * tridiagonal coefficients (SLQ probes; the reference frees their U at :634-636). */
+ * every group has stopped.  Once a group has stopped, the operator runs on the remaining
+ * columns only when that needs fewer MFMA tiles (SLEMM_LSEED_COMPACT=0 keeps every column).
+ * keep_cols: the leading keep_cols columns keep every basis vector U_j in HBM (needed by
+ * slemm_gpu_lsolve); the other columns keep only their tridiagonal coefficients (SLQ probes;
+ * the reference frees their U at :634-636). */
 int slemm_gpu_lseed(slemm_geno* h, int kind, const int* col_kind, const int* col_group, float tau0,
                     const float* B, int64_t ldb, int t, float tol, int maxit, int keep_cols,
                     int precision, int verbose, void* stream, slemm_lseed** out);
