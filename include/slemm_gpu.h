@@ -16,6 +16,8 @@
  *  - `stream` is a hipStream_t passed as void* (NULL = the default stream).  Calls that
  *    return host results synchronise that stream before returning.
  *  - A handle is used by one host thread at a time.
+ *  - The REML drivers keep small page-locked host buffers per calling thread (per-step scalars,
+ *    probe sign bits, Monte Carlo draws), grown on demand and held until the process ends.
  */
 #ifndef SLEMM_GPU_H_
 #define SLEMM_GPU_H_
