@@ -44,6 +44,14 @@ namespace slemm {
 
 constexpr int kRowsPerPart = 4096;
 
+// The per-column passes below read whole float4 groups: a column starts at k ld with ld = n_pad, a
+// multiple of kPad, and a part at a multiple of kRowsPerPart, so every group is 16-byte aligned.
+// Each thread requests its four groups of a part (one per 256-thread sweep) before using any, so a
+// wave keeps four 1 KB loads in flight per array; the rows past the last whole group of a column
+// (n not a multiple of 4) are taken one by one.
+static_assert(kPad % 4 == 0 && kRowsPerPart % 1024 == 0, "float4 groups of the Lanczos passes");
+constexpr int kLzSweeps = 4;  // float4 groups per thread per pass over 1024 groups
+
 // tmp_k -= b_k * Uprev_k (if Uprev); partial dot(Ucur_k, tmp_k).  Inactive columns untouched.
 __global__ __launch_bounds__(256) void k_lz_a(const float* __restrict__ Uprev, const float* __restrict__ Ucur,
                                               float* __restrict__ tmp, int64_t ld, int64_t n,
@@ -56,7 +64,42 @@ __global__ __launch_bounds__(256) void k_lz_a(const float* __restrict__ Uprev, c
   double s = 0.0;
   if (active[k]) {
     const float bk = Uprev ? (float)b[k] : 0.f;
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    const int64_t c0 = (int64_t)k * ld + i0;
+    const int64_t nq = (i1 - i0) >> 2;
+    float4* t4 = reinterpret_cast<float4*>(tmp + c0);
+    const float4* u4 = reinterpret_cast<const float4*>(Ucur + c0);
+    const float4* p4 = Uprev ? reinterpret_cast<const float4*>(Uprev + c0) : nullptr;
+    for (int64_t q0 = threadIdx.x; q0 < nq; q0 += 256 * kLzSweeps) {
+      float4 v[kLzSweeps], u[kLzSweeps], w[kLzSweeps];
+#pragma unroll
+      for (int r = 0; r < kLzSweeps; ++r) {
+        const int64_t q = q0 + 256 * r;
+        if (q < nq) {
+          v[r] = t4[q];
+          u[r] = u4[q];
+          if (p4) w[r] = p4[q];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kLzSweeps; ++r) {
+        const int64_t q = q0 + 256 * r;
+        if (q < nq) {
+          float4 x = v[r];
+          if (p4) {
+            x.x -= w[r].x * bk;
+            x.y -= w[r].y * bk;
+            x.z -= w[r].z * bk;
+            x.w -= w[r].w * bk;
+          }
+          t4[q] = x;
+          s += (double)u[r].x * (double)x.x;
+          s += (double)u[r].y * (double)x.y;
+          s += (double)u[r].z * (double)x.z;
+          s += (double)u[r].w * (double)x.w;
+        }
+      }
+    }
+    for (int64_t i = i0 + 4 * nq + threadIdx.x; i < i1; i += 256) {
       const int64_t o = i + (int64_t)k * ld;
       float v = tmp[o];
       if (Uprev) v -= Uprev[o] * bk;
@@ -84,7 +127,38 @@ __global__ __launch_bounds__(256) void k_lz_b(const float* __restrict__ Ucur, fl
   double s = 0.0;
   if (active[k]) {
     const float dk = (float)d[k];
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    const int64_t c0 = (int64_t)k * ld + i0;
+    const int64_t nq = (i1 - i0) >> 2;
+    float4* t4 = reinterpret_cast<float4*>(tmp + c0);
+    const float4* u4 = reinterpret_cast<const float4*>(Ucur + c0);
+    for (int64_t q0 = threadIdx.x; q0 < nq; q0 += 256 * kLzSweeps) {
+      float4 v[kLzSweeps], u[kLzSweeps];
+#pragma unroll
+      for (int r = 0; r < kLzSweeps; ++r) {
+        const int64_t q = q0 + 256 * r;
+        if (q < nq) {
+          v[r] = t4[q];
+          u[r] = u4[q];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kLzSweeps; ++r) {
+        const int64_t q = q0 + 256 * r;
+        if (q < nq) {
+          float4 x;
+          x.x = v[r].x - u[r].x * dk;
+          x.y = v[r].y - u[r].y * dk;
+          x.z = v[r].z - u[r].z * dk;
+          x.w = v[r].w - u[r].w * dk;
+          t4[q] = x;
+          s += (double)x.x * (double)x.x;
+          s += (double)x.y * (double)x.y;
+          s += (double)x.z * (double)x.z;
+          s += (double)x.w * (double)x.w;
+        }
+      }
+    }
+    for (int64_t i = i0 + 4 * nq + threadIdx.x; i < i1; i += 256) {
       const int64_t o = i + (int64_t)k * ld;
       const float v = tmp[o] - Ucur[o] * dk;
       tmp[o] = v;
@@ -100,24 +174,34 @@ __global__ __launch_bounds__(256) void k_lz_b(const float* __restrict__ Ucur, fl
   if (threadIdx.x == 0) part[(int64_t)blockIdx.x * t + k] = red[0];
 }
 
-// sums partials to res[k] (double), optionally sqrt
-__global__ void k_lz_sum(const double* __restrict__ part, int64_t nparts, int t, double* __restrict__ res,
-                         int take_sqrt) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= t) return;
+// res[k] = the sum of column k's partials (double), optionally its square root.  One wave per
+// column: lane l adds parts l, l + 64, ..., then a butterfly over the wave (at a million rows a
+// column has 245 parts, too long a chain of dependent loads for one thread).
+__global__ __launch_bounds__(64) void k_lz_sum(const double* __restrict__ part, int64_t nparts, int t,
+                                               double* __restrict__ res, int take_sqrt) {
+  const int k = blockIdx.x;
   double s = 0.0;
-  for (int64_t p = 0; p < nparts; ++p) s += part[p * t + k];
-  res[k] = take_sqrt ? sqrt(s) : s;
+  for (int64_t p = threadIdx.x; p < nparts; p += 64) s += part[p * t + k];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (threadIdx.x == 0) res[k] = take_sqrt ? sqrt(s) : s;
 }
 
-// Unext_k = tmp_k / b_k (active columns); scale = 1/b in FP32 like the reference's division
+// Unext_k = tmp_k / b_k (active columns), a division in FP32 like the reference's; thread q takes
+// rows 4q .. 4q + 3 (those below n)
 __global__ void k_lz_c(const float* __restrict__ tmp, float* __restrict__ Unext, int64_t ld, int64_t n,
                        const double* __restrict__ b, const int* __restrict__ active) {
   const int k = blockIdx.y;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= n || !active[k]) return;
   const int64_t o = i + (int64_t)k * ld;
-  Unext[o] = tmp[o] / (float)b[k];
+  const float bk = (float)b[k];
+  if (i + 4 <= n) {
+    const float4 v = *reinterpret_cast<const float4*>(tmp + o);
+    *reinterpret_cast<float4*>(Unext + o) = make_float4(v.x / bk, v.y / bk, v.z / bk, v.w / bk);
+  } else {
+    for (int64_t e = 0; i + e < n; ++e) Unext[o + e] = tmp[o + e] / bk;
+  }
 }
 
 // copies n x t (ldb) into n x t (ld) and scales column k by 1/bnorm_k
@@ -171,6 +255,7 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
     REQUIRE(tol > 0, "tolerance must be positive");
     HIPCHECK(hipSetDevice(h->device));
     const int64_t n = h->n, ld = h->n_pad;
+    REQUIRE(ld % 4 == 0, "the Lanczos passes take rows in float4 groups");
 
     std::vector<int> group(t, 0);
     int ngroups = 1;
@@ -220,7 +305,7 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
       k_lz_a<<<dim3((unsigned)nparts, (unsigned)t), 256, 0, st>>>(nullptr, bcopy.as<float>(), bcopy.as<float>(), ld,
                                                                  n, d_dvec, dact.as<int>(), part.as<double>(), t);
       HIPCHECK(hipGetLastError());
-      k_lz_sum<<<(t + 63) / 64, 64, 0, st>>>(part.as<double>(), nparts, t, d_dvec, 1);
+      k_lz_sum<<<t, 64, 0, st>>>(part.as<double>(), nparts, t, d_dvec, 1);
       HIPCHECK(hipGetLastError());
       HIPCHECK(hipMemcpyAsync(bnorm.data(), d_dvec, sizeof(double) * t, hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
@@ -289,12 +374,12 @@ slemm_lseed* lseed_run(slemm_geno* h, const LanczosOp& apply_op, const int* col_
       k_lz_a<<<dim3((unsigned)nparts, (unsigned)t), 256, 0, s>>>(Uprev, Ucur, tmp.as<float>(), ld, n, d_dvec + t,
                                                                 dact.as<int>(), part.as<double>(), t);
       HIPCHECK(hipGetLastError());
-      k_lz_sum<<<(t + 63) / 64, 64, 0, s>>>(part.as<double>(), nparts, t, d_dvec, 0);  // delta_j
+      k_lz_sum<<<t, 64, 0, s>>>(part.as<double>(), nparts, t, d_dvec, 0);  // delta_j
       k_lz_b<<<dim3((unsigned)nparts, (unsigned)t), 256, 0, s>>>(Ucur, tmp.as<float>(), ld, n, d_dvec,
                                                                 dact.as<int>(), part.as<double>(), t);
       HIPCHECK(hipGetLastError());
-      k_lz_sum<<<(t + 63) / 64, 64, 0, s>>>(part.as<double>(), nparts, t, d_dvec + t, 1);  // beta_{j+1}
-      k_lz_c<<<dim3((unsigned)((n + 255) / 256), (unsigned)t), 256, 0, s>>>(tmp.as<float>(), Unext, ld, n,
+      k_lz_sum<<<t, 64, 0, s>>>(part.as<double>(), nparts, t, d_dvec + t, 1);  // beta_{j+1}
+      k_lz_c<<<dim3((unsigned)((n + 1023) / 1024), (unsigned)t), 256, 0, s>>>(tmp.as<float>(), Unext, ld, n,
                                                                            d_dvec + t, dact.as<int>());
       HIPCHECK(hipGetLastError());
     };
