@@ -282,11 +282,26 @@ void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const floa
   const Layout L = layout_for(h, t, precision);
   bool any_quad = false;
   const int* kinds = upload_kinds(h, kind, col_kind, t, any_quad, st);
+  // the span [qlo, qlo + tq) of the QUADFORM columns: the projections and the finish run on it
+  // alone (LDET columns, the probes, need neither)
+  int qlo = 0, tq = 0;
+  if (any_quad) {
+    int qhi = -1;
+    qlo = t;
+    for (int k = 0; k < t; ++k)
+      if (h->kinds_host[k] == SLEMM_OP_QUADFORM) {
+        qlo = std::min(qlo, k);
+        qhi = k;
+      }
+    tq = qhi - qlo + 1;
+  }
 
   const double* qtv = nullptr;
   if (any_quad && h->c > 0) {
-    h->qtv.reserve(sizeof(double) * h->c * L.KP);  // entries k < t are all written and the only ones read
-    tn_product(h->Q, h->n_pad, h->c, V, ldv, t, h->n, h->qtv.as<double>(), L.KP, h->redpart, st);
+    // entries k in the span are written; pack_pass1 reads them for QUADFORM columns only
+    h->qtv.reserve(sizeof(double) * h->c * L.KP);
+    tn_product(h->Q, h->n_pad, h->c, V + (int64_t)qlo * ldv, ldv, tq, h->n, h->qtv.as<double>() + qlo, L.KP,
+               h->redpart, st);
     qtv = h->qtv.as<double>();
   }
   pack_pass1(h, L, V, ldv, t, h->rr, qtv, kinds, precision, st);
@@ -332,10 +347,12 @@ void run_op(slemm_geno* h, int kind, const int* col_kind, float tau0, const floa
     double* bq = nullptr;
     if (h->c > 0) {
       h->qtv.reserve(sizeof(double) * h->c * L.KP);
-      tn_product(h->Q, h->n_pad, h->c, out, ldo, t, h->n, h->qtv.as<double>(), L.KP, h->redpart, st);
-      bq = h->qtv.as<double>();
+      tn_product(h->Q, h->n_pad, h->c, out + (int64_t)qlo * ldo, ldo, tq, h->n, h->qtv.as<double>() + qlo, L.KP,
+                 h->redpart, st);
+      bq = h->qtv.as<double>() + qlo;
     }
-    launch_quad_finish(out, ldo, V, ldv, h->Q, h->n_pad, h->c, bq, L.KP, kinds, t, h->n, tau0, st);
+    launch_quad_finish(out + (int64_t)qlo * ldo, ldo, V + (int64_t)qlo * ldv, ldv, h->Q, h->n_pad, h->c, bq, L.KP,
+                       kinds + qlo, tq, h->n, tau0, st);
   }
 }
 
